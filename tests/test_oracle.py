@@ -1,0 +1,326 @@
+"""Pins of the CPU oracle (the checker) against the reference's own fixtures and the mathematics.
+
+Reference anchors: tests/test_fmm.py:14-59 (golden direct fields + FMM error columns),
+tests/test_functionality.py:38-46 (nuclear-multipole energy), tests/test_fields.py:17-31
+(nuclear field closed form), tests/test_solver.py:36-146 (CG vs dense inverse, damped and not).
+"""
+import functools
+import itertools
+import os
+
+import numpy as np
+import pytest
+import sympy as sp
+
+import oracle as O
+
+from .conftest import GOLDEN
+
+
+@functools.lru_cache(None)
+def loprop():
+    return O.read_potfile(os.path.join(GOLDEN, "loprop_solvated_20.pot.gz"))
+
+
+@functools.lru_cache(None)
+def loprop_ref():
+    d = np.load(os.path.join(GOLDEN, "loprop_fmm_ref.npz"))
+    return d["fmuls"], list(d["err_columns"]), d["errors"]
+
+
+@functools.lru_cache(None)
+def pna():
+    m = np.load(os.path.join(GOLDEN, "molecules.npz"))
+    return m["pna_z"].astype(float), m["pna_xyz"]
+
+
+# ---------------------------------------------------------------------------------------------
+# symbolic references (sympy): Thole-screened potential f_V(aR)/R, gen_tensors.py:47-66
+# ---------------------------------------------------------------------------------------------
+X, Y, Z, A = sp.symbols("x y z a", real=True)
+RR = sp.sqrt(X**2 + Y**2 + Z**2)
+G_PLAIN = 1 / RR
+G_THOLE = (1 - (A * RR / 2 + 1) * sp.exp(-A * RR)) / RR
+XYZ = (X, Y, Z)
+
+
+def _multi(k):
+    return [(a, b, k - a - b) for a in range(k, -1, -1) for b in range(k - a, -1, -1)]
+
+
+@functools.lru_cache(None)
+def _deriv_fn(damped, a, b, c):
+    G = G_THOLE if damped else G_PLAIN
+    expr = sp.diff(G, X, a, Y, b, Z, c) if a + b + c else G
+    return sp.lambdify((X, Y, Z, A), expr, "math")
+
+
+def sym_field(m, d, S, a=0.0, damped=False):
+    """F = -grad sum_j S_j d^j G evaluated symbolically (j over order-m multi-indices)."""
+    F = np.zeros(3)
+    for q, (ja, jb, jc) in enumerate(_multi(m)):
+        for comp in range(3):
+            e = [ja, jb, jc]
+            e[comp] += 1
+            F[comp] -= S[q] * _deriv_fn(damped, *e)(d[0], d[1], d[2], a)
+    return F
+
+
+def sym_T2(d, a=0.0, damped=False):
+    T = np.zeros((3, 3))
+    for i in range(3):
+        for j in range(3):
+            e = [0, 0, 0]
+            e[i] += 1
+            e[j] += 1
+            T[i, j] = _deriv_fn(damped, *e)(d[0], d[1], d[2], a)
+    return T
+
+
+# ---------------------------------------------------------------------------------------------
+def test_direct_multipole_fields_golden():
+    """Direct static multipole fields reproduce fmuls_loprop_solvated_20.txt (test_fmm.py:19-24)."""
+    fmuls, _, _ = loprop_ref()
+    F = O.multipole_fields(loprop())
+    np.testing.assert_allclose(F, fmuls, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("theta,order", [(0.5, 3), (0.5, 5), (0.7, 3), (0.7, 5), (0.99, 3), (0.99, 5),
+                                         (0.5, 7), (0.2, 5), (0.3, 3)])
+def test_fmm_error_columns(theta, order):
+    """Per-site relative FMM error equals ref_fmm_errors.csv (test_fmm.py:57-59, atol 1e-8).
+
+    Any difference in tree cells, MAC decisions, list membership or operator truncation moves
+    these errors far beyond 1e-8."""
+    fmuls, cols, errs = loprop_ref()
+    F = O.multipole_fields(loprop(), scheme="fmm", theta=theta, order=order)
+    fs = fmuls.reshape(-1, 3)
+    err = np.linalg.norm(fs - F.reshape(-1, 3), axis=1) / np.linalg.norm(fs, axis=1)
+    np.testing.assert_allclose(errs[:, cols.index(f"err_F_{theta}_{order}")], err, atol=1e-8)
+
+
+def test_nuclear_multipole_energy_golden():
+    z, xyz = pna()
+    pot = O.read_potfile(os.path.join(GOLDEN, "pna_6w.pot"))
+    E = O.nuclear_multipole_energy(pot, z, xyz)
+    np.testing.assert_almost_equal(E, -0.321349401430, decimal=9)
+
+
+def test_nuclear_fields_closed_form():
+    z, xyz = pna()
+    pot = O.read_potfile(os.path.join(GOLDEN, "pna_6w.pot"))
+    F = O.nuclear_fields(pot.pos, z, xyz)
+    ref = np.zeros((pot.n, 3))
+    for i in range(pot.n):
+        for a in range(len(z)):
+            d = pot.pos[i] - xyz[a]
+            ref[i] += z[a] * d / np.linalg.norm(d) ** 3
+    np.testing.assert_allclose(F, ref.ravel(), atol=1e-14)
+
+
+@pytest.mark.parametrize("m", [0, 1, 2])
+@pytest.mark.parametrize("damped", [False, True])
+def test_pair_field_vs_sympy(m, damped):
+    rng = np.random.default_rng(10 + m)
+    for _ in range(12):
+        d = rng.normal(size=3) * rng.uniform(0.5, 6.0)
+        S = rng.normal(size=O.ncomp(m))
+        a = rng.uniform(0.2, 2.0)
+        F = O.pair_field(m, d, S, a, damped)
+        Fs = sym_field(m, d, S, a, damped)
+        np.testing.assert_allclose(F, Fs, rtol=1e-12, atol=1e-14 * np.abs(Fs).max())
+
+
+def test_coulomb_derivatives_vs_sympy():
+    rng = np.random.default_rng(3)
+    p = 6
+    for _ in range(3):
+        r = rng.normal(size=3) * 3
+        D = O.coulomb_derivs(p, r)
+        k = 0
+        for order in range(p + 1):
+            for (a, b, c) in _multi(order):
+                ref = _deriv_fn(False, a, b, c)(r[0], r[1], r[2], 0.0)
+                assert abs(D[k] - ref) <= 1e-12 * max(1.0, abs(ref)), (a, b, c)
+                k += 1
+
+
+def _dense_B_sympy(pos, pol, excl, damping):
+    """Dense B from the symbolic T2 (independent of the oracle's closed form), as the
+    reference test builds it (test_solver.py:50-64,105-121)."""
+    n = len(pos)
+    B = np.zeros((3 * n, 3 * n))
+    iso = (pol[:, 0] + pol[:, 3] + pol[:, 5]) / 3.0
+    for i in range(n):
+        a = np.array([[pol[i, 0], pol[i, 1], pol[i, 2]], [pol[i, 1], pol[i, 3], pol[i, 4]],
+                      [pol[i, 2], pol[i, 4], pol[i, 5]]])
+        B[3 * i:3 * i + 3, 3 * i:3 * i + 3] = np.linalg.inv(a)
+        for j in range(n):
+            if i == j or j in excl[i]:
+                continue
+            v = damping / (iso[i] * iso[j]) ** (1 / 6) if damping > 0 else 0.0
+            B[3 * i:3 * i + 3, 3 * j:3 * j + 3] = -sym_T2(pos[j] - pos[i], v, damping > 0)
+    return B
+
+
+@pytest.mark.parametrize("damping", [0.0, 2.1304])
+def test_solver_vs_dense_inverse(damping):
+    z, xyz = pna()
+    pot = O.read_potfile(os.path.join(GOLDEN, "pna_6w.pot"))
+    F = O.nuclear_fields(pot.pos, z, xyz) + O.multipole_fields(pot)
+    Bs = _dense_B_sympy(pot.pos, pot.pol, pot.excl, damping)
+    B = O.BSystem(pot.pos, pot.pol, pot.excl_csr(), damping=damping)
+    np.testing.assert_allclose(B.dense(), Bs, atol=1e-13)
+    np.testing.assert_allclose(B.apply(F), Bs @ F, atol=1e-12)
+    mu, it, norms = B.solve(F, thresh=1e-12)
+    ref = np.linalg.inv(Bs) @ F
+    np.testing.assert_almost_equal(mu, ref, decimal=9)
+    np.testing.assert_allclose(mu, ref, atol=1e-10)
+    assert norms[-1] < 1e-12 and it == len(norms)
+
+
+def test_solver_nonconvergence_message():
+    pot = O.read_potfile(os.path.join(GOLDEN, "pna_6w.pot"))
+    z, xyz = pna()
+    F = O.nuclear_fields(pot.pos, z, xyz)
+    B = O.BSystem(pot.pos, pot.pol, pot.excl_csr())
+    with pytest.raises(RuntimeError, match="Failed to converge induced dipole moments."):
+        B.solve(F, maxiter=1, thresh=1e-14)
+
+
+def test_tree_invariants():
+    """Every site in exactly one leaf, leaves hold < ncrit sites, a cell is split iff its region
+    holds >= ncrit sites, and P2P + M2L cover each ordered (target, source) site pair once."""
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(-20, 20, size=(3000, 3))
+    pos[:500] *= 0.2  # denser core -> uneven depth
+    t = O.Tree(pos, ncrit=32, order=4, theta=0.5)
+    geo, meta, leafs = t.cells()
+    nchild = meta[:, 2]
+    nleaf = meta[:, 3]
+    is_leaf = nchild == 0
+    members = leafs[is_leaf]
+    members = members[members >= 0]
+    assert np.array_equal(np.sort(members), np.arange(3000))
+    assert (nleaf[is_leaf] < 32).all() and (nleaf[~is_leaf] >= 32).all()
+    # region counts: a site is inside a cell's cube (strict > towards the upper half)
+    site_leaf = np.empty(3000, dtype=np.int64)
+    for c in np.nonzero(is_leaf)[0]:
+        for s in leafs[c][leafs[c] >= 0]:
+            site_leaf[s] = c
+    # cover: for every leaf pair in P2P or M2L (with descendants) count each site pair once
+    m2l, p2p = t.lists()
+    parent = meta[:, 0]
+    anc = [[] for _ in range(len(geo))]
+    for c in range(len(geo)):
+        a = c
+        chain = [c]
+        while a != 0:
+            a = parent[a]
+            chain.append(a)
+        anc[c] = chain
+    pair_cells = set()
+    for (B_, A_) in m2l:
+        pair_cells.add((int(A_), int(B_)))
+    for (A_, B_) in p2p:
+        pair_cells.add((int(A_), int(B_)))
+    sample = rng.integers(0, 3000, size=(4000, 2))
+    for i, j in sample:
+        hits = 0
+        for ca in anc[site_leaf[i]]:
+            for cb in anc[site_leaf[j]]:
+                if (ca, cb) in pair_cells:
+                    hits += 1
+        assert hits == 1
+
+
+def test_operators_vs_reference_kernels():
+    """The oracle's generic Cartesian operators equal the reference's generated fmmgen kernels
+    (compiled from the reference sources into oracle/_ref) on random shifts."""
+    R = O.ref_kernels()
+    if R is None:
+        pytest.skip("oracle/_ref not built (reference checkout absent)")
+    import ctypes  # noqa: F401
+    rng = np.random.default_rng(7)
+    for m in range(3):
+        for p in (3, 5):
+            # build an FMM on a tiny random system with the reference's P2M..L2P and compare
+            pos = rng.uniform(-4, 4, size=(400, 3))
+            S = rng.normal(size=(400, O.ncomp(m)))
+            tree = O.Tree(pos, ncrit=16, order=p, theta=0.6)
+            F_orc = tree.field(m, S)
+            F_ref = _fmm_with_ref_kernels(R, tree, pos, S, m, p)
+            np.testing.assert_allclose(F_orc, F_ref, rtol=1e-11, atol=1e-12 * np.abs(F_ref).max())
+
+
+def _nterms(p):
+    return (p + 1) * (p + 2) * (p + 3) // 6
+
+
+def _fmm_with_ref_kernels(R, tree, pos, S, m, p):
+    """Drive the reference's operators over the oracle's tree (tree.cc:359-385 pass order)."""
+    dp = O._d
+    geo, meta, leafs = tree.cells()
+    m2l, p2p = tree.lists()
+    nc = len(geo)
+    ms = _nterms(p) - _nterms(m - 1)
+    ls = _nterms(p - m)
+    M = np.zeros((nc, ms))
+    L = np.zeros((nc, ls))
+    F = np.zeros((len(pos), 3))
+    leafcells = [c for c in range(nc) if meta[c, 2] == 0]
+    for c in leafcells:
+        for l in leafs[c][leafs[c] >= 0]:
+            Q = np.zeros(ms)
+            Q[:O.ncomp(m)] = S[l]
+            d = geo[c, :3] - pos[l]
+            R.ref_m2m(m, p, *d, Q.ctypes.data_as(dp), M[c].ctypes.data_as(dp))
+    for c in range(nc - 1, 0, -1):
+        pa = meta[c, 0]
+        d = geo[pa, :3] - geo[c, :3]
+        Mc = M[c].copy()
+        Mp = M[pa].copy()
+        R.ref_m2m(m, p, *d, Mc.ctypes.data_as(dp), Mp.ctypes.data_as(dp))
+        M[pa] = Mp
+    for (B_, A_) in m2l:
+        d = geo[A_, :3] - geo[B_, :3]
+        Mb = M[B_].copy()
+        La = L[A_].copy()
+        R.ref_m2l(m, p, *d, Mb.ctypes.data_as(dp), La.ctypes.data_as(dp))
+        L[A_] = La
+    for (A_, B_) in p2p:
+        for l1 in leafs[A_][leafs[A_] >= 0]:
+            for l2 in leafs[B_][leafs[B_] >= 0]:
+                if l1 == l2:
+                    continue
+                Sl = S[l2].copy()
+                f = np.zeros(3)
+                R.ref_p2p(m, *(pos[l1] - pos[l2]), Sl.ctypes.data_as(dp), f.ctypes.data_as(dp))
+                F[l1] += f
+    children = {}
+    for c in range(1, nc):
+        children.setdefault(int(meta[c, 0]), []).append(c)
+    for pa in range(nc):
+        for c in children.get(pa, []):
+            d = geo[c, :3] - geo[pa, :3]
+            Lp = L[pa].copy()
+            Lc = L[c].copy()
+            R.ref_l2l(m, p, *d, Lp.ctypes.data_as(dp), Lc.ctypes.data_as(dp))
+            L[c] = Lc
+    for c in leafcells:
+        for l in leafs[c][leafs[c] >= 0]:
+            f = np.zeros(3)
+            Lc = L[c].copy()
+            R.ref_l2p(m, p, *(pos[l] - geo[c, :3]), Lc.ctypes.data_as(dp), f.ctypes.data_as(dp))
+            F[l] += f
+    return F.ravel()
+
+
+def test_potfile_reader_fields():
+    pot = O.read_potfile(os.path.join(GOLDEN, "pna_6w.pot"))
+    assert pot.n == 18 and pot.polarizable.all()
+    np.testing.assert_allclose(pot.pos[0], np.array([9.371, 2.953, -6.078]) * O.ANG2BOHR)
+    assert pot.excl[0] == [1, 2] and pot.excl[17] == [15, 16]
+    # quadrupole trace removed on read (potential.hh:23-34)
+    th = pot.mul[:, 4:10]
+    np.testing.assert_allclose(th[:, 0] + th[:, 3] + th[:, 5], 0.0, atol=1e-14)
