@@ -1,0 +1,235 @@
+"""ctypes binding of the C ABI in include/cppe_hip.h (libcppe_hip.so, built in-tree).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libcppe_hip.so")
+
+OK, ERR_ARG, ERR_DEVICE, ERR_NOCONV, ERR_TREE, ERR_UNSUPPORTED = 0, 1, 2, 3, 4, 5
+SUM_DIRECT, SUM_FMM = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+
+EXPORTS = [
+    "cppe_hip_last_error", "cppe_hip_device_count", "cppe_hip_system_create", "cppe_hip_system_destroy",
+    "cppe_hip_field", "cppe_hip_multipole_fields", "cppe_hip_apply_b", "cppe_hip_apply_alpha",
+    "cppe_hip_apply_alpha_inv", "cppe_hip_dense_b", "cppe_hip_dense_b_inverse", "cppe_hip_solve", "cppe_hip_upload_rhs",
+    "cppe_hip_solve_resident", "cppe_hip_download_mu", "cppe_hip_nuclear_fields",
+    "cppe_hip_nuclear_multipole_energy", "cppe_hip_dot", "cppe_hip_tree_stats", "cppe_hip_tree_export",
+]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("summation", ctypes.c_int), ("ncrit", ctypes.c_int), ("order", ctypes.c_int),
+                ("theta", ctypes.c_double), ("damping", ctypes.c_double), ("precision", ctypes.c_int)]
+
+
+class CppeHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+_lib = None
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int)
+_ll = ctypes.POINTER(ctypes.c_longlong)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m cppe_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.cppe_hip_last_error.restype = ctypes.c_char_p
+        vp = ctypes.c_void_p
+        L.cppe_hip_device_count.argtypes = [_i]
+        L.cppe_hip_system_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, _d, _d, _d, _i, _i,
+                                             ctypes.POINTER(Options)]
+        L.cppe_hip_system_destroy.argtypes = [vp]
+        L.cppe_hip_field.argtypes = [vp, ctypes.c_int, _d, _d]
+        L.cppe_hip_multipole_fields.argtypes = [vp, _d, _d, _d, _d]
+        for f in ("cppe_hip_apply_b", "cppe_hip_apply_alpha", "cppe_hip_apply_alpha_inv"):
+            getattr(L, f).argtypes = [vp, _d, _d]
+        L.cppe_hip_dense_b.argtypes = [vp, _i, _d]
+        L.cppe_hip_dense_b_inverse.argtypes = [vp, _i, _d]
+        L.cppe_hip_solve.argtypes = [vp, _d, _d, ctypes.c_int, ctypes.c_double, ctypes.c_int, _d, _i, _d]
+        L.cppe_hip_upload_rhs.argtypes = [vp, _d]
+        L.cppe_hip_solve_resident.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _i, _d, _d]
+        L.cppe_hip_download_mu.argtypes = [vp, _d]
+        L.cppe_hip_nuclear_fields.argtypes = [ctypes.c_int, _d, ctypes.c_int, _d, _d, _d]
+        L.cppe_hip_nuclear_multipole_energy.argtypes = [ctypes.c_int, _d, _i, _d, ctypes.c_int, _d, _d, _d]
+        L.cppe_hip_dot.argtypes = [ctypes.c_longlong, _d, _d, _d]
+        L.cppe_hip_tree_stats.argtypes = [vp, _ll]
+        L.cppe_hip_tree_export.argtypes = [vp, _d, _ll, _ll, _i, _ll, _ll]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != OK:
+        raise CppeHipError(rc, lib().cppe_hip_last_error().decode())
+
+
+def _p(a, t=_d):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def device_count():
+    c = ctypes.c_int(0)
+    lib().cppe_hip_device_count(ctypes.byref(c))
+    return c.value
+
+
+class System:
+    """One device-resident set of sites (cppe_hip_system)."""
+
+    def __init__(self, pos, alpha6=None, alpha_iso=None, excl=None, summation="direct", ncrit=64, order=5,
+                 theta=0.5, damping=0.0, precision="fp64"):
+        self.pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        self.n = self.pos.size // 3
+        a6 = None if alpha6 is None else np.ascontiguousarray(alpha6, dtype=np.float64).reshape(-1)
+        ai = None if alpha_iso is None else np.ascontiguousarray(alpha_iso, dtype=np.float64)
+        off = idx = None
+        if excl is not None:
+            off = np.ascontiguousarray(excl[0], dtype=np.int32)
+            idx = np.ascontiguousarray(excl[1], dtype=np.int32)
+            if idx.size == 0:
+                idx = np.zeros(1, dtype=np.int32)
+        self.opts = Options(SUM_FMM if summation == "fmm" else SUM_DIRECT, int(ncrit), int(order), float(theta),
+                            float(damping), PREC_MIXED if precision == "mixed" else PREC_FP64)
+        h = ctypes.c_void_p()
+        _check(lib().cppe_hip_system_create(ctypes.byref(h), self.n, _p(self.pos), _p(a6), _p(ai), _p(off, _i),
+                                            _p(idx, _i), ctypes.byref(self.opts)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cppe_hip_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _vec_op(self, fn, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(3 * self.n)
+        _check(getattr(lib(), fn)(self.h, _p(x), _p(out)))
+        return out
+
+    def apply_b(self, x):
+        return self._vec_op("cppe_hip_apply_b", x)
+
+    def apply_alpha(self, x):
+        return self._vec_op("cppe_hip_apply_alpha", x)
+
+    def apply_alpha_inv(self, x):
+        return self._vec_op("cppe_hip_apply_alpha_inv", x)
+
+    def field(self, m, S):
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        out = np.zeros(3 * self.n)
+        _check(lib().cppe_hip_field(self.h, int(m), _p(S), _p(out)))
+        return out
+
+    def multipole_fields(self, q, mu=None, theta6=None):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        mu = None if mu is None else np.ascontiguousarray(mu, dtype=np.float64)
+        th = None if theta6 is None else np.ascontiguousarray(theta6, dtype=np.float64)
+        out = np.zeros(3 * self.n)
+        _check(lib().cppe_hip_multipole_fields(self.h, _p(q), _p(mu), _p(th), _p(out)))
+        return out
+
+    def dense_b(self, index=None):
+        index = np.ascontiguousarray(np.arange(self.n) if index is None else index, dtype=np.int32)
+        out = np.zeros((3 * self.n, 3 * self.n))
+        _check(lib().cppe_hip_dense_b(self.h, _p(index, _i), _p(out)))
+        return out
+
+    def dense_b_inverse(self, index=None):
+        index = np.ascontiguousarray(np.arange(self.n) if index is None else index, dtype=np.int32)
+        out = np.zeros((3 * self.n, 3 * self.n))
+        _check(lib().cppe_hip_dense_b_inverse(self.h, _p(index, _i), _p(out)))
+        return out
+
+    def solve(self, rhs, guess=None, make_guess=True, thresh=1e-8, maxiter=50, raise_on_noconv=True):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        g = None if guess is None else np.ascontiguousarray(guess, dtype=np.float64)
+        mu = np.zeros(3 * self.n)
+        it = ctypes.c_int(0)
+        norms = np.zeros(max(maxiter, 1))
+        rc = lib().cppe_hip_solve(self.h, _p(rhs), _p(g), int(make_guess), float(thresh), int(maxiter), _p(mu),
+                                  ctypes.byref(it), _p(norms))
+        if rc != OK and (raise_on_noconv or rc != ERR_NOCONV):
+            _check(rc)
+        return mu, it.value, norms[:it.value]
+
+    def upload_rhs(self, rhs):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        _check(lib().cppe_hip_upload_rhs(self.h, _p(rhs)))
+
+    def solve_resident(self, guess_mode=0, thresh=1e-8, maxiter=50):
+        it = ctypes.c_int(0)
+        norms = np.zeros(max(maxiter, 1))
+        ms = ctypes.c_double(0.0)
+        _check(lib().cppe_hip_solve_resident(self.h, int(guess_mode), float(thresh), int(maxiter),
+                                             ctypes.byref(it), _p(norms), ctypes.byref(ms)))
+        return it.value, norms[:it.value], ms.value
+
+    def download_mu(self):
+        mu = np.zeros(3 * self.n)
+        _check(lib().cppe_hip_download_mu(self.h, _p(mu)))
+        return mu
+
+    def tree_stats(self):
+        s = np.zeros(6, dtype=np.int64)
+        _check(lib().cppe_hip_tree_stats(self.h, _p(s, _ll)))
+        return dict(ncells=int(s[0]), nleaves=int(s[1]), n_m2l=int(s[2]), n_p2p=int(s[3]), max_level=int(s[4]),
+                    p2p_site_pairs=int(s[5]))
+
+    def tree_export(self):
+        st = self.tree_stats()
+        nc = st["ncells"]
+        geo = np.zeros((nc, 5))
+        meta = np.zeros((nc, 4), dtype=np.int64)
+        start = np.zeros(nc, dtype=np.int64)
+        perm = np.zeros(self.n, dtype=np.int32)
+        m2l = np.zeros((max(st["n_m2l"], 1), 2), dtype=np.int64)
+        p2p = np.zeros((max(st["n_p2p"], 1), 2), dtype=np.int64)
+        _check(lib().cppe_hip_tree_export(self.h, _p(geo), _p(meta, _ll), _p(start, _ll), _p(perm, _i),
+                                          _p(m2l, _ll), _p(p2p, _ll)))
+        return dict(geo=geo, meta=meta, start=start, perm=perm, m2l=m2l[:st["n_m2l"]], p2p=p2p[:st["n_p2p"]])
+
+
+def nuclear_fields(pos, charges, atom_pos):
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+    ch = np.ascontiguousarray(charges, dtype=np.float64)
+    ap = np.ascontiguousarray(atom_pos, dtype=np.float64).reshape(-1)
+    F = np.zeros(pos.size)
+    _check(lib().cppe_hip_nuclear_fields(pos.size // 3, _p(pos), ch.size, _p(ch), _p(ap), _p(F)))
+    return F
+
+
+def nuclear_multipole_energy(pos, kmax, mul, charges, atom_pos):
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+    km = np.ascontiguousarray(kmax, dtype=np.int32)
+    mul = np.ascontiguousarray(mul, dtype=np.float64).reshape(-1)
+    ch = np.ascontiguousarray(charges, dtype=np.float64)
+    ap = np.ascontiguousarray(atom_pos, dtype=np.float64).reshape(-1)
+    e = ctypes.c_double(0.0)
+    _check(lib().cppe_hip_nuclear_multipole_energy(pos.size // 3, _p(pos), _p(km, _i), _p(mul), ch.size, _p(ch),
+                                                   _p(ap), ctypes.byref(e)))
+    return e.value
+
+
+def dot(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = ctypes.c_double(0.0)
+    _check(lib().cppe_hip_dot(a.size, _p(a), _p(b), ctypes.byref(out)))
+    return out.value

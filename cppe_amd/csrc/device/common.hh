@@ -1,0 +1,59 @@
+// Shared device/host helpers of the gfx950 PE field library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace cppe_dev {
+
+struct hip_error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define CPPE_HIP_CHECK(expr)                                                                 \
+  do {                                                                                       \
+    hipError_t _e = (expr);                                                                  \
+    if (_e != hipSuccess)                                                                    \
+      throw ::cppe_dev::hip_error(std::string(#expr) + ": " + hipGetErrorString(_e) + " (" + \
+                                  __FILE__ + ":" + std::to_string(__LINE__) + ")");          \
+  } while (0)
+
+inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------------------------------
+// Scalar math, FP64 and FP32.
+// ------------------------------------------------------------------------------------------
+
+// 1/sqrt(r2) in FP64: v_rsq_f64 seed (relative error below 2^-22) refined by one third-order
+// Newton-Halley step y1 = y0 (1 + e/2 + 3e^2/8), e = 1 - r2 y0^2, which leaves an error of
+// order e^3 (below 2^-66) plus rounding.
+__device__ __forceinline__ double rsqrt_f64(double r2) {
+  double y = __builtin_amdgcn_rsq(r2);
+  double e = __builtin_fma(-r2 * y, y, 1.0);
+  return __builtin_fma(y * e, __builtin_fma(0.375, e, 0.5), y);
+}
+
+__device__ __forceinline__ float rsqrt_f32(float r2) { return __builtin_amdgcn_rsqf(r2); }
+
+// exp(-u) in FP32 through the hardware base-2 exponential.
+__device__ __forceinline__ float expm_f32(float u) {
+  return __builtin_amdgcn_exp2f(-u * 1.4426950408889634f);
+}
+
+// Thole screening factors for the rank-1/2 tensors of the screened potential f_V(v)/R,
+// f_V = 1 - (1 + v/2) e^-v:  f_E = 1 - (1 + v + v^2/2) e^-v,  f_T = f_E - v^3/6 e^-v.
+// e = e^-v (0 when the pair is undamped). Rank 3: f_D = f_T - v^4/30 e^-v.
+template <typename R>
+__device__ __forceinline__ void thole_ET(R v, R e, R& fE, R& fT) {
+  const R v2 = v * v;
+  fE = R(1) - (R(1) + v + R(0.5) * v2) * e;
+  fT = fE - (v2 * v * R(1.0 / 6.0)) * e;
+}
+
+// Above this scaled distance e^-v * v^4 is below 2^-64 relative; the pair is treated as
+// undamped (screening factors equal 1 to FP64 precision).
+constexpr double kTholeCutoff = 50.0;
+
+}  // namespace cppe_dev

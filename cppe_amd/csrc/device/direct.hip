@@ -1,0 +1,524 @@
+// Direct-summation field kernels on gfx950 (the reference's compute_field_exact path,
+// src/core/fmm/calculate.cc:262-300, and the nuclear / energy / dense-B helpers).
+//
+// Layout: sites in device (Morton) order, every per-site quantity SoA. One thread owns one
+// target site; the block (256 threads = 4 wavefronts) streams source tiles of 256 sites
+// through LDS, so each source value is read once per block and broadcast from LDS.
+// Exclusions: a tile is checked per lane against its own index and CSR exclusion list; only
+// wavefronts with a hit in the tile run the masked inner loop (rare with Morton order).
+// Thole damping (pair parameter a = damping * s_i * s_j, s = alpha_iso^(-1/6), s = 0 for
+// undamped sites): e^-aR is evaluated only when aR < 50; beyond, the screening factors equal
+// 1 to FP64 precision.
+#include "system.hh"
+
+namespace cppe_dev {
+
+constexpr int kTile = 256;
+
+// ---------------------------------------------------------------------------------------------
+// caller AoS (caller order) <-> device SoA (device order)
+// ---------------------------------------------------------------------------------------------
+__global__ void k_aos_to_soa(int n, int nc, const int* __restrict__ perm, const double* __restrict__ aos,
+                             double* __restrict__ soa) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = perm[i];
+  for (int c = 0; c < nc; ++c) soa[(size_t)c * n + i] = aos[(size_t)nc * j + c];
+}
+
+__global__ void k_soa_to_aos(int n, int nc, const int* __restrict__ perm, const double* __restrict__ soa,
+                             double* __restrict__ aos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = perm[i];
+  for (int c = 0; c < nc; ++c) aos[(size_t)nc * j + c] = soa[(size_t)c * n + i];
+}
+
+void aos_to_soa(System& S, const double* aos, double* soa, int nc) {
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(ceil_div(S.n, 256)), dim3(256), 0, S.stream, S.n, nc, S.perm.p, aos,
+                     soa);
+}
+
+void soa_to_aos(System& S, const double* soa, double* aos, int nc) {
+  hipLaunchKernelGGL(k_soa_to_aos, dim3(ceil_div(S.n, 256)), dim3(256), 0, S.stream, S.n, nc, S.perm.p, soa,
+                     aos);
+}
+
+// ---------------------------------------------------------------------------------------------
+// exclusion helpers
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ bool tile_hits(int i, int j0, int j1, const int* __restrict__ eoff,
+                                          const int* __restrict__ eidx) {
+  bool hit = (i >= j0 && i < j1);
+  if (eoff) {
+    const int e0 = eoff[i], e1 = eoff[i + 1];
+    for (int k = e0; k < e1; ++k) {
+      const int e = eidx[k];
+      hit |= (e >= j0 && e < j1);
+    }
+  }
+  return hit;
+}
+
+__device__ __forceinline__ bool excluded(int i, int j, const int* __restrict__ eoff,
+                                         const int* __restrict__ eidx) {
+  if (i == j) return true;
+  if (eoff) {
+    const int e0 = eoff[i], e1 = eoff[i + 1];
+    for (int k = e0; k < e1; ++k)
+      if (eidx[k] == j) return true;
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Pair kernels. d = r_target - r_source.
+// ---------------------------------------------------------------------------------------------
+// -T2 mu with Thole damping: fE mu / R^3 - 3 fT (d.mu) d / R^5  (P2P<1,3>, field_m1.cc:127-193)
+template <bool DAMP>
+__device__ __forceinline__ void pair_m1_f64(double dx, double dy, double dz, double mx, double my,
+                                            double mz, double a, double& fx, double& fy, double& fz) {
+  const double r2   = dx * dx + dy * dy + dz * dz;
+  const double rinv = rsqrt_f64(r2);
+  const double ri2  = rinv * rinv;
+  const double ri3  = rinv * ri2;
+  double fE = 1.0, fT = 1.0;
+  if (DAMP) {
+    const double u = a * (r2 * rinv);
+    if (u < kTholeCutoff && a > 0.0) thole_ET(u, exp(-u), fE, fT);
+  }
+  const double dmu = dx * mx + dy * my + dz * mz;
+  const double c3  = fE * ri3;
+  const double c5  = 3.0 * fT * dmu * ri3 * ri2;
+  fx += c3 * mx - c5 * dx;
+  fy += c3 * my - c5 * dy;
+  fz += c3 * mz - c5 * dz;
+}
+
+template <bool DAMP>
+__device__ __forceinline__ void pair_m1_f32(float dx, float dy, float dz, float mx, float my, float mz,
+                                            float a, float& fx, float& fy, float& fz) {
+  const float r2   = dx * dx + dy * dy + dz * dz;
+  const float rinv = rsqrt_f32(r2);
+  const float ri2  = rinv * rinv;
+  const float ri3  = rinv * ri2;
+  float fE = 1.0f, fT = 1.0f;
+  if (DAMP) {
+    const float u = a * (r2 * rinv);
+    const float e = (u < 50.0f && a > 0.0f) ? expm_f32(u) : 0.0f;
+    thole_ET(u, e, fE, fT);
+  }
+  const float dmu = dx * mx + dy * my + dz * mz;
+  const float c3  = fE * ri3;
+  const float c5  = 3.0f * fT * dmu * ri3 * ri2;
+  fx += c3 * mx - c5 * dx;
+  fy += c3 * my - c5 * dy;
+  fz += c3 * mz - c5 * dz;
+}
+
+// Static field of charge + dipole + packed quadrupole S (half diagonal), combined as
+// F(m0) - F(m1) + F(m2) (electric_fields.cc:163,174,186), with Thole factors from f_V/R:
+//   m0: +q fE d/R^3
+//   -m1: 3 fT (d.mu) d/R^5 - fE mu/R^3
+//   m2: 15 fD P d/R^7 - 3 fT (tr d + Q d)/R^5, P = sum_q S_q d^q, tr = S0+S3+S5,
+//       (Q d) = (2S0 dx + S1 dy + S2 dz, S1 dx + 2S3 dy + S4 dz, S2 dx + S4 dy + 2S5 dz)
+__device__ __forceinline__ void pair_static(double dx, double dy, double dz, double q, double mx,
+                                            double my, double mz, const double* Sq, double a,
+                                            bool damp, double& fx, double& fy, double& fz) {
+  const double r2   = dx * dx + dy * dy + dz * dz;
+  const double rinv = rsqrt_f64(r2);
+  const double ri2  = rinv * rinv;
+  const double ri3  = rinv * ri2;
+  const double ri5  = ri3 * ri2;
+  const double ri7  = ri5 * ri2;
+  double fE = 1.0, fT = 1.0, fD = 1.0;
+  if (damp) {
+    const double u = a * (r2 * rinv);
+    if (u < kTholeCutoff && a > 0.0) {
+      const double e = exp(-u);
+      thole_ET(u, e, fE, fT);
+      const double u2 = u * u;
+      fD = fT - (u2 * u2 * (1.0 / 30.0)) * e;
+    }
+  }
+  const double dmu = dx * mx + dy * my + dz * mz;
+  const double P   = Sq[0] * dx * dx + Sq[1] * dx * dy + Sq[2] * dx * dz + Sq[3] * dy * dy +
+                   Sq[4] * dy * dz + Sq[5] * dz * dz;
+  const double tr  = Sq[0] + Sq[3] + Sq[5];
+  const double qx  = 2.0 * Sq[0] * dx + Sq[1] * dy + Sq[2] * dz;
+  const double qy  = Sq[1] * dx + 2.0 * Sq[3] * dy + Sq[4] * dz;
+  const double qz  = Sq[2] * dx + Sq[4] * dy + 2.0 * Sq[5] * dz;
+  const double cd  = q * fE * ri3 + 3.0 * fT * dmu * ri5 + 15.0 * fD * P * ri7 - 3.0 * fT * tr * ri5;
+  const double cm  = -fE * ri3;
+  const double cq  = -3.0 * fT * ri5;
+  fx += cd * dx + cm * mx + cq * qx;
+  fy += cd * dy + cm * my + cq * qy;
+  fz += cd * dz + cm * mz + cq * qz;
+}
+
+// Field of order-m sources (general, FP64): used for cppe_hip_field (m = 0, 1, 2).
+__device__ __forceinline__ void pair_m(int m, double dx, double dy, double dz, const double* Sq, double a,
+                                       bool damp, double& fx, double& fy, double& fz) {
+  if (m == 1) {
+    if (damp)
+      pair_m1_f64<true>(dx, dy, dz, Sq[0], Sq[1], Sq[2], a, fx, fy, fz);
+    else
+      pair_m1_f64<false>(dx, dy, dz, Sq[0], Sq[1], Sq[2], a, fx, fy, fz);
+    return;
+  }
+  double zero6[6] = {0, 0, 0, 0, 0, 0};
+  if (m == 0)
+    pair_static(dx, dy, dz, Sq[0], 0.0, 0.0, 0.0, zero6, a, damp, fx, fy, fz);
+  else
+    pair_static(dx, dy, dz, 0.0, 0.0, 0.0, 0.0, Sq, a, damp, fx, fy, fz);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Generic direct field: m in {0,1,2} (nsrc = 1,3,6 components) or static (nsrc = 10: q, mu,
+// S6, combined with the sign convention of MultipoleFields). FP64, one-shot paths.
+// ---------------------------------------------------------------------------------------------
+template <int KIND>  // 0,1,2 = order m ; 3 = static combined
+__global__ __launch_bounds__(256) void k_direct_generic(int n, const double* __restrict__ x,
+                                                        const double* __restrict__ y,
+                                                        const double* __restrict__ z,
+                                                        const double* __restrict__ src,
+                                                        const double* __restrict__ s, double damping,
+                                                        const int* __restrict__ eoff,
+                                                        const int* __restrict__ eidx,
+                                                        double* __restrict__ F) {
+  constexpr int NS = KIND == 0 ? 1 : (KIND == 1 ? 3 : (KIND == 2 ? 6 : 10));
+  __shared__ double sx[kTile], sy[kTile], sz[kTile], ss[kTile];
+  __shared__ double sS[NS][kTile];
+  const int t  = threadIdx.x;
+  const int i  = blockIdx.x * kTile + t;
+  const bool act = i < n;
+  const double xi = act ? x[i] : 0.0, yi = act ? y[i] : 0.0, zi = act ? z[i] : 0.0;
+  const double di = act && s ? damping * s[i] : 0.0;
+  const bool damp = damping > 0.0 && s != nullptr;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  for (int j0 = 0; j0 < n; j0 += kTile) {
+    const int tn = min(kTile, n - j0);
+    __syncthreads();
+    if (t < tn) {
+      sx[t] = x[j0 + t];
+      sy[t] = y[j0 + t];
+      sz[t] = z[j0 + t];
+      ss[t] = s ? s[j0 + t] : 0.0;
+      for (int c = 0; c < NS; ++c) sS[c][t] = src[(size_t)c * n + j0 + t];
+    }
+    __syncthreads();
+    const bool hit = act && tile_hits(i, j0, j0 + tn, eoff, eidx);
+    const bool masked = __ballot(hit) != 0;
+    for (int jj = 0; jj < tn; ++jj) {
+      if (masked && (!act || excluded(i, j0 + jj, eoff, eidx))) continue;
+      const double dx = xi - sx[jj], dy = yi - sy[jj], dz = zi - sz[jj];
+      const double a  = di * ss[jj];
+      double Sq[NS];
+      for (int c = 0; c < NS; ++c) Sq[c] = sS[c][jj];
+      if (KIND == 3)
+        pair_static(dx, dy, dz, Sq[0], Sq[1], Sq[2], Sq[3], &Sq[NS == 10 ? 4 : 0], a, damp, fx, fy, fz);
+      else
+        pair_m(KIND, dx, dy, dz, Sq, a, damp, fx, fy, fz);
+    }
+  }
+  if (act) {
+    F[i]         = fx;
+    F[n + i]     = fy;
+    F[2 * n + i] = fz;
+  }
+}
+
+void direct_field(System& S, int m, const double* src, double* F, double damping) {
+  const dim3 g(ceil_div(S.n, kTile)), b(kTile);
+  const double* s = damping > 0.0 ? S.s.p : nullptr;
+  const int* eo = S.excl_off.p;
+  const int* ei = S.excl_idx.p;
+  if (m == 0)
+    hipLaunchKernelGGL(k_direct_generic<0>, g, b, 0, S.stream, S.n, S.x.p, S.y.p, S.z.p, src, s, damping, eo, ei, F);
+  else if (m == 1)
+    hipLaunchKernelGGL(k_direct_generic<1>, g, b, 0, S.stream, S.n, S.x.p, S.y.p, S.z.p, src, s, damping, eo, ei, F);
+  else
+    hipLaunchKernelGGL(k_direct_generic<2>, g, b, 0, S.stream, S.n, S.x.p, S.y.p, S.z.p, src, s, damping, eo, ei, F);
+}
+
+// src layout for the static field: q (n), mu (3n), S6 (6n) consecutive SoA blocks.
+void direct_static(System& S, const double* q, const double* mu, const double* th, double* F,
+                   double damping) {
+  (void)mu;
+  (void)th;
+  const double* s = damping > 0.0 ? S.s.p : nullptr;
+  hipLaunchKernelGGL(k_direct_generic<3>, dim3(ceil_div(S.n, kTile)), dim3(kTile), 0, S.stream, S.n, S.x.p,
+                     S.y.p, S.z.p, q, s, damping, S.excl_off.p, S.excl_idx.p, F);
+}
+
+// ---------------------------------------------------------------------------------------------
+// B apply, the hot kernel of the direct path: out = -T2 mu (damped) + alpha^-1 mu.
+// FP64 form: everything FP64. FP32 form: sources staged in LDS as FP32 offsets from a per-block
+// FP64 anchor (the block's first target), dipoles and damping parameters in FP32, FP32 pair
+// arithmetic with hardware rsqrt / exp2, per-tile FP32 partial sums promoted to FP64.
+// ---------------------------------------------------------------------------------------------
+template <bool DAMP>
+__global__ __launch_bounds__(256) void k_apply_b_f64(int n, const double* __restrict__ x,
+                                                     const double* __restrict__ y,
+                                                     const double* __restrict__ z,
+                                                     const double* __restrict__ s, double damping,
+                                                     const int* __restrict__ eoff, const int* __restrict__ eidx,
+                                                     const double* __restrict__ mu, const double* __restrict__ ainv,
+                                                     double* __restrict__ out) {
+  __shared__ double sx[kTile], sy[kTile], sz[kTile], sm[3][kTile], ss[kTile];
+  const int t    = threadIdx.x;
+  const int i    = blockIdx.x * kTile + t;
+  const bool act = i < n;
+  const double xi = act ? x[i] : 0.0, yi = act ? y[i] : 0.0, zi = act ? z[i] : 0.0;
+  const double di = (DAMP && act) ? damping * s[i] : 0.0;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  for (int j0 = 0; j0 < n; j0 += kTile) {
+    const int tn = min(kTile, n - j0);
+    __syncthreads();
+    if (t < tn) {
+      const int j = j0 + t;
+      sx[t]    = x[j];
+      sy[t]    = y[j];
+      sz[t]    = z[j];
+      sm[0][t] = mu[j];
+      sm[1][t] = mu[n + j];
+      sm[2][t] = mu[2 * n + j];
+      ss[t]    = DAMP ? s[j] : 0.0;
+    }
+    __syncthreads();
+    const bool hit    = act && tile_hits(i, j0, j0 + tn, eoff, eidx);
+    const bool masked = __ballot(hit) != 0;
+    if (!masked) {
+#pragma unroll 4
+      for (int jj = 0; jj < tn; ++jj)
+        pair_m1_f64<DAMP>(xi - sx[jj], yi - sy[jj], zi - sz[jj], sm[0][jj], sm[1][jj], sm[2][jj],
+                          di * ss[jj], fx, fy, fz);
+    } else {
+      for (int jj = 0; jj < tn; ++jj) {
+        if (!act || (hit && excluded(i, j0 + jj, eoff, eidx))) continue;
+        pair_m1_f64<DAMP>(xi - sx[jj], yi - sy[jj], zi - sz[jj], sm[0][jj], sm[1][jj], sm[2][jj],
+                          di * ss[jj], fx, fy, fz);
+      }
+    }
+  }
+  if (act) {
+    const double m0 = mu[i], m1 = mu[n + i], m2 = mu[2 * n + i];
+    const double a0 = ainv[i], a1 = ainv[n + i], a2 = ainv[2 * n + i], a3 = ainv[3 * n + i],
+                 a4 = ainv[4 * n + i], a5 = ainv[5 * n + i];
+    out[i]         = fx + (a0 * m0 + a1 * m1 + a2 * m2);
+    out[n + i]     = fy + (a1 * m0 + a3 * m1 + a4 * m2);
+    out[2 * n + i] = fz + (a2 * m0 + a4 * m1 + a5 * m2);
+  }
+}
+
+template <bool DAMP>
+__global__ __launch_bounds__(256) void k_apply_b_f32(int n, const double* __restrict__ x,
+                                                     const double* __restrict__ y,
+                                                     const double* __restrict__ z,
+                                                     const double* __restrict__ s, double damping,
+                                                     const int* __restrict__ eoff, const int* __restrict__ eidx,
+                                                     const double* __restrict__ mu, const double* __restrict__ ainv,
+                                                     double* __restrict__ out) {
+  __shared__ float sx[kTile], sy[kTile], sz[kTile], sm[3][kTile], ss[kTile];
+  const int t    = threadIdx.x;
+  const int i    = blockIdx.x * kTile + t;
+  const bool act = i < n;
+  const int i0   = blockIdx.x * kTile;
+  const double ax = x[i0], ay = y[i0], az = z[i0];  // FP64 anchor of this block
+  const float xi = act ? (float)(x[i] - ax) : 0.f, yi = act ? (float)(y[i] - ay) : 0.f,
+              zi = act ? (float)(z[i] - az) : 0.f;
+  const float di = (DAMP && act) ? (float)(damping * s[i]) : 0.f;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  for (int j0 = 0; j0 < n; j0 += kTile) {
+    const int tn = min(kTile, n - j0);
+    __syncthreads();
+    if (t < tn) {
+      const int j = j0 + t;
+      sx[t]    = (float)(x[j] - ax);
+      sy[t]    = (float)(y[j] - ay);
+      sz[t]    = (float)(z[j] - az);
+      sm[0][t] = (float)mu[j];
+      sm[1][t] = (float)mu[n + j];
+      sm[2][t] = (float)mu[2 * n + j];
+      ss[t]    = DAMP ? (float)s[j] : 0.f;
+    }
+    __syncthreads();
+    const bool hit    = act && tile_hits(i, j0, j0 + tn, eoff, eidx);
+    const bool masked = __ballot(hit) != 0;
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    if (!masked) {
+#pragma unroll 4
+      for (int jj = 0; jj < tn; ++jj)
+        pair_m1_f32<DAMP>(xi - sx[jj], yi - sy[jj], zi - sz[jj], sm[0][jj], sm[1][jj], sm[2][jj],
+                          di * ss[jj], gx, gy, gz);
+    } else {
+      for (int jj = 0; jj < tn; ++jj) {
+        if (!act || (hit && excluded(i, j0 + jj, eoff, eidx))) continue;
+        pair_m1_f32<DAMP>(xi - sx[jj], yi - sy[jj], zi - sz[jj], sm[0][jj], sm[1][jj], sm[2][jj],
+                          di * ss[jj], gx, gy, gz);
+      }
+    }
+    fx += gx;
+    fy += gy;
+    fz += gz;
+  }
+  if (act) {
+    const double m0 = mu[i], m1 = mu[n + i], m2 = mu[2 * n + i];
+    const double a0 = ainv[i], a1 = ainv[n + i], a2 = ainv[2 * n + i], a3 = ainv[3 * n + i],
+                 a4 = ainv[4 * n + i], a5 = ainv[5 * n + i];
+    out[i]         = fx + (a0 * m0 + a1 * m1 + a2 * m2);
+    out[n + i]     = fy + (a1 * m0 + a3 * m1 + a4 * m2);
+    out[2 * n + i] = fz + (a2 * m0 + a4 * m1 + a5 * m2);
+  }
+}
+
+void apply_b_direct(System& S, const double* xin, double* out, bool fp32) {
+  const dim3 g(ceil_div(S.n, kTile)), b(kTile);
+  const bool damp = S.opts.damping > 0.0;
+  const double d  = S.opts.damping;
+  if (fp32) {
+    if (damp)
+      hipLaunchKernelGGL(k_apply_b_f32<true>, g, b, 0, S.stream, S.n, S.x.p, S.y.p, S.z.p, S.s.p, d,
+                         S.excl_off.p, S.excl_idx.p, xin, S.ainv6.p, out);
+    else
+      hipLaunchKernelGGL(k_apply_b_f32<false>, g, b, 0, S.stream, S.n, S.x.p, S.y.p, S.z.p, S.s.p, d,
+                         S.excl_off.p, S.excl_idx.p, xin, S.ainv6.p, out);
+  } else {
+    if (damp)
+      hipLaunchKernelGGL(k_apply_b_f64<true>, g, b, 0, S.stream, S.n, S.x.p, S.y.p, S.z.p, S.s.p, d,
+                         S.excl_off.p, S.excl_idx.p, xin, S.ainv6.p, out);
+    else
+      hipLaunchKernelGGL(k_apply_b_f64<false>, g, b, 0, S.stream, S.n, S.x.p, S.y.p, S.z.p, S.s.p, d,
+                         S.excl_off.p, S.excl_idx.p, xin, S.ainv6.p, out);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Nuclear fields (electric_fields.cc:58-75) and the nuclear-multipole energy
+// (multipole_expansion.cc:10-33), caller order, atoms staged in LDS.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_nuclear(int n, const double* __restrict__ pos, int na,
+                                                 const double* __restrict__ ch, const double* __restrict__ apos,
+                                                 double* __restrict__ F) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  for (int a = 0; a < na; ++a) {
+    const double dx = px - apos[3 * a], dy = py - apos[3 * a + 1], dz = pz - apos[3 * a + 2];
+    const double r  = sqrt(dx * dx + dy * dy + dz * dz);
+    const double c  = ch[a] / (r * r * r);
+    fx += c * dx;
+    fy += c * dy;
+    fz += c * dz;
+  }
+  F[3 * i]     = fx;
+  F[3 * i + 1] = fy;
+  F[3 * i + 2] = fz;
+}
+
+// E_i = sum_a Z_a sum_k (-1)^k/k! sum_(abc) k!/(a!b!c!) M_k(abc) d^(abc) (1/R), d = R_a - r_i
+__global__ __launch_bounds__(256) void k_nuc_mul_energy(int n, const double* __restrict__ pos,
+                                                        const int* __restrict__ kmax,
+                                                        const double* __restrict__ mul, int na,
+                                                        const double* __restrict__ ch,
+                                                        const double* __restrict__ apos,
+                                                        double* __restrict__ part) {
+  __shared__ double red[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e    = 0.0;
+  if (i < n) {
+    const int km = kmax[i];
+    const double* M = &mul[10 * i];
+    for (int a = 0; a < na; ++a) {
+      const double dx = apos[3 * a] - pos[3 * i], dy = apos[3 * a + 1] - pos[3 * i + 1],
+                   dz = apos[3 * a + 2] - pos[3 * i + 2];
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double r  = sqrt(r2);
+      const double r1 = 1.0 / r, r3 = r1 / r2, r5 = r3 / r2;
+      double ea = 0.0;
+      if (km >= 0) ea += M[0] * r1;
+      if (km >= 1) ea += -(M[1] * (-dx * r3) + M[2] * (-dy * r3) + M[3] * (-dz * r3));
+      if (km >= 2) {
+        // T2 components xx,xy,xz,yy,yz,zz; symmetry factors 1,2,2,1,2,1; prefactor +1/2
+        const double txx = 3 * dx * dx * r5 - r3, txy = 3 * dx * dy * r5, txz = 3 * dx * dz * r5;
+        const double tyy = 3 * dy * dy * r5 - r3, tyz = 3 * dy * dz * r5, tzz = 3 * dz * dz * r5 - r3;
+        ea += 0.5 * (M[4] * txx + 2 * M[5] * txy + 2 * M[6] * txz + M[7] * tyy + 2 * M[8] * tyz + M[9] * tzz);
+      }
+      e += ea * ch[a];
+    }
+  }
+  red[threadIdx.x] = e;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_sum_partials(int np, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double red[256];
+  double v = 0.0;
+  for (int k = threadIdx.x; k < np; k += blockDim.x) v += part[k];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_dot_part(long long n, const double* __restrict__ a,
+                                                  const double* __restrict__ b, double* __restrict__ part) {
+  __shared__ double red[256];
+  double v = 0.0;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+    v += a[k] * b[k];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Dense B (bmatrix.cc:64-92), caller order; one thread per (i, j) block of 3x3.
+__global__ __launch_bounds__(256) void k_dense_b(int n, const double* __restrict__ pos,
+                                                 const double* __restrict__ iso,
+                                                 const double* __restrict__ ainv_aos, double damping,
+                                                 const int* __restrict__ eoff, const int* __restrict__ eidx,
+                                                 const int* __restrict__ index, double* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)n * n) return;
+  const int i = (int)(t / n), j = (int)(t % n);
+  const size_t N = 3 * (size_t)n;
+  if (i == j) {
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) out[(3 * i + a) * N + 3 * j + c] = ainv_aos[9 * i + 3 * a + c];
+    return;
+  }
+  bool ex = false;
+  if (eoff)
+    for (int k = eoff[i]; k < eoff[i + 1]; ++k) ex |= (eidx[k] == index[j]);
+  double T[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  if (!ex) {
+    const double d[3] = {pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1], pos[3 * j + 2] - pos[3 * i + 2]};
+    const double r    = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    double fE = 1.0, fT = 1.0;
+    if (damping > 0.0) {
+      const double v = damping / pow(iso[i] * iso[j], 1.0 / 6.0);
+      const double u = v * r;
+      thole_ET(u, exp(-u), fE, fT);
+    }
+    const double r3 = 1.0 / (r * r * r), r5 = r3 / (r * r);
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) T[a][c] = -(3.0 * fT * d[a] * d[c] * r5 - (a == c ? fE * r3 : 0.0));
+  }
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) out[(3 * i + a) * N + 3 * j + c] = T[a][c];
+}
+
+}  // namespace cppe_dev

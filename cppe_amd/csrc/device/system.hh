@@ -1,0 +1,103 @@
+// Device-resident state behind one cppe_hip_system handle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../../include/cppe_hip.h"
+#include "common.hh"
+
+namespace cppe_dev {
+
+// Device buffer with RAII release.
+template <typename T>
+struct dbuf {
+  T* p = nullptr;
+  size_t n = 0;
+  dbuf() = default;
+  dbuf(const dbuf&) = delete;
+  dbuf& operator=(const dbuf&) = delete;
+  ~dbuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    release();
+    if (count == 0) count = 1;
+    CPPE_HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  void zero(hipStream_t s) { CPPE_HIP_CHECK(hipMemsetAsync(p, 0, n * sizeof(T), s)); }
+  operator T*() const { return p; }
+};
+
+// Octree in level order (level 0 = root). Sites are sorted by Morton key ("device order");
+// every cell covers the contiguous device-order range [start, start + count).
+struct Octree {
+  int ncells = 0, nleaves = 0, max_level = 0;
+  std::vector<int> level_offset;  // host copy: first cell index of each level (+ end)
+  dbuf<double> cx, cy, cz, cr, crmax;
+  dbuf<int> start, count, parent, level, child_first, nchild, octmask;
+  dbuf<int> leaf_ids;  // indices of leaf cells
+  // interaction lists grouped by target (owner computes): CSR over target cells
+  long long n_m2l = 0, n_p2p = 0, p2p_site_pairs = 0;
+  dbuf<int> m2l_off, m2l_src;  // ncells+1 offsets, sources
+  dbuf<int> p2p_off, p2p_src;  // ncells+1 offsets (non-empty only for leaves), source leaves
+  bool built = false;
+};
+
+struct FmmState;  // FMM expansions and operator scratch (fmm.hip)
+
+struct System {
+  int n = 0;
+  cppe_hip_options opts{};
+  hipStream_t stream = nullptr;
+  bool has_alpha = false;
+
+  // site data in device order (Morton-sorted)
+  dbuf<double> x, y, z;          // positions
+  dbuf<double> s;                // alpha_iso^(-1/6), 0 = undamped site
+  dbuf<double> alpha6, ainv6;    // packed 3x3 (xx,xy,xz,yy,yz,zz), SoA [k*n + i]
+  dbuf<int> excl_off, excl_idx;  // CSR in device order
+  dbuf<int> perm;                // device order -> caller index
+  dbuf<int> iperm;               // caller index -> device order
+  std::vector<int> h_perm;
+  std::vector<int> h_excl_off, h_excl_idx;  // caller order, values as given
+
+  Octree tree;
+  FmmState* fmm = nullptr;
+
+  // solver work vectors: 3n SoA (component-major) in device order
+  dbuf<double> v_mu, v_rhs, v_r, v_z, v_p, v_Ap, v_t, v_d;
+  dbuf<double> red;    // reduction partials
+  dbuf<double> scal;   // device scalars of the CG recurrence
+  dbuf<double> io_a, io_b;  // AoS3 staging in caller order
+  dbuf<float> f32_src;      // FP32 staging for mixed-precision applies
+  bool mu_valid = false;
+
+  ~System();
+};
+
+// ---- kernels/launchers implemented across the .hip translation units -----------------------
+void build_octree(System& S);
+void build_lists(System& S);
+
+// field of order-m sources given in device-order SoA (ncomp(m) arrays of n) into F (3n SoA)
+void direct_field(System& S, int m, const double* src, double* F, double damping);
+void direct_static(System& S, const double* q, const double* mu, const double* th, double* F,
+                   double damping);
+// out = field_m1(x) + alpha^-1 x, FP64 (or FP32 arithmetic when fp32 is set)
+void apply_b(System& S, const double* x, double* out, bool fp32);
+
+void fmm_field(System& S, int m, const double* src, double* F, double damping, bool fp32);
+void fmm_static(System& S, const double* q, const double* mu, const double* th, double* F, double damping);
+void fmm_destroy(System& S);
+
+// layout conversions between caller AoS (caller order) and device SoA (device order)
+void aos_to_soa(System& S, const double* aos, double* soa, int ncomp);
+void soa_to_aos(System& S, const double* soa, double* aos, int ncomp);
+
+}  // namespace cppe_dev

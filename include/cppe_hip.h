@@ -1,0 +1,157 @@
+/*
+ * cppe_hip.h -- C ABI of the MI355X (gfx950) polarizable-embedding field / induced-dipole path.
+ *
+ * Plain pointers and sizes only. Host-side vectors use the reference's layouts: "AoS3" is an
+ * FP64 array of length 3*n indexed 3*i+c, in the caller's site order. Every entry point returns
+ * CPPE_HIP_OK (0) or an error code; cppe_hip_last_error() then holds a thread-local message.
+ *
+ * A cppe_hip_system owns one set of sites resident in HBM (positions, polarizabilities, damping
+ * parameters, exclusions, the octree and its interaction lists, and the solver work vectors).
+ * It replaces, for the reference (maxscheurer/cppe), the per-call pair
+ *   build_shared_tree<m,3>(...)            src/core/fmm/tree.cc:134-237
+ *   Tree::compute_field_fmm / _exact(F)    src/core/fmm/tree.cc:359-394
+ * and the objects built around it (BMatrix src/core/bmatrix.hh:19-36, InducedMoments
+ * src/core/electric_fields.hh:51-80). The reference rebuilds the tree on every B apply
+ * (src/core/bmatrix.cc:28-29); here it is built once per system.
+ */
+#ifndef CPPE_HIP_H
+#define CPPE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPPE_HIP_OK 0
+#define CPPE_HIP_ERR_ARG 1        /* invalid argument (reference: std::invalid_argument) */
+#define CPPE_HIP_ERR_DEVICE 2     /* HIP runtime error or no device */
+#define CPPE_HIP_ERR_NOCONV 3     /* "Failed to converge induced dipole moments." */
+#define CPPE_HIP_ERR_TREE 4       /* octree could not be built (e.g. coincident sites) */
+#define CPPE_HIP_ERR_UNSUPPORTED 5
+
+#define CPPE_HIP_SUM_DIRECT 0
+#define CPPE_HIP_SUM_FMM 1
+
+#define CPPE_HIP_PREC_FP64 0      /* plain form: FP64 throughout */
+#define CPPE_HIP_PREC_MIXED 1     /* FP32 field applies inside FP64 iterative refinement */
+
+typedef struct cppe_hip_options {
+  int summation;    /* PeOptions::summation_induced_fields: 0 "direct", 1 "fmm" (pe_options.hh:28-29) */
+  int ncrit;        /* PeOptions::tree_ncrit (pe_options.hh:30) */
+  int order;        /* PeOptions::tree_expansion_order (pe_options.hh:31) */
+  double theta;     /* PeOptions::theta (pe_options.hh:32) */
+  double damping;   /* Thole factor used by this system's fields; 0 = undamped
+                       (bmatrix.cc:18-21, electric_fields.cc:116-120) */
+  int precision;    /* CPPE_HIP_PREC_* used by cppe_hip_solve */
+} cppe_hip_options;
+
+typedef struct cppe_hip_system cppe_hip_system;
+
+/* Thread-local message of the last failing call on this thread ("" if none). */
+const char* cppe_hip_last_error(void);
+
+/* Number of visible HIP devices (0 on a host without a GPU; never fails). */
+int cppe_hip_device_count(int* count);
+
+/*
+ * Create a device-resident system of n sites.
+ *   pos         AoS3 positions (bohr)
+ *   alpha6      n*6 packed polarizabilities (xx,xy,xz,yy,yz,zz) or NULL (no response operator)
+ *   alpha_iso   n isotropic polarizabilities used for Thole damping (0 = site not damped), or NULL
+ *   excl_off    n+1 CSR offsets, excl_idx values: exclusions compared with the *position* of the
+ *               other site in this system, as the tree-side test does (fmm/calculate.cc:38,
+ *               fmm/tree.hh:23-26). Values outside [0,n) never match. May both be NULL.
+ * Replaces build_shared_tree(potentials, S, ncrit, order, theta, damping) (tree.cc:134-237)
+ * together with the BMatrix constructor (bmatrix.hh:19-36).
+ */
+int cppe_hip_system_create(cppe_hip_system** out, int n, const double* pos, const double* alpha6,
+                           const double* alpha_iso, const int* excl_off, const int* excl_idx,
+                           const cppe_hip_options* opts);
+int cppe_hip_system_destroy(cppe_hip_system* sys);
+
+/*
+ * Field at every site from order-m sources S (n*ncomp(m) values per site, the reference's
+ * packing: q | mu_xyz | half-diagonal quadrupole), overwriting F (AoS3). Convention of the
+ * reference's P2P<m,3> (fmm/field_m{0,1,2}.cc): m=1 returns the *negative* dipole field.
+ * Replaces Tree<m,3>::compute_field_exact / compute_field_fmm (tree.cc:359-394).
+ */
+int cppe_hip_field(cppe_hip_system* sys, int m, const double* S, double* F);
+
+/*
+ * Static multipole field F(q) - F(mu) + F(theta') at every site, one fused device pass.
+ * q, mu (AoS3), theta6 (n*6, full quadrupole with its trace removed; the half-diagonal packing
+ * of electric_fields.cc:144-149 is applied inside). Any of mu/theta6 may be NULL.
+ * Replaces MultipoleFields::compute (electric_fields.cc:106-189).
+ */
+int cppe_hip_multipole_fields(cppe_hip_system* sys, const double* q, const double* mu,
+                              const double* theta6, double* F);
+
+/* out = B x = field_m1(x) + alpha^-1 x  (BMatrix::apply, bmatrix.cc:9-40) */
+int cppe_hip_apply_b(cppe_hip_system* sys, const double* x, double* out);
+/* out = alpha x (BMatrix::apply_diagonal_inverse, bmatrix.cc:42-51) */
+int cppe_hip_apply_alpha(cppe_hip_system* sys, const double* x, double* out);
+/* out = alpha^-1 x (BMatrix::apply_diagonal, bmatrix.cc:53-62) */
+int cppe_hip_apply_alpha_inv(cppe_hip_system* sys, const double* x, double* out);
+
+/*
+ * Dense B (3n x 3n row-major): diagonal alpha^-1, off-diagonal -T2 (Thole-damped when the
+ * system's damping > 0) with pairs zeroed where excl of i contains index[j] (the potfile index,
+ * as bmatrix.cc:73 compares). BMatrix::to_dense_matrix (bmatrix.cc:64-92).
+ */
+int cppe_hip_dense_b(cppe_hip_system* sys, const int* index, double* out);
+
+/* Inverse of the dense B by Gauss-Jordan elimination with partial pivoting on the device
+ * (BMatrix::direct_inverse, bmatrix.cc:94-97). Test-size systems. */
+int cppe_hip_dense_b_inverse(cppe_hip_system* sys, const int* index, double* out);
+
+/*
+ * Jacobi-preconditioned CG on B mu = rhs (InducedMoments::compute, electric_fields.cc:191-239).
+ * make_guess != 0: x0 = alpha rhs; otherwise x0 = guess. Stops when ||r||_2 < thresh (absolute,
+ * unpreconditioned). norms (length >= maxiter, may be NULL) receives the residual norm of each
+ * iteration; *iters the number of iterations run. Returns CPPE_HIP_ERR_NOCONV after maxiter.
+ * With opts.precision == CPPE_HIP_PREC_MIXED the field applies run in FP32 inside FP64
+ * iterative refinement; the stopping test is the same FP64 residual norm.
+ */
+int cppe_hip_solve(cppe_hip_system* sys, const double* rhs, const double* guess, int make_guess,
+                   double thresh, int maxiter, double* mu, int* iters, double* norms);
+
+/*
+ * Device-resident variants (SCF-loop residency): rhs/guess are uploaded once and mu stays in
+ * HBM between calls; cppe_hip_solve_resident times only device work (inputs already resident).
+ * guess_mode: 0 = alpha rhs, 1 = previous mu held by the system, 2 = zero.
+ * *device_ms (may be NULL) receives the solve's device time measured with HIP events.
+ */
+int cppe_hip_upload_rhs(cppe_hip_system* sys, const double* rhs);
+int cppe_hip_solve_resident(cppe_hip_system* sys, int guess_mode, double thresh, int maxiter,
+                            int* iters, double* norms, double* device_ms);
+int cppe_hip_download_mu(cppe_hip_system* sys, double* mu);
+
+/* NuclearFields::compute (electric_fields.cc:58-75): F_i = sum_a Z_a (r_i - R_a)/|r_i - R_a|^3 */
+int cppe_hip_nuclear_fields(int n, const double* pos, int natoms, const double* charges,
+                            const double* atom_pos, double* F);
+
+/*
+ * MultipoleExpansion::interaction_energy (multipole_expansion.cc:10-33): kmax[i] highest order
+ * present at site i (-1: none), mul = 10 values per site (q, mu, full traceless quadrupole).
+ */
+int cppe_hip_nuclear_multipole_energy(int n, const double* pos, const int* kmax, const double* mul,
+                                      int natoms, const double* charges, const double* atom_pos,
+                                      double* energy);
+
+/* out = sum_i a_i b_i in FP64 on the device (polarization energies, cppe_state.cc:97-108) */
+int cppe_hip_dot(long long n, const double* a, const double* b, double* out);
+
+/*
+ * Octree introspection for parity tests.
+ * stats: [ncells, nleaves, n_m2l, n_p2p_cell_pairs, max_level, p2p_site_pairs]
+ * geo (ncells*5: x,y,z,r,rmax), meta (ncells*4: parent, level, child mask, count),
+ * leaf_start (ncells: first site in device order), perm (n: device order -> caller index),
+ * m2l (n_m2l*2: source, target), p2p (n_p2p*2: target, source). Cells are in level order.
+ */
+int cppe_hip_tree_stats(cppe_hip_system* sys, long long* stats);
+int cppe_hip_tree_export(cppe_hip_system* sys, double* geo, long long* meta, long long* cell_start,
+                         int* perm, long long* m2l, long long* p2p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CPPE_HIP_H */
