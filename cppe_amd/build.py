@@ -45,12 +45,25 @@ def _pymod_path():
     return os.path.join(PKG, "pycppe" + suffix)
 
 
+FMM_ORDERS = range(2, 9)
+
+
 def build_device(jobs=8):
     os.makedirs(BUILD, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "device", "*.hip")))
-    objs = [os.path.join(BUILD, os.path.basename(s) + ".o") for s in srcs]
+    dev = os.path.join(CSRC, "device")
+    srcs = sorted(s for s in glob.glob(os.path.join(dev, "*.hip")) if not s.endswith("fmm_order.hip"))
+    steps = [([HIPCC, "-c", *HIP_FLAGS, s, "-o", os.path.join(BUILD, os.path.basename(s) + ".o")],
+              os.path.join(BUILD, os.path.basename(s) + ".o")) for s in srcs]
+    # the FMM kernels: one object per expansion order, compiled in parallel
+    for p in FMM_ORDERS:
+        o = os.path.join(BUILD, "fmm_order_p%d.o" % p)
+        steps.append(([HIPCC, "-c", *HIP_FLAGS, "-DCPPE_FMM_ORDER=%d" % p, os.path.join(dev, "fmm_order.hip"),
+                       "-o", o], o))
+    # longest first: the octree unit pulls in hipCUB
+    steps.sort(key=lambda st: 0 if "octree" in st[1] else 1)
+    objs = [o for _, o in steps]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        list(ex.map(lambda so: _run([HIPCC, "-c", *HIP_FLAGS, so[0], "-o", so[1]]), zip(srcs, objs)))
+        list(ex.map(lambda st: _run(st[0]), steps))
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", LIB_HIP,
           "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     return LIB_HIP

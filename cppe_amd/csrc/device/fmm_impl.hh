@@ -1,0 +1,492 @@
+// Fast multipole method on gfx950: kernels and pass drivers, templated on the expansion order.
+//
+// Passes (the reference's Tree::compute_field_fmm, src/core/fmm/tree.cc:359-385, with its
+// shift conventions, fmm/calculate.cc:129-260), all owner-computes (no atomics, deterministic):
+//   P2M   one thread per leaf: point multipoles of its sites shifted to the leaf centre
+//   M2M   one thread per internal cell, level by level from the deepest level to the root
+//   M2L   one wavefront per target cell; lanes take sources of its target-grouped M2L list,
+//         each lane evaluates the Coulomb derivatives and contracts them with the source M;
+//         the 64 partial expansions are summed by wavefront shuffles
+//   L2L   one thread per cell, level by level from the root
+//   P2P+L2P one wavefront per target leaf: lanes split as (target site t, source slot s) with
+//         s < 64/T; the leaf's near list is walked 64 source leaves at a time (wavefront prefix
+//         scan of leaf sizes) and its sites are staged through LDS 64 at a time; the slot
+//         partials are summed through LDS, then each target adds its L2P term (and, for the
+//         response operator, alpha^-1 mu)
+// Damping (Thole) applies to near-field pairs only; the far field is undamped, as in the
+// reference (fmm/calculate.cc:48-72). Exclusions apply only to near-field pairs (ibid.).
+// FP32 form: M, L and pair arithmetic in FP32 with coordinates taken relative to cell / leaf
+// centres in FP64 before rounding, so FP32 sees only local offsets.
+#pragma once
+#include "fmm_ops.gen.hh"
+#include "pairs.hh"
+#include "system.hh"
+
+namespace cppe_dev {
+
+enum FmmKind { KIND_M1 = 0, KIND_MC = 1 };
+enum P2PMode { MODE_BAPPLY = 0, MODE_FIELD1 = 1, MODE_STATIC = 2 };
+
+struct FmmState {
+  int order = 0;
+  dbuf<double> M64, L64;  // dipole kind
+  dbuf<float> M32, L32;
+  dbuf<double> Mc, Lc;    // combined kind (static fields, m = 0 / 2)
+  dbuf<double> src10;     // staging of combined sources
+};
+
+template <int KIND, int P, typename R>
+struct Ops;
+
+#define CPPE_DEFINE_OPS(P)                                                                                    \
+  template <typename R>                                                                                       \
+  struct Ops<KIND_M1, P, R> {                                                                                 \
+    static constexpr int MS = fmm_msize(0, P), LS = fmm_lsize(0, P), NS = 3;                                  \
+    static __device__ __forceinline__ void p2m(R x, R y, R z, const R* S, R* M) {                             \
+      fmmops::p2m_m1_p##P<R>(x, y, z, S, M);                                                                  \
+    }                                                                                                         \
+    static __device__ __forceinline__ void m2m(R x, R y, R z, const R* Mc, R* Mp) {                           \
+      fmmops::m2m_m1_p##P<R>(x, y, z, Mc, Mp);                                                                \
+    }                                                                                                         \
+    static __device__ __forceinline__ void m2l(double x, double y, double z, const double* M, double* L) {    \
+      fmmops::m2l_m1_p##P##_f64(x, y, z, M, L);                                                               \
+    }                                                                                                         \
+    static __device__ __forceinline__ void m2l(float x, float y, float z, const float* M, float* L) {         \
+      fmmops::m2l_m1_p##P##_f32(x, y, z, M, L);                                                               \
+    }                                                                                                         \
+    static __device__ __forceinline__ void l2l(R x, R y, R z, const R* Lp, R* Lc) {                           \
+      fmmops::l2l_m1_p##P<R>(x, y, z, Lp, Lc);                                                                \
+    }                                                                                                         \
+    static __device__ __forceinline__ void l2p(R x, R y, R z, const R* L, R& fx, R& fy, R& fz) {              \
+      fmmops::l2p_m1_p##P<R>(x, y, z, L, fx, fy, fz);                                                         \
+    }                                                                                                         \
+  };                                                                                                          \
+  template <typename R>                                                                                       \
+  struct Ops<KIND_MC, P, R> {                                                                                 \
+    static constexpr int MS = fmm_msize(1, P), LS = fmm_lsize(1, P), NS = 10;                                 \
+    static __device__ __forceinline__ void p2m(R x, R y, R z, const R* S, R* M) {                             \
+      fmmops::p2m_mc_p##P<R>(x, y, z, S, M);                                                                  \
+    }                                                                                                         \
+    static __device__ __forceinline__ void m2m(R x, R y, R z, const R* Mc, R* Mp) {                           \
+      fmmops::m2m_mc_p##P<R>(x, y, z, Mc, Mp);                                                                \
+    }                                                                                                         \
+    static __device__ __forceinline__ void m2l(double x, double y, double z, const double* M, double* L) {    \
+      fmmops::m2l_mc_p##P##_f64(x, y, z, M, L);                                                               \
+    }                                                                                                         \
+    static __device__ __forceinline__ void m2l(float x, float y, float z, const float* M, float* L) {         \
+      fmmops::m2l_mc_p##P##_f32(x, y, z, M, L);                                                               \
+    }                                                                                                         \
+    static __device__ __forceinline__ void l2l(R x, R y, R z, const R* Lp, R* Lc) {                           \
+      fmmops::l2l_mc_p##P<R>(x, y, z, Lp, Lc);                                                                \
+    }                                                                                                         \
+    static __device__ __forceinline__ void l2p(R x, R y, R z, const R* L, R& fx, R& fy, R& fz) {              \
+      fmmops::l2p_mc_p##P<R>(x, y, z, L, fx, fy, fz);                                                         \
+    }                                                                                                         \
+  };
+
+#define CPPE_DEFINE_OPS_EXPAND(P) CPPE_DEFINE_OPS(P)
+#ifdef CPPE_FMM_ORDER
+CPPE_DEFINE_OPS_EXPAND(CPPE_FMM_ORDER)
+#endif
+
+// Sources in device order, SoA: dipole kind = 3 arrays (mu); combined kind = 10 arrays
+// (q, mu, packed quadrupole); the far field of the combined kind uses -mu (static field
+// convention F(q) - F(mu) + F(theta')).
+template <int KIND, typename R>
+__device__ __forceinline__ void load_source(const double* __restrict__ src, int n, int i, R* S) {
+  if (KIND == KIND_M1) {
+    S[0] = (R)src[i];
+    S[1] = (R)src[n + i];
+    S[2] = (R)src[2 * (size_t)n + i];
+  } else {
+    S[0] = (R)src[i];
+    for (int c = 1; c < 4; ++c) S[c] = -(R)src[(size_t)c * n + i];
+    for (int c = 4; c < 10; ++c) S[c] = (R)src[(size_t)c * n + i];
+  }
+}
+
+template <int KIND, int P, typename R>
+__global__ __launch_bounds__(64) void k_fmm_p2m(int nleaves, const int* __restrict__ leaf_ids,
+                                                const int* __restrict__ start, const int* __restrict__ count,
+                                                const double* __restrict__ cx, const double* __restrict__ cy,
+                                                const double* __restrict__ cz, const double* __restrict__ x,
+                                                const double* __restrict__ y, const double* __restrict__ z,
+                                                const double* __restrict__ src, int n, R* __restrict__ M) {
+  using O     = Ops<KIND, P, R>;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nleaves) return;
+  const int c = leaf_ids[t];
+  R m[O::MS];
+#pragma unroll
+  for (int k = 0; k < O::MS; ++k) m[k] = R(0);
+  const int i0 = start[c], i1 = i0 + count[c];
+  const double ccx = cx[c], ccy = cy[c], ccz = cz[c];
+  for (int i = i0; i < i1; ++i) {
+    R S[O::NS];
+    load_source<KIND, R>(src, n, i, S);
+    O::p2m((R)(ccx - x[i]), (R)(ccy - y[i]), (R)(ccz - z[i]), S, m);
+  }
+#pragma unroll
+  for (int k = 0; k < O::MS; ++k) M[(size_t)c * O::MS + k] = m[k];
+}
+
+template <int KIND, int P, typename R>
+__global__ __launch_bounds__(64) void k_fmm_m2m(int c0, int c1, const int* __restrict__ nchild,
+                                                const int* __restrict__ child_first, const double* __restrict__ cx,
+                                                const double* __restrict__ cy, const double* __restrict__ cz,
+                                                R* __restrict__ M) {
+  using O     = Ops<KIND, P, R>;
+  const int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= c1) return;
+  const int nc = nchild[c];
+  if (nc == 0) return;
+  R m[O::MS];
+#pragma unroll
+  for (int k = 0; k < O::MS; ++k) m[k] = R(0);
+  const int f = child_first[c];
+  for (int ch = f; ch < f + nc; ++ch)
+    O::m2m((R)(cx[c] - cx[ch]), (R)(cy[c] - cy[ch]), (R)(cz[c] - cz[ch]), &M[(size_t)ch * O::MS], m);
+#pragma unroll
+  for (int k = 0; k < O::MS; ++k) M[(size_t)c * O::MS + k] = m[k];
+}
+
+template <typename R>
+__device__ __forceinline__ R wave_sum(R v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+template <int KIND, int P, typename R>
+__global__ __launch_bounds__(64) void k_fmm_m2l(const int* __restrict__ off, const int* __restrict__ srcs,
+                                                const double* __restrict__ cx, const double* __restrict__ cy,
+                                                const double* __restrict__ cz, const R* __restrict__ M,
+                                                R* __restrict__ L) {
+  using O        = Ops<KIND, P, R>;
+  const int A    = blockIdx.x;
+  const int lane = threadIdx.x;
+  R l[O::LS];
+#pragma unroll
+  for (int k = 0; k < O::LS; ++k) l[k] = R(0);
+  const int o0 = off[A], o1 = off[A + 1];
+  const double ax = cx[A], ay = cy[A], az = cz[A];
+  for (int k = o0 + lane; k < o1; k += 64) {
+    const int B = srcs[k];
+    O::m2l((R)(ax - cx[B]), (R)(ay - cy[B]), (R)(az - cz[B]), &M[(size_t)B * O::MS], l);
+  }
+  constexpr int NCH = (O::LS + 63) / 64;
+  R mine[NCH];
+#pragma unroll
+  for (int k = 0; k < NCH; ++k) mine[k] = R(0);
+#pragma unroll
+  for (int k = 0; k < O::LS; ++k) {
+    const R v = wave_sum(l[k]);
+    if (lane == (k & 63)) mine[k >> 6] = v;
+  }
+#pragma unroll
+  for (int q = 0; q < NCH; ++q) {
+    const int k = q * 64 + lane;
+    if (k < O::LS) L[(size_t)A * O::LS + k] = mine[q];
+  }
+}
+
+template <int KIND, int P, typename R>
+__global__ __launch_bounds__(64) void k_fmm_l2l(int c0, int c1, const int* __restrict__ parent,
+                                                const double* __restrict__ cx, const double* __restrict__ cy,
+                                                const double* __restrict__ cz, R* __restrict__ L) {
+  using O     = Ops<KIND, P, R>;
+  const int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= c1) return;
+  const int pa = parent[c];
+  R l[O::LS];
+#pragma unroll
+  for (int k = 0; k < O::LS; ++k) l[k] = L[(size_t)c * O::LS + k];
+  O::l2l((R)(cx[c] - cx[pa]), (R)(cy[c] - cy[pa]), (R)(cz[c] - cz[pa]), &L[(size_t)pa * O::LS], l);
+#pragma unroll
+  for (int k = 0; k < O::LS; ++k) L[(size_t)c * O::LS + k] = l[k];
+}
+
+struct P2PArgs {
+  const int* leaf_ids;
+  const int* start;
+  const int* count;
+  const int* p2p_off;
+  const int* p2p_src;
+  const double *cx, *cy, *cz;
+  const double *x, *y, *z;
+  const double* s;  // alpha_iso^(-1/6) (0: undamped site)
+  double damping;
+  const int* eoff;
+  const int* eidx;
+  const double* src;   // sources, SoA in device order (3 or 10 arrays)
+  const double* ainv;  // alpha^-1 packed SoA (response operator only)
+  int n;
+  double* out;         // 3n SoA
+};
+
+template <int KIND, int P, typename R, bool DAMP, int MODE>
+__global__ __launch_bounds__(64) void k_fmm_p2p(P2PArgs a, const R* __restrict__ L) {
+  using O            = Ops<KIND, P, R>;
+  constexpr int NS   = (MODE == MODE_STATIC) ? 10 : 3;
+  __shared__ R sx[64], sy[64], sz[64], ss[64];
+  __shared__ R sS[NS][64];
+  __shared__ int sj[64], spre[64], sst[64];
+  __shared__ double px[64], py[64], pz[64];
+  const int lane = threadIdx.x;
+  const int A    = a.leaf_ids[blockIdx.x];
+  const int T    = a.count[A], i0 = a.start[A];
+  const int o0 = a.p2p_off[A], o1 = a.p2p_off[A + 1];
+  const double cax = a.cx[A], cay = a.cy[A], caz = a.cz[A];
+  const int n = a.n;
+  for (int tb = 0; tb < T; tb += 64) {
+    const int Tb = min(64, T - tb), Sb = 64 / Tb;
+    const int t = lane % Tb, sl = lane / Tb;
+    const bool act = sl < Sb;
+    const int i    = i0 + tb + t;
+    // target: FP64 kernels use absolute coordinates, FP32 kernels offsets from the leaf centre
+    const R xi = sizeof(R) == 8 ? (R)a.x[i] : (R)(a.x[i] - cax);
+    const R yi = sizeof(R) == 8 ? (R)a.y[i] : (R)(a.y[i] - cay);
+    const R zi = sizeof(R) == 8 ? (R)a.z[i] : (R)(a.z[i] - caz);
+    const R di = DAMP ? (R)(a.damping * a.s[i]) : R(0);
+    int lo = i, hi = i;
+    if (a.eoff)
+      for (int k = a.eoff[i]; k < a.eoff[i + 1]; ++k) {
+        const int e = a.eidx[k];
+        if (e >= 0) {
+          lo = min(lo, e);
+          hi = max(hi, e);
+        }
+      }
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int g0 = o0; g0 < o1; g0 += 64) {
+      const int gl = min(64, o1 - g0);
+      int cnt = 0, st = 0;
+      if (lane < gl) {
+        const int B = a.p2p_src[g0 + lane];
+        cnt         = a.count[B];
+        st          = a.start[B];
+      }
+      int incl = cnt;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+      }
+      const int total = __shfl(incl, 63);
+      __syncthreads();
+      spre[lane] = incl - cnt;
+      sst[lane]  = st;
+      __syncthreads();
+      for (int c0 = 0; c0 < total; c0 += 64) {
+        const int q = c0 + lane;
+        if (q < total) {
+          int lo2 = 0, hi2 = gl;  // largest k with spre[k] <= q
+          while (hi2 - lo2 > 1) {
+            const int mid = (lo2 + hi2) >> 1;
+            if (spre[mid] <= q)
+              lo2 = mid;
+            else
+              hi2 = mid;
+          }
+          const int j = sst[lo2] + (q - spre[lo2]);
+          sj[lane]    = j;
+          if (sizeof(R) == 8) {
+            sx[lane] = (R)a.x[j];
+            sy[lane] = (R)a.y[j];
+            sz[lane] = (R)a.z[j];
+          } else {
+            sx[lane] = (R)(a.x[j] - cax);
+            sy[lane] = (R)(a.y[j] - cay);
+            sz[lane] = (R)(a.z[j] - caz);
+          }
+          ss[lane] = DAMP ? (R)a.s[j] : R(0);
+#pragma unroll
+          for (int c = 0; c < NS; ++c) sS[c][lane] = (R)a.src[(size_t)c * n + j];
+        }
+        __syncthreads();
+        const int cn = min(64, total - c0);
+        if (act) {
+          R gx = R(0), gy = R(0), gz = R(0);
+          for (int u = sl; u < cn; u += Sb) {
+            const int j = sj[u];
+            if (j >= lo && j <= hi && excluded(i, j, a.eoff, a.eidx)) continue;
+            const R dx = xi - sx[u], dy = yi - sy[u], dz = zi - sz[u];
+            if (MODE == MODE_STATIC) {
+              double Sq[6];
+#pragma unroll
+              for (int c = 0; c < 6; ++c) Sq[c] = (double)sS[4 + c][u];
+              double hx = 0.0, hy = 0.0, hz = 0.0;
+              pair_static((double)dx, (double)dy, (double)dz, (double)sS[0][u], (double)sS[1][u],
+                          (double)sS[2][u], (double)sS[3][u], Sq, (double)(di * ss[u]), DAMP, hx, hy, hz);
+              gx += (R)hx;
+              gy += (R)hy;
+              gz += (R)hz;
+            } else if (sizeof(R) == 8) {
+              double hx = 0.0, hy = 0.0, hz = 0.0;
+              pair_m1_f64<DAMP>((double)dx, (double)dy, (double)dz, (double)sS[0][u], (double)sS[1][u],
+                                (double)sS[2][u], (double)(di * ss[u]), hx, hy, hz);
+              gx += (R)hx;
+              gy += (R)hy;
+              gz += (R)hz;
+            } else {
+              float hx = 0.f, hy = 0.f, hz = 0.f;
+              pair_m1_f32<DAMP>((float)dx, (float)dy, (float)dz, (float)sS[0][u], (float)sS[1][u],
+                                (float)sS[2][u], (float)(di * ss[u]), hx, hy, hz);
+              gx += (R)hx;
+              gy += (R)hy;
+              gz += (R)hz;
+            }
+          }
+          fx += (double)gx;
+          fy += (double)gy;
+          fz += (double)gz;
+        }
+        __syncthreads();
+      }
+    }
+    px[lane] = act ? fx : 0.0;
+    py[lane] = act ? fy : 0.0;
+    pz[lane] = act ? fz : 0.0;
+    __syncthreads();
+    if (lane < Tb) {
+      double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+      for (int s2 = 0; s2 < Sb; ++s2) {
+        Fx += px[lane + Tb * s2];
+        Fy += py[lane + Tb * s2];
+        Fz += pz[lane + Tb * s2];
+      }
+      R lx = R(0), ly = R(0), lz = R(0);
+      O::l2p((R)(a.x[i] - cax), (R)(a.y[i] - cay), (R)(a.z[i] - caz), &L[(size_t)A * O::LS], lx, ly, lz);
+      Fx += (double)lx;
+      Fy += (double)ly;
+      Fz += (double)lz;
+      if (MODE == MODE_BAPPLY) {
+        const double m0 = a.src[i], m1 = a.src[n + i], m2 = a.src[2 * (size_t)n + i];
+        const double* ai = a.ainv;
+        const double a0 = ai[i], a1 = ai[n + i], a2 = ai[2 * (size_t)n + i], a3 = ai[3 * (size_t)n + i],
+                     a4 = ai[4 * (size_t)n + i], a5 = ai[5 * (size_t)n + i];
+        Fx += a0 * m0 + a1 * m1 + a2 * m2;
+        Fy += a1 * m0 + a3 * m1 + a4 * m2;
+        Fz += a2 * m0 + a4 * m1 + a5 * m2;
+      }
+      a.out[i]                 = Fx;
+      a.out[n + i]             = Fy;
+      a.out[2 * (size_t)n + i] = Fz;
+    }
+    __syncthreads();
+  }
+}
+
+// ---- host drivers ----------------------------------------------------------------------------
+template <int KIND, int P, typename R>
+void fmm_far_pass(System& S, const double* src, R* M, R* L) {
+  using O         = Ops<KIND, P, R>;
+  const Octree& T = S.tree;
+  hipStream_t st  = S.stream;
+  hipLaunchKernelGGL((k_fmm_p2m<KIND, P, R>), dim3(ceil_div(T.nleaves, 64)), dim3(64), 0, st, T.nleaves,
+                     T.leaf_ids.p, T.start.p, T.count.p, T.cx.p, T.cy.p, T.cz.p, S.x.p, S.y.p, S.z.p, src, S.n, M);
+  const auto& lo = T.level_offset;
+  for (int lev = T.max_level - 1; lev >= 0; --lev) {
+    const int c0 = lo[lev], c1 = lo[lev + 1];
+    hipLaunchKernelGGL((k_fmm_m2m<KIND, P, R>), dim3(ceil_div(c1 - c0, 64)), dim3(64), 0, st, c0, c1,
+                       T.nchild.p, T.child_first.p, T.cx.p, T.cy.p, T.cz.p, M);
+  }
+  hipLaunchKernelGGL((k_fmm_m2l<KIND, P, R>), dim3(T.ncells), dim3(64), 0, st, T.m2l_off.p, T.m2l_src.p, T.cx.p,
+                     T.cy.p, T.cz.p, M, L);
+  for (int lev = 1; lev <= T.max_level; ++lev) {
+    const int c0 = lo[lev], c1 = lo[lev + 1];
+    hipLaunchKernelGGL((k_fmm_l2l<KIND, P, R>), dim3(ceil_div(c1 - c0, 64)), dim3(64), 0, st, c0, c1,
+                       T.parent.p, T.cx.p, T.cy.p, T.cz.p, L);
+  }
+  (void)sizeof(O);
+}
+
+inline P2PArgs make_p2p_args(System& S, const double* src, double damping, double* out) {
+  const Octree& T = S.tree;
+  P2PArgs a;
+  a.leaf_ids = T.leaf_ids.p;
+  a.start    = T.start.p;
+  a.count    = T.count.p;
+  a.p2p_off  = T.p2p_off.p;
+  a.p2p_src  = T.p2p_src.p;
+  a.cx = T.cx.p;
+  a.cy = T.cy.p;
+  a.cz = T.cz.p;
+  a.x = S.x.p;
+  a.y = S.y.p;
+  a.z = S.z.p;
+  a.s       = S.s.p;
+  a.damping = damping;
+  a.eoff    = S.excl_off.p;
+  a.eidx    = S.excl_idx.p;
+  a.src     = src;
+  a.ainv    = S.ainv6.p;
+  a.n       = S.n;
+  a.out     = out;
+  return a;
+}
+
+template <int KIND, int P, typename R, int MODE>
+void fmm_p2p_pass(System& S, const double* src, double damping, const R* L, double* out) {
+  P2PArgs a      = make_p2p_args(S, src, damping, out);
+  const int nl   = S.tree.nleaves;
+  hipStream_t st = S.stream;
+  if (damping > 0.0)
+    hipLaunchKernelGGL((k_fmm_p2p<KIND, P, R, true, MODE>), dim3(nl), dim3(64), 0, st, a, L);
+  else
+    hipLaunchKernelGGL((k_fmm_p2p<KIND, P, R, false, MODE>), dim3(nl), dim3(64), 0, st, a, L);
+}
+
+template <typename T>
+void ensure_buf(dbuf<T>& b, size_t count) {
+  if (b.n < count || !b.p) b.alloc(count);
+}
+
+template <int P>
+void fmm_apply_b_t(System& S, const double* x, double* out, bool fp32) {
+  FmmState& F      = *S.fmm;
+  const size_t nc  = S.tree.ncells;
+  using O64        = Ops<KIND_M1, P, double>;
+  const double dmp = S.opts.damping;
+  if (fp32) {
+    ensure_buf(F.M32, nc * O64::MS);
+    ensure_buf(F.L32, nc * O64::LS);
+    fmm_far_pass<KIND_M1, P, float>(S, x, F.M32.p, F.L32.p);
+    fmm_p2p_pass<KIND_M1, P, float, MODE_BAPPLY>(S, x, dmp, F.L32.p, out);
+  } else {
+    ensure_buf(F.M64, nc * O64::MS);
+    ensure_buf(F.L64, nc * O64::LS);
+    fmm_far_pass<KIND_M1, P, double>(S, x, F.M64.p, F.L64.p);
+    fmm_p2p_pass<KIND_M1, P, double, MODE_BAPPLY>(S, x, dmp, F.L64.p, out);
+  }
+}
+
+template <int P>
+void fmm_field1_t(System& S, const double* src, double* out, double damping) {
+  FmmState& F     = *S.fmm;
+  const size_t nc = S.tree.ncells;
+  using O64       = Ops<KIND_M1, P, double>;
+  ensure_buf(F.M64, nc * O64::MS);
+  ensure_buf(F.L64, nc * O64::LS);
+  fmm_far_pass<KIND_M1, P, double>(S, src, F.M64.p, F.L64.p);
+  fmm_p2p_pass<KIND_M1, P, double, MODE_FIELD1>(S, src, damping, F.L64.p, out);
+}
+
+// src10: 10 SoA arrays (q, mu, packed quadrupole), device order
+template <int P>
+void fmm_static_t(System& S, const double* src10, double* out, double damping) {
+  FmmState& F     = *S.fmm;
+  const size_t nc = S.tree.ncells;
+  using OC        = Ops<KIND_MC, P, double>;
+  ensure_buf(F.Mc, nc * OC::MS);
+  ensure_buf(F.Lc, nc * OC::LS);
+  fmm_far_pass<KIND_MC, P, double>(S, src10, F.Mc.p, F.Lc.p);
+  fmm_p2p_pass<KIND_MC, P, double, MODE_STATIC>(S, src10, damping, F.Lc.p, out);
+}
+
+#define CPPE_INSTANTIATE_FMM(P)                                                          \
+  template void fmm_apply_b_t<P>(System&, const double*, double*, bool);                 \
+  template void fmm_field1_t<P>(System&, const double*, double*, double);                \
+  template void fmm_static_t<P>(System&, const double*, double*, double);
+
+}  // namespace cppe_dev
