@@ -20,6 +20,7 @@ EXPORTS = [
     "cppe_hip_apply_alpha_inv", "cppe_hip_dense_b", "cppe_hip_dense_b_inverse", "cppe_hip_solve", "cppe_hip_upload_rhs",
     "cppe_hip_solve_resident", "cppe_hip_download_mu", "cppe_hip_nuclear_fields",
     "cppe_hip_nuclear_multipole_energy", "cppe_hip_dot", "cppe_hip_tree_stats", "cppe_hip_tree_export",
+    "cppe_hip_profile_apply",
 ]
 
 
@@ -67,6 +68,7 @@ def lib():
         L.cppe_hip_dot.argtypes = [ctypes.c_longlong, _d, _d, _d]
         L.cppe_hip_tree_stats.argtypes = [vp, _ll]
         L.cppe_hip_tree_export.argtypes = [vp, _d, _ll, _ll, _i, _ll, _ll]
+        L.cppe_hip_profile_apply.argtypes = [vp, ctypes.c_int, ctypes.c_int, _d]
         _lib = L
     return _lib
 
@@ -185,6 +187,12 @@ class System:
         mu = np.zeros(3 * self.n)
         _check(lib().cppe_hip_download_mu(self.h, _p(mu)))
         return mu
+
+    def profile_apply(self, fp32=False, reps=5):
+        """Mean ms per apply: upward, m2l, l2l, near (P2P+L2P or direct), total."""
+        out = np.zeros(5)
+        _check(lib().cppe_hip_profile_apply(self.h, int(fp32), int(reps), _p(out)))
+        return dict(upward=out[0], m2l=out[1], l2l=out[2], near=out[3], total=out[4])
 
     def tree_stats(self):
         s = np.zeros(6, dtype=np.int64)

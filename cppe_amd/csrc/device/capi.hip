@@ -500,6 +500,53 @@ int cppe_hip_download_mu(cppe_hip_system* sys, double* mu) {
   CPPE_API_END
 }
 
+__global__ void k_fill_pattern(int n3, double* v) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n3) v[k] = 0.1 * sin(0.37 * (double)k + 1.0);  // fixed pseudo-random dipoles
+}
+
+int cppe_hip_profile_apply(cppe_hip_system* sys, int fp32, int reps, double* phase_ms) {
+  CPPE_API_BEGIN
+  if (!sys || reps <= 0 || !phase_ms) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
+  System& S = sys->S;
+  if (!S.has_alpha) return fail(CPPE_HIP_ERR_ARG, "system has no polarizabilities");
+  hipStream_t st = S.stream;
+  const int n3   = 3 * S.n;
+  hipLaunchKernelGGL(k_fill_pattern, dim3(ceil_div(n3, 256)), dim3(256), 0, st, n3, S.v_d.p);
+  apply_b(S, S.v_d.p, S.v_t.p, fp32 != 0);  // warm (allocations)
+  CPPE_HIP_CHECK(hipStreamSynchronize(st));
+  hipEvent_t ev[6];
+  for (auto& e : ev) CPPE_HIP_CHECK(hipEventCreate(&e));
+  double acc[5] = {0, 0, 0, 0, 0};
+  const bool fmm = S.opts.summation == CPPE_HIP_SUM_FMM;
+  for (int r = 0; r < reps; ++r) {
+    if (fmm) S.prof = ev;
+    CPPE_HIP_CHECK(hipEventRecord(ev[5], st));
+    if (!fmm) CPPE_HIP_CHECK(hipEventRecord(ev[3], st));
+    apply_b(S, S.v_d.p, S.v_t.p, fp32 != 0);
+    if (!fmm) CPPE_HIP_CHECK(hipEventRecord(ev[4], st));
+    S.prof = nullptr;
+    CPPE_HIP_CHECK(hipEventSynchronize(ev[4]));
+    float t = 0.f;
+    if (fmm) {
+      CPPE_HIP_CHECK(hipEventElapsedTime(&t, ev[0], ev[1]));
+      acc[0] += t;
+      CPPE_HIP_CHECK(hipEventElapsedTime(&t, ev[1], ev[2]));
+      acc[1] += t;
+      CPPE_HIP_CHECK(hipEventElapsedTime(&t, ev[2], ev[3]));
+      acc[2] += t;
+    }
+    CPPE_HIP_CHECK(hipEventElapsedTime(&t, ev[3], ev[4]));
+    acc[3] += t;
+    CPPE_HIP_CHECK(hipEventElapsedTime(&t, ev[5], ev[4]));
+    acc[4] += t;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (int k = 0; k < 5; ++k) phase_ms[k] = acc[k] / reps;
+  return CPPE_HIP_OK;
+  CPPE_API_END
+}
+
 int cppe_hip_nuclear_fields(int n, const double* pos, int natoms, const double* charges, const double* atom_pos,
                             double* F) {
   CPPE_API_BEGIN

@@ -383,21 +383,25 @@ void fmm_far_pass(System& S, const double* src, R* M, R* L) {
   using O         = Ops<KIND, P, R>;
   const Octree& T = S.tree;
   hipStream_t st  = S.stream;
+  const auto& lo = T.level_offset;
+  if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[0], st));
   hipLaunchKernelGGL((k_fmm_p2m<KIND, P, R>), dim3(ceil_div(T.nleaves, 64)), dim3(64), 0, st, T.nleaves,
                      T.leaf_ids.p, T.start.p, T.count.p, T.cx.p, T.cy.p, T.cz.p, S.x.p, S.y.p, S.z.p, src, S.n, M);
-  const auto& lo = T.level_offset;
   for (int lev = T.max_level - 1; lev >= 0; --lev) {
     const int c0 = lo[lev], c1 = lo[lev + 1];
     hipLaunchKernelGGL((k_fmm_m2m<KIND, P, R>), dim3(ceil_div(c1 - c0, 64)), dim3(64), 0, st, c0, c1,
                        T.nchild.p, T.child_first.p, T.cx.p, T.cy.p, T.cz.p, M);
   }
+  if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[1], st));
   hipLaunchKernelGGL((k_fmm_m2l<KIND, P, R>), dim3(T.ncells), dim3(64), 0, st, T.m2l_off.p, T.m2l_src.p, T.cx.p,
                      T.cy.p, T.cz.p, M, L);
+  if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[2], st));
   for (int lev = 1; lev <= T.max_level; ++lev) {
     const int c0 = lo[lev], c1 = lo[lev + 1];
     hipLaunchKernelGGL((k_fmm_l2l<KIND, P, R>), dim3(ceil_div(c1 - c0, 64)), dim3(64), 0, st, c0, c1,
                        T.parent.p, T.cx.p, T.cy.p, T.cz.p, L);
   }
+  if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[3], st));
   (void)sizeof(O);
 }
 
@@ -435,6 +439,7 @@ void fmm_p2p_pass(System& S, const double* src, double damping, const R* L, doub
     hipLaunchKernelGGL((k_fmm_p2p<KIND, P, R, true, MODE>), dim3(nl), dim3(64), 0, st, a, L);
   else
     hipLaunchKernelGGL((k_fmm_p2p<KIND, P, R, false, MODE>), dim3(nl), dim3(64), 0, st, a, L);
+  if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[4], st));
 }
 
 template <typename T>

@@ -77,6 +77,7 @@ struct System {
   dbuf<double> io_a, io_b;  // AoS3 staging in caller order
   dbuf<float> f32_src;      // FP32 staging for mixed-precision applies
   bool mu_valid = false;
+  hipEvent_t* prof = nullptr;  // when set: events [0..4] recorded at the FMM phase boundaries
 
   ~System();
 };

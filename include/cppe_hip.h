@@ -125,6 +125,14 @@ int cppe_hip_solve_resident(cppe_hip_system* sys, int guess_mode, double thresh,
                             int* iters, double* norms, double* device_ms);
 int cppe_hip_download_mu(cppe_hip_system* sys, double* mu);
 
+/*
+ * Times `reps` B applies of a fixed pseudo-random vector on the system's stream, with HIP
+ * events around each phase. phase_ms (5 values, mean per apply): [0] upward pass (P2M+M2M),
+ * [1] M2L, [2] L2L, [3] near field (P2P+L2P, or the direct pair kernel), [4] whole apply.
+ * No counterpart in the reference (measurement support).
+ */
+int cppe_hip_profile_apply(cppe_hip_system* sys, int fp32, int reps, double* phase_ms);
+
 /* NuclearFields::compute (electric_fields.cc:58-75): F_i = sum_a Z_a (r_i - R_a)/|r_i - R_a|^3 */
 int cppe_hip_nuclear_fields(int n, const double* pos, int natoms, const double* charges,
                             const double* atom_pos, double* F);
