@@ -33,6 +33,9 @@ struct FmmState {
   dbuf<float> M32, L32;
   dbuf<double> Mc, Lc;    // combined kind (static fields, m = 0 / 2)
   dbuf<double> src10;     // staging of combined sources
+  dbuf<double4> pos4;     // near-field source records: x, y, z, alpha_iso^(-1/6)
+  dbuf<double4> mu4;      // near-field dipole records (FP64 form)
+  dbuf<float4> mu4f;      // near-field dipole records (FP32 form)
 };
 
 template <int KIND, int P, typename R>
@@ -377,6 +380,116 @@ __global__ __launch_bounds__(64) void k_fmm_p2p(P2PArgs a, const R* __restrict__
   }
 }
 
+// Near field for dipole sources, one thread per target site (device order). Each lane walks the
+// near list of its own leaf with a private cursor over the concatenated source sites; lanes of
+// the same leaf (consecutive in Morton order) load identical records, which the memory
+// pipeline coalesces, so no LDS staging or barriers are needed. Source records are 32-byte
+// (x, y, z, s) and (mu, 0) vectors. FP32 form: the displacement is formed in FP64 and rounded,
+// the rest of the pair arithmetic is FP32.
+struct NearArgs {
+  const int* leaf_of;
+  const int* start;
+  const int* count;
+  const int* p2p_off;
+  const int* p2p_src;
+  const double *cx, *cy, *cz;
+  const double4* pos4;
+  const int* eoff;
+  const int* eidx;
+  const double* src;   // mu, 3 SoA arrays (alpha^-1 epilogue)
+  const double* ainv;  // packed alpha^-1, 6 SoA arrays
+  double damping;
+  int n;
+  double* out;
+};
+
+template <int P, typename R, bool DAMP, int MODE, typename MU4>
+__global__ __launch_bounds__(256) void k_fmm_near(NearArgs a, const MU4* __restrict__ mu4, const R* __restrict__ L) {
+  using O     = Ops<KIND_M1, P, R>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int A     = a.leaf_of[i];
+  const double4 pi = a.pos4[i];
+  const R di       = DAMP ? (R)(a.damping * pi.w) : R(0);
+  int lo = i, hi = i;
+  if (a.eoff)
+    for (int k = a.eoff[i]; k < a.eoff[i + 1]; ++k) {
+      const int e = a.eidx[k];
+      if (e >= 0) {
+        lo = min(lo, e);
+        hi = max(hi, e);
+      }
+    }
+  int k = a.p2p_off[A];
+  const int kend = a.p2p_off[A + 1];
+  R fx = R(0), fy = R(0), fz = R(0);
+  if (k < kend) {
+    int B = a.p2p_src[k];
+    int j = a.start[B], jend = j + a.count[B];
+    while (true) {
+      const double4 pj = a.pos4[j];
+      const MU4 mj     = mu4[j];
+      if (!(j >= lo && j <= hi && excluded(i, j, a.eoff, a.eidx))) {
+        if constexpr (sizeof(R) == 8) {
+          double hx = 0.0, hy = 0.0, hz = 0.0;
+          pair_m1_f64<DAMP>(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, (double)mj.x, (double)mj.y, (double)mj.z,
+                            (double)di * pj.w, hx, hy, hz);
+          fx += (R)hx;
+          fy += (R)hy;
+          fz += (R)hz;
+        } else {
+          float hx = 0.f, hy = 0.f, hz = 0.f;
+          pair_m1_f32<DAMP>((float)(pi.x - pj.x), (float)(pi.y - pj.y), (float)(pi.z - pj.z), (float)mj.x,
+                            (float)mj.y, (float)mj.z, (float)di * (float)pj.w, hx, hy, hz);
+          fx += (R)hx;
+          fy += (R)hy;
+          fz += (R)hz;
+        }
+      }
+      if (++j == jend) {
+        if (++k == kend) break;
+        B    = a.p2p_src[k];
+        j    = a.start[B];
+        jend = j + a.count[B];
+      }
+    }
+  }
+  R lx = R(0), ly = R(0), lz = R(0);
+  O::l2p((R)(pi.x - a.cx[A]), (R)(pi.y - a.cy[A]), (R)(pi.z - a.cz[A]), &L[(size_t)A * O::LS], lx, ly, lz);
+  double Fx = (double)fx + (double)lx, Fy = (double)fy + (double)ly, Fz = (double)fz + (double)lz;
+  const int n = a.n;
+  if (MODE == MODE_BAPPLY) {
+    const double m0 = a.src[i], m1 = a.src[n + i], m2 = a.src[2 * (size_t)n + i];
+    const double* ai = a.ainv;
+    const double a0 = ai[i], a1 = ai[n + i], a2 = ai[2 * (size_t)n + i], a3 = ai[3 * (size_t)n + i],
+                 a4 = ai[4 * (size_t)n + i], a5 = ai[5 * (size_t)n + i];
+    Fx += a0 * m0 + a1 * m1 + a2 * m2;
+    Fy += a1 * m0 + a3 * m1 + a4 * m2;
+    Fz += a2 * m0 + a4 * m1 + a5 * m2;
+  }
+  a.out[i]                 = Fx;
+  a.out[n + i]             = Fy;
+  a.out[2 * (size_t)n + i] = Fz;
+}
+
+static __global__ void k_pack_pos4(int n, const double* __restrict__ x, const double* __restrict__ y,
+                            const double* __restrict__ z, const double* __restrict__ s, double4* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_double4(x[i], y[i], z[i], s ? s[i] : 0.0);
+}
+
+template <typename MU4>
+__global__ void k_pack_mu4(int n, const double* __restrict__ mu, MU4* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  MU4 v;
+  v.x    = mu[i];
+  v.y    = mu[n + i];
+  v.z    = mu[2 * (size_t)n + i];
+  v.w    = 0;
+  out[i] = v;
+}
+
 // ---- host drivers ----------------------------------------------------------------------------
 template <int KIND, int P, typename R>
 void fmm_far_pass(System& S, const double* src, R* M, R* L) {
@@ -447,6 +560,43 @@ void ensure_buf(dbuf<T>& b, size_t count) {
   if (b.n < count || !b.p) b.alloc(count);
 }
 
+inline void ensure_pos4(System& S) {
+  FmmState& F = *S.fmm;
+  if (F.pos4.p && F.pos4.n >= (size_t)S.n) return;
+  F.pos4.alloc(S.n);
+  hipLaunchKernelGGL(k_pack_pos4, dim3(ceil_div(S.n, 256)), dim3(256), 0, S.stream, S.n, S.x.p, S.y.p, S.z.p, S.s.p,
+                     F.pos4.p);
+}
+
+template <int P, typename R, int MODE>
+void fmm_near_pass(System& S, const double* mu, double damping, const R* L, double* out) {
+  FmmState& F     = *S.fmm;
+  const Octree& T = S.tree;
+  hipStream_t st  = S.stream;
+  ensure_pos4(S);
+  NearArgs a{T.leaf_of.p, T.start.p, T.count.p, T.p2p_off.p, T.p2p_src.p, T.cx.p, T.cy.p, T.cz.p, F.pos4.p,
+             S.excl_off.p, S.excl_idx.p, mu, S.ainv6.p, damping, S.n, out};
+  const dim3 g(ceil_div(S.n, 256)), b(256);
+  if constexpr (sizeof(R) == 8) {
+    ensure_buf(F.mu4, S.n);
+    hipLaunchKernelGGL(k_pack_mu4<double4>, g, b, 0, st, S.n, mu, F.mu4.p);
+    if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[3], st));
+    if (damping > 0.0)
+      hipLaunchKernelGGL((k_fmm_near<P, R, true, MODE, double4>), g, b, 0, st, a, F.mu4.p, L);
+    else
+      hipLaunchKernelGGL((k_fmm_near<P, R, false, MODE, double4>), g, b, 0, st, a, F.mu4.p, L);
+  } else {
+    ensure_buf(F.mu4f, S.n);
+    hipLaunchKernelGGL(k_pack_mu4<float4>, g, b, 0, st, S.n, mu, F.mu4f.p);
+    if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[3], st));
+    if (damping > 0.0)
+      hipLaunchKernelGGL((k_fmm_near<P, R, true, MODE, float4>), g, b, 0, st, a, F.mu4f.p, L);
+    else
+      hipLaunchKernelGGL((k_fmm_near<P, R, false, MODE, float4>), g, b, 0, st, a, F.mu4f.p, L);
+  }
+  if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[4], st));
+}
+
 template <int P>
 void fmm_apply_b_t(System& S, const double* x, double* out, bool fp32) {
   FmmState& F      = *S.fmm;
@@ -457,12 +607,12 @@ void fmm_apply_b_t(System& S, const double* x, double* out, bool fp32) {
     ensure_buf(F.M32, nc * O64::MS);
     ensure_buf(F.L32, nc * O64::LS);
     fmm_far_pass<KIND_M1, P, float>(S, x, F.M32.p, F.L32.p);
-    fmm_p2p_pass<KIND_M1, P, float, MODE_BAPPLY>(S, x, dmp, F.L32.p, out);
+    fmm_near_pass<P, float, MODE_BAPPLY>(S, x, dmp, F.L32.p, out);
   } else {
     ensure_buf(F.M64, nc * O64::MS);
     ensure_buf(F.L64, nc * O64::LS);
     fmm_far_pass<KIND_M1, P, double>(S, x, F.M64.p, F.L64.p);
-    fmm_p2p_pass<KIND_M1, P, double, MODE_BAPPLY>(S, x, dmp, F.L64.p, out);
+    fmm_near_pass<P, double, MODE_BAPPLY>(S, x, dmp, F.L64.p, out);
   }
 }
 
@@ -474,7 +624,7 @@ void fmm_field1_t(System& S, const double* src, double* out, double damping) {
   ensure_buf(F.M64, nc * O64::MS);
   ensure_buf(F.L64, nc * O64::LS);
   fmm_far_pass<KIND_M1, P, double>(S, src, F.M64.p, F.L64.p);
-  fmm_p2p_pass<KIND_M1, P, double, MODE_FIELD1>(S, src, damping, F.L64.p, out);
+  fmm_near_pass<P, double, MODE_FIELD1>(S, src, damping, F.L64.p, out);
 }
 
 // src10: 10 SoA arrays (q, mu, packed quadrupole), device order

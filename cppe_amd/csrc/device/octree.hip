@@ -35,28 +35,27 @@ constexpr int kMaxLevel = 21;
 __global__ __launch_bounds__(256) void k_root(int n, const double* __restrict__ x,
                                               const double* __restrict__ y,
                                               const double* __restrict__ z, double* __restrict__ out) {
+  constexpr int CH = 2048;
   __shared__ double avg[3];
   __shared__ double red[3][256];
+  __shared__ double buf[3][CH];
   const int t = threadIdx.x;
-  if (t < 3) {
-    const double* a = t == 0 ? x : (t == 1 ? y : z);
-    double s = 0.0;
-    int i = 0;
-    for (; i + 8 <= n; i += 8) {  // loads hoisted, additions kept in index order
-      double v0 = a[i], v1 = a[i + 1], v2 = a[i + 2], v3 = a[i + 3];
-      double v4 = a[i + 4], v5 = a[i + 5], v6 = a[i + 6], v7 = a[i + 7];
-      s = s + v0;
-      s = s + v1;
-      s = s + v2;
-      s = s + v3;
-      s = s + v4;
-      s = s + v5;
-      s = s + v6;
-      s = s + v7;
+  // sums in index order (one thread per axis) over LDS-staged chunks: the rounding of each
+  // addition is that of the reference's sequential loop
+  double s = 0.0;
+  for (int base = 0; base < n; base += CH) {
+    const int m = min(CH, n - base);
+    __syncthreads();
+    for (int k = t; k < m; k += blockDim.x) {
+      buf[0][k] = x[base + k];
+      buf[1][k] = y[base + k];
+      buf[2][k] = z[base + k];
     }
-    for (; i < n; ++i) s = s + a[i];
-    avg[t] = s / (double)n;
+    __syncthreads();
+    if (t < 3)
+      for (int k = 0; k < m; ++k) s = s + buf[t][k];
   }
+  if (t < 3) avg[t] = s / (double)n;
   __syncthreads();
   double m0 = 0.0, m1 = 0.0, m2 = 0.0;
   for (int i = t; i < n; i += blockDim.x) {
@@ -247,6 +246,14 @@ __global__ void k_leaf_flags(int ncells, const int* nchild, int* flag) {
   if (c < ncells) flag[c] = nchild[c] == 0;
 }
 
+__global__ void k_leaf_of(int nleaves, const int* __restrict__ leaf_ids, const int* __restrict__ start,
+                          const int* __restrict__ count, int* __restrict__ leaf_of) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nleaves) return;
+  const int c = leaf_ids[t];
+  for (int i = start[c]; i < start[c] + count[c]; ++i) leaf_of[i] = c;
+}
+
 __global__ void k_scatter_ids(int n, const int* flag, const int* rank, int* out) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < n && flag[c]) out[rank[c]] = c;
@@ -394,6 +401,9 @@ void build_octree(System& S) {
   T.leaf_ids.alloc(T.nleaves);
   hipLaunchKernelGGL(k_scatter_ids, dim3(ceil_div(ncells, 256)), dim3(256), 0, st, ncells, lf.p, lr.p,
                      T.leaf_ids.p);
+  T.leaf_of.alloc(n);
+  hipLaunchKernelGGL(k_leaf_of, dim3(ceil_div(T.nleaves, 256)), dim3(256), 0, st, T.nleaves, T.leaf_ids.p,
+                     T.start.p, T.count.p, T.leaf_of.p);
   S.h_perm.resize(n);
   CPPE_HIP_CHECK(hipMemcpyAsync(S.h_perm.data(), S.perm.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
   CPPE_HIP_CHECK(hipStreamSynchronize(st));

@@ -42,6 +42,7 @@ struct Octree {
   dbuf<double> cx, cy, cz, cr, crmax;
   dbuf<int> start, count, parent, level, child_first, nchild, octmask;
   dbuf<int> leaf_ids;  // indices of leaf cells
+  dbuf<int> leaf_of;   // leaf cell of each site (device order)
   // interaction lists grouped by target (owner computes): CSR over target cells
   long long n_m2l = 0, n_p2p = 0, p2p_site_pairs = 0;
   dbuf<int> m2l_off, m2l_src;  // ncells+1 offsets, sources
