@@ -96,6 +96,7 @@ def main():
     ap.add_argument("--summation", default="fmm", choices=["fmm", "direct"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=5)
+    ap.add_argument("--no-ml", action="store_true", help="headline form without the learned warm start")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the induced-dipole path is built and timed on a single MI355X "
@@ -121,12 +122,34 @@ def main():
     S32.upload_rhs(rhs)
     log("setup (octree + lists) ms", round(t_setup, 1))
 
-    ml_used = False
-    guess_mode_head = 0
+    from cppe_amd import warmstart
+    model = None if args.no_ml else warmstart.load()
+    ml_used = model is not None
     wall64, dev64, its64 = time_solves(S64, args.steps, args.warmup, 0)
     log("fp64 solve ms", wall64, "iters", its64)
-    wall32, dev32, its32 = time_solves(S32, args.steps, args.warmup, guess_mode_head)
-    log("mixed solve ms", wall32, "iters", its32)
+    guess_ms = []
+    if ml_used:
+        # headline step = learned guess (descriptors + network + combination) + mixed solve
+        def head_solve():
+            g = capi.ml_guess(S32, model)
+            it, norms, ms = S32.solve_resident(1, 1e-8, 200)
+            return it, ms + g, g
+        for _ in range(args.warmup):
+            head_solve()
+        wall32, dev32, its32 = [], [], []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            it, ms, g = head_solve()
+            wall32.append((time.perf_counter() - t0) * 1e3)
+            dev32.append(ms)
+            its32.append(it)
+            guess_ms.append(g)
+        wall32, dev32 = float(np.mean(wall32)), float(np.mean(dev32))
+    else:
+        wall32, dev32, its32 = time_solves(S32, args.steps, args.warmup, 0)
+    log("headline solve ms", wall32, "iters", its32, "ml", ml_used)
+    # the same mixed solve from alpha F, for the ML gain
+    wallm, devm, itsm = time_solves(S32, max(1, args.steps // 2), 0, 0)
     mu64 = S64.download_mu()
     mu32 = S32.download_mu()
     e64 = -0.5 * float(mu64 @ rhs)
@@ -188,6 +211,9 @@ def main():
         "device_ms": {"headline": round(dev32, 3), "fp64": round(dev64, 3)},
         "iterations": {"headline": its32, "fp64": its64},
         "ml_warm_start": ml_used,
+        "ml_guess_ms": round(float(np.mean(guess_ms)), 3) if guess_ms else None,
+        "mixed_alphaF_ms": round(wallm, 3),
+        "mixed_alphaF_iterations": itsm,
         "epol_fp64": e64,
         "epol_rel_diff": abs(e32 - e64) / abs(e64),
         "sites_per_s": round(w.n / (wall32 * 1e-3), 1),

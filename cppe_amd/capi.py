@@ -21,7 +21,12 @@ EXPORTS = [
     "cppe_hip_solve_resident", "cppe_hip_download_mu", "cppe_hip_nuclear_fields",
     "cppe_hip_nuclear_multipole_energy", "cppe_hip_dot", "cppe_hip_tree_stats", "cppe_hip_tree_export",
     "cppe_hip_profile_apply",
+    "cppe_hip_mlp_create", "cppe_hip_mlp_destroy", "cppe_hip_mlp_set", "cppe_hip_mlp_get", "cppe_hip_ml_features",
+    "cppe_hip_mlp_forward", "cppe_hip_mlp_loss_grad", "cppe_hip_mlp_reserve", "cppe_hip_mlp_collect",
+    "cppe_hip_mlp_prepare", "cppe_hip_mlp_train", "cppe_hip_ml_guess",
 ]
+MLP_IN = 64
+MLP_PARAMS = 25219
 
 
 class Options(ctypes.Structure):
@@ -69,6 +74,19 @@ def lib():
         L.cppe_hip_tree_stats.argtypes = [vp, _ll]
         L.cppe_hip_tree_export.argtypes = [vp, _d, _ll, _ll, _i, _ll, _ll]
         L.cppe_hip_profile_apply.argtypes = [vp, ctypes.c_int, ctypes.c_int, _d]
+        _f = ctypes.POINTER(ctypes.c_float)
+        L.cppe_hip_mlp_create.argtypes = [ctypes.POINTER(vp)]
+        L.cppe_hip_mlp_destroy.argtypes = [vp]
+        L.cppe_hip_mlp_set.argtypes = [vp, _f, _f, _f]
+        L.cppe_hip_mlp_get.argtypes = [vp, _f, _f, _f]
+        L.cppe_hip_ml_features.argtypes = [vp, _f, _d]
+        L.cppe_hip_mlp_forward.argtypes = [vp, _f, ctypes.c_longlong, _f]
+        L.cppe_hip_mlp_loss_grad.argtypes = [vp, _f, _f, _f, _f, ctypes.c_longlong, _f, _d]
+        L.cppe_hip_mlp_reserve.argtypes = [vp, ctypes.c_longlong]
+        L.cppe_hip_mlp_collect.argtypes = [vp, vp]
+        L.cppe_hip_mlp_prepare.argtypes = [vp]
+        L.cppe_hip_mlp_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint, _d]
+        L.cppe_hip_ml_guess.argtypes = [vp, vp, _d]
         _lib = L
     return _lib
 
@@ -241,3 +259,89 @@ def dot(a, b):
     out = ctypes.c_double(0.0)
     _check(lib().cppe_hip_dot(a.size, _p(a), _p(b), ctypes.byref(out)))
     return out.value
+
+
+_f = ctypes.POINTER(ctypes.c_float)
+
+
+def _pf(a):
+    return None if a is None else a.ctypes.data_as(_f)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Mlp:
+    """Device-resident warm-start network (cppe_hip_mlp): parameters, scaling, training set."""
+
+    def __init__(self, params=None, mean=None, inv_std=None):
+        h = ctypes.c_void_p()
+        _check(lib().cppe_hip_mlp_create(ctypes.byref(h)))
+        self.h = h
+        if params is not None:
+            self.set(params, mean, inv_std)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cppe_hip_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set(self, params, mean=None, inv_std=None):
+        p = _f32(params)
+        assert p.size == MLP_PARAMS
+        m = None if mean is None else _f32(mean)
+        s = None if inv_std is None else _f32(inv_std)
+        _check(lib().cppe_hip_mlp_set(self.h, _pf(p), _pf(m), _pf(s)))
+
+    def get(self):
+        p = np.zeros(MLP_PARAMS, dtype=np.float32)
+        m = np.zeros(MLP_IN, dtype=np.float32)
+        s = np.zeros(MLP_IN, dtype=np.float32)
+        _check(lib().cppe_hip_mlp_get(self.h, _pf(p), _pf(m), _pf(s)))
+        return p, m, s
+
+    def forward(self, X):
+        X = _f32(X)
+        ns = X.shape[0]
+        y = np.zeros((ns, 3), dtype=np.float32)
+        _check(lib().cppe_hip_mlp_forward(self.h, _pf(X), ns, _pf(y)))
+        return y
+
+    def loss_grad(self, X, V, T, w):
+        X, V, T, w = _f32(X), _f32(V), _f32(T), _f32(w)
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        loss = ctypes.c_double(0.0)
+        _check(lib().cppe_hip_mlp_loss_grad(self.h, _pf(X), _pf(V), _pf(T), _pf(w), X.shape[0], _pf(g),
+                                            ctypes.byref(loss)))
+        return loss.value, g
+
+    def reserve(self, ns):
+        _check(lib().cppe_hip_mlp_reserve(self.h, int(ns)))
+
+    def collect(self, system):
+        _check(lib().cppe_hip_mlp_collect(self.h, system.h))
+
+    def prepare(self):
+        _check(lib().cppe_hip_mlp_prepare(self.h))
+
+    def train(self, epochs, batch=4096, lr=1e-3, seed=0):
+        losses = np.zeros(max(epochs, 1))
+        _check(lib().cppe_hip_mlp_train(self.h, int(epochs), int(batch), float(lr), int(seed), _p(losses)))
+        return losses[:epochs]
+
+
+def ml_features(system):
+    X = np.zeros((system.n, MLP_IN), dtype=np.float32)
+    V = np.zeros((system.n, 9))
+    _check(lib().cppe_hip_ml_features(system.h, _pf(X), _p(V)))
+    return X, V
+
+
+def ml_guess(system, mlp):
+    ms = ctypes.c_double(0.0)
+    _check(lib().cppe_hip_ml_guess(system.h, mlp.h, ctypes.byref(ms)))
+    return ms.value

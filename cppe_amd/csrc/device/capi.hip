@@ -96,6 +96,10 @@ static int fail(int code, const std::string& msg) {
   return code;
 }
 
+namespace cppe_dev {
+int capi_fail(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace cppe_dev
+
 #define CPPE_API_BEGIN try {
 #define CPPE_API_END                                               \
   }                                                                \

@@ -1,0 +1,105 @@
+"""Learned CG warm start: training-set generation, device training and the shipped model.
+
+The network (see include/cppe_hip.h, ``cppe_hip_mlp_*``) is trained on induced dipoles that the
+FP64 solver itself produces: seeded synthetic water boxes of 3k-30k sites (the benchmark
+recipe with per-snapshot seeds), static fields as right-hand side, FMM summation (theta 0.5,
+order 5, ncrit 64), Thole damping 2.1304, CG to 1e-8. Every site of every snapshot is a sample.
+
+    python -m cppe_amd.warmstart --snapshots 2000 --epochs 8 --out cppe_amd/data/warmstart_mlp.npz
+
+The model file holds the FP32 parameters, the feature scaling and the training metadata.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import capi, synthetic
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_MODEL = os.path.join(PKG, "data", "warmstart_mlp.npz")
+TRAIN_SEED0 = 20000
+DAMPING = 2.1304
+
+
+def static_rhs(w, summation="fmm"):
+    """Static field at the sites (nuclei + multipoles), computed on the device."""
+    Fn = capi.nuclear_fields(w.pos, w.atoms_z.astype(float), w.atoms_pos)
+    Sm = capi.System(w.pos, excl=w.excl_csr(), summation=summation)
+    Fm = Sm.multipole_fields(w.charges, w.dipoles, w.quadrupoles)
+    Sm.close()
+    return Fn + Fm
+
+
+def snapshot_sizes(n, seed=TRAIN_SEED0, lo=1000, hi=10000):
+    """Water counts of the training snapshots: log-uniform in [lo, hi] waters (3k-30k sites)."""
+    rng = np.random.default_rng(seed)
+    return np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=n))).astype(int)
+
+
+def solved_system(w, thresh=1e-8, summation="fmm"):
+    S = capi.System(w.pos, w.pol, w.alpha_iso(), w.excl_csr(), summation=summation, damping=DAMPING)
+    S.upload_rhs(static_rhs(w, summation))
+    it, _, _ = S.solve_resident(0, thresh, 500)
+    return S, it
+
+
+def train(n_snapshots=2000, epochs=8, batch=4096, lr=1e-3, log=print):
+    sizes = snapshot_sizes(n_snapshots)
+    M = capi.Mlp()
+    M.reserve(int(3 * sizes.sum()))
+    t0 = time.time()
+    iters = []
+    for k, nw in enumerate(sizes):
+        w = synthetic.water_box(int(nw), TRAIN_SEED0 + 1 + k, anisotropic=True)
+        S, it = solved_system(w)
+        iters.append(it)
+        M.collect(S)
+        S.close()
+        if (k + 1) % 100 == 0:
+            log("collected %d/%d snapshots (%.0f s)" % (k + 1, n_snapshots, time.time() - t0))
+    M.prepare()
+    log("training on %d sites" % int(3 * sizes.sum()))
+    losses = M.train(epochs, batch, lr, seed=1)
+    for e, l in enumerate(losses):
+        log("epoch %d loss %.5e" % (e, l))
+    meta = dict(n_snapshots=int(n_snapshots), sites=int(3 * sizes.sum()), epochs=int(epochs), batch=int(batch),
+                lr=float(lr), losses=[float(x) for x in losses], mean_cg_iterations=float(np.mean(iters)),
+                seconds=time.time() - t0)
+    return M, meta
+
+
+def save(M, meta, path=DEFAULT_MODEL):
+    p, mean, istd = M.get()
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.savez(path, params=p, mean=mean, inv_std=istd, meta=json.dumps(meta))
+
+
+def load(path=DEFAULT_MODEL):
+    """The shipped model as a device network, or None when no model file exists."""
+    if not os.path.exists(path):
+        return None
+    d = np.load(path, allow_pickle=False)
+    return capi.Mlp(d["params"], d["mean"], d["inv_std"])
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--snapshots", type=int, default=2000)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--out", default=DEFAULT_MODEL)
+    args = ap.parse_args()
+    M, meta = train(args.snapshots, args.epochs, args.batch, args.lr,
+                    log=lambda *a: print(*a, file=sys.stderr, flush=True))
+    save(M, meta, args.out)
+    print(json.dumps(meta))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

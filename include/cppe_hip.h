@@ -159,6 +159,44 @@ int cppe_hip_tree_stats(cppe_hip_system* sys, long long* stats);
 int cppe_hip_tree_export(cppe_hip_system* sys, double* geo, long long* meta, long long* cell_start,
                          int* perm, long long* m2l, long long* p2p);
 
+/* ---------------------------------------------------------------------------------------------
+ * Learned CG warm start (no counterpart in the reference). A per-site MLP 64 -> 128 -> 128 -> 3
+ * (SiLU, BF16 MFMA with FP32 accumulation) maps 64 invariant descriptors of a site's
+ * neighbourhood (6 A) and permanent field to coefficients c of three equivariant vectors
+ * (v1 = alpha F, v2 and v3 = local damped Neumann terms); the guess is sum_a c_a v_a.
+ * CG from that guess is unchanged and still stops on the FP64 residual; a zero or alpha F guess
+ * remains available (guess_mode 0 / 2 of cppe_hip_solve_resident).
+ * ------------------------------------------------------------------------------------------- */
+#define CPPE_HIP_MLP_IN 64
+#define CPPE_HIP_MLP_PARAMS 25219 /* W1[64][128] b1[128] W2[128][128] b2[128] W3[128][3] b3[3] */
+
+typedef struct cppe_hip_mlp cppe_hip_mlp;
+
+int cppe_hip_mlp_create(cppe_hip_mlp** out);
+int cppe_hip_mlp_destroy(cppe_hip_mlp* m);
+/* FP32 parameters (CPPE_HIP_MLP_PARAMS) and the feature scaling (64 means, 64 inverse std). */
+int cppe_hip_mlp_set(cppe_hip_mlp* m, const float* params, const float* mean, const float* inv_std);
+int cppe_hip_mlp_get(cppe_hip_mlp* m, float* params, float* mean, float* inv_std);
+/* Descriptors X (n*64 FP32) and bases V (n*9: v1, v2, v3 as xyz triples) of the system's
+ * resident rhs (cppe_hip_upload_rhs), caller order. */
+int cppe_hip_ml_features(cppe_hip_system* sys, float* X, double* V);
+/* y (ns*3) = MLP of raw features X (ns*64), scaled with the model's feature scaling. */
+int cppe_hip_mlp_forward(cppe_hip_mlp* m, const float* X, long long ns, float* y);
+/* Mean loss sum_s w_s |sum_a y_sa V_sa - T_s|^2 / ns and its gradient (CPPE_HIP_MLP_PARAMS). */
+int cppe_hip_mlp_loss_grad(cppe_hip_mlp* m, const float* X, const float* V, const float* T, const float* w,
+                           long long ns, float* grad, double* loss);
+/* Training set in HBM: reserve room for ns samples; append every site of a system (features of its
+ * resident rhs, bases, and its resident moments as targets); fix the feature scaling. */
+int cppe_hip_mlp_reserve(cppe_hip_mlp* m, long long ns);
+int cppe_hip_mlp_collect(cppe_hip_mlp* m, cppe_hip_system* sys);
+int cppe_hip_mlp_prepare(cppe_hip_mlp* m);
+/* Adam epochs over the collected set (minibatches of `batch` samples, tiles shuffled with `seed`);
+ * losses receives the mean training loss of each epoch. */
+int cppe_hip_mlp_train(cppe_hip_mlp* m, int epochs, int batch, float lr, unsigned seed, double* losses);
+/* Learned guess into the system's resident moments (use guess_mode 1 of cppe_hip_solve_resident);
+ * *device_ms receives the time of features + network + guess. */
+int cppe_hip_ml_guess(cppe_hip_system* sys, cppe_hip_mlp* m, double* device_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,105 @@
+"""GPU checks of the learned warm start against the CPU reference (oracle/mlp_ref.py).
+
+* descriptors and bases: GPU kernels vs brute force (FP32 features rtol 1e-4, bases 1e-9)
+* forward: BF16-MFMA network vs the FP64 reference with the same bfloat16 rounding points
+* gradient: GPU backward vs the FP64 reference gradient (bf16-emulated tight, pure FP64 loose)
+* training + guess: a small training run lowers the loss and the learned guess needs fewer CG
+  iterations than alpha F on a held-out snapshot, with the same 1e-8 FP64 stopping test.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from cppe_amd import capi, synthetic
+from oracle import mlp_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def rhs_of(w):
+    Fn = capi.nuclear_fields(w.pos, w.atoms_z.astype(float), w.atoms_pos)
+    Sm = capi.System(w.pos, excl=w.excl_csr())
+    return Fn + Sm.multipole_fields(w.charges, w.dipoles, w.quadrupoles)
+
+
+def test_features_vs_reference():
+    w = synthetic.water_box(250, 21, anisotropic=True)
+    F = rhs_of(w)
+    S = capi.System(w.pos, w.pol, w.alpha_iso(), w.excl_csr(), damping=2.1304)
+    S.upload_rhs(F)
+    X, V = capi.ml_features(S)
+    Xr, Vr = R.features(w.pos, w.pol, F.reshape(-1, 3), w.excl_csr(), 2.1304)
+    np.testing.assert_allclose(V, Vr, rtol=1e-9, atol=1e-12 * np.abs(Vr).max())
+    scale = np.maximum(np.abs(Xr).max(axis=0), 1e-3)
+    assert (np.abs(X - Xr) / scale).max() < 2e-5
+
+
+def _random_model(seed):
+    rng = np.random.default_rng(seed)
+    p = (rng.normal(size=R.NPARAMS) * 0.15).astype(np.float32)
+    mean = rng.normal(size=64).astype(np.float32) * 0.1
+    istd = rng.uniform(0.5, 1.5, size=64).astype(np.float32)
+    return p, mean, istd
+
+
+def test_forward_vs_reference():
+    p, mean, istd = _random_model(3)
+    M = capi.Mlp(p, mean, istd)
+    X = np.random.default_rng(4).normal(size=(300, 64)).astype(np.float32)
+    y = M.forward(X)
+    yr = R.forward(p, mean, istd, X, emulate_bf16=True)
+    np.testing.assert_allclose(y, yr, rtol=2e-3, atol=2e-3 * np.abs(yr).max())
+    y64 = R.forward(p, mean, istd, X, emulate_bf16=False)
+    assert np.abs(y - y64).max() < 3e-2 * np.abs(y64).max()
+
+
+def test_gradient_vs_reference():
+    p, mean, istd = _random_model(5)
+    M = capi.Mlp(p, mean, istd)
+    rng = np.random.default_rng(6)
+    ns = 203  # ragged last tile
+    X = rng.normal(size=(ns, 64)).astype(np.float32)
+    V = rng.normal(size=(ns, 9)).astype(np.float32)
+    T = rng.normal(size=(ns, 3)).astype(np.float32)
+    w = rng.uniform(0.5, 2.0, size=ns).astype(np.float32)
+    loss, g = M.loss_grad(X, V, T, w)
+    lr_, gr = R.loss_grad(p, mean, istd, X, V, T, w, emulate_bf16=True)
+    assert abs(loss - lr_) <= 1e-4 * abs(lr_)
+    for lo, hi in [(R.OFF_W1, R.OFF_B1), (R.OFF_B1, R.OFF_W2), (R.OFF_W2, R.OFF_B2), (R.OFF_B2, R.OFF_W3),
+                   (R.OFF_W3, R.OFF_B3), (R.OFF_B3, R.NPARAMS)]:
+        a, b = g[lo:hi], gr[lo:hi]
+        assert np.abs(a - b).max() <= 5e-3 * np.abs(b).max() + 1e-7, (lo, hi)
+    _, g64 = R.loss_grad(p, mean, istd, X, V, T, w, emulate_bf16=False)
+    assert np.linalg.norm(g - g64) <= 5e-2 * np.linalg.norm(g64)
+
+
+def test_training_and_guess_gain():
+    """Train on a few small boxes; the learned guess must save CG iterations on a held-out box."""
+    M = capi.Mlp()
+    boxes = [synthetic.water_box(int(nw), 3000 + k, anisotropic=True) for k, nw in enumerate([500, 700, 600, 800])]
+    M.reserve(sum(b.n for b in boxes))
+    for b in boxes:
+        S = capi.System(b.pos, b.pol, b.alpha_iso(), b.excl_csr(), damping=2.1304)
+        S.upload_rhs(rhs_of(b))
+        S.solve_resident(0, 1e-10, 200)
+        M.collect(S)
+    M.prepare()
+    losses = M.train(epochs=30, batch=1024, lr=2e-3, seed=0)
+    assert losses[-1] < 0.5 * losses[0]
+    test = synthetic.water_box(700, 3999, anisotropic=True)
+    F = rhs_of(test)
+    S = capi.System(test.pos, test.pol, test.alpha_iso(), test.excl_csr(), damping=2.1304)
+    S.upload_rhs(F)
+    it0, _, _ = S.solve_resident(0, 1e-8, 200)
+    mu_ref = S.download_mu()
+    capi.ml_guess(S, M)
+    g = S.download_mu()
+    it1, _, _ = S.solve_resident(1, 1e-8, 200)
+    mu = S.download_mu()
+    err_alphaF = np.linalg.norm(S.apply_alpha(F) - mu_ref)
+    err_ml = np.linalg.norm(g - mu_ref)
+    print("guess error alpha F %.3e  learned %.3e  iterations %d -> %d" % (err_alphaF, err_ml, it0, it1))
+    assert err_ml < 0.5 * err_alphaF
+    assert it1 < it0
+    B = O.BSystem(test.pos, test.pol, test.excl_csr(), damping=2.1304)
+    assert np.linalg.norm(F - B.apply(mu)) < 1e-8
