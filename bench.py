@@ -97,6 +97,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=5)
     ap.add_argument("--no-ml", action="store_true", help="headline form without the learned warm start")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuned ones")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the induced-dipole path is built and timed on a single MI355X "
@@ -121,6 +122,12 @@ def main():
     S64.upload_rhs(rhs)
     S32.upload_rhs(rhs)
     log("setup (octree + lists) ms", round(t_setup, 1))
+    tune = None
+    if args.summation == "fmm" and not args.no_tune:
+        t0 = time.perf_counter()
+        tune = {"fp64": S64.autotune(reps=3)["fp64"], "fp32": S32.autotune(reps=3)["fp32"]}
+        tune["seconds"] = round(time.perf_counter() - t0, 2)
+        log("autotune", tune)
 
     from cppe_amd import warmstart
     model = None if args.no_ml else warmstart.load()
@@ -219,6 +226,7 @@ def main():
         "sites_per_s": round(w.n / (wall32 * 1e-3), 1),
         "setup_ms": round(t_setup, 2),
         "tree": st,
+        "autotune": tune,
         "phases_ms": {"headline": {k: round(v, 4) for k, v in prof32.items()},
                       "fp64": {k: round(v, 4) for k, v in prof64.items()}},
         "roofline": roof,

@@ -20,7 +20,8 @@ EXPORTS = [
     "cppe_hip_apply_alpha_inv", "cppe_hip_dense_b", "cppe_hip_dense_b_inverse", "cppe_hip_solve", "cppe_hip_upload_rhs",
     "cppe_hip_solve_resident", "cppe_hip_download_mu", "cppe_hip_nuclear_fields",
     "cppe_hip_nuclear_multipole_energy", "cppe_hip_dot", "cppe_hip_tree_stats", "cppe_hip_tree_export",
-    "cppe_hip_profile_apply",
+    "cppe_hip_profile_apply", "cppe_hip_batch_create", "cppe_hip_segment_results", "cppe_hip_set_graphs",
+    "cppe_hip_autotune", "cppe_hip_set_tune",
     "cppe_hip_mlp_create", "cppe_hip_mlp_destroy", "cppe_hip_mlp_set", "cppe_hip_mlp_get", "cppe_hip_ml_features",
     "cppe_hip_mlp_forward", "cppe_hip_mlp_loss_grad", "cppe_hip_mlp_reserve", "cppe_hip_mlp_collect",
     "cppe_hip_mlp_prepare", "cppe_hip_mlp_train", "cppe_hip_ml_guess",
@@ -74,6 +75,12 @@ def lib():
         L.cppe_hip_tree_stats.argtypes = [vp, _ll]
         L.cppe_hip_tree_export.argtypes = [vp, _d, _ll, _ll, _i, _ll, _ll]
         L.cppe_hip_profile_apply.argtypes = [vp, ctypes.c_int, ctypes.c_int, _d]
+        L.cppe_hip_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, _i, _i, _d, _d, _d, _i, _i,
+                                            ctypes.POINTER(Options)]
+        L.cppe_hip_segment_results.argtypes = [vp, _i, _d, _i]
+        L.cppe_hip_set_graphs.argtypes = [vp, ctypes.c_int]
+        L.cppe_hip_autotune.argtypes = [vp, ctypes.c_int, _i, _d]
+        L.cppe_hip_set_tune.argtypes = [vp, _i]
         _f = ctypes.POINTER(ctypes.c_float)
         L.cppe_hip_mlp_create.argtypes = [ctypes.POINTER(vp)]
         L.cppe_hip_mlp_destroy.argtypes = [vp]
@@ -201,6 +208,32 @@ class System:
                                              ctypes.byref(it), _p(norms), ctypes.byref(ms)))
         return it.value, norms[:it.value], ms.value
 
+    NEAR_CANDIDATES = [(0, 64), (0, 128), (0, 256), (1, 64), (1, 128), (1, 256)]
+    M2L_CANDIDATES = [16, 32, 64]
+
+    def autotune(self, reps=3):
+        """Times the B-apply launch candidates on this system and keeps the fastest (tree path)."""
+        choice = np.zeros(6, dtype=np.int32)
+        ms = np.zeros(18)
+        _check(lib().cppe_hip_autotune(self.h, int(reps), _p(choice, _i), _p(ms)))
+        out = {}
+        for k, prec in enumerate(["fp64", "fp32"]):
+            m = ms[9 * k:9 * k + 9]
+            out[prec] = dict(near_variant=int(choice[3 * k]), near_block=int(choice[3 * k + 1]),
+                             m2l_group=int(choice[3 * k + 2]),
+                             near_ms={"%s/%d" % ("thread" if v == 0 else "leafwave", b): float(t)
+                                      for (v, b), t in zip(self.NEAR_CANDIDATES, m[:6])},
+                             m2l_ms={str(g): float(t) for g, t in zip(self.M2L_CANDIDATES, m[6:])})
+        self.tune_choice = choice
+        return out
+
+    def set_tune(self, choice):
+        c = np.ascontiguousarray(choice, dtype=np.int32)
+        _check(lib().cppe_hip_set_tune(self.h, _p(c, _i)))
+
+    def set_graphs(self, enable=True):
+        _check(lib().cppe_hip_set_graphs(self.h, int(bool(enable))))
+
     def download_mu(self):
         mu = np.zeros(3 * self.n)
         _check(lib().cppe_hip_download_mu(self.h, _p(mu)))
@@ -230,6 +263,48 @@ class System:
         _check(lib().cppe_hip_tree_export(self.h, _p(geo), _p(meta, _ll), _p(start, _ll), _p(perm, _i),
                                           _p(m2l, _ll), _p(p2p, _ll)))
         return dict(geo=geo, meta=meta, start=start, perm=perm, m2l=m2l[:st["n_m2l"]], p2p=p2p[:st["n_p2p"]])
+
+
+class Batch(System):
+    """Independent snapshots held by one handle (cppe_hip_batch_create): one tree per snapshot, shared
+    field applies, a CG recurrence per snapshot. ``direct[g]`` puts snapshot g in a single leaf (exact
+    direct sum); the others use the tree with ``ncrit``."""
+
+    def __init__(self, snapshots, direct, ncrit=64, order=5, theta=0.5, damping=0.0, precision="fp64"):
+        sizes = [int(s.n) for s in snapshots]
+        self.seg_off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+        self.nseg = len(sizes)
+        self.pos = np.ascontiguousarray(np.concatenate([s.pos for s in snapshots]), dtype=np.float64).reshape(-1)
+        self.n = self.pos.size // 3
+        a6 = np.ascontiguousarray(np.concatenate([s.pol for s in snapshots]), dtype=np.float64).reshape(-1)
+        ai = np.ascontiguousarray(np.concatenate([s.alpha_iso() for s in snapshots]), dtype=np.float64)
+        offs, idxs, base_e = [np.zeros(1, dtype=np.int64)], [], 0
+        for s, o in zip(snapshots, self.seg_off[:-1]):
+            eo, ei = s.excl_csr()
+            offs.append(eo[1:].astype(np.int64) + base_e)
+            idxs.append(np.where(ei >= 0, ei + o, ei))
+            base_e += int(eo[-1])
+        off = np.ascontiguousarray(np.concatenate(offs), dtype=np.int32)
+        idx = np.ascontiguousarray(np.concatenate(idxs) if base_e else np.zeros(1), dtype=np.int32)
+        nc = np.ascontiguousarray([0 if d else int(ncrit) for d in direct], dtype=np.int32)
+        self.opts = Options(SUM_FMM, int(ncrit), int(order), float(theta), float(damping),
+                            PREC_MIXED if precision == "mixed" else PREC_FP64)
+        h = ctypes.c_void_p()
+        _check(lib().cppe_hip_batch_create(ctypes.byref(h), self.nseg, _p(self.seg_off, _i), _p(nc, _i),
+                                           _p(self.pos), _p(a6), _p(ai), _p(off, _i), _p(idx, _i),
+                                           ctypes.byref(self.opts)))
+        self.h = h
+
+    def segment_results(self):
+        it = np.zeros(self.nseg, dtype=np.int32)
+        rn = np.zeros(self.nseg)
+        cv = np.zeros(self.nseg, dtype=np.int32)
+        _check(lib().cppe_hip_segment_results(self.h, _p(it, _i), _p(rn), _p(cv, _i)))
+        return it, rn, cv.astype(bool)
+
+    def split(self, v):
+        """Per-snapshot views of a concatenated AoS3 vector."""
+        return [v[3 * a:3 * b] for a, b in zip(self.seg_off[:-1], self.seg_off[1:])]
 
 
 def nuclear_fields(pos, charges, atom_pos):

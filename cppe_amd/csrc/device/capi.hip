@@ -12,11 +12,9 @@
 
 namespace cppe_dev {
 
-int device_cg(System& S, const double* rhs, double* x, double thresh, int maxiter, bool mixed,
-              std::vector<double>& norms, bool& converged);
-void precondition(System& S, const double* in, double* out);
-void apply_b(System& S, const double* x, double* out, bool fp32);
 void dense_inverse(hipStream_t st, int N, double* A, double* X);
+void fmm_autotune(System& S, int reps, int* choice, double* ms);
+void fmm_set_tune(System& S, const int* choice);
 
 __global__ void k_scatter_xyz(int n, const double* __restrict__ aos, double* x, double* y, double* z) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -77,6 +75,8 @@ __global__ void k_excl_fill(int n, const int* __restrict__ perm, const int* __re
 }
 
 System::~System() {
+  cg_graphs_release(*this);
+  if (h_status) (void)hipHostFree(h_status);
   if (fmm) fmm_destroy(*this);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -130,25 +130,21 @@ int cppe_hip_device_count(int* count) {
   return CPPE_HIP_OK;
 }
 
-int cppe_hip_system_create(cppe_hip_system** out, int n, const double* pos, const double* alpha6,
-                           const double* alpha_iso, const int* excl_off, const int* excl_idx,
-                           const cppe_hip_options* opts) {
-  CPPE_API_BEGIN
-  *out = nullptr;
-  if (n <= 0 || !pos || !opts) return fail(CPPE_HIP_ERR_ARG, "invalid system arguments");
-  if (opts->ncrit < 2) return fail(CPPE_HIP_ERR_ARG, "tree_ncrit must be >= 2");
-  if (opts->summation == CPPE_HIP_SUM_FMM && (opts->order < 1 || opts->order > 8))
-    return fail(CPPE_HIP_ERR_UNSUPPORTED, "tree_expansion_order must be in 1..8");
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-    return fail(CPPE_HIP_ERR_DEVICE, "no HIP device available");
-  auto* H  = new cppe_hip_system();
+}  // extern "C"
+
+// Builds a system (one segment) or a batch of snapshots (several segments, FMM machinery with a
+// per-segment ncrit). Positions and per-site data are in caller order.
+static cppe_hip_system* create_system(int n, const double* pos, const double* alpha6, const double* alpha_iso,
+                                      const int* excl_off, const int* excl_idx, const cppe_hip_options* opts,
+                                      const std::vector<int>& seg_off, const std::vector<int>& seg_ncrit) {
+  auto* H   = new cppe_hip_system();
   System& S = H->S;
   try {
     S.n    = n;
     S.opts = *opts;
     CPPE_HIP_CHECK(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
     hipStream_t st = S.stream;
+    setup_segments(S, seg_off, seg_ncrit);
     dbuf<double> aos;
     upload(aos, pos, 3 * (size_t)n, st);
     S.x.alloc(n);
@@ -204,15 +200,65 @@ int cppe_hip_system_create(cppe_hip_system** out, int n, const double* pos, cons
     S.v_d.alloc(n3 * 4);  // room for up to 10 source components (static fields)
     S.io_a.alloc(n3 * 4);
     S.io_b.alloc(n3);
-    S.red.alloc(4096);
-    S.scal.alloc(64);
-    S.scal.zero(st);
     CPPE_HIP_CHECK(hipStreamSynchronize(st));
   } catch (...) {
     delete H;
     throw;
   }
-  *out = H;
+  return H;
+}
+
+static int check_create_args(int n, const double* pos, const cppe_hip_options* opts) {
+  if (n <= 0 || !pos || !opts) return fail(CPPE_HIP_ERR_ARG, "invalid system arguments");
+  if (opts->ncrit < 2) return fail(CPPE_HIP_ERR_ARG, "tree_ncrit must be >= 2");
+  if (opts->summation == CPPE_HIP_SUM_FMM && (opts->order < 1 || opts->order > 8))
+    return fail(CPPE_HIP_ERR_UNSUPPORTED, "tree_expansion_order must be in 1..8");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(CPPE_HIP_ERR_DEVICE, "no HIP device available");
+  return CPPE_HIP_OK;
+}
+
+extern "C" {
+
+int cppe_hip_system_create(cppe_hip_system** out, int n, const double* pos, const double* alpha6,
+                           const double* alpha_iso, const int* excl_off, const int* excl_idx,
+                           const cppe_hip_options* opts) {
+  CPPE_API_BEGIN
+  if (out) *out = nullptr;
+  if (!out) return fail(CPPE_HIP_ERR_ARG, "invalid system arguments");
+  const int rc = check_create_args(n, pos, opts);
+  if (rc != CPPE_HIP_OK) return rc;
+  *out = create_system(n, pos, alpha6, alpha_iso, excl_off, excl_idx, opts, {0, n}, {opts->ncrit});
+  return CPPE_HIP_OK;
+  CPPE_API_END
+}
+
+int cppe_hip_batch_create(cppe_hip_system** out, int nseg, const int* seg_off, const int* seg_ncrit,
+                          const double* pos, const double* alpha6, const double* alpha_iso, const int* excl_off,
+                          const int* excl_idx, const cppe_hip_options* opts) {
+  CPPE_API_BEGIN
+  if (out) *out = nullptr;
+  if (!out || nseg <= 0 || !seg_off || !seg_ncrit) return fail(CPPE_HIP_ERR_ARG, "invalid batch arguments");
+  const int n = seg_off[nseg];
+  if (seg_off[0] != 0) return fail(CPPE_HIP_ERR_ARG, "segment offsets must start at 0");
+  std::vector<int> off(seg_off, seg_off + nseg + 1), nc(nseg);
+  for (int g = 0; g < nseg; ++g) {
+    const int sz = off[g + 1] - off[g];
+    if (sz <= 0) return fail(CPPE_HIP_ERR_ARG, "every snapshot needs at least one site");
+    if (seg_ncrit[g] != 0 && seg_ncrit[g] < 2) return fail(CPPE_HIP_ERR_ARG, "tree_ncrit must be >= 2");
+    nc[g] = seg_ncrit[g] == 0 ? sz + 1 : seg_ncrit[g];  // 0: whole snapshot in one leaf (direct sum)
+  }
+  if (excl_off && excl_idx)
+    for (int g = 0; g < nseg; ++g)
+      for (int i = off[g]; i < off[g + 1]; ++i)
+        for (int k = excl_off[i]; k < excl_off[i + 1]; ++k)
+          if (excl_idx[k] >= 0 && (excl_idx[k] < off[g] || excl_idx[k] >= off[g + 1]))
+            return fail(CPPE_HIP_ERR_ARG, "an exclusion joins two snapshots");
+  cppe_hip_options o = *opts;
+  o.summation        = CPPE_HIP_SUM_FMM;
+  const int rc       = check_create_args(n, pos, &o);
+  if (rc != CPPE_HIP_OK) return rc;
+  *out = create_system(n, pos, alpha6, alpha_iso, excl_off, excl_idx, &o, off, nc);
   return CPPE_HIP_OK;
   CPPE_API_END
 }
@@ -239,6 +285,7 @@ int cppe_hip_field(cppe_hip_system* sys, int m, const double* src, double* F) {
   CPPE_API_BEGIN
   if (!sys || m < 0 || m > 2 || !src || !F) return fail(CPPE_HIP_ERR_ARG, "invalid field arguments");
   System& S    = sys->S;
+  if (S.nseg > 1 && m != 1) return fail(CPPE_HIP_ERR_UNSUPPORTED, "order-0/2 fields of a snapshot batch: use multipole_fields");
   const int nc = (m + 1) * (m + 2) / 2;
   host_to_soa(S, src, nc, S.v_d.p);
   if (S.opts.summation == CPPE_HIP_SUM_FMM)
@@ -411,6 +458,7 @@ static void dense_b_device(System& S, const int* index, dbuf<double>& dout) {
 int cppe_hip_dense_b(cppe_hip_system* sys, const int* index, double* out) {
   CPPE_API_BEGIN
   if (!sys || !index || !out) return fail(CPPE_HIP_ERR_ARG, "invalid dense arguments");
+  if (sys->S.nseg > 1) return fail(CPPE_HIP_ERR_UNSUPPORTED, "dense B is not available for snapshot batches");
   dbuf<double> d;
   dense_b_device(sys->S, index, d);
   const size_t N = 3 * (size_t)sys->S.n;
@@ -422,6 +470,7 @@ int cppe_hip_dense_b(cppe_hip_system* sys, const int* index, double* out) {
 int cppe_hip_dense_b_inverse(cppe_hip_system* sys, const int* index, double* out) {
   CPPE_API_BEGIN
   if (!sys || !index || !out) return fail(CPPE_HIP_ERR_ARG, "invalid dense arguments");
+  if (sys->S.nseg > 1) return fail(CPPE_HIP_ERR_UNSUPPORTED, "dense B is not available for snapshot batches");
   if (sys->S.n > 4000) return fail(CPPE_HIP_ERR_UNSUPPORTED, "dense inverse limited to 4000 sites");
   dbuf<double> d, inv;
   dense_b_device(sys->S, index, d);
@@ -445,11 +494,9 @@ static int run_solve(System& S, int guess_mode, const double* guess_host, double
   if (guess_mode == 0)
     precondition(S, S.v_rhs.p, S.v_mu.p);
   else if (guess_mode == 2 || (guess_mode == 1 && !S.mu_valid))
-    CPPE_HIP_CHECK(hipMemsetAsync(S.v_mu.p, 0, sizeof(double) * n3, st));
-  std::vector<double> hn;
-  bool conv = false;
-  int it = device_cg(S, S.v_rhs.p, S.v_mu.p, thresh, maxiter,
-                     S.opts.precision == CPPE_HIP_PREC_MIXED, hn, conv);
+    fill_f64(S, S.v_mu.p, n3, 0.0);
+  CgResult& res = S.last;
+  device_cg(S, thresh, maxiter, S.opts.precision == CPPE_HIP_PREC_MIXED, res);
   CPPE_HIP_CHECK(hipEventRecord(e1, st));
   CPPE_HIP_CHECK(hipEventSynchronize(e1));
   float ms = 0.f;
@@ -457,11 +504,11 @@ static int run_solve(System& S, int guess_mode, const double* guess_host, double
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (device_ms) *device_ms = ms;
-  if (iters) *iters = it;
+  if (iters) *iters = S.nseg == 1 ? res.iters[0] : res.loop_iters;
   if (norms)
-    for (size_t k = 0; k < hn.size() && (int)k < maxiter; ++k) norms[k] = hn[k];
+    for (size_t k = 0; k < res.norms0.size() && (int)k < maxiter; ++k) norms[k] = res.norms0[k];
   S.mu_valid = true;
-  if (!conv) return fail(CPPE_HIP_ERR_NOCONV, "Failed to converge induced dipole moments.");
+  if (!res.all_converged) return fail(CPPE_HIP_ERR_NOCONV, "Failed to converge induced dipole moments.");
   return CPPE_HIP_OK;
 }
 
@@ -493,6 +540,53 @@ int cppe_hip_solve_resident(cppe_hip_system* sys, int guess_mode, double thresh,
   CPPE_API_BEGIN
   if (!sys || guess_mode < 0 || guess_mode > 2) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
   return run_solve(sys->S, guess_mode, nullptr, thresh, maxiter, iters, norms, device_ms);
+  CPPE_API_END
+}
+
+int cppe_hip_segment_results(cppe_hip_system* sys, int* iters, double* rnorm, int* converged) {
+  CPPE_API_BEGIN
+  if (!sys) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
+  const CgResult& r = sys->S.last;
+  if ((int)r.iters.size() != sys->S.nseg) return fail(CPPE_HIP_ERR_ARG, "no solve has run on this system");
+  for (int g = 0; g < sys->S.nseg; ++g) {
+    if (iters) iters[g] = r.iters[g];
+    if (rnorm) rnorm[g] = r.rnorm[g];
+    if (converged) converged[g] = r.converged[g];
+  }
+  return CPPE_HIP_OK;
+  CPPE_API_END
+}
+
+int cppe_hip_autotune(cppe_hip_system* sys, int reps, int* choice, double* ms) {
+  CPPE_API_BEGIN
+  if (!sys || reps <= 0 || !choice || !ms) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
+  System& S = sys->S;
+  if (S.opts.summation != CPPE_HIP_SUM_FMM) return fail(CPPE_HIP_ERR_UNSUPPORTED, "autotuning covers the tree path only");
+  if (!S.has_alpha) return fail(CPPE_HIP_ERR_ARG, "system has no polarizabilities");
+  fmm_autotune(S, reps, choice, ms);
+  return CPPE_HIP_OK;
+  CPPE_API_END
+}
+
+int cppe_hip_set_tune(cppe_hip_system* sys, const int* choice) {
+  CPPE_API_BEGIN
+  if (!sys || !choice) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
+  if (sys->S.opts.summation != CPPE_HIP_SUM_FMM) return fail(CPPE_HIP_ERR_UNSUPPORTED, "autotuning covers the tree path only");
+  for (int prec = 0; prec < 2; ++prec) {
+    const int v = choice[3 * prec], b = choice[3 * prec + 1], g = choice[3 * prec + 2];
+    if ((v != 0 && v != 1) || (b != 64 && b != 128 && b != 256) || (g != 16 && g != 32 && g != 64))
+      return fail(CPPE_HIP_ERR_ARG, "invalid launch shape");
+  }
+  fmm_set_tune(sys->S, choice);
+  return CPPE_HIP_OK;
+  CPPE_API_END
+}
+
+int cppe_hip_set_graphs(cppe_hip_system* sys, int enable) {
+  CPPE_API_BEGIN
+  if (!sys) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
+  sys->S.use_graphs = enable != 0;
+  return CPPE_HIP_OK;
   CPPE_API_END
 }
 

@@ -79,6 +79,100 @@ void fmm_static(System& S, const double* q, const double* mu, const double* th, 
 #undef CALL
 }
 
+// Allocates every buffer an FMM B apply touches (both precisions), so that applies can be
+// captured into a graph.
+void fmm_reserve(System& S) {
+  FmmState& F     = state(S);
+  const size_t nc = S.tree.ncells;
+  const int p     = S.opts.order;
+  ensure_buf(F.M64, nc * fmm_msize(0, p));
+  ensure_buf(F.L64, nc * fmm_lsize(0, p));
+  ensure_buf(F.M32, nc * fmm_msize(0, p));
+  ensure_buf(F.L32, nc * fmm_lsize(0, p));
+  ensure_buf(F.mu4, S.n);
+  ensure_buf(F.mu4f, S.n);
+  ensure_pos4(S);
+  ensure_posf4(S);
+  ensure_near_items(S);
+  CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
+}
+
+// Autotuner of the B-apply launch shapes: for each precision, every near-field candidate
+// (variant x block size) and every M2L group width is timed on this system's own tree with HIP
+// events around the kernel (mean of reps applies after one warm-up), and the fastest is kept.
+// ms receives the timings: per precision (FP64 then FP32) kNearCand near values then kM2lCand
+// M2L values; choice receives (near variant, near block, m2l group) per precision.
+static const int kNearCand[][2] = {{0, 64}, {0, 128}, {0, 256}, {1, 64}, {1, 128}, {1, 256}};
+static const int kM2lCand[]     = {16, 32, 64};
+constexpr int kNNear = 6, kNM2l = 3;
+
+void fmm_autotune(System& S, int reps, int* choice, double* ms) {
+  fmm_reserve(S);
+  FmmState& F    = *S.fmm;
+  hipStream_t st = S.stream;
+  hipEvent_t ev[6];
+  for (auto& e : ev) CPPE_HIP_CHECK(hipEventCreate(&e));
+  auto timed = [&](bool fp32, int phase) {  // phase 0: near (events 3-4), 1: m2l (events 1-2)
+    apply_b_fmm(S, S.v_d.p, S.v_t.p, fp32);  // warm-up
+    double acc = 0.0;
+    for (int r = 0; r < reps; ++r) {
+      S.prof = ev;
+      apply_b_fmm(S, S.v_d.p, S.v_t.p, fp32);
+      S.prof = nullptr;
+      CPPE_HIP_CHECK(hipEventSynchronize(ev[4]));
+      float t = 0.f;
+      if (phase == 0)
+        CPPE_HIP_CHECK(hipEventElapsedTime(&t, ev[3], ev[4]));
+      else
+        CPPE_HIP_CHECK(hipEventElapsedTime(&t, ev[1], ev[2]));
+      acc += t;
+    }
+    return acc / reps;
+  };
+  try {
+    fill_f64(S, S.v_d.p, 3 * (size_t)S.n, 0.01);
+    for (int prec = 0; prec < 2; ++prec) {
+      const bool fp32 = prec == 1;
+      double* m       = ms + prec * (kNNear + kNM2l);
+      int best        = 0;
+      for (int c = 0; c < kNNear; ++c) {
+        F.tune.near_variant[prec] = kNearCand[c][0];
+        F.tune.near_block[prec]   = kNearCand[c][1];
+        m[c]                      = timed(fp32, 0);
+        if (m[c] < m[best]) best = c;
+      }
+      F.tune.near_variant[prec] = kNearCand[best][0];
+      F.tune.near_block[prec]   = kNearCand[best][1];
+      int bm                    = 0;
+      for (int c = 0; c < kNM2l; ++c) {
+        F.tune.m2l_group[prec] = kM2lCand[c];
+        m[kNNear + c]          = timed(fp32, 1);
+        if (m[kNNear + c] < m[kNNear + bm]) bm = c;
+      }
+      F.tune.m2l_group[prec] = kM2lCand[bm];
+      choice[3 * prec]       = F.tune.near_variant[prec];
+      choice[3 * prec + 1]   = F.tune.near_block[prec];
+      choice[3 * prec + 2]   = F.tune.m2l_group[prec];
+    }
+  } catch (...) {
+    S.prof = nullptr;
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    throw;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  cg_graphs_release(S);  // captured graphs hold the previous launch shapes
+}
+
+void fmm_set_tune(System& S, const int* choice) {
+  FmmState& F = state(S);
+  for (int prec = 0; prec < 2; ++prec) {
+    F.tune.near_variant[prec] = choice[3 * prec];
+    F.tune.near_block[prec]   = choice[3 * prec + 1];
+    F.tune.m2l_group[prec]    = choice[3 * prec + 2];
+  }
+  cg_graphs_release(S);
+}
+
 void fmm_destroy(System& S) {
   delete S.fmm;
   S.fmm = nullptr;

@@ -27,8 +27,20 @@ namespace cppe_dev {
 enum FmmKind { KIND_M1 = 0, KIND_MC = 1 };
 enum P2PMode { MODE_BAPPLY = 0, MODE_FIELD1 = 1, MODE_STATIC = 2 };
 
+// Launch shapes of the B-apply kernels, chosen by the autotuner (cppe_hip_autotune); index 0 is
+// the FP64 apply, 1 the FP32 apply.
+struct FmmTune {
+  int near_variant[2] = {0, 0};  // 0: one thread per target site, 1: one wavefront per leaf chunk
+  int near_block[2]   = {256, 256};
+  int m2l_group[2]    = {64, 64};  // lanes per target cell in M2L (16, 32 or 64)
+};
+
 struct FmmState {
   int order = 0;
+  FmmTune tune;
+  dbuf<int2> near_items;  // (leaf, first target) of each 64-target chunk of every leaf
+  int n_items = 0;
+  dbuf<float4> posf4;     // FP32 source records: offset from the site's leaf centre, alpha_iso^(-1/6)
   dbuf<double> M64, L64;  // dipole kind
   dbuf<float> M32, L32;
   dbuf<double> Mc, Lc;    // combined kind (static fields, m = 0 / 2)
@@ -110,6 +122,7 @@ __device__ __forceinline__ void load_source(const double* __restrict__ src, int 
 
 template <int KIND, int P, typename R>
 __global__ __launch_bounds__(64) void k_fmm_p2m(int nleaves, const int* __restrict__ leaf_ids,
+                                                const int* __restrict__ level,
                                                 const int* __restrict__ start, const int* __restrict__ count,
                                                 const double* __restrict__ cx, const double* __restrict__ cy,
                                                 const double* __restrict__ cz, const double* __restrict__ x,
@@ -119,6 +132,7 @@ __global__ __launch_bounds__(64) void k_fmm_p2m(int nleaves, const int* __restri
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nleaves) return;
   const int c = leaf_ids[t];
+  if (level[c] == 0) return;  // a root's expansion never enters an M2L pair
   R m[O::MS];
 #pragma unroll
   for (int k = 0; k < O::MS; ++k) m[k] = R(0);
@@ -160,36 +174,52 @@ __device__ __forceinline__ R wave_sum(R v) {
   return v;
 }
 
-template <int KIND, int P, typename R>
-__global__ __launch_bounds__(64) void k_fmm_m2l(const int* __restrict__ off, const int* __restrict__ srcs,
+// G lanes per target cell (G = 16, 32, 64; 64/G cells per wavefront). Lanes of a group take the
+// sources of the target-grouped list in turn; the G partial expansions are summed by xor
+// shuffles inside the group and the owner lane of each coefficient stores it.
+template <int G, typename R>
+__device__ __forceinline__ R group_sum(R v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+template <int KIND, int P, typename R, int G>
+__global__ __launch_bounds__(64) void k_fmm_m2l(int ncells, const int* __restrict__ off, const int* __restrict__ srcs,
                                                 const double* __restrict__ cx, const double* __restrict__ cy,
                                                 const double* __restrict__ cz, const R* __restrict__ M,
                                                 R* __restrict__ L) {
-  using O        = Ops<KIND, P, R>;
-  const int A    = blockIdx.x;
-  const int lane = threadIdx.x;
+  using O         = Ops<KIND, P, R>;
+  const int lane  = threadIdx.x;
+  const int gl    = lane % G;
+  const int A     = blockIdx.x * (64 / G) + lane / G;
+  const bool valid = A < ncells;
   R l[O::LS];
 #pragma unroll
   for (int k = 0; k < O::LS; ++k) l[k] = R(0);
-  const int o0 = off[A], o1 = off[A + 1];
-  const double ax = cx[A], ay = cy[A], az = cz[A];
-  for (int k = o0 + lane; k < o1; k += 64) {
-    const int B = srcs[k];
-    O::m2l((R)(ax - cx[B]), (R)(ay - cy[B]), (R)(az - cz[B]), &M[(size_t)B * O::MS], l);
+  if (valid) {
+    const int o0 = off[A], o1 = off[A + 1];
+    const double ax = cx[A], ay = cy[A], az = cz[A];
+    for (int k = o0 + gl; k < o1; k += G) {
+      const int B = srcs[k];
+      O::m2l((R)(ax - cx[B]), (R)(ay - cy[B]), (R)(az - cz[B]), &M[(size_t)B * O::MS], l);
+    }
   }
-  constexpr int NCH = (O::LS + 63) / 64;
+  constexpr int NCH = (O::LS + G - 1) / G;
   R mine[NCH];
 #pragma unroll
   for (int k = 0; k < NCH; ++k) mine[k] = R(0);
 #pragma unroll
   for (int k = 0; k < O::LS; ++k) {
-    const R v = wave_sum(l[k]);
-    if (lane == (k & 63)) mine[k >> 6] = v;
+    const R v = group_sum<G>(l[k]);
+    if (gl == (k % G)) mine[k / G] = v;
   }
+  if (valid) {
 #pragma unroll
-  for (int q = 0; q < NCH; ++q) {
-    const int k = q * 64 + lane;
-    if (k < O::LS) L[(size_t)A * O::LS + k] = mine[q];
+    for (int q = 0; q < NCH; ++q) {
+      const int k = q * G + gl;
+      if (k < O::LS) L[(size_t)A * O::LS + k] = mine[q];
+    }
   }
 }
 
@@ -472,6 +502,98 @@ __global__ __launch_bounds__(256) void k_fmm_near(NearArgs a, const MU4* __restr
   a.out[2 * (size_t)n + i] = Fz;
 }
 
+// Near field, one wavefront per 64-target chunk of a leaf (work item = (leaf, first target)).
+// The source walk is uniform across the wavefront -- every lane visits the same source site at
+// the same time -- so the source records are read once per wavefront (scalar loads) and the
+// lanes differ only in their target. FP32 form: per source leaf, each lane forms its target's
+// offset from that leaf's centre in FP64 and rounds it; sources carry FP32 offsets from their
+// own leaf centre (posf4), so FP32 only ever sees offsets of a few leaf widths.
+template <int P, typename R, bool DAMP, int MODE, typename MU4, typename POSR>
+__global__ __launch_bounds__(256) void k_fmm_near_lw(NearArgs a, const int2* __restrict__ items, int nitems,
+                                                     const POSR* __restrict__ posr, const MU4* __restrict__ mu4,
+                                                     const R* __restrict__ L) {
+  using O        = Ops<KIND_M1, P, R>;
+  const int w    = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  if (w >= nitems) return;  // uniform per wavefront
+  const int2 item  = items[w];
+  const int A      = __builtin_amdgcn_readfirstlane(item.x);
+  const int t      = __builtin_amdgcn_readfirstlane(item.y) + lane;
+  const bool act   = t < a.count[A];
+  const int i      = a.start[A] + (act ? t : 0);  // idle lanes shadow the first target, store nothing
+  const double4 pi = a.pos4[i];
+  const R di       = DAMP ? (R)(a.damping * pi.w) : R(0);
+  int lo = i, hi = i;
+  if (a.eoff)
+    for (int k = a.eoff[i]; k < a.eoff[i + 1]; ++k) {
+      const int e = a.eidx[k];
+      if (e >= 0) {
+        lo = min(lo, e);
+        hi = max(hi, e);
+      }
+    }
+  R fx = R(0), fy = R(0), fz = R(0);
+  const int k1 = a.p2p_off[A + 1];
+  for (int k = a.p2p_off[A]; k < k1; ++k) {
+    const int B  = a.p2p_src[k];
+    const int j0 = a.start[B], j1 = j0 + a.count[B];
+    R tx, ty, tz;
+    if constexpr (sizeof(R) == 8) {
+      tx = pi.x;
+      ty = pi.y;
+      tz = pi.z;
+    } else {
+      tx = (R)(pi.x - a.cx[B]);
+      ty = (R)(pi.y - a.cy[B]);
+      tz = (R)(pi.z - a.cz[B]);
+    }
+    for (int j = j0; j < j1; ++j) {
+      const POSR pj = posr[j];
+      const MU4 mj  = mu4[j];
+      if (j >= lo && j <= hi && excluded(i, j, a.eoff, a.eidx)) continue;
+      if constexpr (sizeof(R) == 8) {
+        double hx = 0.0, hy = 0.0, hz = 0.0;
+        pair_m1_f64<DAMP>(tx - pj.x, ty - pj.y, tz - pj.z, (double)mj.x, (double)mj.y, (double)mj.z,
+                          (double)di * pj.w, hx, hy, hz);
+        fx += hx;
+        fy += hy;
+        fz += hz;
+      } else {
+        pair_m1_f32<DAMP>(tx - pj.x, ty - pj.y, tz - pj.z, mj.x, mj.y, mj.z, di * pj.w, fx, fy, fz);
+      }
+    }
+  }
+  R lx = R(0), ly = R(0), lz = R(0);
+  O::l2p((R)(pi.x - a.cx[A]), (R)(pi.y - a.cy[A]), (R)(pi.z - a.cz[A]), &L[(size_t)A * O::LS], lx, ly, lz);
+  double Fx = (double)fx + (double)lx, Fy = (double)fy + (double)ly, Fz = (double)fz + (double)lz;
+  if (!act) return;
+  const int n = a.n;
+  if (MODE == MODE_BAPPLY) {
+    const double m0 = a.src[i], m1 = a.src[n + i], m2 = a.src[2 * (size_t)n + i];
+    const double* ai = a.ainv;
+    const double a0 = ai[i], a1 = ai[n + i], a2 = ai[2 * (size_t)n + i], a3 = ai[3 * (size_t)n + i],
+                 a4 = ai[4 * (size_t)n + i], a5 = ai[5 * (size_t)n + i];
+    Fx += a0 * m0 + a1 * m1 + a2 * m2;
+    Fy += a1 * m0 + a3 * m1 + a4 * m2;
+    Fz += a2 * m0 + a4 * m1 + a5 * m2;
+  }
+  a.out[i]                 = Fx;
+  a.out[n + i]             = Fy;
+  a.out[2 * (size_t)n + i] = Fz;
+}
+
+// FP32 source records: offset of each site from its leaf centre, and alpha_iso^(-1/6).
+static __global__ void k_pack_posf4(int n, const double* __restrict__ x, const double* __restrict__ y,
+                                    const double* __restrict__ z, const double* __restrict__ s,
+                                    const int* __restrict__ leaf_of, const double* __restrict__ cx,
+                                    const double* __restrict__ cy, const double* __restrict__ cz,
+                                    float4* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = leaf_of[i];
+  out[i] = make_float4((float)(x[i] - cx[c]), (float)(y[i] - cy[c]), (float)(z[i] - cz[c]), s ? (float)s[i] : 0.f);
+}
+
 static __global__ void k_pack_pos4(int n, const double* __restrict__ x, const double* __restrict__ y,
                             const double* __restrict__ z, const double* __restrict__ s, double4* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -499,15 +621,30 @@ void fmm_far_pass(System& S, const double* src, R* M, R* L) {
   const auto& lo = T.level_offset;
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[0], st));
   hipLaunchKernelGGL((k_fmm_p2m<KIND, P, R>), dim3(ceil_div(T.nleaves, 64)), dim3(64), 0, st, T.nleaves,
-                     T.leaf_ids.p, T.start.p, T.count.p, T.cx.p, T.cy.p, T.cz.p, S.x.p, S.y.p, S.z.p, src, S.n, M);
-  for (int lev = T.max_level - 1; lev >= 0; --lev) {
+                     T.leaf_ids.p, T.level.p, T.start.p, T.count.p, T.cx.p, T.cy.p, T.cz.p, S.x.p, S.y.p, S.z.p, src, S.n, M);
+  // roots (level 0) are skipped: their expansions never enter an M2L pair
+  for (int lev = T.max_level - 1; lev >= 1; --lev) {
     const int c0 = lo[lev], c1 = lo[lev + 1];
     hipLaunchKernelGGL((k_fmm_m2m<KIND, P, R>), dim3(ceil_div(c1 - c0, 64)), dim3(64), 0, st, c0, c1,
                        T.nchild.p, T.child_first.p, T.cx.p, T.cy.p, T.cz.p, M);
   }
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[1], st));
-  hipLaunchKernelGGL((k_fmm_m2l<KIND, P, R>), dim3(T.ncells), dim3(64), 0, st, T.m2l_off.p, T.m2l_src.p, T.cx.p,
-                     T.cy.p, T.cz.p, M, L);
+  const int grp = S.fmm ? S.fmm->tune.m2l_group[sizeof(R) == 4] : 64;
+  if constexpr (KIND == KIND_M1) {
+    if (grp == 16)
+      hipLaunchKernelGGL((k_fmm_m2l<KIND, P, R, 16>), dim3(ceil_div(T.ncells, 4)), dim3(64), 0, st, T.ncells,
+                         T.m2l_off.p, T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, M, L);
+    else if (grp == 32)
+      hipLaunchKernelGGL((k_fmm_m2l<KIND, P, R, 32>), dim3(ceil_div(T.ncells, 2)), dim3(64), 0, st, T.ncells,
+                         T.m2l_off.p, T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, M, L);
+    else
+      hipLaunchKernelGGL((k_fmm_m2l<KIND, P, R, 64>), dim3(T.ncells), dim3(64), 0, st, T.ncells, T.m2l_off.p,
+                         T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, M, L);
+  } else {
+    (void)grp;
+    hipLaunchKernelGGL((k_fmm_m2l<KIND, P, R, 64>), dim3(T.ncells), dim3(64), 0, st, T.ncells, T.m2l_off.p,
+                       T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, M, L);
+  }
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[2], st));
   for (int lev = 1; lev <= T.max_level; ++lev) {
     const int c0 = lo[lev], c1 = lo[lev + 1];
@@ -568,6 +705,58 @@ inline void ensure_pos4(System& S) {
                      F.pos4.p);
 }
 
+// Work items of the leaf-wave near kernel: (leaf, first target) per 64-target chunk of a leaf.
+inline void ensure_near_items(System& S) {
+  FmmState& F     = *S.fmm;
+  const Octree& T = S.tree;
+  if (F.near_items.p && F.n_items > 0) return;
+  std::vector<int> ids(T.nleaves), cnt(T.ncells);
+  CPPE_HIP_CHECK(hipMemcpyAsync(ids.data(), T.leaf_ids.p, sizeof(int) * T.nleaves, hipMemcpyDeviceToHost, S.stream));
+  CPPE_HIP_CHECK(hipMemcpyAsync(cnt.data(), T.count.p, sizeof(int) * T.ncells, hipMemcpyDeviceToHost, S.stream));
+  CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
+  std::vector<int2> items;
+  items.reserve(T.nleaves);
+  for (int c : ids)
+    for (int t0 = 0; t0 < cnt[c]; t0 += 64) items.push_back(make_int2(c, t0));
+  F.near_items.alloc(items.size());
+  CPPE_HIP_CHECK(hipMemcpyAsync(F.near_items.p, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice,
+                                S.stream));
+  CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
+  F.n_items = (int)items.size();
+}
+
+inline void ensure_posf4(System& S) {
+  FmmState& F     = *S.fmm;
+  const Octree& T = S.tree;
+  if (F.posf4.p && F.posf4.n >= (size_t)S.n) return;
+  F.posf4.alloc(S.n);
+  hipLaunchKernelGGL(k_pack_posf4, dim3(ceil_div(S.n, 256)), dim3(256), 0, S.stream, S.n, S.x.p, S.y.p, S.z.p, S.s.p,
+                     T.leaf_of.p, T.cx.p, T.cy.p, T.cz.p, F.posf4.p);
+}
+
+template <int P, typename R, int MODE, bool DAMP, typename MU4>
+void launch_near(System& S, const NearArgs& a, const MU4* mu4, const R* L) {
+  FmmState& F     = *S.fmm;
+  const int prec  = sizeof(R) == 4;
+  const int block = F.tune.near_block[prec];
+  hipStream_t st  = S.stream;
+  if (F.tune.near_variant[prec] == 1) {
+    ensure_near_items(S);
+    const int waves = block / 64;
+    const dim3 g(ceil_div(F.n_items, waves)), b(block);
+    if constexpr (sizeof(R) == 8)
+      hipLaunchKernelGGL((k_fmm_near_lw<P, R, DAMP, MODE, MU4, double4>), g, b, 0, st, a, F.near_items.p, F.n_items,
+                         F.pos4.p, mu4, L);
+    else {
+      ensure_posf4(S);
+      hipLaunchKernelGGL((k_fmm_near_lw<P, R, DAMP, MODE, MU4, float4>), g, b, 0, st, a, F.near_items.p, F.n_items,
+                         F.posf4.p, mu4, L);
+    }
+  } else {
+    hipLaunchKernelGGL((k_fmm_near<P, R, DAMP, MODE, MU4>), dim3(ceil_div(S.n, block)), dim3(block), 0, st, a, mu4, L);
+  }
+}
+
 template <int P, typename R, int MODE>
 void fmm_near_pass(System& S, const double* mu, double damping, const R* L, double* out) {
   FmmState& F     = *S.fmm;
@@ -582,17 +771,17 @@ void fmm_near_pass(System& S, const double* mu, double damping, const R* L, doub
     hipLaunchKernelGGL(k_pack_mu4<double4>, g, b, 0, st, S.n, mu, F.mu4.p);
     if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[3], st));
     if (damping > 0.0)
-      hipLaunchKernelGGL((k_fmm_near<P, R, true, MODE, double4>), g, b, 0, st, a, F.mu4.p, L);
+      launch_near<P, R, MODE, true>(S, a, F.mu4.p, L);
     else
-      hipLaunchKernelGGL((k_fmm_near<P, R, false, MODE, double4>), g, b, 0, st, a, F.mu4.p, L);
+      launch_near<P, R, MODE, false>(S, a, F.mu4.p, L);
   } else {
     ensure_buf(F.mu4f, S.n);
     hipLaunchKernelGGL(k_pack_mu4<float4>, g, b, 0, st, S.n, mu, F.mu4f.p);
     if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[3], st));
     if (damping > 0.0)
-      hipLaunchKernelGGL((k_fmm_near<P, R, true, MODE, float4>), g, b, 0, st, a, F.mu4f.p, L);
+      launch_near<P, R, MODE, true>(S, a, F.mu4f.p, L);
     else
-      hipLaunchKernelGGL((k_fmm_near<P, R, false, MODE, float4>), g, b, 0, st, a, F.mu4f.p, L);
+      launch_near<P, R, MODE, false>(S, a, F.mu4f.p, L);
   }
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[4], st));
 }

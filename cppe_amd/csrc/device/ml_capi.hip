@@ -177,6 +177,8 @@ int cppe_hip_mlp_get(cppe_hip_mlp* m, float* params, float* mean, float* inv_std
 
 int cppe_hip_ml_features(cppe_hip_system* sys, float* X, double* V) {
   ML_BEGIN
+  if (sys && sys->S.nseg > 1)
+    return capi_fail(CPPE_HIP_ERR_UNSUPPORTED, "descriptors of a snapshot batch are not supported");
   if (!sys || !X || !V) return capi_fail(CPPE_HIP_ERR_ARG, "invalid arguments");
   System& S = sys->S;
   if (!S.has_alpha) return capi_fail(CPPE_HIP_ERR_ARG, "system has no polarizabilities");
@@ -258,6 +260,8 @@ int cppe_hip_mlp_reserve(cppe_hip_mlp* m, long long ns) {
 
 int cppe_hip_mlp_collect(cppe_hip_mlp* m, cppe_hip_system* sys) {
   ML_BEGIN
+  if (sys && sys->S.nseg > 1)
+    return capi_fail(CPPE_HIP_ERR_UNSUPPORTED, "descriptors of a snapshot batch are not supported");
   if (!m || !sys) return capi_fail(CPPE_HIP_ERR_ARG, "invalid arguments");
   Mlp& M    = m->M;
   System& S = sys->S;
@@ -343,6 +347,8 @@ int cppe_hip_mlp_train(cppe_hip_mlp* m, int epochs, int batch, float lr, unsigne
 
 int cppe_hip_ml_guess(cppe_hip_system* sys, cppe_hip_mlp* m, double* device_ms) {
   ML_BEGIN
+  if (sys && sys->S.nseg > 1)
+    return capi_fail(CPPE_HIP_ERR_UNSUPPORTED, "descriptors of a snapshot batch are not supported");
   if (!sys || !m) return capi_fail(CPPE_HIP_ERR_ARG, "invalid arguments");
   System& S = sys->S;
   if (!S.has_alpha) return capi_fail(CPPE_HIP_ERR_ARG, "system has no polarizabilities");

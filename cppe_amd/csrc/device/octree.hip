@@ -20,6 +20,13 @@
 //      rule; lists are then sorted by (target, source) into per-target CSR (owner computes).
 // Cell identity (level, centre) and list membership are bit-identical to the reference; only
 // the cell numbering differs (level order here, creation order there).
+//
+// Forests: a system may hold several independent snapshots ("segments", contiguous site ranges
+// with their own ncrit). Each gets its own root (steps 1-2 per segment), the sort is segmented,
+// cell ranges never cross a segment boundary, and the traversal starts from every (root, root)
+// pair, so no list entry joins two snapshots. A segment whose ncrit exceeds its size is a single
+// leaf whose near list is itself: its far field is empty and its near field is the exact direct
+// sum. One segment with the system's ncrit is the reference's tree.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -32,14 +39,17 @@ namespace cppe_dev {
 constexpr int kMaxLevel = 21;
 
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_root(int n, const double* __restrict__ x,
+// One block per snapshot (segment) of sites [off[g], off[g+1]).
+__global__ __launch_bounds__(256) void k_root(const int* __restrict__ seg_off, const double* __restrict__ x,
                                               const double* __restrict__ y,
                                               const double* __restrict__ z, double* __restrict__ out) {
   constexpr int CH = 2048;
   __shared__ double avg[3];
   __shared__ double red[3][256];
   __shared__ double buf[3][CH];
-  const int t = threadIdx.x;
+  const int t  = threadIdx.x;
+  const int g  = blockIdx.x;
+  const int i0 = seg_off[g], n = seg_off[g + 1] - i0;
   // sums in index order (one thread per axis) over LDS-staged chunks: the rounding of each
   // addition is that of the reference's sequential loop
   double s = 0.0;
@@ -47,9 +57,9 @@ __global__ __launch_bounds__(256) void k_root(int n, const double* __restrict__ 
     const int m = min(CH, n - base);
     __syncthreads();
     for (int k = t; k < m; k += blockDim.x) {
-      buf[0][k] = x[base + k];
-      buf[1][k] = y[base + k];
-      buf[2][k] = z[base + k];
+      buf[0][k] = x[i0 + base + k];
+      buf[1][k] = y[i0 + base + k];
+      buf[2][k] = z[i0 + base + k];
     }
     __syncthreads();
     if (t < 3)
@@ -58,7 +68,8 @@ __global__ __launch_bounds__(256) void k_root(int n, const double* __restrict__ 
   if (t < 3) avg[t] = s / (double)n;
   __syncthreads();
   double m0 = 0.0, m1 = 0.0, m2 = 0.0;
-  for (int i = t; i < n; i += blockDim.x) {
+  for (int k = t; k < n; k += blockDim.x) {
+    const int i = i0 + k;
     double dx = fabs(x[i] - avg[0]);
     double dy = fabs(y[i] - avg[1]);
     double dz = fabs(z[i] - avg[2]);
@@ -78,10 +89,10 @@ __global__ __launch_bounds__(256) void k_root(int n, const double* __restrict__ 
   if (t == 0) {
     double xmax = red[0][0], ymax = red[1][0], zmax = red[2][0];
     double r = (xmax > ymax ? (xmax > zmax ? xmax : zmax) : (ymax > zmax ? ymax : zmax)) * 1.001;
-    out[0] = avg[0];
-    out[1] = avg[1];
-    out[2] = avg[2];
-    out[3] = r;
+    out[4 * g + 0] = avg[0];
+    out[4 * g + 1] = avg[1];
+    out[4 * g + 2] = avg[2];
+    out[4 * g + 3] = r;
   }
 }
 
@@ -98,11 +109,12 @@ __device__ __forceinline__ void child_centre(int oct, double& cx, double& cy, do
 __global__ __launch_bounds__(256) void k_keys(int n, const double* __restrict__ x,
                                               const double* __restrict__ y,
                                               const double* __restrict__ z,
-                                              const double* __restrict__ root,
+                                              const double* __restrict__ root, const int* __restrict__ seg_of,
                                               uint64_t* __restrict__ keys, int* __restrict__ idx) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double cx = root[0], cy = root[1], cz = root[2], r = root[3];
+  const double* rt = root + 4 * seg_of[i];
+  double cx = rt[0], cy = rt[1], cz = rt[2], r = rt[3];
   const double px = x[i], py = y[i], pz = z[i];
   uint64_t key = 0;
   for (int l = 0; l < kMaxLevel; ++l) {
@@ -139,11 +151,16 @@ __device__ __forceinline__ int cell_end(const uint64_t* keys, int i, int hi, int
 }
 
 // Leaf level of each sorted site: the first level whose cell holds < ncrit sites.
-__global__ __launch_bounds__(256) void k_leaf_level(int n, int ncrit, const uint64_t* __restrict__ keys,
+__global__ __launch_bounds__(256) void k_leaf_level(int n, const int* __restrict__ seg_of,
+                                                    const int* __restrict__ seg_off,
+                                                    const int* __restrict__ seg_ncrit,
+                                                    const uint64_t* __restrict__ keys,
                                                     int* __restrict__ leaf_level, int* __restrict__ err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  int lo = 0, hi = n;
+  const int g     = seg_of[i];
+  const int ncrit = seg_ncrit[g];
+  int lo = seg_off[g], hi = seg_off[g + 1];
   for (int L = 0; L <= kMaxLevel; ++L) {
     if (L > 0) {
       uint64_t p = prefix_of(keys[i], L);
@@ -161,16 +178,19 @@ __global__ __launch_bounds__(256) void k_leaf_level(int n, int ncrit, const uint
 }
 
 __global__ __launch_bounds__(256) void k_heads(int n, int L, const uint64_t* __restrict__ keys,
+                                               const int* __restrict__ seg_of, const int* __restrict__ seg_off,
                                                const int* __restrict__ leaf_level, int* __restrict__ flag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int f = 0;
-  if (leaf_level[i] >= L) f = (i == 0) || (prefix_of(keys[i], L) != prefix_of(keys[i - 1], L));
+  if (leaf_level[i] >= L)
+    f = (i == seg_off[seg_of[i]]) || (prefix_of(keys[i], L) != prefix_of(keys[i - 1], L));
   flag[i] = f;
 }
 
 __global__ __launch_bounds__(256) void k_fill_cells(
       int n, int L, int cell0, int prev0, int prev_n, const uint64_t* __restrict__ keys,
+      const int* __restrict__ seg_of, const int* __restrict__ seg_off,
       const int* __restrict__ leaf_level, const int* __restrict__ flag, const int* __restrict__ rank,
       const double* __restrict__ root, double* cx, double* cy, double* cz, double* cr, double* crmax,
       int* start, int* count, int* parent, int* level) {
@@ -178,8 +198,10 @@ __global__ __launch_bounds__(256) void k_fill_cells(
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || !flag[i]) return;
   const int c    = cell0 + rank[i];
+  const int g    = seg_of[i];
   uint64_t key   = keys[i];
-  double x = root[0], y = root[1], z = root[2], r = root[3];
+  const double* rt = root + 4 * g;
+  double x = rt[0], y = rt[1], z = rt[2], r = rt[3];
   for (int l = 1; l <= L; ++l) {
     int oct = (int)((key >> (3 * (kMaxLevel - l))) & 7ull);
     child_centre(oct, x, y, z, r);
@@ -190,10 +212,10 @@ __global__ __launch_bounds__(256) void k_fill_cells(
   cr[c]    = r;
   crmax[c] = sqrt(3 * r * r);
   start[c] = i;
-  count[c] = cell_end(keys, i, n, L) - i;
+  count[c] = cell_end(keys, i, seg_off[g + 1], L) - i;
   level[c] = L;
   if (L == 0) {
-    parent[c] = 0;
+    parent[c] = c;  // a root (one per snapshot) is its own parent
   } else {
     // parent = last cell of the previous level whose start <= i
     int lo = 0, hi = prev_n;
@@ -214,7 +236,7 @@ __global__ __launch_bounds__(256) void k_link(int ncells, const int* __restrict_
                                               const double* __restrict__ cz, int* child_first, int* nchild,
                                               int* octmask) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c <= 0 || c >= ncells) return;
+  if (c >= ncells || level[c] == 0) return;
   const int p = parent[c];
   atomicMin(&child_first[p], c);
   atomicAdd(&nchild[p], 1);
@@ -272,11 +294,11 @@ static void exclusive_scan(const T* in, T* out, int n, hipStream_t s, dbuf<char>
 void build_octree(System& S) {
   const int n       = S.n;
   hipStream_t st    = S.stream;
-  const int ncrit   = S.opts.ncrit;
+  const int nseg    = S.nseg;
   Octree& T         = S.tree;
   dbuf<double> root;
-  root.alloc(4);
-  hipLaunchKernelGGL(k_root, dim3(1), dim3(256), 0, st, n, S.x.p, S.y.p, S.z.p, root.p);
+  root.alloc(4 * (size_t)nseg);
+  hipLaunchKernelGGL(k_root, dim3(nseg), dim3(256), 0, st, S.seg_off.p, S.x.p, S.y.p, S.z.p, root.p);
   dbuf<uint64_t> keys_in, keys;
   dbuf<int> idx_in, idx;
   keys_in.alloc(n);
@@ -284,14 +306,25 @@ void build_octree(System& S) {
   idx_in.alloc(n);
   idx.alloc(n);
   hipLaunchKernelGGL(k_keys, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, S.x.p, S.y.p, S.z.p, root.p,
-                     keys_in.p, idx_in.p);
+                     S.seg_of.p, keys_in.p, idx_in.p);
   dbuf<char> tmp;
   size_t bytes = 0;
-  CPPE_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys_in.p, keys.p, idx_in.p, idx.p, n,
-                                                    0, 3 * kMaxLevel, st));
-  tmp.alloc(bytes + 16);
-  CPPE_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, keys_in.p, keys.p, idx_in.p, idx.p, n, 0,
-                                                    3 * kMaxLevel, st));
+  if (nseg == 1) {
+    CPPE_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys_in.p, keys.p, idx_in.p, idx.p, n,
+                                                      0, 3 * kMaxLevel, st));
+    tmp.alloc(bytes + 16);
+    CPPE_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, keys_in.p, keys.p, idx_in.p, idx.p, n, 0,
+                                                      3 * kMaxLevel, st));
+  } else {
+    // snapshots stay where they are; each is sorted by key on its own (stable)
+    CPPE_HIP_CHECK(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, bytes, keys_in.p, keys.p, idx_in.p,
+                                                               idx.p, n, nseg, S.seg_off.p, S.seg_off.p + 1, 0,
+                                                               3 * kMaxLevel, st));
+    tmp.alloc(bytes + 16);
+    CPPE_HIP_CHECK(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp.p, bytes, keys_in.p, keys.p, idx_in.p, idx.p,
+                                                               n, nseg, S.seg_off.p, S.seg_off.p + 1, 0,
+                                                               3 * kMaxLevel, st));
+  }
   // sorted positions, permutation
   dbuf<double> xs, ys, zs;
   xs.alloc(n);
@@ -310,8 +343,8 @@ void build_octree(System& S) {
   leaf_level.alloc(n);
   err.alloc(1);
   err.zero(st);
-  hipLaunchKernelGGL(k_leaf_level, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, ncrit, keys.p,
-                     leaf_level.p, err.p);
+  hipLaunchKernelGGL(k_leaf_level, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, S.seg_of.p, S.seg_off.p,
+                     S.seg_ncrit.p, keys.p, leaf_level.p, err.p);
   int h_err = 0;
   CPPE_HIP_CHECK(hipMemcpyAsync(&h_err, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
   CPPE_HIP_CHECK(hipStreamSynchronize(st));
@@ -357,8 +390,8 @@ void build_octree(System& S) {
   int ncells = 0;
   int L      = 0;
   for (; L <= kMaxLevel; ++L) {
-    hipLaunchKernelGGL(k_heads, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, L, keys.p, leaf_level.p,
-                       flag.p);
+    hipLaunchKernelGGL(k_heads, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, L, keys.p, S.seg_of.p,
+                       S.seg_off.p, leaf_level.p, flag.p);
     exclusive_scan(flag.p, rank.p, n, st, tmp);
     int last_rank = 0, last_flag = 0;
     CPPE_HIP_CHECK(hipMemcpyAsync(&last_rank, rank.p + n - 1, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -370,7 +403,7 @@ void build_octree(System& S) {
     const int prev0 = L > 0 ? level_off[L - 1] : 0;
     const int prevn = L > 0 ? level_off[L] - level_off[L - 1] : 0;
     hipLaunchKernelGGL(k_fill_cells, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, L, ncells, prev0, prevn,
-                       keys.p, leaf_level.p, flag.p, rank.p, root.p, T.cx.p, T.cy.p, T.cz.p, T.cr.p,
+                       keys.p, S.seg_of.p, S.seg_off.p, leaf_level.p, flag.p, rank.p, root.p, T.cx.p, T.cy.p, T.cz.p, T.cr.p,
                        T.crmax.p, T.start.p, T.count.p, T.parent.p, T.level.p);
     ncells += nl;
     level_off.push_back(ncells);
@@ -523,7 +556,7 @@ void build_lists(System& S) {
   const double theta = S.opts.theta;
   long long cap_m2l  = 512ll * T.ncells + 4096;
   long long cap_p2p  = 256ll * T.nleaves + 4096;
-  long long cap_next = cap_m2l;
+  long long cap_next = std::max<long long>(cap_m2l, T.level_offset[1]);
   dbuf<int> counters;
   counters.alloc(4);
   for (int attempt = 0; attempt < 6; ++attempt) {
@@ -533,11 +566,14 @@ void build_lists(System& S) {
     fb.alloc(cap_next);
     m2l.alloc(cap_m2l);
     p2p.alloc(cap_p2p);
-    int2 seed = make_int2(0, 0);
-    CPPE_HIP_CHECK(hipMemcpyAsync(fa.p, &seed, sizeof(int2), hipMemcpyHostToDevice, st));
+    // one traversal per snapshot, all started together from the (root, root) pairs
+    const int nroots = T.level_offset[1];
+    std::vector<int2> seed(nroots);
+    for (int c = 0; c < nroots; ++c) seed[c] = make_int2(c, c);
+    CPPE_HIP_CHECK(hipMemcpyAsync(fa.p, seed.data(), sizeof(int2) * nroots, hipMemcpyHostToDevice, st));
     int h_cnt[4] = {0, 0, 0, 0};
     CPPE_HIP_CHECK(hipMemcpyAsync(counters.p, h_cnt, sizeof(h_cnt), hipMemcpyHostToDevice, st));
-    int nfront = 1;
+    int nfront = nroots;
     bool overflow = false;
     while (nfront > 0) {
       TravOut o{fb.p, m2l.p, p2p.p, counters.p, (int)cap_next, (int)cap_m2l, (int)cap_p2p};

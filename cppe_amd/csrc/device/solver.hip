@@ -1,15 +1,29 @@
 // Jacobi-preconditioned conjugate gradients on the device (InducedMoments::compute,
-// src/core/electric_fields.cc:191-239). Vectors are 3n SoA in device order; all reductions
-// are two-stage with a fixed grid and a fixed summation tree, so results are deterministic.
+// src/core/electric_fields.cc:191-239), for one system or for a batch of independent snapshots
+// ("segments") solved together.
+//
+// Vectors are 3n SoA in device order. Work is split into chunks of at most kChunk sites that
+// never cross a segment boundary; every reduction is one partial per chunk (fixed block tree)
+// followed by a per-segment sum over its chunks in chunk order, so results are deterministic
+// and a segment's scalars do not depend on the other segments. Each segment runs its own
+// recurrence (its own step lengths, residual norm and convergence test) and is frozen once it
+// has converged; the field applies are shared by all segments.
 //
 // FP64 form: the reference recurrence with every B apply in FP64.
 // Mixed form: the same recurrence with FP32 field applies, made exact by reliable updates
-// (residual replacement; Sleijpen & van der Vorst, Computing 56, 141 (1996)): whenever the
-// recursively updated residual has fallen by kReliableDrop since the last replacement, or
-// below the threshold, the true residual rhs - B x is recomputed with an FP64 apply and
-// replaces it. Convergence is only ever declared on an FP64 true residual, so both forms stop
-// on the same FP64 criterion ||rhs - B x||_2 < thresh.
+// (residual replacement; Sleijpen & van der Vorst, Computing 56, 141 (1996)): whenever a
+// segment's recursively updated residual has fallen by kReliableDrop since its last
+// replacement, or below the threshold, its true residual rhs - B x is recomputed with an FP64
+// apply and replaces it. Convergence is only ever declared on an FP64 true residual, so both
+// forms stop on the same FP64 criterion ||rhs - B x||_2 < thresh, per segment.
+//
+// One iteration is three fixed launch sequences -- step (apply p, step length, x/r update,
+// residual norm), reliable (FP64 apply of x, residual replacement; mixed form, when a segment
+// asks for it) and direction (preconditioner, beta, p update). Each is captured once into a HIP
+// graph and replayed; all scalars live on the device, and the host reads a 4-int status block
+// after the step and reliable graphs to decide what to launch next.
 #include <cmath>
+#include <functional>
 #include <vector>
 
 #include "system.hh"
@@ -18,11 +32,17 @@ namespace cppe_dev {
 
 void apply_b_direct(System& S, const double* xin, double* out, bool fp32);
 void apply_b_fmm(System& S, const double* xin, double* out, bool fp32);
+void fmm_reserve(System& S);
 
-constexpr int kRedBlocks = 512;
+constexpr int kChunk          = 1024;
 constexpr double kReliableDrop = 1e-4;
 
-enum Scal { S_RZ = 0, S_ALPHA, S_BETA, S_RR, S_ZR, S_PAP, S_RNORM, S_COUNT };
+// per-segment doubles (SoA, each nseg long) and ints
+enum SegD { D_RZ = 0, D_ALPHA, D_BETA, D_RNORM, D_RLAST, D_COUNT };
+enum SegI { I_ACTIVE = 0, I_NEEDREL, I_ITERS, I_BAD, I_COUNT };
+// status block: [iteration, active segments, segments asking for a reliable update, bad]
+enum Status { ST_IT = 0, ST_ACTIVE, ST_REL, ST_BAD, ST_COUNT };
+enum FinalMode { F_INIT = 0, F_ALPHA, F_RNORM, F_REL, F_BETA };
 
 void apply_b(System& S, const double* x, double* out, bool fp32) {
   if (S.opts.summation == CPPE_HIP_SUM_FMM)
@@ -44,6 +64,20 @@ __global__ __launch_bounds__(256) void k_alpha_apply(int n, const double* __rest
   out[2 * n + i] = a2 * m0 + a4 * m1 + a5 * m2;
 }
 
+void precondition(System& S, const double* in, double* out) {
+  hipLaunchKernelGGL(k_alpha_apply, dim3(ceil_div(S.n, 256)), dim3(256), 0, S.stream, S.n, S.alpha6.p, in, out);
+}
+
+__global__ void k_fill_f64(size_t n, double* __restrict__ a, double v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+void fill_f64(System& S, double* a, size_t n, double v) {
+  if (n) hipLaunchKernelGGL(k_fill_f64, dim3(ceil_div((long long)n, 256)), dim3(256), 0, S.stream, n, a, v);
+}
+
+// block-wide sum (256 threads); the result is valid in thread 0
 __device__ __forceinline__ double block_sum(double v, double* red) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -56,205 +90,442 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return s;
 }
 
+__device__ __forceinline__ void z_of(const double* __restrict__ a6, int n, int i, double r0, double r1, double r2,
+                                     double& z0, double& z1, double& z2) {
+  const double a0 = a6[i], a1 = a6[n + i], a2 = a6[2 * n + i], a3 = a6[3 * n + i], a4 = a6[4 * n + i],
+               a5 = a6[5 * n + i];
+  z0 = a0 * r0 + a1 * r1 + a2 * r2;
+  z1 = a1 * r0 + a3 * r1 + a4 * r2;
+  z2 = a2 * r0 + a4 * r1 + a5 * r2;
+}
+
+struct CgVecs {
+  int n, nseg, nchunk;
+  const int* clo;   // nchunk + 1 site offsets of the chunks
+  const int* cseg;  // segment of each chunk
+  const double* a6;
+  const double* rhs;
+  double *x, *r, *z, *p, *Ap, *t;
+  double* part;     // 2 * nchunk partials
+  double* sd;       // D_COUNT * nseg
+  int* si;          // I_COUNT * nseg
+};
+
 // r = rhs - t ; z = alpha r ; p = z ; partials (r.z, r.r)
-__global__ __launch_bounds__(256) void k_cg_init(int n, const double* __restrict__ a6,
-                                                 const double* __restrict__ rhs, const double* __restrict__ t,
-                                                 double* __restrict__ r, double* __restrict__ z,
-                                                 double* __restrict__ p, double* __restrict__ part) {
+__global__ __launch_bounds__(256) void k_sg_init(CgVecs v) {
   __shared__ double red[4];
+  const int c = blockIdx.x, n = v.n;
   double rz = 0.0, rr = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double r0 = rhs[i] - t[i], r1 = rhs[n + i] - t[n + i], r2 = rhs[2 * n + i] - t[2 * n + i];
-    const double a0 = a6[i], a1 = a6[n + i], a2 = a6[2 * n + i], a3 = a6[3 * n + i], a4 = a6[4 * n + i],
-                 a5 = a6[5 * n + i];
-    const double z0 = a0 * r0 + a1 * r1 + a2 * r2;
-    const double z1 = a1 * r0 + a3 * r1 + a4 * r2;
-    const double z2 = a2 * r0 + a4 * r1 + a5 * r2;
-    r[i] = r0;
-    r[n + i] = r1;
-    r[2 * n + i] = r2;
-    z[i] = z0;
-    z[n + i] = z1;
-    z[2 * n + i] = z2;
-    p[i] = z0;
-    p[n + i] = z1;
-    p[2 * n + i] = z2;
+  for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x) {
+    const double r0 = v.rhs[i] - v.t[i], r1 = v.rhs[n + i] - v.t[n + i], r2 = v.rhs[2 * n + i] - v.t[2 * n + i];
+    double z0, z1, z2;
+    z_of(v.a6, n, i, r0, r1, r2, z0, z1, z2);
+    v.r[i] = r0;
+    v.r[n + i] = r1;
+    v.r[2 * n + i] = r2;
+    v.z[i] = z0;
+    v.z[n + i] = z1;
+    v.z[2 * n + i] = z2;
+    v.p[i] = z0;
+    v.p[n + i] = z1;
+    v.p[2 * n + i] = z2;
     rz += r0 * z0 + r1 * z1 + r2 * z2;
     rr += r0 * r0 + r1 * r1 + r2 * r2;
   }
-  double s1 = block_sum(rz, red);
-  double s2 = block_sum(rr, red);
+  const double s1 = block_sum(rz, red);
+  const double s2 = block_sum(rr, red);
   if (threadIdx.x == 0) {
-    part[blockIdx.x]             = s1;
-    part[kRedBlocks + blockIdx.x] = s2;
+    v.part[c]            = s1;
+    v.part[v.nchunk + c] = s2;
   }
 }
 
-// partial p.Ap
-__global__ __launch_bounds__(256) void k_dot3(int n3, const double* __restrict__ a, const double* __restrict__ b,
-                                              double* __restrict__ part) {
+// partial p.Ap (active segments)
+__global__ __launch_bounds__(256) void k_sg_pap(CgVecs v) {
   __shared__ double red[4];
-  double v = 0.0;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n3; k += gridDim.x * blockDim.x) v += a[k] * b[k];
-  double s = block_sum(v, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  const int c = blockIdx.x, n = v.n;
+  double s = 0.0;
+  if (v.si[I_ACTIVE * v.nseg + v.cseg[c]])
+    for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x)
+      s += v.p[i] * v.Ap[i] + v.p[n + i] * v.Ap[n + i] + v.p[2 * n + i] * v.Ap[2 * n + i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) v.part[c] = s;
 }
 
-// x += alpha p ; r -= alpha Ap ; partial r.r   (alpha = scal[S_ALPHA])
-__global__ __launch_bounds__(256) void k_xr_update(int n3, const double* __restrict__ scal,
-                                                   const double* __restrict__ p, const double* __restrict__ Ap,
-                                                   double* __restrict__ x, double* __restrict__ r,
-                                                   double* __restrict__ part) {
+// x += alpha p ; r -= alpha Ap ; partial r.r (active segments)
+__global__ __launch_bounds__(256) void k_sg_xr(CgVecs v) {
   __shared__ double red[4];
-  const double al = scal[S_ALPHA];
-  double v = 0.0;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n3; k += gridDim.x * blockDim.x) {
-    x[k] += al * p[k];
-    const double rk = r[k] - al * Ap[k];
-    r[k] = rk;
-    v += rk * rk;
+  const int c = blockIdx.x, n = v.n, g = v.cseg[c];
+  double s = 0.0;
+  if (v.si[I_ACTIVE * v.nseg + g]) {
+    const double al = v.sd[D_ALPHA * v.nseg + g];
+    for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const size_t k = (size_t)q * n + i;
+        v.x[k] += al * v.p[k];
+        const double rk = v.r[k] - al * v.Ap[k];
+        v.r[k] = rk;
+        s += rk * rk;
+      }
   }
-  double s = block_sum(v, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) v.part[c] = s;
 }
 
-// r = rhs - t ; partial r.r (reliable update / initial residual)
-__global__ __launch_bounds__(256) void k_residual(int n3, const double* __restrict__ rhs,
-                                                  const double* __restrict__ t, double* __restrict__ r,
-                                                  double* __restrict__ part) {
+// r = rhs - t for the segments that asked for a reliable update; partial r.r
+__global__ __launch_bounds__(256) void k_sg_resid(CgVecs v) {
   __shared__ double red[4];
-  double v = 0.0;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n3; k += gridDim.x * blockDim.x) {
-    const double rk = rhs[k] - t[k];
-    r[k] = rk;
-    v += rk * rk;
-  }
-  double s = block_sum(v, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  const int c = blockIdx.x, n = v.n, g = v.cseg[c];
+  double s = 0.0;
+  if (v.si[I_NEEDREL * v.nseg + g])
+    for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const size_t k  = (size_t)q * n + i;
+        const double rk = v.rhs[k] - v.t[k];
+        v.r[k] = rk;
+        s += rk * rk;
+      }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) v.part[c] = s;
 }
 
-// z = alpha r ; partial z.r
-__global__ __launch_bounds__(256) void k_z_update(int n, const double* __restrict__ a6,
-                                                  const double* __restrict__ r, double* __restrict__ z,
-                                                  double* __restrict__ part) {
+// z = alpha r ; partial z.r (active segments)
+__global__ __launch_bounds__(256) void k_sg_z(CgVecs v) {
   __shared__ double red[4];
-  double v = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double r0 = r[i], r1 = r[n + i], r2 = r[2 * n + i];
-    const double a0 = a6[i], a1 = a6[n + i], a2 = a6[2 * n + i], a3 = a6[3 * n + i], a4 = a6[4 * n + i],
-                 a5 = a6[5 * n + i];
-    const double z0 = a0 * r0 + a1 * r1 + a2 * r2;
-    const double z1 = a1 * r0 + a3 * r1 + a4 * r2;
-    const double z2 = a2 * r0 + a4 * r1 + a5 * r2;
-    z[i] = z0;
-    z[n + i] = z1;
-    z[2 * n + i] = z2;
-    v += r0 * z0 + r1 * z1 + r2 * z2;
-  }
-  double s = block_sum(v, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  const int c = blockIdx.x, n = v.n;
+  double s = 0.0;
+  if (v.si[I_ACTIVE * v.nseg + v.cseg[c]])
+    for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x) {
+      const double r0 = v.r[i], r1 = v.r[n + i], r2 = v.r[2 * n + i];
+      double z0, z1, z2;
+      z_of(v.a6, n, i, r0, r1, r2, z0, z1, z2);
+      v.z[i] = z0;
+      v.z[n + i] = z1;
+      v.z[2 * n + i] = z2;
+      s += r0 * z0 + r1 * z1 + r2 * z2;
+    }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) v.part[c] = s;
 }
 
-// p = z + beta p  (beta = scal[S_BETA])
-__global__ __launch_bounds__(256) void k_p_update(int n3, const double* __restrict__ scal,
-                                                  const double* __restrict__ z, double* __restrict__ p) {
-  const double be = scal[S_BETA];
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n3; k += gridDim.x * blockDim.x) p[k] = z[k] + be * p[k];
+// p = z + beta p (active segments)
+__global__ __launch_bounds__(256) void k_sg_p(CgVecs v) {
+  const int c = blockIdx.x, n = v.n, g = v.cseg[c];
+  if (!v.si[I_ACTIVE * v.nseg + g]) return;
+  const double be = v.sd[D_BETA * v.nseg + g];
+  for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const size_t k = (size_t)q * n + i;
+      v.p[k]         = v.z[k] + be * v.p[k];
+    }
 }
 
-// Single-block finalisation of the partial sums (fixed order -> deterministic).
-// mode 0: init   (rz, rr)      -> S_RZ, S_RNORM
-// mode 1: alpha  (pAp)         -> S_ALPHA = S_RZ / pAp
-// mode 2: rnorm  (rr)          -> S_RNORM = sqrt(rr)
-// mode 3: beta   (zr)          -> S_BETA = zr / S_RZ ; S_RZ = zr
-__global__ __launch_bounds__(256) void k_finalize(int mode, int nblk, const double* __restrict__ part,
-                                                  double* __restrict__ scal) {
-  __shared__ double red[4];
-  double v1 = 0.0, v2 = 0.0;
-  for (int k = threadIdx.x; k < nblk; k += blockDim.x) {
-    v1 += part[k];
-    if (mode == 0) v2 += part[kRedBlocks + k];
+struct CgFinal {
+  int nseg, nchunk;
+  const int* seg_chunk;  // nseg + 1 chunk offsets of the segments
+  const double* part;
+  double* sd;
+  int* si;
+  int* status;           // ST_COUNT ints
+  const double* params;  // [thresh, reliable drop]
+  double* hist;          // residual norms, iteration-major [it * nseg + g]
+  int hist_cap;          // iterations held by hist
+};
+
+// Per-segment finalisation of the chunk partials, one thread per segment, chunks summed in
+// order (deterministic). Then the status block is refreshed.
+template <int MODE, bool MIXED>
+__global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
+  __shared__ int cnt[3];
+  if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int ns = f.nseg;
+  const int it = f.status[ST_IT];
+  const double thresh = f.params[0], drop = f.params[1];
+  int na = 0, nr = 0, nb = 0;
+  for (int g = threadIdx.x; g < ns; g += blockDim.x) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int c = f.seg_chunk[g]; c < f.seg_chunk[g + 1]; ++c) {
+      s1 += f.part[c];
+      if (MODE == F_INIT) s2 += f.part[f.nchunk + c];
+    }
+    double* sd = f.sd;
+    int* si    = f.si;
+    if (MODE == F_INIT) {
+      const double rn = sqrt(s2);
+      sd[D_RZ * ns + g]    = s1;
+      sd[D_RNORM * ns + g] = rn;
+      sd[D_RLAST * ns + g] = rn;
+      sd[D_ALPHA * ns + g] = 0.0;
+      sd[D_BETA * ns + g]  = 0.0;
+      const bool bad       = !isfinite(rn);
+      si[I_BAD * ns + g]     = bad;
+      si[I_ACTIVE * ns + g]  = !bad && !(rn < thresh);  // an exact guess needs no iteration
+      si[I_NEEDREL * ns + g] = 0;
+      si[I_ITERS * ns + g]   = 0;
+    } else if (MODE == F_ALPHA) {
+      if (si[I_ACTIVE * ns + g]) sd[D_ALPHA * ns + g] = sd[D_RZ * ns + g] / s1;
+    } else if (MODE == F_RNORM) {
+      if (si[I_ACTIVE * ns + g]) {
+        const double rn      = sqrt(s1);
+        sd[D_RNORM * ns + g] = rn;
+        if (!isfinite(rn)) {
+          si[I_BAD * ns + g]    = 1;
+          si[I_ACTIVE * ns + g] = 0;
+          si[I_ITERS * ns + g]  = it + 1;
+          if (it < f.hist_cap) f.hist[(size_t)it * ns + g] = rn;
+        } else if (MIXED && (rn < thresh || rn < drop * sd[D_RLAST * ns + g])) {
+          si[I_NEEDREL * ns + g] = 1;  // recorded after the replacement
+        } else {
+          if (it < f.hist_cap) f.hist[(size_t)it * ns + g] = rn;
+          if (rn < thresh) {
+            si[I_ACTIVE * ns + g] = 0;
+            si[I_ITERS * ns + g]  = it + 1;
+          }
+        }
+      }
+    } else if (MODE == F_REL) {
+      if (si[I_NEEDREL * ns + g]) {
+        const double rn        = sqrt(s1);
+        sd[D_RNORM * ns + g]   = rn;
+        sd[D_RLAST * ns + g]   = rn;
+        si[I_NEEDREL * ns + g] = 0;
+        if (it < f.hist_cap) f.hist[(size_t)it * ns + g] = rn;
+        if (!isfinite(rn)) {
+          si[I_BAD * ns + g]    = 1;
+          si[I_ACTIVE * ns + g] = 0;
+          si[I_ITERS * ns + g]  = it + 1;
+        } else if (rn < thresh) {
+          si[I_ACTIVE * ns + g] = 0;
+          si[I_ITERS * ns + g]  = it + 1;
+        }
+      }
+    } else {  // F_BETA
+      if (si[I_ACTIVE * ns + g]) {
+        sd[D_BETA * ns + g] = s1 / sd[D_RZ * ns + g];
+        sd[D_RZ * ns + g]   = s1;
+      }
+    }
+    na += si[I_ACTIVE * ns + g];
+    nr += si[I_NEEDREL * ns + g];
+    nb += si[I_BAD * ns + g];
   }
-  double s1 = block_sum(v1, red);
-  double s2 = block_sum(v2, red);
+  if (na) atomicAdd(&cnt[0], na);
+  if (nr) atomicAdd(&cnt[1], nr);
+  if (nb) atomicAdd(&cnt[2], nb);
+  __syncthreads();
   if (threadIdx.x == 0) {
-    if (mode == 0) {
-      scal[S_RZ]    = s1;
-      scal[S_RNORM] = sqrt(s2);
-    } else if (mode == 1) {
-      scal[S_PAP]   = s1;
-      scal[S_ALPHA] = scal[S_RZ] / s1;
-    } else if (mode == 2) {
-      scal[S_RR]    = s1;
-      scal[S_RNORM] = sqrt(s1);
-    } else {
-      scal[S_ZR]   = s1;
-      scal[S_BETA] = s1 / scal[S_RZ];
-      scal[S_RZ]   = s1;
+    f.status[ST_ACTIVE] = cnt[0];
+    f.status[ST_REL]    = cnt[1];
+    f.status[ST_BAD]    = cnt[2];
+    if (MODE == F_INIT) f.status[ST_IT] = 0;
+    if (MODE == F_BETA) f.status[ST_IT] = it + 1;
+  }
+}
+
+// ---- host side -------------------------------------------------------------------------------
+void setup_segments(System& S, const std::vector<int>& seg_off, const std::vector<int>& seg_ncrit) {
+  const int n = S.n, ns = (int)seg_off.size() - 1;
+  S.nseg      = ns;
+  S.h_seg_off = seg_off;
+  std::vector<int> of(n), clo, cseg, schunk(ns + 1, 0);
+  for (int g = 0; g < ns; ++g) {
+    for (int i = seg_off[g]; i < seg_off[g + 1]; ++i) of[i] = g;
+    schunk[g] = (int)cseg.size();
+    for (int i = seg_off[g]; i < seg_off[g + 1]; i += kChunk) {
+      clo.push_back(i);
+      cseg.push_back(g);
     }
   }
+  schunk[ns] = (int)cseg.size();
+  clo.push_back(n);
+  S.nchunk = (int)cseg.size();
+  auto up  = [&](dbuf<int>& b, const std::vector<int>& h) {
+    b.alloc(h.size());
+    CPPE_HIP_CHECK(hipMemcpyAsync(b.p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, S.stream));
+  };
+  up(S.seg_off, seg_off);
+  up(S.seg_ncrit, seg_ncrit);
+  up(S.seg_of, of);
+  up(S.chunk_lo, clo);
+  up(S.chunk_seg, cseg);
+  up(S.seg_chunk, schunk);
+  S.cg_part.alloc(2 * (size_t)S.nchunk);
+  S.cg_sd.alloc(D_COUNT * (size_t)ns);
+  S.cg_si.alloc(I_COUNT * (size_t)ns);
+  S.cg_status.alloc(ST_COUNT);
+  S.cg_params.alloc(2);
+  if (!S.h_status) CPPE_HIP_CHECK(hipHostMalloc((void**)&S.h_status, sizeof(int) * ST_COUNT));
+  CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
 }
 
-static int red_blocks(long long m) { return (int)std::min<long long>(kRedBlocks, (m + 255) / 256); }
-
-static double read_scalar(System& S, int which) {
-  double v = 0.0;
-  CPPE_HIP_CHECK(hipMemcpyAsync(&v, S.scal.p + which, sizeof(double), hipMemcpyDeviceToHost, S.stream));
-  CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
+static CgVecs vecs(System& S) {
+  CgVecs v;
+  v.n = S.n;
+  v.nseg = S.nseg;
+  v.nchunk = S.nchunk;
+  v.clo = S.chunk_lo.p;
+  v.cseg = S.chunk_seg.p;
+  v.a6 = S.alpha6.p;
+  v.rhs = S.v_rhs.p;
+  v.x = S.v_mu.p;
+  v.r = S.v_r.p;
+  v.z = S.v_z.p;
+  v.p = S.v_p.p;
+  v.Ap = S.v_Ap.p;
+  v.t = S.v_t.p;
+  v.part = S.cg_part.p;
+  v.sd = S.cg_sd.p;
+  v.si = S.cg_si.p;
   return v;
 }
 
-void precondition(System& S, const double* in, double* out) {
-  hipLaunchKernelGGL(k_alpha_apply, dim3(ceil_div(S.n, 256)), dim3(256), 0, S.stream, S.n, S.alpha6.p, in, out);
+static CgFinal finals(System& S) {
+  CgFinal f;
+  f.nseg = S.nseg;
+  f.nchunk = S.nchunk;
+  f.seg_chunk = S.seg_chunk.p;
+  f.part = S.cg_part.p;
+  f.sd = S.cg_sd.p;
+  f.si = S.cg_si.p;
+  f.status = S.cg_status.p;
+  f.params = S.cg_params.p;
+  f.hist = S.cg_hist.p;
+  f.hist_cap = S.hist_cap;
+  return f;
 }
 
-// Solve B x = rhs for device vectors (x holds the initial guess on entry).
-// Returns the number of iterations and fills norms (host vector) per iteration.
-int device_cg(System& S, const double* rhs, double* x, double thresh, int maxiter, bool mixed,
-              std::vector<double>& norms, bool& converged) {
-  const int n = S.n, n3 = 3 * n;
-  hipStream_t st = S.stream;
-  const int gb3 = red_blocks(n3), gb1 = red_blocks(n);
-  double* r = S.v_r.p;
-  double* z = S.v_z.p;
-  double* p = S.v_p.p;
-  double* Ap = S.v_Ap.p;
-  double* t = S.v_t.p;
-  double* part = S.red.p;
-  double* scal = S.scal.p;
-  norms.clear();
-  converged = false;
-  // r0 = rhs - B x0 (FP64), z0 = alpha r0, p = z0
-  apply_b(S, x, t, false);
-  hipLaunchKernelGGL(k_cg_init, dim3(gb1), dim3(256), 0, st, n, S.alpha6.p, rhs, t, r, z, p, part);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, 0, gb1, part, scal);
-  double r_last = read_scalar(S, S_RNORM);
-  for (int k = 0; k < maxiter; ++k) {
-    apply_b(S, p, Ap, mixed);
-    hipLaunchKernelGGL(k_dot3, dim3(gb3), dim3(256), 0, st, n3, p, Ap, part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, 1, gb3, part, scal);
-    hipLaunchKernelGGL(k_xr_update, dim3(gb3), dim3(256), 0, st, n3, scal, p, Ap, x, r, part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, 2, gb3, part, scal);
-    double rn = read_scalar(S, S_RNORM);
-    if (mixed && (rn < thresh || rn < kReliableDrop * r_last)) {
-      // reliable update: replace the recursive residual by the FP64 true residual
-      apply_b(S, x, t, false);
-      hipLaunchKernelGGL(k_residual, dim3(gb3), dim3(256), 0, st, n3, rhs, t, r, part);
-      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, 2, gb3, part, scal);
-      rn     = read_scalar(S, S_RNORM);
-      r_last = rn;
+template <int MODE>
+static void launch_final(System& S, bool mixed) {
+  if (mixed)
+    hipLaunchKernelGGL((k_sg_final<MODE, true>), dim3(1), dim3(256), 0, S.stream, finals(S));
+  else
+    hipLaunchKernelGGL((k_sg_final<MODE, false>), dim3(1), dim3(256), 0, S.stream, finals(S));
+}
+
+static void seq_step(System& S, bool mixed) {
+  const dim3 g(S.nchunk), b(256);
+  apply_b(S, S.v_p.p, S.v_Ap.p, mixed);
+  hipLaunchKernelGGL(k_sg_pap, g, b, 0, S.stream, vecs(S));
+  launch_final<F_ALPHA>(S, mixed);
+  hipLaunchKernelGGL(k_sg_xr, g, b, 0, S.stream, vecs(S));
+  launch_final<F_RNORM>(S, mixed);
+}
+
+static void seq_reliable(System& S) {
+  apply_b(S, S.v_mu.p, S.v_t.p, false);
+  hipLaunchKernelGGL(k_sg_resid, dim3(S.nchunk), dim3(256), 0, S.stream, vecs(S));
+  launch_final<F_REL>(S, true);
+}
+
+static void seq_direction(System& S, bool mixed) {
+  const dim3 g(S.nchunk), b(256);
+  hipLaunchKernelGGL(k_sg_z, g, b, 0, S.stream, vecs(S));
+  launch_final<F_BETA>(S, mixed);
+  hipLaunchKernelGGL(k_sg_p, g, b, 0, S.stream, vecs(S));
+}
+
+void cg_graphs_release(System& S) {
+  for (auto* g : {&S.g_step[0], &S.g_step[1], &S.g_rel, &S.g_dir[0], &S.g_dir[1]})
+    if (*g) {
+      (void)hipGraphExecDestroy(*g);
+      *g = nullptr;
     }
-    norms.push_back(rn);
-    if (!std::isfinite(rn)) break;
-    if (rn < thresh) {
-      converged = true;
-      return k + 1;
-    }
-    hipLaunchKernelGGL(k_z_update, dim3(gb1), dim3(256), 0, st, n, S.alpha6.p, r, z, part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, 3, gb1, part, scal);
-    hipLaunchKernelGGL(k_p_update, dim3(gb3), dim3(256), 0, st, n3, scal, z, p);
+}
+
+static void capture(System& S, hipGraphExec_t& exec, const std::function<void()>& body) {
+  hipGraph_t g       = nullptr;
+  hipEvent_t* prof   = S.prof;
+  S.prof             = nullptr;  // no event records inside a graph
+  CPPE_HIP_CHECK(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
+  try {
+    body();
+  } catch (...) {
+    (void)hipStreamEndCapture(S.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    S.prof = prof;
+    throw;
   }
-  return (int)norms.size();
+  CPPE_HIP_CHECK(hipStreamEndCapture(S.stream, &g));
+  S.prof = prof;
+  const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  CPPE_HIP_CHECK(e);
+}
+
+static void run_graph(System& S, hipGraphExec_t& exec, const std::function<void()>& body) {
+  if (!S.use_graphs) {
+    body();
+    return;
+  }
+  if (!exec) capture(S, exec, body);
+  CPPE_HIP_CHECK(hipGraphLaunch(exec, S.stream));
+}
+
+static void read_status(System& S) {
+  CPPE_HIP_CHECK(hipMemcpyAsync(S.h_status, S.cg_status.p, sizeof(int) * ST_COUNT, hipMemcpyDeviceToHost, S.stream));
+  CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
+}
+
+// Solves B x = rhs for S.v_rhs into S.v_mu (which holds the initial guess). Results per segment
+// are left in the device state; norms0 receives the residual norms of segment 0.
+int device_cg(System& S, double thresh, int maxiter, bool mixed, CgResult& res) {
+  hipStream_t st = S.stream;
+  const int ns   = S.nseg;
+  if (S.hist_cap < maxiter || !S.cg_hist.p) {
+    S.hist_cap = std::max(maxiter, 64);
+    S.cg_hist.alloc((size_t)S.hist_cap * ns);
+    cg_graphs_release(S);  // the graphs hold the history buffer
+  }
+  if (S.opts.summation == CPPE_HIP_SUM_FMM) fmm_reserve(S);  // no allocation inside a graph
+  const double params[2] = {thresh, kReliableDrop};
+  CPPE_HIP_CHECK(hipMemcpyAsync(S.cg_params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
+  // r0 = rhs - B x0 (FP64), z0 = alpha r0, p = z0
+  apply_b(S, S.v_mu.p, S.v_t.p, false);
+  hipLaunchKernelGGL(k_sg_init, dim3(S.nchunk), dim3(256), 0, st, vecs(S));
+  launch_final<F_INIT>(S, mixed);
+  read_status(S);
+  int k = 0;
+  const int m = mixed ? 1 : 0;
+  while (S.h_status[ST_ACTIVE] > 0 && k < maxiter) {
+    run_graph(S, S.g_step[m], [&] { seq_step(S, mixed); });
+    read_status(S);
+    if (S.h_status[ST_REL] > 0) {
+      run_graph(S, S.g_rel, [&] { seq_reliable(S); });
+      read_status(S);
+    }
+    ++k;
+    if (S.h_status[ST_ACTIVE] == 0) break;
+    run_graph(S, S.g_dir[m], [&] { seq_direction(S, mixed); });
+  }
+  // per-segment results
+  std::vector<int> si((size_t)I_COUNT * ns);
+  std::vector<double> sd((size_t)D_COUNT * ns);
+  CPPE_HIP_CHECK(hipMemcpyAsync(si.data(), S.cg_si.p, sizeof(int) * si.size(), hipMemcpyDeviceToHost, st));
+  CPPE_HIP_CHECK(hipMemcpyAsync(sd.data(), S.cg_sd.p, sizeof(double) * sd.size(), hipMemcpyDeviceToHost, st));
+  const int nh = std::min(k, S.hist_cap);
+  std::vector<double> hist((size_t)nh * ns);
+  if (nh)
+    CPPE_HIP_CHECK(hipMemcpyAsync(hist.data(), S.cg_hist.p, sizeof(double) * hist.size(), hipMemcpyDeviceToHost, st));
+  CPPE_HIP_CHECK(hipStreamSynchronize(st));
+  res.iters.assign(ns, 0);
+  res.rnorm.assign(ns, 0.0);
+  res.converged.assign(ns, 0);
+  res.norms0.clear();
+  res.all_converged = true;
+  for (int g = 0; g < ns; ++g) {
+    const bool active = si[I_ACTIVE * ns + g] != 0;
+    res.iters[g]      = active ? k : si[I_ITERS * ns + g];
+    res.rnorm[g]      = sd[D_RNORM * ns + g];
+    res.converged[g]  = !active && !si[I_BAD * ns + g] && res.rnorm[g] < thresh;
+    res.all_converged = res.all_converged && res.converged[g];
+  }
+  for (int it = 0; it < std::min(res.iters[0], nh); ++it) res.norms0.push_back(hist[(size_t)it * ns]);
+  res.loop_iters = k;
+  return k;
 }
 
 }  // namespace cppe_dev

@@ -52,6 +52,15 @@ struct Octree {
 
 struct FmmState;  // FMM expansions and operator scratch (fmm.hip)
 
+// Outcome of a device CG solve, per segment.
+struct CgResult {
+  std::vector<int> iters, converged;
+  std::vector<double> rnorm;
+  std::vector<double> norms0;  // residual norm per iteration of segment 0
+  int loop_iters = 0;          // iterations of the shared loop (max over segments)
+  bool all_converged = false;
+};
+
 struct System {
   int n = 0;
   cppe_hip_options opts{};
@@ -71,10 +80,24 @@ struct System {
   Octree tree;
   FmmState* fmm = nullptr;
 
+  // snapshots ("segments"): contiguous site ranges that form independent systems (own root
+  // cell, own ncrit, own CG recurrence). A plain system is one segment.
+  int nseg = 1;
+  std::vector<int> h_seg_off;            // nseg + 1 (caller order; the same ranges in device order)
+  dbuf<int> seg_off, seg_ncrit, seg_of;  // seg_of: segment of each site
+
   // solver work vectors: 3n SoA (component-major) in device order
   dbuf<double> v_mu, v_rhs, v_r, v_z, v_p, v_Ap, v_t, v_d;
-  dbuf<double> red;    // reduction partials
-  dbuf<double> scal;   // device scalars of the CG recurrence
+  // CG work split and state (solver.hip)
+  int nchunk = 0;
+  dbuf<int> chunk_lo, chunk_seg, seg_chunk;
+  dbuf<double> cg_part, cg_sd, cg_params, cg_hist;
+  dbuf<int> cg_si, cg_status;
+  int hist_cap = 0;
+  int* h_status = nullptr;  // pinned host copy of the status block
+  bool use_graphs = true;
+  CgResult last;            // outcome of the last solve
+  hipGraphExec_t g_step[2] = {nullptr, nullptr}, g_rel = nullptr, g_dir[2] = {nullptr, nullptr};
   dbuf<double> io_a, io_b;  // AoS3 staging in caller order
   dbuf<float> f32_src;      // FP32 staging for mixed-precision applies
   bool mu_valid = false;
@@ -93,6 +116,13 @@ void direct_static(System& S, const double* q, const double* mu, const double* t
                    double damping);
 // out = field_m1(x) + alpha^-1 x, FP64 (or FP32 arithmetic when fp32 is set)
 void apply_b(System& S, const double* x, double* out, bool fp32);
+void precondition(System& S, const double* in, double* out);
+void fill_f64(System& S, double* a, size_t n, double v);
+// segment tables and CG state; seg_off has nseg + 1 entries
+void setup_segments(System& S, const std::vector<int>& seg_off, const std::vector<int>& seg_ncrit);
+// CG on S.v_rhs from the guess in S.v_mu; returns the shared loop's iteration count
+int device_cg(System& S, double thresh, int maxiter, bool mixed, CgResult& res);
+void cg_graphs_release(System& S);
 
 void fmm_field(System& S, int m, const double* src, double* F, double damping, bool fp32);
 void fmm_static(System& S, const double* q, const double* mu, const double* th, double* F, double damping);

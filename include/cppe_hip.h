@@ -126,6 +126,41 @@ int cppe_hip_solve_resident(cppe_hip_system* sys, int guess_mode, double thresh,
 int cppe_hip_download_mu(cppe_hip_system* sys, double* mu);
 
 /*
+ * Snapshot batches (no counterpart in the reference, whose InducedMoments::compute,
+ * electric_fields.cc:191-239, solves one environment per call). nseg independent systems, site
+ * ranges [seg_off[g], seg_off[g+1]) of the concatenated caller arrays, are held by one handle:
+ * one tree per snapshot (own root; seg_ncrit[g] = ncrit, or 0 for a single leaf = exact direct
+ * sum), every field apply shared by all snapshots, and a CG recurrence per snapshot (own step
+ * lengths, own FP64 stopping test ||F_g - B_g mu_g|| < thresh, frozen once converged).
+ * Exclusion indices are global (caller positions in the concatenation) and must stay inside
+ * their snapshot. The summation is always the tree path (opts.summation is ignored).
+ * upload_rhs / solve_resident / download_mu / multipole_fields / apply_b act on all snapshots at
+ * once; solve_resident's *iters is the shared loop's count (max over snapshots).
+ * Not available on a batch: dense B, fields of order 0/2 through cppe_hip_field, descriptors.
+ */
+int cppe_hip_batch_create(cppe_hip_system** out, int nseg, const int* seg_off, const int* seg_ncrit,
+                          const double* pos, const double* alpha6, const double* alpha_iso, const int* excl_off,
+                          const int* excl_idx, const cppe_hip_options* opts);
+/* Per-snapshot outcome of the last solve (arrays of nseg; any may be NULL). */
+int cppe_hip_segment_results(cppe_hip_system* sys, int* iters, double* rnorm, int* converged);
+/* CG iterations replayed as captured HIP graphs (default 1) or launched kernel by kernel (0). */
+int cppe_hip_set_graphs(cppe_hip_system* sys, int enable);
+
+/*
+ * Autotuner of the tree path's B-apply launch shapes (no counterpart in the reference). For the
+ * FP64 and the FP32 apply in turn, times on this system's own tree (HIP events, mean of reps
+ * after a warm-up) every near-field candidate -- variant 0 (one thread per target site) or 1
+ * (one wavefront per 64-target leaf chunk, uniform source walk) x block 64/128/256 -- then every
+ * M2L group width (16/32/64 lanes per target cell), and keeps the fastest of each.
+ * choice[6]: (near variant, near block, m2l group) for FP64 then FP32.
+ * ms[18]: per precision, the 6 near timings then the 3 M2L timings (ms per launch).
+ * Launch shapes never change results beyond FP summation order.
+ */
+int cppe_hip_autotune(cppe_hip_system* sys, int reps, int* choice, double* ms);
+/* Applies a launch-shape choice (e.g. a cached autotune result). */
+int cppe_hip_set_tune(cppe_hip_system* sys, const int* choice);
+
+/*
  * Times `reps` B applies of a fixed pseudo-random vector on the system's stream, with HIP
  * events around each phase. phase_ms (5 values, mean per apply): [0] upward pass (P2M+M2M),
  * [1] M2L, [2] L2L, [3] near field (P2P+L2P, or the direct pair kernel), [4] whole apply.

@@ -1,0 +1,73 @@
+"""GPU checks of the launch-shape autotuner and of every launch shape it can pick.
+
+Each near-field variant x block size and each M2L group width must give the same B apply as the
+default shape (FP64: 1e-13 relative, summation order only; FP32: 1e-6 relative), and a solve with
+the tuned shapes must stop on the FP64 residual with dipoles equal to the default's to 1e-6 (both stop at 1e-8).
+"""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+from cppe_amd import capi, synthetic
+
+pytestmark = pytest.mark.gpu
+DAMP = 2.1304
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def system(precision):
+    w = synthetic.water_box(1500, 7200, anisotropic=True)
+    S = capi.System(w.pos, w.pol, w.alpha_iso(), w.excl_csr(), summation="fmm", ncrit=64, damping=DAMP,
+                    precision=precision)
+    return w, S
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_every_launch_shape_same_apply(precision):
+    w, S = system(precision)
+    x = np.random.default_rng(11).normal(size=3 * w.n)
+    ref = S.apply_b(x)
+    tol = 1e-13 if precision == "fp64" else 1e-6
+    prec = 0 if precision == "fp64" else 1
+    for (v, b), g in itertools.product(capi.System.NEAR_CANDIDATES, capi.System.M2L_CANDIDATES):
+        choice = np.array([0, 256, 64, 0, 256, 64], dtype=np.int32)
+        choice[3 * prec:3 * prec + 3] = (v, b, g)
+        say("shape", precision, v, b, g)
+        S.set_tune(choice)
+        y = S.apply_b(x)
+        assert np.abs(y - ref).max() < tol * np.abs(ref).max(), (v, b, g)
+
+
+def test_autotune_then_solve():
+    w, S = system("mixed")
+    Fn = capi.nuclear_fields(w.pos, w.atoms_z.astype(float), w.atoms_pos)
+    Sm = capi.System(w.pos, excl=w.excl_csr(), summation="fmm")
+    F = Fn + Sm.multipole_fields(w.charges, w.dipoles, w.quadrupoles)
+    S.upload_rhs(F)
+    S.solve_resident(0, 1e-8, 100)
+    mu0 = S.download_mu()
+    rep = S.autotune(reps=2)
+    say("autotune", rep)
+    for prec in ("fp64", "fp32"):
+        r = rep[prec]
+        assert (r["near_variant"], r["near_block"]) in capi.System.NEAR_CANDIDATES
+        assert r["m2l_group"] in capi.System.M2L_CANDIDATES
+        assert all(t > 0 for t in r["near_ms"].values()) and all(t > 0 for t in r["m2l_ms"].values())
+    it, norms, _ = S.solve_resident(0, 1e-8, 100)
+    assert norms[-1] < 1e-8
+    mu = S.download_mu()
+    assert np.abs(mu - mu0).max() < 1e-6 * np.abs(mu0).max()
+
+
+def test_set_tune_rejects_unknown_shapes():
+    _, S = system("fp64")
+    with pytest.raises(capi.CppeHipError):
+        S.set_tune([2, 256, 64, 0, 256, 64])
+    with pytest.raises(capi.CppeHipError):
+        S.set_tune([0, 96, 64, 0, 256, 64])
