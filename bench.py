@@ -98,6 +98,11 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=5)
     ap.add_argument("--no-ml", action="store_true", help="headline form without the learned warm start")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuned ones")
+    ap.add_argument("--apply-damping", type=float, default=2.1304, help="Thole factor of --apply-only runs")
+    ap.add_argument("--apply-ncrit", type=int, default=64, help="tree_ncrit of --apply-only runs")
+    ap.add_argument("--apply-only", metavar="SHAPE", default=None,
+                    help="profiling aid: only run FMM B applies of the workload with the launch shape "
+                         "'variant,block,m2l' for both precisions (e.g. 2,64,16), print phase times, exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the induced-dipole path is built and timed on a single MI355X "
@@ -113,6 +118,19 @@ def main():
     log("generating workload", args.workload)
     w = synthetic.water_box(**spec)
     log("sites", w.n)
+    if args.apply_only:
+        v, b, g = (int(t) for t in args.apply_only.split(","))
+        for prec in ("mixed", "fp64"):
+            S = capi.System(w.pos, w.pol, w.alpha_iso(), w.excl_csr(), precision=prec, summation="fmm",
+                            ncrit=args.apply_ncrit, order=5, theta=0.5, damping=args.apply_damping)
+            S.set_tune([v, b, g, v, b, g])
+            ph = S.profile_apply(fp32=prec == "mixed", reps=args.profile_reps)
+            st = S.tree_stats()
+            log(prec, "shape", args.apply_only, "damping", args.apply_damping, "ncrit", args.apply_ncrit,
+                "near ms %.4f m2l ms %.4f total ms %.4f" % (ph["near"], ph["m2l"], ph["total"]),
+                "pairs", st["p2p_site_pairs"], "m2l", st["n_m2l"], "leaves", st["nleaves"])
+            S.close()
+        return 0
     rhs = build_rhs(capi, w, args.summation)
     common = dict(summation=args.summation, ncrit=ncrit, order=order, theta=theta, damping=damping)
     t0 = time.perf_counter()

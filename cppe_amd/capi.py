@@ -208,22 +208,27 @@ class System:
                                              ctypes.byref(it), _p(norms), ctypes.byref(ms)))
         return it.value, norms[:it.value], ms.value
 
-    NEAR_CANDIDATES = [(0, 64), (0, 128), (0, 256), (1, 64), (1, 128), (1, 256)]
+    NEAR_CANDIDATES = [(0, 64), (0, 128), (0, 256), (1, 64), (1, 128), (1, 256), (2, 64), (3, 64), (3, 128),
+                       (3, 256), (4, 64), (4, 128), (4, 256), (5, 64), (5, 128), (5, 256), (6, 64), (6, 128),
+                       (6, 256), (7, 64), (7, 128), (7, 256)]
+    NEAR_NAMES = {0: "thread", 1: "leafwave", 2: "ldsleaf", 3: "slotted", 4: "cluster", 5: "staged64", 6: "staged32",
+                  7: "staged16"}
     M2L_CANDIDATES = [16, 32, 64]
 
     def autotune(self, reps=3):
         """Times the B-apply launch candidates on this system and keeps the fastest (tree path)."""
         choice = np.zeros(6, dtype=np.int32)
-        ms = np.zeros(18)
+        nn, nm = len(self.NEAR_CANDIDATES), len(self.M2L_CANDIDATES)
+        ms = np.zeros(2 * (nn + nm))
         _check(lib().cppe_hip_autotune(self.h, int(reps), _p(choice, _i), _p(ms)))
         out = {}
         for k, prec in enumerate(["fp64", "fp32"]):
-            m = ms[9 * k:9 * k + 9]
+            m = ms[(nn + nm) * k:(nn + nm) * (k + 1)]
             out[prec] = dict(near_variant=int(choice[3 * k]), near_block=int(choice[3 * k + 1]),
                              m2l_group=int(choice[3 * k + 2]),
-                             near_ms={"%s/%d" % ("thread" if v == 0 else "leafwave", b): float(t)
-                                      for (v, b), t in zip(self.NEAR_CANDIDATES, m[:6])},
-                             m2l_ms={str(g): float(t) for g, t in zip(self.M2L_CANDIDATES, m[6:])})
+                             near_ms={"%s/%d" % (self.NEAR_NAMES[v], b): round(float(t), 4)
+                                      for (v, b), t in zip(self.NEAR_CANDIDATES, m[:nn])},
+                             m2l_ms={str(g): round(float(t), 4) for g, t in zip(self.M2L_CANDIDATES, m[nn:])})
         self.tune_choice = choice
         return out
 

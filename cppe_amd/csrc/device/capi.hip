@@ -574,7 +574,8 @@ int cppe_hip_set_tune(cppe_hip_system* sys, const int* choice) {
   if (sys->S.opts.summation != CPPE_HIP_SUM_FMM) return fail(CPPE_HIP_ERR_UNSUPPORTED, "autotuning covers the tree path only");
   for (int prec = 0; prec < 2; ++prec) {
     const int v = choice[3 * prec], b = choice[3 * prec + 1], g = choice[3 * prec + 2];
-    if ((v != 0 && v != 1) || (b != 64 && b != 128 && b != 256) || (g != 16 && g != 32 && g != 64))
+    if (v < 0 || v > 7 || (b != 64 && b != 128 && b != 256) || (v == 2 && b != 64) ||
+        (g != 16 && g != 32 && g != 64))
       return fail(CPPE_HIP_ERR_ARG, "invalid launch shape");
   }
   fmm_set_tune(sys->S, choice);

@@ -2,6 +2,9 @@
 #define CPPE_FMMOPS_ONLY_P 0  // no operator code in the dispatch unit
 #include "fmm_impl.hh"
 
+#include <algorithm>
+#include <vector>
+
 namespace cppe_dev {
 
 // instantiated per order in fmm_order.hip
@@ -79,6 +82,89 @@ void fmm_static(System& S, const double* q, const double* mu, const double* th, 
 #undef CALL
 }
 
+// Target clusters for the cluster near-field kernel: leaves in device order packed greedily into
+// runs of <= 64 sites (a leaf with more sites is split into 64-site chunks); per cluster, the
+// union of its leaves' near lists in memory order of the source leaves, each with the mask of
+// lanes (targets) for which it is near.
+void ensure_clusters(System& S) {
+  FmmState& F     = *S.fmm;
+  const Octree& T = S.tree;
+  if (F.n_clusters > 0) return;
+  hipStream_t st = S.stream;
+  const int nc   = T.ncells;
+  std::vector<int> ids(T.nleaves), start(nc), count(nc), off(nc + 1), src(std::max<long long>(T.n_p2p, 1));
+  CPPE_HIP_CHECK(hipMemcpyAsync(ids.data(), T.leaf_ids.p, sizeof(int) * T.nleaves, hipMemcpyDeviceToHost, st));
+  CPPE_HIP_CHECK(hipMemcpyAsync(start.data(), T.start.p, sizeof(int) * nc, hipMemcpyDeviceToHost, st));
+  CPPE_HIP_CHECK(hipMemcpyAsync(count.data(), T.count.p, sizeof(int) * nc, hipMemcpyDeviceToHost, st));
+  CPPE_HIP_CHECK(hipMemcpyAsync(off.data(), T.p2p_off.p, sizeof(int) * (nc + 1), hipMemcpyDeviceToHost, st));
+  if (T.n_p2p)
+    CPPE_HIP_CHECK(hipMemcpyAsync(src.data(), T.p2p_src.p, sizeof(int) * T.n_p2p, hipMemcpyDeviceToHost, st));
+  CPPE_HIP_CHECK(hipStreamSynchronize(st));
+  std::sort(ids.begin(), ids.end(), [&](int a, int b) { return start[a] < start[b]; });
+  std::vector<int2> ranges;
+  std::vector<int> coff{0}, csrc;
+  std::vector<unsigned long long> cmask;
+  std::vector<int> slot(nc, -1);
+  struct Member { int leaf, lane0, n; };
+  std::vector<Member> cur;
+  int cur_first = 0, cur_n = 0;
+  std::vector<std::pair<int, unsigned long long>> uni;  // (source leaf, mask) of the open cluster
+  auto flush = [&]() {
+    if (cur.empty()) return;
+    uni.clear();
+    for (const Member& m : cur) {
+      const unsigned long long bits = (m.n >= 64 ? ~0ull : ((1ull << m.n) - 1)) << m.lane0;
+      for (int k = off[m.leaf]; k < off[m.leaf + 1]; ++k) {
+        const int B = src[k];
+        if (slot[B] < 0) {
+          slot[B] = (int)uni.size();
+          uni.push_back({B, 0ull});
+        }
+        uni[slot[B]].second |= bits;
+      }
+    }
+    for (auto& e : uni) slot[e.first] = -1;
+    std::sort(uni.begin(), uni.end(), [&](const auto& x, const auto& y) { return start[x.first] < start[y.first]; });
+    for (auto& e : uni) {
+      csrc.push_back(e.first);
+      cmask.push_back(e.second);
+    }
+    coff.push_back((int)csrc.size());
+    ranges.push_back(make_int2(cur_first, cur_n));
+    cur.clear();
+    cur_n = 0;
+  };
+  for (int a : ids) {
+    const int ca = count[a];
+    if (ca > 64) {  // chunks of a large leaf, each a cluster of its own
+      flush();
+      for (int t0 = 0; t0 < ca; t0 += 64) {
+        cur_first = start[a] + t0;
+        cur_n     = std::min(64, ca - t0);
+        cur.push_back({a, 0, cur_n});
+        flush();
+      }
+      continue;
+    }
+    if (cur_n + ca > 64) flush();
+    if (cur.empty()) cur_first = start[a];
+    cur.push_back({a, cur_n, ca});
+    cur_n += ca;
+  }
+  flush();
+  auto up = [&](auto& b, const auto& h) {
+    b.alloc(std::max<size_t>(h.size(), 1));
+    if (!h.empty())
+      CPPE_HIP_CHECK(hipMemcpyAsync(b.p, h.data(), sizeof(h[0]) * h.size(), hipMemcpyHostToDevice, st));
+  };
+  up(F.cl_range, ranges);
+  up(F.cl_off, coff);
+  up(F.cl_src, csrc);
+  up(F.cl_mask, cmask);
+  CPPE_HIP_CHECK(hipStreamSynchronize(st));
+  F.n_clusters = (int)ranges.size();
+}
+
 // Allocates every buffer an FMM B apply touches (both precisions), so that applies can be
 // captured into a graph.
 void fmm_reserve(System& S) {
@@ -93,18 +179,33 @@ void fmm_reserve(System& S) {
   ensure_buf(F.mu4f, S.n);
   ensure_pos4(S);
   ensure_posf4(S);
-  ensure_near_items(S);
+  ensure_near_items(S, 0);
+  for (int prec = 0; prec < 2; ++prec) {
+    const int v = F.tune.near_variant[prec];
+    if (v == 3 || v >= 5) build_near_expanded(S);
+    if (v >= 5) ensure_near_items(S, v - 5);
+  }
+  if (F.tune.near_variant[0] == 4 || F.tune.near_variant[1] == 4) ensure_clusters(S);
   CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
 }
 
 // Autotuner of the B-apply launch shapes: for each precision, every near-field candidate
 // (variant x block size) and every M2L group width is timed on this system's own tree with HIP
 // events around the kernel (mean of reps applies after one warm-up), and the fastest is kept.
+// Near-field variants: 0 thread per target, 1 wavefront per leaf chunk with a uniform source
+// walk, 2 wavefront per leaf with LDS-staged sources, 3 wavefront per leaf chunk with (target,
+// source slot) lanes over the expanded near list, 4 wavefront per target cluster (<= 64 sites of
+// consecutive leaves) over the union of their near lists with lane masks, 5 wavefront per leaf
+// chunk (5: 64 targets, 6: 32, 7: 16), several per block, sources of the expanded near list staged
+// through wave-private LDS.
 // ms receives the timings: per precision (FP64 then FP32) kNearCand near values then kM2lCand
 // M2L values; choice receives (near variant, near block, m2l group) per precision.
-static const int kNearCand[][2] = {{0, 64}, {0, 128}, {0, 256}, {1, 64}, {1, 128}, {1, 256}};
+static const int kNearCand[][2] = {{0, 64}, {0, 128}, {0, 256}, {1, 64}, {1, 128}, {1, 256},
+                                   {2, 64}, {3, 64},  {3, 128}, {3, 256}, {4, 64}, {4, 128},
+                                   {4, 256}, {5, 64}, {5, 128}, {5, 256}, {6, 64}, {6, 128},
+                                   {6, 256}, {7, 64}, {7, 128}, {7, 256}};
 static const int kM2lCand[]     = {16, 32, 64};
-constexpr int kNNear = 6, kNM2l = 3;
+constexpr int kNNear = 22, kNM2l = 3;
 
 void fmm_autotune(System& S, int reps, int* choice, double* ms) {
   fmm_reserve(S);

@@ -38,8 +38,15 @@ struct FmmTune {
 struct FmmState {
   int order = 0;
   FmmTune tune;
-  dbuf<int2> near_items;  // (leaf, first target) of each 64-target chunk of every leaf
-  int n_items = 0;
+  // work items (leaf, first target) of the leaf-chunk near kernels, for chunks of 64, 32, 16 targets
+  dbuf<int2> near_items[3];
+  int n_items[3] = {0, 0, 0};
+  // target clusters: runs of whole leaves (device order) of <= 64 sites, and per cluster the
+  // union of its leaves' near lists with, per source leaf, the mask of lanes it is near for
+  int n_clusters = 0;
+  dbuf<int2> cl_range;               // (first site, number of sites)
+  dbuf<int> cl_off, cl_src;          // CSR: n_clusters+1 offsets, source leaves
+  dbuf<unsigned long long> cl_mask;  // lane masks
   dbuf<float4> posf4;     // FP32 source records: offset from the site's leaf centre, alpha_iso^(-1/6)
   dbuf<double> M64, L64;  // dipole kind
   dbuf<float> M32, L32;
@@ -582,6 +589,315 @@ __global__ __launch_bounds__(256) void k_fmm_near_lw(NearArgs a, const int2* __r
   a.out[2 * (size_t)n + i] = Fz;
 }
 
+// Near field, one wavefront per 64-target chunk of a leaf, lanes split as (target t, source slot
+// s): T = targets of the chunk, S = 64 / T slots, so up to T*S lanes evaluate a pair each step.
+// The sources are the leaf's expanded near list (the concatenated site ranges of its near leaves,
+// built once per tree); slot s takes positions s, s + S, ... of it, so consecutive slots read
+// consecutive records. Slot partials are summed through LDS at the end. The pair arithmetic is
+// that of the thread-per-target kernel (FP32 form: displacement formed in FP64, then rounded).
+template <int P, typename R, bool DAMP, int MODE, typename MU4>
+__global__ __launch_bounds__(256) void k_fmm_near_sl(NearArgs a, const int2* __restrict__ items, int nitems,
+                                                     const long long* __restrict__ xoff,
+                                                     const int* __restrict__ xsrc, const MU4* __restrict__ mu4,
+                                                     const R* __restrict__ L) {
+  using O = Ops<KIND_M1, P, R>;
+  __shared__ R red[3][256];
+  const int wl   = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int w    = blockIdx.x * (blockDim.x >> 6) + wl;
+  const bool wv  = w < nitems;
+  int A = 0, t0 = 0, T = 1;
+  if (wv) {
+    const int2 item = items[w];
+    A               = item.x;
+    t0              = item.y;
+    T               = min(64, a.count[A] - t0);
+  }
+  const int S    = 64 / T;
+  const int t    = lane % T, sl = lane / T;
+  const bool act = wv && sl < S;
+  const int i    = a.start[A] + t0 + t;
+  R fx = R(0), fy = R(0), fz = R(0);
+  double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (wv) pi = a.pos4[i];
+  if (act) {
+    const R di = DAMP ? (R)(a.damping * pi.w) : R(0);
+    int lo = i, hi = i;
+    if (a.eoff)
+      for (int k = a.eoff[i]; k < a.eoff[i + 1]; ++k) {
+        const int e = a.eidx[k];
+        if (e >= 0) {
+          lo = min(lo, e);
+          hi = max(hi, e);
+        }
+      }
+    const long long q1 = xoff[A + 1];
+    for (long long q = xoff[A] + sl; q < q1; q += S) {
+      const int j       = xsrc[q];
+      const double4 pj  = a.pos4[j];
+      const MU4 mj      = mu4[j];
+      if (j >= lo && j <= hi && excluded(i, j, a.eoff, a.eidx)) continue;
+      if constexpr (sizeof(R) == 8) {
+        double hx = 0.0, hy = 0.0, hz = 0.0;
+        pair_m1_f64<DAMP>(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, (double)mj.x, (double)mj.y, (double)mj.z,
+                          (double)di * pj.w, hx, hy, hz);
+        fx += hx;
+        fy += hy;
+        fz += hz;
+      } else {
+        pair_m1_f32<DAMP>((float)(pi.x - pj.x), (float)(pi.y - pj.y), (float)(pi.z - pj.z), (float)mj.x,
+                          (float)mj.y, (float)mj.z, di * (float)pj.w, fx, fy, fz);
+      }
+    }
+  }
+  red[0][threadIdx.x] = fx;
+  red[1][threadIdx.x] = fy;
+  red[2][threadIdx.x] = fz;
+  __syncthreads();
+  if (!wv || lane >= T) return;
+  const int base = wl * 64 + lane;
+  double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+  for (int s2 = 0; s2 < S; ++s2) {
+    Fx += (double)red[0][base + T * s2];
+    Fy += (double)red[1][base + T * s2];
+    Fz += (double)red[2][base + T * s2];
+  }
+  R lx = R(0), ly = R(0), lz = R(0);
+  O::l2p((R)(pi.x - a.cx[A]), (R)(pi.y - a.cy[A]), (R)(pi.z - a.cz[A]), &L[(size_t)A * O::LS], lx, ly, lz);
+  Fx += (double)lx;
+  Fy += (double)ly;
+  Fz += (double)lz;
+  const int n = a.n;
+  if (MODE == MODE_BAPPLY) {
+    const double m0 = a.src[i], m1 = a.src[n + i], m2 = a.src[2 * (size_t)n + i];
+    const double* ai = a.ainv;
+    const double a0 = ai[i], a1 = ai[n + i], a2 = ai[2 * (size_t)n + i], a3 = ai[3 * (size_t)n + i],
+                 a4 = ai[4 * (size_t)n + i], a5 = ai[5 * (size_t)n + i];
+    Fx += a0 * m0 + a1 * m1 + a2 * m2;
+    Fy += a1 * m0 + a3 * m1 + a4 * m2;
+    Fz += a2 * m0 + a4 * m1 + a5 * m2;
+  }
+  a.out[i]                 = Fx;
+  a.out[n + i]             = Fy;
+  a.out[2 * (size_t)n + i] = Fz;
+}
+
+// Near field over target clusters: one wavefront per cluster of whole leaves (<= 64 consecutive
+// target sites, one per lane). The cluster walks the union of its leaves' near lists; each entry
+// carries the mask of lanes for which that source leaf is near (the others take it through M2L
+// and skip it). The walk is uniform across the wavefront, so source records arrive through
+// scalar loads and lanes differ only in their target. Lanes whose exclusion range touches the
+// source leaf take a checked loop; all others a loop without tests. FP32 form: target offsets
+// from the source leaf centre are formed in FP64 and rounded once per source leaf; sources carry
+// FP32 offsets from their own leaf centre.
+template <typename R, bool DAMP, typename MU4, typename POSR, bool CHECK>
+__device__ __forceinline__ void cluster_leaf(int i, int lo, int hi, int j0, int j1, R tx, R ty, R tz, R di,
+                                             const POSR* __restrict__ posr, const MU4* __restrict__ mu4,
+                                             const int* __restrict__ eoff, const int* __restrict__ eidx,
+                                             R& fx, R& fy, R& fz) {
+  for (int j = j0; j < j1; ++j) {
+    const POSR pj = posr[j];
+    const MU4 mj  = mu4[j];
+    if (CHECK && j >= lo && j <= hi && excluded(i, j, eoff, eidx)) continue;
+    if constexpr (sizeof(R) == 8) {
+      pair_m1_f64<DAMP>(tx - pj.x, ty - pj.y, tz - pj.z, mj.x, mj.y, mj.z, di * pj.w, fx, fy, fz);
+    } else {
+      pair_m1_f32<DAMP>(tx - pj.x, ty - pj.y, tz - pj.z, mj.x, mj.y, mj.z, di * pj.w, fx, fy, fz);
+    }
+  }
+}
+
+template <int P, typename R, bool DAMP, int MODE, typename MU4, typename POSR>
+__global__ __launch_bounds__(256) void k_fmm_near_cl(NearArgs a, int ncl, const int2* __restrict__ range,
+                                                     const int* __restrict__ uoff, const int* __restrict__ usrc,
+                                                     const unsigned long long* __restrict__ umask,
+                                                     const POSR* __restrict__ posr, const MU4* __restrict__ mu4,
+                                                     const R* __restrict__ L) {
+  using O        = Ops<KIND_M1, P, R>;
+  const int g    = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  if (g >= ncl) return;  // uniform per wavefront
+  const int2 rg   = range[g];
+  const int g0    = __builtin_amdgcn_readfirstlane(rg.x);
+  const int nt    = __builtin_amdgcn_readfirstlane(rg.y);
+  const bool act  = lane < nt;
+  const int i     = g0 + (act ? lane : 0);
+  const double4 pi = a.pos4[i];
+  const R di       = DAMP ? (R)(a.damping * pi.w) : R(0);
+  int lo = i, hi = i;
+  if (a.eoff)
+    for (int k = a.eoff[i]; k < a.eoff[i + 1]; ++k) {
+      const int e = a.eidx[k];
+      if (e >= 0) {
+        lo = min(lo, e);
+        hi = max(hi, e);
+      }
+    }
+  const unsigned long long bit = 1ull << lane;
+  R fx = R(0), fy = R(0), fz = R(0);
+  const int k1 = uoff[g + 1];
+  for (int k = uoff[g]; k < k1; ++k) {
+    const int B = usrc[k];
+    if (!act || !(umask[k] & bit)) continue;
+    const int j0 = a.start[B], j1 = j0 + a.count[B];
+    R tx, ty, tz;
+    if constexpr (sizeof(R) == 8) {
+      tx = pi.x;
+      ty = pi.y;
+      tz = pi.z;
+    } else {
+      tx = (R)(pi.x - a.cx[B]);
+      ty = (R)(pi.y - a.cy[B]);
+      tz = (R)(pi.z - a.cz[B]);
+    }
+    if (lo < j1 && hi >= j0)
+      cluster_leaf<R, DAMP, MU4, POSR, true>(i, lo, hi, j0, j1, tx, ty, tz, di, posr, mu4, a.eoff, a.eidx, fx, fy,
+                                             fz);
+    else
+      cluster_leaf<R, DAMP, MU4, POSR, false>(i, lo, hi, j0, j1, tx, ty, tz, di, posr, mu4, a.eoff, a.eidx, fx,
+                                              fy, fz);
+  }
+  if (!act) return;
+  const int A = a.leaf_of[i];
+  R lx = R(0), ly = R(0), lz = R(0);
+  O::l2p((R)(pi.x - a.cx[A]), (R)(pi.y - a.cy[A]), (R)(pi.z - a.cz[A]), &L[(size_t)A * O::LS], lx, ly, lz);
+  double Fx = (double)fx + (double)lx, Fy = (double)fy + (double)ly, Fz = (double)fz + (double)lz;
+  const int n = a.n;
+  if (MODE == MODE_BAPPLY) {
+    const double m0 = a.src[i], m1 = a.src[n + i], m2 = a.src[2 * (size_t)n + i];
+    const double* ai = a.ainv;
+    const double a0 = ai[i], a1 = ai[n + i], a2 = ai[2 * (size_t)n + i], a3 = ai[3 * (size_t)n + i],
+                 a4 = ai[4 * (size_t)n + i], a5 = ai[5 * (size_t)n + i];
+    Fx += a0 * m0 + a1 * m1 + a2 * m2;
+    Fy += a1 * m0 + a3 * m1 + a4 * m2;
+    Fz += a2 * m0 + a4 * m1 + a5 * m2;
+  }
+  a.out[i]                 = Fx;
+  a.out[n + i]             = Fy;
+  a.out[2 * (size_t)n + i] = Fz;
+}
+
+// Orders LDS accesses of one wavefront (no cross-wavefront synchronisation).
+__device__ __forceinline__ void wave_lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS source record of the staged near-field kernel: position (FP32: offset from the target
+// leaf centre; FP64: absolute), alpha_iso^(-1/6), dipole, site index.
+template <typename R>
+struct StagedRec {
+  R x, y, z, s;
+  R mx, my, mz;
+  int j;
+};
+
+// Near field, one wavefront per leaf chunk of at most `chunk` (64, 32 or 16) targets, several
+// wavefronts per block, each with its own LDS region and no block-wide barrier. Lanes are (target
+// t, source slot s): T = targets of the chunk, S = 64 / T; smaller chunks spread a large leaf over
+// several wavefronts with more slots each. The leaf's expanded near list is consumed 64 sources at a time: each
+// lane gathers one source record into LDS (coalesced index load), then slot s evaluates sources
+// s, s + S, ... of the chunk against its target. FP32 form: positions staged as offsets from the
+// target leaf centre, formed in FP64 and rounded. Slot partials are summed through LDS.
+template <int P, typename R, bool DAMP, int MODE, typename MU4>
+__global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __restrict__ items, int nitems,
+                                                     int chunk, const long long* __restrict__ xoff,
+                                                     const int* __restrict__ xsrc, const MU4* __restrict__ mu4,
+                                                     const R* __restrict__ L) {
+  using O = Ops<KIND_M1, P, R>;
+  __shared__ StagedRec<R> rec[4][64];
+  __shared__ R red[3][4][64];
+  const int wl   = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int w    = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + wl);
+  if (w >= nitems) return;  // uniform per wavefront; no block-wide barrier follows
+  const int2 item = items[w];
+  const int A     = __builtin_amdgcn_readfirstlane(item.x);
+  const int t0    = __builtin_amdgcn_readfirstlane(item.y);
+  const int T     = min(chunk, a.count[A] - t0);
+  const int S     = 64 / T;
+  const int t = lane % T, sl = lane / T;
+  const bool act = sl < S;
+  const int i    = a.start[A] + t0 + t;
+  const double4 pi = a.pos4[i];
+  const double cax = a.cx[A], cay = a.cy[A], caz = a.cz[A];
+  const R xi = sizeof(R) == 8 ? (R)pi.x : (R)(pi.x - cax);
+  const R yi = sizeof(R) == 8 ? (R)pi.y : (R)(pi.y - cay);
+  const R zi = sizeof(R) == 8 ? (R)pi.z : (R)(pi.z - caz);
+  const R di = DAMP ? (R)(a.damping * pi.w) : R(0);
+  int lo = i, hi = i;
+  if (a.eoff)
+    for (int k = a.eoff[i]; k < a.eoff[i + 1]; ++k) {
+      const int e = a.eidx[k];
+      if (e >= 0) {
+        lo = min(lo, e);
+        hi = max(hi, e);
+      }
+    }
+  StagedRec<R>* sr = rec[wl];
+  R fx = R(0), fy = R(0), fz = R(0);
+  const long long q1 = xoff[A + 1];
+  for (long long q0 = xoff[A]; q0 < q1; q0 += 64) {
+    const int cn = (int)min<long long>(64, q1 - q0);
+    if (lane < cn) {
+      const int j      = xsrc[q0 + lane];
+      const double4 pj = a.pos4[j];
+      const MU4 mj     = mu4[j];
+      StagedRec<R> r;
+      r.x  = sizeof(R) == 8 ? (R)pj.x : (R)(pj.x - cax);
+      r.y  = sizeof(R) == 8 ? (R)pj.y : (R)(pj.y - cay);
+      r.z  = sizeof(R) == 8 ? (R)pj.z : (R)(pj.z - caz);
+      r.s  = (R)pj.w;
+      r.mx = (R)mj.x;
+      r.my = (R)mj.y;
+      r.mz = (R)mj.z;
+      r.j  = j;
+      sr[lane] = r;
+    }
+    wave_lds_barrier();
+    if (act)
+      for (int u = sl; u < cn; u += S) {
+        const StagedRec<R> r = sr[u];
+        if (r.j >= lo && r.j <= hi && excluded(i, r.j, a.eoff, a.eidx)) continue;
+        if constexpr (sizeof(R) == 8)
+          pair_m1_f64<DAMP>(xi - r.x, yi - r.y, zi - r.z, r.mx, r.my, r.mz, di * r.s, fx, fy, fz);
+        else
+          pair_m1_f32<DAMP>(xi - r.x, yi - r.y, zi - r.z, r.mx, r.my, r.mz, di * r.s, fx, fy, fz);
+      }
+    wave_lds_barrier();
+  }
+  red[0][wl][lane] = act ? fx : R(0);
+  red[1][wl][lane] = act ? fy : R(0);
+  red[2][wl][lane] = act ? fz : R(0);
+  wave_lds_barrier();
+  if (lane >= T) return;
+  double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+  for (int s2 = 0; s2 < S; ++s2) {
+    Fx += (double)red[0][wl][lane + T * s2];
+    Fy += (double)red[1][wl][lane + T * s2];
+    Fz += (double)red[2][wl][lane + T * s2];
+  }
+  R lx = R(0), ly = R(0), lz = R(0);
+  O::l2p((R)(pi.x - cax), (R)(pi.y - cay), (R)(pi.z - caz), &L[(size_t)A * O::LS], lx, ly, lz);
+  Fx += (double)lx;
+  Fy += (double)ly;
+  Fz += (double)lz;
+  const int n = a.n;
+  if (MODE == MODE_BAPPLY) {
+    const double m0 = a.src[i], m1 = a.src[n + i], m2 = a.src[2 * (size_t)n + i];
+    const double* ai = a.ainv;
+    const double a0 = ai[i], a1 = ai[n + i], a2 = ai[2 * (size_t)n + i], a3 = ai[3 * (size_t)n + i],
+                 a4 = ai[4 * (size_t)n + i], a5 = ai[5 * (size_t)n + i];
+    Fx += a0 * m0 + a1 * m1 + a2 * m2;
+    Fy += a1 * m0 + a3 * m1 + a4 * m2;
+    Fz += a2 * m0 + a4 * m1 + a5 * m2;
+  }
+  a.out[i]                 = Fx;
+  a.out[n + i]             = Fy;
+  a.out[2 * (size_t)n + i] = Fz;
+}
+
 // FP32 source records: offset of each site from its leaf centre, and alpha_iso^(-1/6).
 static __global__ void k_pack_posf4(int n, const double* __restrict__ x, const double* __restrict__ y,
                                     const double* __restrict__ z, const double* __restrict__ s,
@@ -705,11 +1021,13 @@ inline void ensure_pos4(System& S) {
                      F.pos4.p);
 }
 
-// Work items of the leaf-wave near kernel: (leaf, first target) per 64-target chunk of a leaf.
-inline void ensure_near_items(System& S) {
+// Work items of the leaf-chunk near kernels: (leaf, first target) per chunk of at most C targets of
+// every leaf (C = 64 >> ci). Returns the item count.
+inline int ensure_near_items(System& S, int ci = 0) {
   FmmState& F     = *S.fmm;
   const Octree& T = S.tree;
-  if (F.near_items.p && F.n_items > 0) return;
+  if (F.near_items[ci].p && F.n_items[ci] > 0) return F.n_items[ci];
+  const int C = 64 >> ci;
   std::vector<int> ids(T.nleaves), cnt(T.ncells);
   CPPE_HIP_CHECK(hipMemcpyAsync(ids.data(), T.leaf_ids.p, sizeof(int) * T.nleaves, hipMemcpyDeviceToHost, S.stream));
   CPPE_HIP_CHECK(hipMemcpyAsync(cnt.data(), T.count.p, sizeof(int) * T.ncells, hipMemcpyDeviceToHost, S.stream));
@@ -717,12 +1035,13 @@ inline void ensure_near_items(System& S) {
   std::vector<int2> items;
   items.reserve(T.nleaves);
   for (int c : ids)
-    for (int t0 = 0; t0 < cnt[c]; t0 += 64) items.push_back(make_int2(c, t0));
-  F.near_items.alloc(items.size());
-  CPPE_HIP_CHECK(hipMemcpyAsync(F.near_items.p, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice,
+    for (int t0 = 0; t0 < cnt[c]; t0 += C) items.push_back(make_int2(c, t0));
+  F.near_items[ci].alloc(items.size());
+  CPPE_HIP_CHECK(hipMemcpyAsync(F.near_items[ci].p, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice,
                                 S.stream));
   CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
-  F.n_items = (int)items.size();
+  F.n_items[ci] = (int)items.size();
+  return F.n_items[ci];
 }
 
 inline void ensure_posf4(System& S) {
@@ -734,24 +1053,55 @@ inline void ensure_posf4(System& S) {
                      T.leaf_of.p, T.cx.p, T.cy.p, T.cz.p, F.posf4.p);
 }
 
+void ensure_clusters(System& S);  // fmm.hip
+
 template <int P, typename R, int MODE, bool DAMP, typename MU4>
 void launch_near(System& S, const NearArgs& a, const MU4* mu4, const R* L) {
   FmmState& F     = *S.fmm;
   const int prec  = sizeof(R) == 4;
   const int block = F.tune.near_block[prec];
+  const int var   = F.tune.near_variant[prec];
   hipStream_t st  = S.stream;
-  if (F.tune.near_variant[prec] == 1) {
-    ensure_near_items(S);
-    const int waves = block / 64;
-    const dim3 g(ceil_div(F.n_items, waves)), b(block);
-    if constexpr (sizeof(R) == 8)
-      hipLaunchKernelGGL((k_fmm_near_lw<P, R, DAMP, MODE, MU4, double4>), g, b, 0, st, a, F.near_items.p, F.n_items,
-                         F.pos4.p, mu4, L);
-    else {
+  if (var >= 5) {  // 5, 6, 7: staged, chunks of 64, 32, 16 targets
+    const int ci = var - 5;
+    const int ni = ensure_near_items(S, ci);
+    build_near_expanded(S);
+    const Octree& T = S.tree;
+    const dim3 g(ceil_div(ni, block / 64)), b(block);
+    hipLaunchKernelGGL((k_fmm_near_st<P, R, DAMP, MODE, MU4>), g, b, 0, st, a, F.near_items[ci].p, ni, 64 >> ci,
+                       T.xoff.p, T.xsrc.p, mu4, L);
+  } else if (var == 4) {
+    ensure_clusters(S);
+    const dim3 g(ceil_div(F.n_clusters, block / 64)), b(block);
+    if constexpr (sizeof(R) == 8) {
+      hipLaunchKernelGGL((k_fmm_near_cl<P, R, DAMP, MODE, MU4, double4>), g, b, 0, st, a, F.n_clusters, F.cl_range.p,
+                         F.cl_off.p, F.cl_src.p, F.cl_mask.p, F.pos4.p, mu4, L);
+    } else {
       ensure_posf4(S);
-      hipLaunchKernelGGL((k_fmm_near_lw<P, R, DAMP, MODE, MU4, float4>), g, b, 0, st, a, F.near_items.p, F.n_items,
+      hipLaunchKernelGGL((k_fmm_near_cl<P, R, DAMP, MODE, MU4, float4>), g, b, 0, st, a, F.n_clusters, F.cl_range.p,
+                         F.cl_off.p, F.cl_src.p, F.cl_mask.p, F.posf4.p, mu4, L);
+    }
+  } else if (var == 1 || var == 3) {
+    const int ni    = ensure_near_items(S, 0);
+    const int waves = block / 64;
+    const dim3 g(ceil_div(ni, waves)), b(block);
+    if (var == 3) {
+      build_near_expanded(S);
+      const Octree& T = S.tree;
+      hipLaunchKernelGGL((k_fmm_near_sl<P, R, DAMP, MODE, MU4>), g, b, 0, st, a, F.near_items[0].p, ni,
+                         T.xoff.p, T.xsrc.p, mu4, L);
+    } else if constexpr (sizeof(R) == 8) {
+      hipLaunchKernelGGL((k_fmm_near_lw<P, R, DAMP, MODE, MU4, double4>), g, b, 0, st, a, F.near_items[0].p, ni,
+                         F.pos4.p, mu4, L);
+    } else {
+      ensure_posf4(S);
+      hipLaunchKernelGGL((k_fmm_near_lw<P, R, DAMP, MODE, MU4, float4>), g, b, 0, st, a, F.near_items[0].p, ni,
                          F.posf4.p, mu4, L);
     }
+  } else if (var == 2) {
+    // LDS-staged kernel (one wavefront per leaf); its sources are the SoA dipoles
+    P2PArgs pa = make_p2p_args(S, a.src, a.damping, a.out);
+    hipLaunchKernelGGL((k_fmm_p2p<KIND_M1, P, R, DAMP, MODE>), dim3(S.tree.nleaves), dim3(64), 0, st, pa, L);
   } else {
     hipLaunchKernelGGL((k_fmm_near<P, R, DAMP, MODE, MU4>), dim3(ceil_div(S.n, block)), dim3(block), 0, st, a, mu4, L);
   }

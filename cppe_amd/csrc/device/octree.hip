@@ -550,6 +550,53 @@ static void sort_keys(dbuf<unsigned long long>& keys, long long n, int ncells, h
   std::swap(keys.n, out.n);
 }
 
+// Expanded near lists (one entry per near source site of each target leaf) for the slotted
+// near-field kernel.
+__global__ void k_xcount(int ncells, const int* __restrict__ off, const int* __restrict__ src,
+                         const int* __restrict__ count, long long* __restrict__ xcnt) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  long long t = 0;
+  for (int k = off[c]; k < off[c + 1]; ++k) t += count[src[k]];
+  xcnt[c] = t;
+}
+
+__global__ __launch_bounds__(64) void k_xfill(const int* __restrict__ off, const int* __restrict__ src,
+                                              const int* __restrict__ start, const int* __restrict__ count,
+                                              const long long* __restrict__ xoff, int* __restrict__ xsrc) {
+  const int c  = blockIdx.x;
+  long long pos = xoff[c];
+  for (int k = off[c]; k < off[c + 1]; ++k) {
+    const int B = src[k], j0 = start[B], nb = count[B];
+    for (int u = threadIdx.x; u < nb; u += 64) xsrc[pos + u] = j0 + u;
+    pos += nb;
+  }
+}
+
+void build_near_expanded(System& S) {
+  Octree& T = S.tree;
+  if (T.nx >= 0) return;
+  hipStream_t st = S.stream;
+  const int nc   = T.ncells;
+  dbuf<long long> cnt;
+  cnt.alloc(nc + 1);
+  hipLaunchKernelGGL(k_xcount, dim3(ceil_div(nc, 256)), dim3(256), 0, st, nc, T.p2p_off.p, T.p2p_src.p, T.count.p,
+                     cnt.p);
+  long long zero = 0;
+  CPPE_HIP_CHECK(hipMemcpyAsync(cnt.p + nc, &zero, sizeof(long long), hipMemcpyHostToDevice, st));
+  T.xoff.alloc(nc + 1);
+  dbuf<char> tmp;
+  exclusive_scan(cnt.p, T.xoff.p, nc + 1, st, tmp);
+  long long total = 0;
+  CPPE_HIP_CHECK(hipMemcpyAsync(&total, T.xoff.p + nc, sizeof(long long), hipMemcpyDeviceToHost, st));
+  CPPE_HIP_CHECK(hipStreamSynchronize(st));
+  T.xsrc.alloc(std::max<long long>(total, 1));
+  hipLaunchKernelGGL(k_xfill, dim3(nc), dim3(64), 0, st, T.p2p_off.p, T.p2p_src.p, T.start.p, T.count.p, T.xoff.p,
+                     T.xsrc.p);
+  CPPE_HIP_CHECK(hipStreamSynchronize(st));
+  T.nx = total;
+}
+
 void build_lists(System& S) {
   Octree& T      = S.tree;
   hipStream_t st = S.stream;

@@ -47,6 +47,10 @@ struct Octree {
   long long n_m2l = 0, n_p2p = 0, p2p_site_pairs = 0;
   dbuf<int> m2l_off, m2l_src;  // ncells+1 offsets, sources
   dbuf<int> p2p_off, p2p_src;  // ncells+1 offsets (non-empty only for leaves), source leaves
+  // expanded near lists: per target leaf, the concatenated site ranges of its near leaves
+  long long nx = -1;           // total length (-1: not built)
+  dbuf<long long> xoff;        // ncells+1 offsets
+  dbuf<int> xsrc;              // source sites (device order)
   bool built = false;
 };
 
@@ -109,6 +113,7 @@ struct System {
 // ---- kernels/launchers implemented across the .hip translation units -----------------------
 void build_octree(System& S);
 void build_lists(System& S);
+void build_near_expanded(System& S);
 
 // field of order-m sources given in device-order SoA (ncomp(m) arrays of n) into F (3n SoA)
 void direct_field(System& S, int m, const double* src, double* F, double damping);

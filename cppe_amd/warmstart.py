@@ -63,13 +63,44 @@ def train(n_snapshots=2000, epochs=8, batch=4096, lr=1e-3, log=print):
             log("collected %d/%d snapshots (%.0f s)" % (k + 1, n_snapshots, time.time() - t0))
     M.prepare()
     log("training on %d sites" % int(3 * sizes.sum()))
-    losses = M.train(epochs, batch, lr, seed=1)
-    for e, l in enumerate(losses):
-        log("epoch %d loss %.5e" % (e, l))
+    losses = []
+    for e in range(epochs):  # cosine learning-rate schedule, one epoch per call (Adam state persists)
+        lr_e = lr * (0.05 + 0.95 * 0.5 * (1.0 + np.cos(np.pi * e / max(epochs, 1))))
+        losses += list(M.train(1, batch, float(lr_e), seed=1 + e))
+        log("epoch %d lr %.2e loss %.5e (%.0f s)" % (e, lr_e, losses[-1], time.time() - t0))
     meta = dict(n_snapshots=int(n_snapshots), sites=int(3 * sizes.sum()), epochs=int(epochs), batch=int(batch),
-                lr=float(lr), losses=[float(x) for x in losses], mean_cg_iterations=float(np.mean(iters)),
-                seconds=time.time() - t0)
+                lr=float(lr), lr_schedule="cosine to 5%", losses=[float(x) for x in losses],
+                mean_cg_iterations=float(np.mean(iters)), seconds=time.time() - t0)
+    meta["validation"] = validate(M, log=log)
     return M, meta
+
+
+VALID_SEED0 = 40000
+
+
+def validate(M, n=16, log=print):
+    """Held-out snapshots (seeds outside the training range): dipole error of the learned guess and
+    of alpha F against the FP64 solution, and CG iterations from each guess to 1e-8."""
+    rows = []
+    for k, nw in enumerate(snapshot_sizes(n, seed=VALID_SEED0)):
+        w = synthetic.water_box(int(nw), VALID_SEED0 + 1 + k, anisotropic=True)
+        S, it_aF = solved_system(w)
+        mu = S.download_mu()
+        capi.ml_guess(S, M)
+        g = S.download_mu()
+        it_ml, _, _ = S.solve_resident(1, 1e-8, 500)
+        aF = S.apply_alpha(static_rhs(w))
+        rows.append(dict(sites=int(w.n), err_ml=float(np.linalg.norm(g - mu) / np.linalg.norm(mu)),
+                         err_alphaF=float(np.linalg.norm(aF - mu) / np.linalg.norm(mu)),
+                         iters_alphaF=int(it_aF), iters_ml=int(it_ml)))
+        S.close()
+    out = dict(snapshots=rows, mean_err_ml=float(np.mean([r["err_ml"] for r in rows])),
+               mean_err_alphaF=float(np.mean([r["err_alphaF"] for r in rows])),
+               mean_iters_alphaF=float(np.mean([r["iters_alphaF"] for r in rows])),
+               mean_iters_ml=float(np.mean([r["iters_ml"] for r in rows])))
+    log("validation: relative dipole error %.3e (alpha F %.3e), CG iterations %.2f (alpha F %.2f)"
+        % (out["mean_err_ml"], out["mean_err_alphaF"], out["mean_iters_ml"], out["mean_iters_alphaF"]))
+    return out
 
 
 def save(M, meta, path=DEFAULT_MODEL):

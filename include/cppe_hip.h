@@ -149,11 +149,20 @@ int cppe_hip_set_graphs(cppe_hip_system* sys, int enable);
 /*
  * Autotuner of the tree path's B-apply launch shapes (no counterpart in the reference). For the
  * FP64 and the FP32 apply in turn, times on this system's own tree (HIP events, mean of reps
- * after a warm-up) every near-field candidate -- variant 0 (one thread per target site) or 1
- * (one wavefront per 64-target leaf chunk, uniform source walk) x block 64/128/256 -- then every
- * M2L group width (16/32/64 lanes per target cell), and keeps the fastest of each.
+ * after a warm-up) every near-field candidate, then every M2L group width (16/32/64 lanes per
+ * target cell), and keeps the fastest of each. Near-field candidates (variant, block):
+ *   0: one thread per target site, block 64/128/256;
+ *   1: one wavefront per 64-target leaf chunk, uniform (scalar) source walk, block 64/128/256;
+ *   2: one wavefront per leaf, sources staged through LDS, block 64;
+ *   3: one wavefront per leaf chunk, lanes = (target, source slot) over the leaf's expanded near
+ *      list (built on first use: one int per near source site of each leaf), block 64/128/256;
+ *   4: one wavefront per target cluster (whole consecutive leaves, <= 64 sites) walking the union
+ *      of their near lists with per-source-leaf lane masks, block 64/128/256;
+ *   5, 6, 7: one wavefront per leaf chunk of at most 64, 32, 16 targets, several per block, the
+ *      expanded near list staged 64 sources at a time through wavefront-private LDS, lanes =
+ *      (target, source slot), block 64/128/256.
  * choice[6]: (near variant, near block, m2l group) for FP64 then FP32.
- * ms[18]: per precision, the 6 near timings then the 3 M2L timings (ms per launch).
+ * ms[50]: per precision, the 22 near timings (order above) then the 3 M2L timings (ms/launch).
  * Launch shapes never change results beyond FP summation order.
  */
 int cppe_hip_autotune(cppe_hip_system* sys, int reps, int* choice, double* ms);
