@@ -78,6 +78,7 @@ System::~System() {
   cg_graphs_release(*this);
   if (h_status) (void)hipHostFree(h_status);
   if (fmm) fmm_destroy(*this);
+  if (ml_grid) ml_grid_destroy(*this);
   if (stream) (void)hipStreamDestroy(stream);
 }
 

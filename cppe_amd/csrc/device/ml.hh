@@ -31,6 +31,29 @@ struct Mlp {
   ~Mlp();
 };
 
+struct GridInfo {
+  double x0, y0, z0;  // lower corner
+  int nx, ny, nz;
+};
+
+// Neighbour grid of one system for the descriptors, built on first use (positions never change),
+// and the guess's work buffers.
+struct MlGrid {
+  GridInfo g;
+  dbuf<int> cstart, sorted;
+  dbuf<unsigned> cellof;
+  // neighbour list within rc (CSR over sites, device order), geometry only
+  long long npairs = 0;
+  dbuf<long long> noff;  // n + 1
+  dbuf<int> nj;          // neighbour index
+  dbuf<float4> ngeo;     // (r, unit vector i -> j)
+  dbuf<double> nk;       // 6 per pair: fc(r) T2d_ij packed xx,xy,xz,yy,yz,zz (0 for excluded pairs)
+  dbuf<float> iso;       // alpha_iso of each site
+  dbuf<float> X, y;      // n x 64 features, n x 3 network outputs
+  dbuf<double> V;        // 9n bases
+};
+MlGrid& ml_grid(System& S);
+
 void ml_features(System& S, float* X, double* V);
 void mlp_refresh_bf16(Mlp& M);
 // y (ns x 3) = MLP(normalise(X)) ; X raw FP32 features (device), ns samples

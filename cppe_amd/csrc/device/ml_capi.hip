@@ -357,16 +357,15 @@ int cppe_hip_ml_guess(cppe_hip_system* sys, cppe_hip_mlp* m, double* device_ms) 
   hipEvent_t e0, e1;
   CPPE_HIP_CHECK(hipEventCreate(&e0));
   CPPE_HIP_CHECK(hipEventCreate(&e1));
-  dbuf<float> dX, dy;
-  dbuf<double> dV;
-  dX.alloc((size_t)n * kMlpIn);
-  dy.alloc((size_t)n * 3);
-  dV.alloc(9 * (size_t)n);
+  MlGrid& G = ml_grid(S);  // neighbour grid and work buffers, kept by the system
+  G.X.alloc((size_t)n * kMlpIn);
+  G.y.alloc((size_t)n * 3);
+  G.V.alloc(9 * (size_t)n);
   CPPE_HIP_CHECK(hipEventRecord(e0, st));
-  ml_features(S, dX.p, dV.p);
+  ml_features(S, G.X.p, G.V.p);
   CPPE_HIP_CHECK(hipStreamSynchronize(m->M.stream));
-  mlp_forward(m->M, dX.p, n, dy.p, st);
-  hipLaunchKernelGGL(k_ml_combine, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, dy.p, dV.p, S.v_mu.p);
+  mlp_forward(m->M, G.X.p, n, G.y.p, st);
+  hipLaunchKernelGGL(k_ml_combine, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, G.y.p, G.V.p, S.v_mu.p);
   CPPE_HIP_CHECK(hipEventRecord(e1, st));
   CPPE_HIP_CHECK(hipEventSynchronize(e1));
   float ms = 0.f;

@@ -19,7 +19,13 @@
 //     54    sum_j fc(r),  55 sum_j fc(r) / r^3 (bohr^-3 x 100)
 //     56    (v2 . v1)/|v1|^2,  57 |v2|/|v1|,  58 (v3 . v1)/|v1|^2,  59 |v3|/|v1|
 //     60    (v3 . v2)/(|v2||v3|),  61 log|v1|,  62 (v1 . F)/(|v1||F|),  63 1
-// Neighbours come from a uniform grid of cell size rc built on the device (sort by cell).
+// Neighbours come from a uniform grid of cell size rc/2 built on the device once per system
+// (sort by cell; a neighbourhood is 25 contiguous rows of 5 cells), stored as a neighbour list
+// with each pair's distance, unit vector and damped tensor block fc T2d (geometry only, built
+// once per system like the octree). The descriptors are
+// evaluated in FP32 (the eight radial Gaussians by the recurrence g_{k+1} = g_k A B_k,
+// A = exp(2 eta dr (r - r_0)), B_k = exp(-eta dr^2 (2k + 1)), two exponentials per neighbour),
+// the bases in FP64 (Thole-damped dipole tensor as in the response operator).
 #include <hipcub/hipcub.hpp>
 
 #include "ml.hh"
@@ -27,24 +33,26 @@
 
 namespace cppe_dev {
 
-constexpr double kRc    = 11.338356747600001;  // 6 A in bohr (6 * 1.8897261246)
-constexpr double kRk0   = 1.5;                 // first radial centre (bohr)
-constexpr double kRkStep = 1.4;                // centre spacing (bohr)
-constexpr double kEta   = 0.5;                 // Gaussian width parameter (bohr^-2)
+constexpr double kRc     = 11.338356747600001;  // 6 A in bohr (6 * 1.8897261246)
+constexpr double kRk0    = 1.5;                 // first radial centre (bohr)
+constexpr double kRkStep = 1.4;                 // centre spacing (bohr)
+constexpr double kEta    = 0.5;                 // Gaussian width parameter (bohr^-2)
+constexpr double kCell   = kRc / 2;             // grid cell edge
+constexpr int kReach     = 2;                   // cells scanned on each side
 
-struct GridInfo {
-  double x0, y0, z0;  // lower corner
-  int nx, ny, nz;
-};
+void ml_grid_destroy(System& S) {
+  delete S.ml_grid;
+  S.ml_grid = nullptr;
+}
 
 __global__ void k_grid_cell(int n, const double* __restrict__ x, const double* __restrict__ y,
                             const double* __restrict__ z, GridInfo g, unsigned* __restrict__ cell,
                             int* __restrict__ idx) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  int cx = min(g.nx - 1, max(0, (int)floor((x[i] - g.x0) / kRc)));
-  int cy = min(g.ny - 1, max(0, (int)floor((y[i] - g.y0) / kRc)));
-  int cz = min(g.nz - 1, max(0, (int)floor((z[i] - g.z0) / kRc)));
+  int cx = min(g.nx - 1, max(0, (int)floor((x[i] - g.x0) / kCell)));
+  int cy = min(g.ny - 1, max(0, (int)floor((y[i] - g.y0) / kCell)));
+  int cz = min(g.nz - 1, max(0, (int)floor((z[i] - g.z0) / kCell)));
   cell[i] = (unsigned)((cz * g.ny + cy) * g.nx + cx);
   idx[i]  = i;
 }
@@ -76,23 +84,6 @@ __device__ __forceinline__ void amul(const double a[6], double vx, double vy, do
   oz = a[2] * vx + a[4] * vy + a[5] * vz;
 }
 
-// T2d(d) w with Thole damping (same screening as the response operator): 3 fT d (d.w)/r^5 - fE w/r^3
-__device__ __forceinline__ void t2_apply(double dx, double dy, double dz, double r, double a, double wx,
-                                         double wy, double wz, double& ox, double& oy, double& oz) {
-  const double ri  = 1.0 / r;
-  const double ri3 = ri * ri * ri;
-  double fE = 1.0, fT = 1.0;
-  if (a > 0.0) {
-    const double u = a * r;
-    if (u < kTholeCutoff) thole_ET(u, exp(-u), fE, fT);
-  }
-  const double dw = dx * wx + dy * wy + dz * wz;
-  const double c5 = 3.0 * fT * dw * ri3 * ri * ri;
-  ox              = c5 * dx - fE * ri3 * wx;
-  oy              = c5 * dy - fE * ri3 * wy;
-  oz              = c5 * dz - fE * ri3 * wz;
-}
-
 struct FeatArgs {
   int n;
   const double *x, *y, *z, *s, *a6, *F;  // F: 3n SoA
@@ -107,152 +98,283 @@ struct FeatArgs {
   double* V;           // 9n: v1 (3n SoA), v2, v3
 };
 
-// pass 1: descriptors 0-55, v1, v2.  pass 2: v3, descriptors 56-63.
-template <int PASS>
-__global__ __launch_bounds__(128) void k_ml_features(FeatArgs a) {
+// Visits every site j != i within kRc of site i (rows of contiguous cells of the grid);
+// f(j, dx, dy, dz, r2) with d = r_j - r_i.
+template <typename Fn>
+__device__ __forceinline__ void for_neighbours(const FeatArgs& a, int i, double xi, double yi, double zi, Fn&& f) {
+  const unsigned c = a.cellof[i];
+  const int cx = (int)(c % a.g.nx), cy = (int)((c / a.g.nx) % a.g.ny), cz = (int)(c / ((unsigned)a.g.nx * a.g.ny));
+  const int x0 = max(0, cx - kReach), x1 = min(a.g.nx - 1, cx + kReach);
+  for (int qz = max(0, cz - kReach); qz <= min(a.g.nz - 1, cz + kReach); ++qz)
+    for (int qy = max(0, cy - kReach); qy <= min(a.g.ny - 1, cy + kReach); ++qy) {
+      const int row = (qz * a.g.ny + qy) * a.g.nx;
+      const int t1  = a.cstart[row + x1 + 1];
+      for (int t = a.cstart[row + x0]; t < t1; ++t) {
+        const int j = a.sorted[t];
+        if (j == i) continue;
+        const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 >= kRc * kRc) continue;
+        f(j, dx, dy, dz, r2);
+      }
+    }
+}
+
+// Neighbour list: count, then fill (pairs within rc, j != i; excluded pairs kept with a zero
+// tensor block so that the densities still see them).
+__global__ __launch_bounds__(128) void k_ml_ncount(FeatArgs a, long long* __restrict__ cnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
+  long long c = 0;
+  for_neighbours(a, i, a.x[i], a.y[i], a.z[i], [&](int, double, double, double, double) { ++c; });
+  cnt[i] = c;
+}
+
+// fc(r) T2d(d), packed xx,xy,xz,yy,yz,zz: (3 fT d_a d_b / r^5 - fE delta_ab / r^3) fc
+__device__ __forceinline__ void t2_block(double dx, double dy, double dz, double r, double a, double fc, double* k) {
+  const double ri  = 1.0 / r;
+  const double ri3 = ri * ri * ri;
+  double fE = 1.0, fT = 1.0;
+  if (a > 0.0) {
+    const double u = a * r;
+    if (u < kTholeCutoff) thole_ET(u, exp(-u), fE, fT);
+  }
+  const double c5 = 3.0 * fT * ri3 * ri * ri * fc, c3 = fE * ri3 * fc;
+  k[0] = c5 * dx * dx - c3;
+  k[1] = c5 * dx * dy;
+  k[2] = c5 * dx * dz;
+  k[3] = c5 * dy * dy - c3;
+  k[4] = c5 * dy * dz;
+  k[5] = c5 * dz * dz - c3;
+}
+
+__global__ __launch_bounds__(128) void k_ml_nfill(FeatArgs a, const long long* __restrict__ noff, int* __restrict__ nj,
+                                                  float4* __restrict__ ngeo, double* __restrict__ nk,
+                                                  float* __restrict__ iso) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int n = a.n;
+  iso[i] = (float)((a.a6[i] + a.a6[3 * (size_t)n + i] + a.a6[5 * (size_t)n + i]) / 3.0);
+  const double di = a.damping * a.s[i];
+  const int e0 = a.eoff ? a.eoff[i] : 0, e1 = a.eoff ? a.eoff[i + 1] : 0;
+  long long p = noff[i];
+  for_neighbours(a, i, a.x[i], a.y[i], a.z[i], [&](int j, double dx, double dy, double dz, double r2) {
+    const double r  = sqrt(r2);
+    const double fc = 0.5 * (cos(M_PI * r / kRc) + 1.0);
+    bool ex = false;
+    for (int k = e0; k < e1; ++k) ex |= (a.eidx[k] == j);
+    nj[p]   = j;
+    ngeo[p] = make_float4((float)r, (float)(dx / r), (float)(dy / r), (float)(dz / r));
+    double* k = nk + 6 * p;
+    if (ex) {
+      for (int c = 0; c < 6; ++c) k[c] = 0.0;
+    } else {
+      t2_block(dx, dy, dz, r, di * a.s[j], fc, k);
+    }
+    ++p;
+  });
+}
+
+struct ListArgs {
+  int n;
+  const long long* noff;
+  const int* nj;
+  const float4* ngeo;
+  const double* nk;
+  const float* iso;
+  const double *x, *y, *z, *a6, *F;
+  const int* eoff;
+  const int* eidx;
+  float* X;
+  double* V;
+};
+
+// Lanes per site of the list kernels: consecutive lanes take consecutive pairs of a site's
+// neighbour list (coalesced), partial sums are combined by a fixed xor-shuffle tree.
+constexpr int kDescLanes = 4, kBasisLanes = 8;
+
+template <int G>
+__device__ __forceinline__ float lane_sum(float v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+template <int G>
+__device__ __forceinline__ double lane_sum(double v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Descriptors 0-55 (FP32 arithmetic) over the neighbour list, kDescLanes lanes per site.
+__global__ __launch_bounds__(256) void k_ml_desc(ListArgs a) {
+  const int gid  = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i0   = gid / kDescLanes;
+  const int sub  = gid % kDescLanes;
+  const bool ok  = i0 < a.n;
+  const int i    = ok ? i0 : a.n - 1;  // trailing lanes shadow the last site and store nothing
   const int n = a.n;
   const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
   const double Fx = a.F[i], Fy = a.F[n + i], Fz = a.F[2 * (size_t)n + i];
   const double Fn = sqrt(Fx * Fx + Fy * Fy + Fz * Fz) + 1e-30;
   const double fx = Fx / Fn, fy = Fy / Fn, fz = Fz / Fn;
+  const float ffx = (float)fx, ffy = (float)fy, ffz = (float)fz;
+  float feat[48];
+#pragma unroll
+  for (int k = 0; k < 48; ++k) feat[k] = 0.f;
+  float f54 = 0.f, f55 = 0.f;
+  constexpr float kPiRc = (float)(M_PI / kRc);
+  constexpr float kA    = (float)(2.0 * kEta * kRkStep);
+  const long long p1 = ok ? a.noff[i + 1] : 0;
+  for (long long p = a.noff[i] + sub; p < p1; p += kDescLanes) {
+    const int j     = a.nj[p];
+    const float4 gq = a.ngeo[p];
+    const float r = gq.x, ux = gq.y, uy = gq.z, uz = gq.w;
+    const float ri  = 1.0f / r;
+    const float fc  = 0.5f * (cosf(kPiRc * r) + 1.0f);
+    const float Fjx = (float)a.F[j], Fjy = (float)a.F[n + j], Fjz = (float)a.F[2 * (size_t)n + j];
+    const float Fji = 1.0f / (sqrtf(Fjx * Fjx + Fjy * Fjy + Fjz * Fjz) + 1e-30f);
+    const float iso = a.iso[j];
+    const float cth = ux * ffx + uy * ffy + uz * ffz;
+    const float p2  = 1.5f * cth * cth - 0.5f;
+    const float al  = (Fjx * ffx + Fjy * ffy + Fjz * ffz) * Fji;
+    const float bj  = (ux * Fjx + uy * Fjy + uz * Fjz) * Fji;
+    const float u   = r - (float)kRk0;
+    float g         = __expf(-(float)kEta * u * u) * fc;
+    const float A   = __expf(kA * u);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      feat[k] += g;
+      feat[8 + k] += g * iso * 0.2f;
+      feat[16 + k] += g * cth;
+      feat[24 + k] += g * p2;
+      feat[32 + k] += g * al;
+      feat[40 + k] += g * bj;
+      g *= A * (float)exp(-kEta * kRkStep * kRkStep * (2 * k + 1));
+    }
+    f54 += fc;
+    f55 += fc * ri * ri * ri * 100.0f;
+  }
+#pragma unroll
+  for (int k = 0; k < 48; ++k) feat[k] = lane_sum<kDescLanes>(feat[k]);
+  f54 = lane_sum<kDescLanes>(f54);
+  f55 = lane_sum<kDescLanes>(f55);
+  if (!ok || sub != 0) return;
+  float* Xi = a.X + (size_t)i * kMlpIn;
+#pragma unroll
+  for (int k = 0; k < 48; ++k) Xi[k] = feat[k];
+  // molecular frame from the first two exclusion partners (fallback: field frame)
   double ai[6];
   site_alpha(a.a6, n, i, ai);
   const double iso_i = (ai[0] + ai[3] + ai[5]) / 3.0;
-  const double di    = a.damping * a.s[i];
-  float feat[56];
-  if (PASS == 1)
-    for (int k = 0; k < 56; ++k) feat[k] = 0.f;
-  double sx = 0.0, sy = 0.0, sz = 0.0;  // sum fc T2d w_j
-  const double* W = PASS == 1 ? nullptr : a.V + 3 * (size_t)n;  // v2 for pass 2
-  const unsigned c = a.cellof[i];
-  const int cx = (int)(c % a.g.nx), cy = (int)((c / a.g.nx) % a.g.ny), cz = (int)(c / ((unsigned)a.g.nx * a.g.ny));
   const int e0 = a.eoff ? a.eoff[i] : 0, e1 = a.eoff ? a.eoff[i + 1] : 0;
-  for (int dzc = -1; dzc <= 1; ++dzc)
-    for (int dyc = -1; dyc <= 1; ++dyc)
-      for (int dxc = -1; dxc <= 1; ++dxc) {
-        const int qx = cx + dxc, qy = cy + dyc, qz = cz + dzc;
-        if (qx < 0 || qy < 0 || qz < 0 || qx >= a.g.nx || qy >= a.g.ny || qz >= a.g.nz) continue;
-        const int q = (qz * a.g.ny + qy) * a.g.nx + qx;
-        for (int t = a.cstart[q]; t < a.cstart[q + 1]; ++t) {
-          const int j = a.sorted[t];
-          if (j == i) continue;
-          const double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;  // i -> j
-          const double r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 >= kRc * kRc) continue;
-          const double r  = sqrt(r2);
-          const double fc = 0.5 * (cos(M_PI * r / kRc) + 1.0);
-          bool ex = false;
-          for (int k = e0; k < e1; ++k) ex |= (a.eidx[k] == j);
-          if (PASS == 1) {
-            const double Fjx = a.F[j], Fjy = a.F[n + j], Fjz = a.F[2 * (size_t)n + j];
-            const double Fjn = sqrt(Fjx * Fjx + Fjy * Fjy + Fjz * Fjz) + 1e-30;
-            double aj[6];
-            site_alpha(a.a6, n, j, aj);
-            const double iso_j = (aj[0] + aj[3] + aj[5]) / 3.0;
-            const double cth   = (dx * fx + dy * fy + dz * fz) / r;
-            const double p2    = 1.5 * cth * cth - 0.5;
-            const double al    = (Fjx * fx + Fjy * fy + Fjz * fz) / Fjn;
-            const double bj    = (dx * Fjx + dy * Fjy + dz * Fjz) / (r * Fjn);
-            for (int k = 0; k < 8; ++k) {
-              const double rk = kRk0 + kRkStep * k;
-              const double gk = exp(-kEta * (r - rk) * (r - rk)) * fc;
-              feat[k] += (float)gk;
-              feat[8 + k] += (float)(gk * iso_j * 0.2);
-              feat[16 + k] += (float)(gk * cth);
-              feat[24 + k] += (float)(gk * p2);
-              feat[32 + k] += (float)(gk * al);
-              feat[40 + k] += (float)(gk * bj);
-            }
-            feat[54] += (float)fc;
-            feat[55] += (float)(fc / (r2 * r) * 100.0);
-            if (!ex) {
-              double wx, wy, wz, tx, ty, tz;
-              amul(aj, Fjx, Fjy, Fjz, wx, wy, wz);
-              t2_apply(dx, dy, dz, r, di * a.s[j], wx, wy, wz, tx, ty, tz);
-              sx += fc * tx;
-              sy += fc * ty;
-              sz += fc * tz;
-            }
-          } else if (!ex) {
-            double tx, ty, tz;
-            t2_apply(dx, dy, dz, r, di * a.s[j], W[j], W[n + j], W[2 * (size_t)n + j], tx, ty, tz);
-            sx += fc * tx;
-            sy += fc * ty;
-            sz += fc * tz;
-          }
-        }
-      }
-  double v1x, v1y, v1z;
-  amul(ai, Fx, Fy, Fz, v1x, v1y, v1z);
-  double ox, oy, oz;
+  double e1x = fx, e1y = fy, e1z = fz, e2x = 0, e2y = 0, e2z = 0;
+  if (e1 - e0 >= 2) {
+    const int p = a.eidx[e0], q = a.eidx[e0 + 1];
+    if (p >= 0 && q >= 0) {
+      double ux = a.x[p] - xi, uy = a.y[p] - yi, uz = a.z[p] - zi;
+      double un = sqrt(ux * ux + uy * uy + uz * uz) + 1e-30;
+      e1x = ux / un;
+      e1y = uy / un;
+      e1z = uz / un;
+      double wx = a.x[q] - xi, wy = a.y[q] - yi, wz = a.z[q] - zi;
+      double pr = wx * e1x + wy * e1y + wz * e1z;
+      wx -= pr * e1x;
+      wy -= pr * e1y;
+      wz -= pr * e1z;
+      double wn = sqrt(wx * wx + wy * wy + wz * wz) + 1e-30;
+      e2x = wx / wn;
+      e2y = wy / wn;
+      e2z = wz / wn;
+    }
+  }
+  const double e3x = e1y * e2z - e1z * e2y, e3y = e1z * e2x - e1x * e2z, e3z = e1x * e2y - e1y * e2x;
+  Xi[48] = (float)(fx * e1x + fy * e1y + fz * e1z);
+  Xi[49] = (float)(fx * e2x + fy * e2y + fz * e2z);
+  Xi[50] = (float)(fx * e3x + fy * e3y + fz * e3z);
+  Xi[51] = (float)log(Fn);
+  Xi[52] = (float)(iso_i * 0.2);
+  double afx, afy, afz;
+  amul(ai, fx, fy, fz, afx, afy, afz);
+  Xi[53] = (float)((afx * fx + afy * fy + afz * fz) / iso_i);
+  Xi[54] = f54;
+  Xi[55] = f55;
+}
+
+// v1 = alpha F
+__global__ void k_ml_v1(ListArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int n = a.n;
+  double ai[6], ox, oy, oz;
+  site_alpha(a.a6, n, i, ai);
+  amul(ai, a.F[i], a.F[n + i], a.F[2 * (size_t)n + i], ox, oy, oz);
+  a.V[i]                 = ox;
+  a.V[n + i]             = oy;
+  a.V[2 * (size_t)n + i] = oz;
+}
+
+// PASS 1: v2 = alpha_i sum_j K_ij v1_j.  PASS 2: v3 = alpha_i sum_j K_ij v2_j, descriptors 56-63.
+// K_ij = fc(r) T2d_ij from the neighbour list (FP64).
+template <int PASS>
+__global__ __launch_bounds__(256) void k_ml_basis(ListArgs a) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i0  = gid / kBasisLanes;
+  const int sub = gid % kBasisLanes;
+  const bool ok = i0 < a.n;
+  const int i   = ok ? i0 : a.n - 1;  // trailing lanes shadow the last site and store nothing
+  const int n = a.n;
+  const double* W = a.V + (PASS == 1 ? 0 : 3 * (size_t)n);
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  const long long p1 = ok ? a.noff[i + 1] : 0;
+  for (long long p = a.noff[i] + sub; p < p1; p += kBasisLanes) {
+    const int j     = a.nj[p];
+    const double* k = a.nk + 6 * p;
+    const double wx = W[j], wy = W[n + j], wz = W[2 * (size_t)n + j];
+    sx += k[0] * wx + k[1] * wy + k[2] * wz;
+    sy += k[1] * wx + k[3] * wy + k[4] * wz;
+    sz += k[2] * wx + k[4] * wy + k[5] * wz;
+  }
+  sx = lane_sum<kBasisLanes>(sx);
+  sy = lane_sum<kBasisLanes>(sy);
+  sz = lane_sum<kBasisLanes>(sz);
+  if (!ok || sub != 0) return;
+  double ai[6], ox, oy, oz;
+  site_alpha(a.a6, n, i, ai);
   amul(ai, sx, sy, sz, ox, oy, oz);
-  float* Xi = a.X + (size_t)i * kMlpIn;
   if (PASS == 1) {
-    a.V[i] = v1x;
-    a.V[n + i] = v1y;
-    a.V[2 * (size_t)n + i] = v1z;
     a.V[3 * (size_t)n + i] = ox;
     a.V[4 * (size_t)n + i] = oy;
     a.V[5 * (size_t)n + i] = oz;
-    // molecular frame from the first two exclusion partners (fallback: field frame)
-    double e1x = fx, e1y = fy, e1z = fz, e2x = 0, e2y = 0, e2z = 0;
-    if (e1 - e0 >= 2) {
-      const int p = a.eidx[e0], q = a.eidx[e0 + 1];
-      if (p >= 0 && q >= 0) {
-        double ux = a.x[p] - xi, uy = a.y[p] - yi, uz = a.z[p] - zi;
-        double un = sqrt(ux * ux + uy * uy + uz * uz) + 1e-30;
-        e1x = ux / un;
-        e1y = uy / un;
-        e1z = uz / un;
-        double wx = a.x[q] - xi, wy = a.y[q] - yi, wz = a.z[q] - zi;
-        double pr = wx * e1x + wy * e1y + wz * e1z;
-        wx -= pr * e1x;
-        wy -= pr * e1y;
-        wz -= pr * e1z;
-        double wn = sqrt(wx * wx + wy * wy + wz * wz) + 1e-30;
-        e2x = wx / wn;
-        e2y = wy / wn;
-        e2z = wz / wn;
-      }
-    }
-    const double e3x = e1y * e2z - e1z * e2y, e3y = e1z * e2x - e1x * e2z, e3z = e1x * e2y - e1y * e2x;
-    for (int k = 0; k < 48; ++k) Xi[k] = feat[k];
-    Xi[48] = (float)(fx * e1x + fy * e1y + fz * e1z);
-    Xi[49] = (float)(fx * e2x + fy * e2y + fz * e2z);
-    Xi[50] = (float)(fx * e3x + fy * e3y + fz * e3z);
-    Xi[51] = (float)log(Fn);
-    Xi[52] = (float)(iso_i * 0.2);
-    double afx, afy, afz;
-    amul(ai, fx, fy, fz, afx, afy, afz);
-    Xi[53] = (float)((afx * fx + afy * fy + afz * fz) / iso_i);
-    Xi[54] = feat[54];
-    Xi[55] = feat[55];
-  } else {
-    a.V[6 * (size_t)n + i] = ox;
-    a.V[7 * (size_t)n + i] = oy;
-    a.V[8 * (size_t)n + i] = oz;
-    const double v2x = a.V[3 * (size_t)n + i], v2y = a.V[4 * (size_t)n + i], v2z = a.V[5 * (size_t)n + i];
-    const double n1 = sqrt(v1x * v1x + v1y * v1y + v1z * v1z) + 1e-30;
-    const double n2 = sqrt(v2x * v2x + v2y * v2y + v2z * v2z) + 1e-30;
-    const double n3 = sqrt(ox * ox + oy * oy + oz * oz) + 1e-30;
-    Xi[56] = (float)((v2x * v1x + v2y * v1y + v2z * v1z) / (n1 * n1));
-    Xi[57] = (float)(n2 / n1);
-    Xi[58] = (float)((ox * v1x + oy * v1y + oz * v1z) / (n1 * n1));
-    Xi[59] = (float)(n3 / n1);
-    Xi[60] = (float)((ox * v2x + oy * v2y + oz * v2z) / (n2 * n3));
-    Xi[61] = (float)log(n1);
-    Xi[62] = (float)((v1x * Fx + v1y * Fy + v1z * Fz) / (n1 * Fn));
-    Xi[63] = 1.0f;
+    return;
   }
+  a.V[6 * (size_t)n + i] = ox;
+  a.V[7 * (size_t)n + i] = oy;
+  a.V[8 * (size_t)n + i] = oz;
+  const double Fx = a.F[i], Fy = a.F[n + i], Fz = a.F[2 * (size_t)n + i];
+  const double Fn = sqrt(Fx * Fx + Fy * Fy + Fz * Fz) + 1e-30;
+  const double v1x = a.V[i], v1y = a.V[n + i], v1z = a.V[2 * (size_t)n + i];
+  const double v2x = a.V[3 * (size_t)n + i], v2y = a.V[4 * (size_t)n + i], v2z = a.V[5 * (size_t)n + i];
+  const double n1 = sqrt(v1x * v1x + v1y * v1y + v1z * v1z) + 1e-30;
+  const double n2 = sqrt(v2x * v2x + v2y * v2y + v2z * v2z) + 1e-30;
+  const double n3 = sqrt(ox * ox + oy * oy + oz * oz) + 1e-30;
+  float* Xi = a.X + (size_t)i * kMlpIn;
+  Xi[56] = (float)((v2x * v1x + v2y * v1y + v2z * v1z) / (n1 * n1));
+  Xi[57] = (float)(n2 / n1);
+  Xi[58] = (float)((ox * v1x + oy * v1y + oz * v1z) / (n1 * n1));
+  Xi[59] = (float)(n3 / n1);
+  Xi[60] = (float)((ox * v2x + oy * v2y + oz * v2z) / (n2 * n3));
+  Xi[61] = (float)log(n1);
+  Xi[62] = (float)((v1x * Fx + v1y * Fy + v1z * Fz) / (n1 * Fn));
+  Xi[63] = 1.0f;
 }
 
-__global__ __launch_bounds__(256) void k_bbox(int n, const double* __restrict__ x, const double* __restrict__ y,
-                                              const double* __restrict__ z, double* __restrict__ out) {
+// Bounding box of the sites: per-block partial min/max, then one block over the partials.
+__global__ __launch_bounds__(256) void k_bbox_part(int n, const double* __restrict__ x, const double* __restrict__ y,
+                                                   const double* __restrict__ z, double* __restrict__ part) {
   __shared__ double r[6][256];
   double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double v[3] = {x[i], y[i], z[i]};
     for (int c = 0; c < 3; ++c) {
       mn[c] = fmin(mn[c], v[c]);
@@ -272,51 +394,102 @@ __global__ __launch_bounds__(256) void k_bbox(int n, const double* __restrict__ 
       }
     __syncthreads();
   }
-  if (threadIdx.x < 6) out[threadIdx.x] = r[threadIdx.x][0];
+  if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = r[threadIdx.x][0];
+}
+
+MlGrid& ml_grid(System& S) {
+  if (S.ml_grid) return *S.ml_grid;
+  const int n    = S.n;
+  hipStream_t st = S.stream;
+  constexpr int kParts = 256;
+  dbuf<double> part;
+  part.alloc(6 * kParts);
+  hipLaunchKernelGGL(k_bbox_part, dim3(kParts), dim3(256), 0, st, n, S.x.p, S.y.p, S.z.p, part.p);
+  std::vector<double> hp(6 * kParts);
+  CPPE_HIP_CHECK(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, st));
+  CPPE_HIP_CHECK(hipStreamSynchronize(st));
+  double h[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+  for (int b = 0; b < kParts; ++b)
+    for (int c = 0; c < 3; ++c) {
+      h[c]     = std::min(h[c], hp[6 * b + c]);
+      h[3 + c] = std::max(h[3 + c], hp[6 * b + 3 + c]);
+    }
+  auto* G = new MlGrid();
+  try {
+    GridInfo& g = G->g;
+    g.x0 = h[0];
+    g.y0 = h[1];
+    g.z0 = h[2];
+    g.nx = std::max(1, (int)std::ceil((h[3] - h[0]) / kCell + 1e-9));
+    g.ny = std::max(1, (int)std::ceil((h[4] - h[1]) / kCell + 1e-9));
+    g.nz = std::max(1, (int)std::ceil((h[5] - h[2]) / kCell + 1e-9));
+    const long long ncell = (long long)g.nx * g.ny * g.nz;
+    if (ncell > (1ll << 26)) throw std::runtime_error("ml features: neighbour grid too large");
+    dbuf<unsigned> cell;
+    dbuf<int> idx;
+    cell.alloc(n);
+    idx.alloc(n);
+    G->cellof.alloc(n);
+    G->sorted.alloc(n);
+    G->cstart.alloc(ncell + 1);
+    hipLaunchKernelGGL(k_grid_cell, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, S.x.p, S.y.p, S.z.p, g, cell.p,
+                       idx.p);
+    dbuf<char> tmp;
+    size_t bytes = 0;
+    int bits     = 1;
+    while ((1ll << bits) <= ncell) ++bits;
+    dbuf<unsigned> cell_s;
+    cell_s.alloc(n);
+    CPPE_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, cell.p, cell_s.p, idx.p, G->sorted.p, n, 0,
+                                                      bits, st));
+    tmp.alloc(bytes + 16);
+    CPPE_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, cell.p, cell_s.p, idx.p, G->sorted.p, n, 0,
+                                                      bits, st));
+    hipLaunchKernelGGL(k_grid_ranges, dim3(ceil_div(ncell + 1, 256)), dim3(256), 0, st, n, (int)ncell, cell_s.p,
+                       G->cstart.p);
+    CPPE_HIP_CHECK(hipMemcpyAsync(G->cellof.p, cell.p, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
+    // neighbour list (geometry only)
+    FeatArgs fa{n, S.x.p, S.y.p, S.z.p, S.s.p, S.alpha6.p, nullptr, S.excl_off.p, S.excl_idx.p,
+                S.opts.damping, g, G->cstart.p, G->sorted.p, G->cellof.p, nullptr, nullptr};
+    dbuf<long long> cnt;
+    cnt.alloc(n + 1);
+    hipLaunchKernelGGL(k_ml_ncount, dim3(ceil_div(n, 128)), dim3(128), 0, st, fa, cnt.p);
+    long long zero = 0;
+    CPPE_HIP_CHECK(hipMemcpyAsync(cnt.p + n, &zero, sizeof(long long), hipMemcpyHostToDevice, st));
+    G->noff.alloc(n + 1);
+    size_t sb = 0;
+    CPPE_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, cnt.p, G->noff.p, n + 1, st));
+    tmp.alloc(sb + 16);
+    CPPE_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, sb, cnt.p, G->noff.p, n + 1, st));
+    CPPE_HIP_CHECK(hipMemcpyAsync(&G->npairs, G->noff.p + n, sizeof(long long), hipMemcpyDeviceToHost, st));
+    CPPE_HIP_CHECK(hipStreamSynchronize(st));
+    const size_t np = (size_t)std::max<long long>(G->npairs, 1);
+    G->nj.alloc(np);
+    G->ngeo.alloc(np);
+    G->nk.alloc(6 * np);
+    G->iso.alloc(n);
+    hipLaunchKernelGGL(k_ml_nfill, dim3(ceil_div(n, 128)), dim3(128), 0, st, fa, G->noff.p, G->nj.p, G->ngeo.p,
+                       G->nk.p, G->iso.p);
+    CPPE_HIP_CHECK(hipStreamSynchronize(st));
+  } catch (...) {
+    delete G;
+    throw;
+  }
+  S.ml_grid = G;
+  return *G;
 }
 
 // Features X (n x 64, raw) and bases V (9n) for the system's resident rhs.
 void ml_features(System& S, float* X, double* V) {
   const int n    = S.n;
   hipStream_t st = S.stream;
-  dbuf<double> bb;
-  bb.alloc(6);
-  hipLaunchKernelGGL(k_bbox, dim3(1), dim3(256), 0, st, n, S.x.p, S.y.p, S.z.p, bb.p);
-  double h[6];
-  CPPE_HIP_CHECK(hipMemcpyAsync(h, bb.p, sizeof(h), hipMemcpyDeviceToHost, st));
-  CPPE_HIP_CHECK(hipStreamSynchronize(st));
-  GridInfo g;
-  g.x0 = h[0];
-  g.y0 = h[1];
-  g.z0 = h[2];
-  g.nx = std::max(1, (int)std::ceil((h[3] - h[0]) / kRc + 1e-9));
-  g.ny = std::max(1, (int)std::ceil((h[4] - h[1]) / kRc + 1e-9));
-  g.nz = std::max(1, (int)std::ceil((h[5] - h[2]) / kRc + 1e-9));
-  const long long ncell = (long long)g.nx * g.ny * g.nz;
-  if (ncell > (1ll << 26)) throw std::runtime_error("ml features: neighbour grid too large");
-  dbuf<unsigned> cell, cell_s;
-  dbuf<int> idx, idx_s, cstart;
-  cell.alloc(n);
-  cell_s.alloc(n);
-  idx.alloc(n);
-  idx_s.alloc(n);
-  cstart.alloc(ncell + 1);
-  hipLaunchKernelGGL(k_grid_cell, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, S.x.p, S.y.p, S.z.p, g, cell.p,
-                     idx.p);
-  dbuf<char> tmp;
-  size_t bytes = 0;
-  int bits = 1;
-  while ((1ll << bits) <= ncell) ++bits;
-  CPPE_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, cell.p, cell_s.p, idx.p, idx_s.p, n, 0, bits, st));
-  tmp.alloc(bytes + 16);
-  CPPE_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, cell.p, cell_s.p, idx.p, idx_s.p, n, 0, bits, st));
-  hipLaunchKernelGGL(k_grid_ranges, dim3(ceil_div(ncell + 1, 256)), dim3(256), 0, st, n, (int)ncell, cell_s.p,
-                     cstart.p);
-  FeatArgs a{n, S.x.p, S.y.p, S.z.p, S.s.p, S.alpha6.p, S.v_rhs.p, S.excl_off.p, S.excl_idx.p,
-             S.opts.damping, g, cstart.p, idx_s.p, cell.p, X, V};
-  hipLaunchKernelGGL(k_ml_features<1>, dim3(ceil_div(n, 128)), dim3(128), 0, st, a);
-  hipLaunchKernelGGL(k_ml_features<2>, dim3(ceil_div(n, 128)), dim3(128), 0, st, a);
-  CPPE_HIP_CHECK(hipStreamSynchronize(st));
+  MlGrid& G      = ml_grid(S);
+  ListArgs a{n, G.noff.p, G.nj.p, G.ngeo.p, G.nk.p, G.iso.p, S.x.p, S.y.p, S.z.p, S.alpha6.p, S.v_rhs.p,
+             S.excl_off.p, S.excl_idx.p, X, V};
+  hipLaunchKernelGGL(k_ml_desc, dim3(ceil_div((long long)n * kDescLanes, 256)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_ml_v1, dim3(ceil_div(n, 256)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_ml_basis<1>, dim3(ceil_div((long long)n * kBasisLanes, 256)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_ml_basis<2>, dim3(ceil_div((long long)n * kBasisLanes, 256)), dim3(256), 0, st, a);
 }
 
 }  // namespace cppe_dev

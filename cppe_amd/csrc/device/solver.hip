@@ -4,8 +4,8 @@
 //
 // Vectors are 3n SoA in device order. Work is split into chunks of at most kChunk sites that
 // never cross a segment boundary; every reduction is one partial per chunk (fixed block tree)
-// followed by a per-segment sum over its chunks in chunk order, so results are deterministic
-// and a segment's scalars do not depend on the other segments. Each segment runs its own
+// followed by a per-segment sum over its chunks (fixed lane assignment and shuffle tree), so
+// results are deterministic and a segment's scalars do not depend on the other segments. Each segment runs its own
 // recurrence (its own step lengths, residual norm and convergence test) and is frozen once it
 // has converged; the field applies are shared by all segments.
 //
@@ -235,8 +235,10 @@ struct CgFinal {
   int hist_cap;          // iterations held by hist
 };
 
-// Per-segment finalisation of the chunk partials, one thread per segment, chunks summed in
-// order (deterministic). Then the status block is refreshed.
+// Per-segment finalisation of the chunk partials: one wavefront per segment (the block's four
+// wavefronts take segments in turn); lane l sums chunks l, l + 64, ... in order, then a fixed
+// shuffle tree combines the lanes -- deterministic for a given chunk split. Lane 0 then updates
+// the segment's scalars; the status block is refreshed at the end.
 template <int MODE, bool MIXED>
 __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
   __shared__ int cnt[3];
@@ -245,13 +247,20 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
   const int ns = f.nseg;
   const int it = f.status[ST_IT];
   const double thresh = f.params[0], drop = f.params[1];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int na = 0, nr = 0, nb = 0;
-  for (int g = threadIdx.x; g < ns; g += blockDim.x) {
+  for (int g = wv; g < ns; g += 4) {
     double s1 = 0.0, s2 = 0.0;
-    for (int c = f.seg_chunk[g]; c < f.seg_chunk[g + 1]; ++c) {
+    for (int c = f.seg_chunk[g] + lane; c < f.seg_chunk[g + 1]; c += 64) {
       s1 += f.part[c];
       if (MODE == F_INIT) s2 += f.part[f.nchunk + c];
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      s1 += __shfl_xor(s1, off);
+      if (MODE == F_INIT) s2 += __shfl_xor(s2, off);
+    }
+    if (lane != 0) continue;
     double* sd = f.sd;
     int* si    = f.si;
     if (MODE == F_INIT) {

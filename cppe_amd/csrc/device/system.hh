@@ -55,6 +55,7 @@ struct Octree {
 };
 
 struct FmmState;  // FMM expansions and operator scratch (fmm.hip)
+struct MlGrid;    // neighbour grid of the learned warm start (ml_features.hip)
 
 // Outcome of a device CG solve, per segment.
 struct CgResult {
@@ -83,6 +84,7 @@ struct System {
 
   Octree tree;
   FmmState* fmm = nullptr;
+  MlGrid* ml_grid = nullptr;
 
   // snapshots ("segments"): contiguous site ranges that form independent systems (own root
   // cell, own ncrit, own CG recurrence). A plain system is one segment.
@@ -132,6 +134,7 @@ void cg_graphs_release(System& S);
 void fmm_field(System& S, int m, const double* src, double* F, double damping, bool fp32);
 void fmm_static(System& S, const double* q, const double* mu, const double* th, double* F, double damping);
 void fmm_destroy(System& S);
+void ml_grid_destroy(System& S);
 
 // layout conversions between caller AoS (caller order) and device SoA (device order)
 void aos_to_soa(System& S, const double* aos, double* soa, int ncomp);
