@@ -98,11 +98,13 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=5)
     ap.add_argument("--no-ml", action="store_true", help="headline form without the learned warm start")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuned ones")
+    ap.add_argument("--reliable-drop", type=float, default=None,
+                    help="mixed form: residual drop that triggers an FP64 true residual (library default 1e-5)")
     ap.add_argument("--apply-damping", type=float, default=2.1304, help="Thole factor of --apply-only runs")
     ap.add_argument("--apply-ncrit", type=int, default=64, help="tree_ncrit of --apply-only runs")
     ap.add_argument("--apply-only", metavar="SHAPE", default=None,
                     help="profiling aid: only run FMM B applies of the workload with the launch shape "
-                         "'variant,block,m2l' for both precisions (e.g. 2,64,16), print phase times, exit")
+                         "'variant,block,m2l[,overlap]' for both precisions (e.g. 8,256,16), print phase times, exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the induced-dipole path is built and timed on a single MI355X "
@@ -119,11 +121,13 @@ def main():
     w = synthetic.water_box(**spec)
     log("sites", w.n)
     if args.apply_only:
-        v, b, g = (int(t) for t in args.apply_only.split(","))
+        shape = [int(t) for t in args.apply_only.split(",")]
+        v, b, g = shape[:3]
+        o = shape[3] if len(shape) > 3 else 0
         for prec in ("mixed", "fp64"):
             S = capi.System(w.pos, w.pol, w.alpha_iso(), w.excl_csr(), precision=prec, summation="fmm",
                             ncrit=args.apply_ncrit, order=5, theta=0.5, damping=args.apply_damping)
-            S.set_tune([v, b, g, v, b, g])
+            S.set_tune([v, b, g, o, v, b, g, o])
             ph = S.profile_apply(fp32=prec == "mixed", reps=args.profile_reps)
             st = S.tree_stats()
             log(prec, "shape", args.apply_only, "damping", args.apply_damping, "ncrit", args.apply_ncrit,
@@ -139,6 +143,8 @@ def main():
     S32 = capi.System(w.pos, w.pol, w.alpha_iso(), w.excl_csr(), precision="mixed", **common)
     S64.upload_rhs(rhs)
     S32.upload_rhs(rhs)
+    if args.reliable_drop:
+        S32.set_reliable_drop(args.reliable_drop)
     log("setup (octree + lists) ms", round(t_setup, 1))
     tune = None
     if args.summation == "fmm" and not args.no_tune:
@@ -172,7 +178,9 @@ def main():
         wall32, dev32 = float(np.mean(wall32)), float(np.mean(dev32))
     else:
         wall32, dev32, its32 = time_solves(S32, args.steps, args.warmup, 0)
-    log("headline solve ms", wall32, "iters", its32, "ml", ml_used)
+    info32 = S32.last_solve_info()
+    log("headline solve ms", wall32, "iters", its32, "ml", ml_used, "fp64 true-residual applies",
+        info32["reliable_updates"])
     # the same mixed solve from alpha F, for the ML gain
     wallm, devm, itsm = time_solves(S32, max(1, args.steps // 2), 0, 0)
     mu64 = S64.download_mu()
@@ -236,6 +244,7 @@ def main():
         "device_ms": {"headline": round(dev32, 3), "fp64": round(dev64, 3)},
         "iterations": {"headline": its32, "fp64": its64},
         "ml_warm_start": ml_used,
+        "fp64_true_residual_applies": info32["reliable_updates"],
         "ml_guess_ms": round(float(np.mean(guess_ms)), 3) if guess_ms else None,
         "mixed_alphaF_ms": round(wallm, 3),
         "mixed_alphaF_iterations": itsm,

@@ -21,7 +21,7 @@ EXPORTS = [
     "cppe_hip_solve_resident", "cppe_hip_download_mu", "cppe_hip_nuclear_fields",
     "cppe_hip_nuclear_multipole_energy", "cppe_hip_dot", "cppe_hip_tree_stats", "cppe_hip_tree_export",
     "cppe_hip_profile_apply", "cppe_hip_batch_create", "cppe_hip_segment_results", "cppe_hip_set_graphs",
-    "cppe_hip_autotune", "cppe_hip_set_tune",
+    "cppe_hip_autotune", "cppe_hip_set_tune", "cppe_hip_set_reliable_drop", "cppe_hip_last_solve_info",
     "cppe_hip_mlp_create", "cppe_hip_mlp_destroy", "cppe_hip_mlp_set", "cppe_hip_mlp_get", "cppe_hip_ml_features",
     "cppe_hip_mlp_forward", "cppe_hip_mlp_loss_grad", "cppe_hip_mlp_reserve", "cppe_hip_mlp_collect",
     "cppe_hip_mlp_prepare", "cppe_hip_mlp_train", "cppe_hip_ml_guess",
@@ -81,6 +81,8 @@ def lib():
         L.cppe_hip_set_graphs.argtypes = [vp, ctypes.c_int]
         L.cppe_hip_autotune.argtypes = [vp, ctypes.c_int, _i, _d]
         L.cppe_hip_set_tune.argtypes = [vp, _i]
+        L.cppe_hip_set_reliable_drop.argtypes = [vp, ctypes.c_double]
+        L.cppe_hip_last_solve_info.argtypes = [vp, _i]
         _f = ctypes.POINTER(ctypes.c_float)
         L.cppe_hip_mlp_create.argtypes = [ctypes.POINTER(vp)]
         L.cppe_hip_mlp_destroy.argtypes = [vp]
@@ -210,31 +212,41 @@ class System:
 
     NEAR_CANDIDATES = [(0, 64), (0, 128), (0, 256), (1, 64), (1, 128), (1, 256), (2, 64), (3, 64), (3, 128),
                        (3, 256), (4, 64), (4, 128), (4, 256), (5, 64), (5, 128), (5, 256), (6, 64), (6, 128),
-                       (6, 256), (7, 64), (7, 128), (7, 256)]
+                       (6, 256), (7, 64), (7, 128), (7, 256), (8, 64), (8, 128), (8, 256)]
     NEAR_NAMES = {0: "thread", 1: "leafwave", 2: "ldsleaf", 3: "slotted", 4: "cluster", 5: "staged64", 6: "staged32",
-                  7: "staged16"}
+                  7: "staged16", 8: "staged8"}
     M2L_CANDIDATES = [16, 32, 64]
 
     def autotune(self, reps=3):
         """Times the B-apply launch candidates on this system and keeps the fastest (tree path)."""
-        choice = np.zeros(6, dtype=np.int32)
+        choice = np.zeros(8, dtype=np.int32)
         nn, nm = len(self.NEAR_CANDIDATES), len(self.M2L_CANDIDATES)
-        ms = np.zeros(2 * (nn + nm))
+        w = nn + nm + 2
+        ms = np.zeros(2 * w)
         _check(lib().cppe_hip_autotune(self.h, int(reps), _p(choice, _i), _p(ms)))
         out = {}
         for k, prec in enumerate(["fp64", "fp32"]):
-            m = ms[(nn + nm) * k:(nn + nm) * (k + 1)]
-            out[prec] = dict(near_variant=int(choice[3 * k]), near_block=int(choice[3 * k + 1]),
-                             m2l_group=int(choice[3 * k + 2]),
+            m = ms[w * k:w * (k + 1)]
+            out[prec] = dict(near_variant=int(choice[4 * k]), near_block=int(choice[4 * k + 1]),
+                             m2l_group=int(choice[4 * k + 2]), overlap=int(choice[4 * k + 3]),
+                             apply_ms={"series": round(float(m[nn + nm]), 4), "overlap": round(float(m[nn + nm + 1]), 4)},
                              near_ms={"%s/%d" % (self.NEAR_NAMES[v], b): round(float(t), 4)
                                       for (v, b), t in zip(self.NEAR_CANDIDATES, m[:nn])},
-                             m2l_ms={str(g): round(float(t), 4) for g, t in zip(self.M2L_CANDIDATES, m[nn:])})
+                             m2l_ms={str(g): round(float(t), 4) for g, t in zip(self.M2L_CANDIDATES, m[nn:nn + nm])})
         self.tune_choice = choice
         return out
 
     def set_tune(self, choice):
         c = np.ascontiguousarray(choice, dtype=np.int32)
         _check(lib().cppe_hip_set_tune(self.h, _p(c, _i)))
+
+    def set_reliable_drop(self, drop):
+        _check(lib().cppe_hip_set_reliable_drop(self.h, float(drop)))
+
+    def last_solve_info(self):
+        info = np.zeros(2, dtype=np.int32)
+        _check(lib().cppe_hip_last_solve_info(self.h, _p(info, _i)))
+        return dict(iterations=int(info[0]), reliable_updates=int(info[1]))
 
     def set_graphs(self, enable=True):
         _check(lib().cppe_hip_set_graphs(self.h, int(bool(enable))))

@@ -574,12 +574,29 @@ int cppe_hip_set_tune(cppe_hip_system* sys, const int* choice) {
   if (!sys || !choice) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
   if (sys->S.opts.summation != CPPE_HIP_SUM_FMM) return fail(CPPE_HIP_ERR_UNSUPPORTED, "autotuning covers the tree path only");
   for (int prec = 0; prec < 2; ++prec) {
-    const int v = choice[3 * prec], b = choice[3 * prec + 1], g = choice[3 * prec + 2];
-    if (v < 0 || v > 7 || (b != 64 && b != 128 && b != 256) || (v == 2 && b != 64) ||
-        (g != 16 && g != 32 && g != 64))
+    const int v = choice[4 * prec], b = choice[4 * prec + 1], g = choice[4 * prec + 2], o = choice[4 * prec + 3];
+    if (v < 0 || v > 8 || (b != 64 && b != 128 && b != 256) || (v == 2 && b != 64) ||
+        (g != 16 && g != 32 && g != 64) || (o != 0 && o != 1))
       return fail(CPPE_HIP_ERR_ARG, "invalid launch shape");
   }
   fmm_set_tune(sys->S, choice);
+  return CPPE_HIP_OK;
+  CPPE_API_END
+}
+
+int cppe_hip_set_reliable_drop(cppe_hip_system* sys, double drop) {
+  CPPE_API_BEGIN
+  if (!sys || !(drop > 0.0 && drop < 1.0)) return fail(CPPE_HIP_ERR_ARG, "reliable drop must be in (0, 1)");
+  sys->S.reliable_drop = drop;
+  return CPPE_HIP_OK;
+  CPPE_API_END
+}
+
+int cppe_hip_last_solve_info(cppe_hip_system* sys, int* info) {
+  CPPE_API_BEGIN
+  if (!sys || !info) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
+  info[0] = sys->S.last.loop_iters;
+  info[1] = sys->S.last.reliable_updates;
   return CPPE_HIP_OK;
   CPPE_API_END
 }

@@ -35,7 +35,18 @@ CPPE_EXTERN_FMM(8)
 static FmmState& state(System& S) {
   if (!S.tree.built || S.tree.m2l_off.p == nullptr)
     throw std::runtime_error("fmm summation needs the interaction lists of this system");
-  if (!S.fmm) S.fmm = new FmmState();
+  if (!S.fmm) {
+    auto* F = new FmmState();
+    try {
+      CPPE_HIP_CHECK(hipStreamCreateWithFlags(&F->far_stream, hipStreamNonBlocking));
+      CPPE_HIP_CHECK(hipEventCreateWithFlags(&F->ev_fork, hipEventDisableTiming));
+      CPPE_HIP_CHECK(hipEventCreateWithFlags(&F->ev_join, hipEventDisableTiming));
+    } catch (...) {
+      delete F;
+      throw;
+    }
+    S.fmm = F;
+  }
   S.fmm->order = S.opts.order;
   return *S.fmm;
 }
@@ -196,16 +207,20 @@ void fmm_reserve(System& S) {
 // walk, 2 wavefront per leaf with LDS-staged sources, 3 wavefront per leaf chunk with (target,
 // source slot) lanes over the expanded near list, 4 wavefront per target cluster (<= 64 sites of
 // consecutive leaves) over the union of their near lists with lane masks, 5 wavefront per leaf
-// chunk (5: 64 targets, 6: 32, 7: 16), several per block, sources of the expanded near list staged
+// chunk (5: 64 targets, 6: 32, 7: 16, 8: 8), several per block, sources of the expanded near list staged
 // through wave-private LDS.
-// ms receives the timings: per precision (FP64 then FP32) kNearCand near values then kM2lCand
-// M2L values; choice receives (near variant, near block, m2l group) per precision.
+// Then, for a staged near variant, the whole apply is timed with the far field in series and
+// beside the near field on a second stream, and the faster is kept.
+// ms receives the timings: per precision (FP64 then FP32) kNearCand near values, kM2lCand M2L
+// values, then the apply in series and overlapped; choice receives (near variant, near block, m2l
+// group, overlap) per precision.
 static const int kNearCand[][2] = {{0, 64}, {0, 128}, {0, 256}, {1, 64}, {1, 128}, {1, 256},
                                    {2, 64}, {3, 64},  {3, 128}, {3, 256}, {4, 64}, {4, 128},
                                    {4, 256}, {5, 64}, {5, 128}, {5, 256}, {6, 64}, {6, 128},
-                                   {6, 256}, {7, 64}, {7, 128}, {7, 256}};
+                                   {6, 256}, {7, 64}, {7, 128}, {7, 256}, {8, 64}, {8, 128},
+                                   {8, 256}};
 static const int kM2lCand[]     = {16, 32, 64};
-constexpr int kNNear = 22, kNM2l = 3;
+constexpr int kNNear = 25, kNM2l = 3;
 
 void fmm_autotune(System& S, int reps, int* choice, double* ms) {
   fmm_reserve(S);
@@ -230,11 +245,22 @@ void fmm_autotune(System& S, int reps, int* choice, double* ms) {
     }
     return acc / reps;
   };
+  auto timed_total = [&](bool fp32) {
+    apply_b_fmm(S, S.v_d.p, S.v_t.p, fp32);  // warm-up
+    CPPE_HIP_CHECK(hipEventRecord(ev[0], S.stream));
+    for (int r = 0; r < reps; ++r) apply_b_fmm(S, S.v_d.p, S.v_t.p, fp32);
+    CPPE_HIP_CHECK(hipEventRecord(ev[1], S.stream));
+    CPPE_HIP_CHECK(hipEventSynchronize(ev[1]));
+    float t = 0.f;
+    CPPE_HIP_CHECK(hipEventElapsedTime(&t, ev[0], ev[1]));
+    return (double)t / reps;
+  };
   try {
     fill_f64(S, S.v_d.p, 3 * (size_t)S.n, 0.01);
     for (int prec = 0; prec < 2; ++prec) {
       const bool fp32 = prec == 1;
-      double* m       = ms + prec * (kNNear + kNM2l);
+      double* m       = ms + prec * (kNNear + kNM2l + 2);
+      F.tune.overlap[prec] = 0;
       int best        = 0;
       for (int c = 0; c < kNNear; ++c) {
         F.tune.near_variant[prec] = kNearCand[c][0];
@@ -251,9 +277,17 @@ void fmm_autotune(System& S, int reps, int* choice, double* ms) {
         if (m[kNNear + c] < m[kNNear + bm]) bm = c;
       }
       F.tune.m2l_group[prec] = kM2lCand[bm];
-      choice[3 * prec]       = F.tune.near_variant[prec];
-      choice[3 * prec + 1]   = F.tune.near_block[prec];
-      choice[3 * prec + 2]   = F.tune.m2l_group[prec];
+      m[kNNear + kNM2l]      = timed_total(fp32);
+      m[kNNear + kNM2l + 1]  = 0.0;
+      if (F.tune.near_variant[prec] >= 5) {
+        F.tune.overlap[prec]  = 1;
+        m[kNNear + kNM2l + 1] = timed_total(fp32);
+        F.tune.overlap[prec]  = m[kNNear + kNM2l + 1] < m[kNNear + kNM2l] ? 1 : 0;
+      }
+      choice[4 * prec]     = F.tune.near_variant[prec];
+      choice[4 * prec + 1] = F.tune.near_block[prec];
+      choice[4 * prec + 2] = F.tune.m2l_group[prec];
+      choice[4 * prec + 3] = F.tune.overlap[prec];
     }
   } catch (...) {
     S.prof = nullptr;
@@ -267,9 +301,10 @@ void fmm_autotune(System& S, int reps, int* choice, double* ms) {
 void fmm_set_tune(System& S, const int* choice) {
   FmmState& F = state(S);
   for (int prec = 0; prec < 2; ++prec) {
-    F.tune.near_variant[prec] = choice[3 * prec];
-    F.tune.near_block[prec]   = choice[3 * prec + 1];
-    F.tune.m2l_group[prec]    = choice[3 * prec + 2];
+    F.tune.near_variant[prec] = choice[4 * prec];
+    F.tune.near_block[prec]   = choice[4 * prec + 1];
+    F.tune.m2l_group[prec]    = choice[4 * prec + 2];
+    F.tune.overlap[prec]      = choice[4 * prec + 3];
   }
   cg_graphs_release(S);
 }

@@ -33,14 +33,23 @@ struct FmmTune {
   int near_variant[2] = {0, 0};  // 0: one thread per target site, 1: one wavefront per leaf chunk
   int near_block[2]   = {256, 256};
   int m2l_group[2]    = {64, 64};  // lanes per target cell in M2L (16, 32 or 64)
+  int overlap[2]      = {0, 0};    // far field on a second stream beside the near field (staged variants)
 };
 
 struct FmmState {
   int order = 0;
   FmmTune tune;
-  // work items (leaf, first target) of the leaf-chunk near kernels, for chunks of 64, 32, 16 targets
-  dbuf<int2> near_items[3];
-  int n_items[3] = {0, 0, 0};
+  // second stream for the far field (default priority) and the events that fork / join it
+  hipStream_t far_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  ~FmmState() {
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (far_stream) (void)hipStreamDestroy(far_stream);
+  }
+  // work items (leaf, first target) of the leaf-chunk near kernels, for chunks of 64, 32, 16, 8 targets
+  dbuf<int2> near_items[4];
+  int n_items[4] = {0, 0, 0, 0};
   // target clusters: runs of whole leaves (device order) of <= 64 sites, and per cluster the
   // union of its leaves' near lists with, per source leaf, the mask of lanes it is near for
   int n_clusters = 0;
@@ -800,7 +809,7 @@ struct StagedRec {
 // lane gathers one source record into LDS (coalesced index load), then slot s evaluates sources
 // s, s + S, ... of the chunk against its target. FP32 form: positions staged as offsets from the
 // target leaf centre, formed in FP64 and rounded. Slot partials are summed through LDS.
-template <int P, typename R, bool DAMP, int MODE, typename MU4>
+template <int P, typename R, bool DAMP, int MODE, typename MU4, bool WITH_L2P = true>
 __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __restrict__ items, int nitems,
                                                      int chunk, const long long* __restrict__ xoff,
                                                      const int* __restrict__ xsrc, const MU4* __restrict__ mu4,
@@ -878,11 +887,15 @@ __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __r
     Fy += (double)red[1][wl][lane + T * s2];
     Fz += (double)red[2][wl][lane + T * s2];
   }
-  R lx = R(0), ly = R(0), lz = R(0);
-  O::l2p((R)(pi.x - cax), (R)(pi.y - cay), (R)(pi.z - caz), &L[(size_t)A * O::LS], lx, ly, lz);
-  Fx += (double)lx;
-  Fy += (double)ly;
-  Fz += (double)lz;
+  if constexpr (WITH_L2P) {
+    R lx = R(0), ly = R(0), lz = R(0);
+    O::l2p((R)(pi.x - cax), (R)(pi.y - cay), (R)(pi.z - caz), &L[(size_t)A * O::LS], lx, ly, lz);
+    Fx += (double)lx;
+    Fy += (double)ly;
+    Fz += (double)lz;
+  } else {
+    (void)L;
+  }
   const int n = a.n;
   if (MODE == MODE_BAPPLY) {
     const double m0 = a.src[i], m1 = a.src[n + i], m2 = a.src[2 * (size_t)n + i];
@@ -928,12 +941,29 @@ __global__ void k_pack_mu4(int n, const double* __restrict__ mu, MU4* __restrict
   out[i] = v;
 }
 
+// out += L2P of each site's leaf expansion (the far-field term when it runs beside the near field)
+template <int P, typename R>
+__global__ __launch_bounds__(256) void k_fmm_l2p_add(int n, const int* __restrict__ leaf_of,
+                                                     const double4* __restrict__ pos4, const double* __restrict__ cx,
+                                                     const double* __restrict__ cy, const double* __restrict__ cz,
+                                                     const R* __restrict__ L, double* __restrict__ out) {
+  using O     = Ops<KIND_M1, P, R>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int A      = leaf_of[i];
+  const double4 pi = pos4[i];
+  R lx = R(0), ly = R(0), lz = R(0);
+  O::l2p((R)(pi.x - cx[A]), (R)(pi.y - cy[A]), (R)(pi.z - cz[A]), &L[(size_t)A * O::LS], lx, ly, lz);
+  out[i] += (double)lx;
+  out[n + i] += (double)ly;
+  out[2 * (size_t)n + i] += (double)lz;
+}
+
 // ---- host drivers ----------------------------------------------------------------------------
 template <int KIND, int P, typename R>
-void fmm_far_pass(System& S, const double* src, R* M, R* L) {
+void fmm_far_pass(System& S, const double* src, R* M, R* L, hipStream_t st) {
   using O         = Ops<KIND, P, R>;
   const Octree& T = S.tree;
-  hipStream_t st  = S.stream;
   const auto& lo = T.level_offset;
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[0], st));
   hipLaunchKernelGGL((k_fmm_p2m<KIND, P, R>), dim3(ceil_div(T.nleaves, 64)), dim3(64), 0, st, T.nleaves,
@@ -1022,7 +1052,7 @@ inline void ensure_pos4(System& S) {
 }
 
 // Work items of the leaf-chunk near kernels: (leaf, first target) per chunk of at most C targets of
-// every leaf (C = 64 >> ci). Returns the item count.
+// every leaf (C = 64 >> ci, ci = 0..3). Returns the item count.
 inline int ensure_near_items(System& S, int ci = 0) {
   FmmState& F     = *S.fmm;
   const Octree& T = S.tree;
@@ -1056,20 +1086,24 @@ inline void ensure_posf4(System& S) {
 void ensure_clusters(System& S);  // fmm.hip
 
 template <int P, typename R, int MODE, bool DAMP, typename MU4>
-void launch_near(System& S, const NearArgs& a, const MU4* mu4, const R* L) {
+void launch_near(System& S, const NearArgs& a, const MU4* mu4, const R* L, bool with_l2p) {
   FmmState& F     = *S.fmm;
   const int prec  = sizeof(R) == 4;
   const int block = F.tune.near_block[prec];
   const int var   = F.tune.near_variant[prec];
   hipStream_t st  = S.stream;
-  if (var >= 5) {  // 5, 6, 7: staged, chunks of 64, 32, 16 targets
+  if (var >= 5) {  // 5, 6, 7, 8: staged, chunks of 64, 32, 16, 8 targets
     const int ci = var - 5;
     const int ni = ensure_near_items(S, ci);
     build_near_expanded(S);
     const Octree& T = S.tree;
     const dim3 g(ceil_div(ni, block / 64)), b(block);
-    hipLaunchKernelGGL((k_fmm_near_st<P, R, DAMP, MODE, MU4>), g, b, 0, st, a, F.near_items[ci].p, ni, 64 >> ci,
-                       T.xoff.p, T.xsrc.p, mu4, L);
+    if (with_l2p)
+      hipLaunchKernelGGL((k_fmm_near_st<P, R, DAMP, MODE, MU4, true>), g, b, 0, st, a, F.near_items[ci].p, ni,
+                         64 >> ci, T.xoff.p, T.xsrc.p, mu4, L);
+    else
+      hipLaunchKernelGGL((k_fmm_near_st<P, R, DAMP, MODE, MU4, false>), g, b, 0, st, a, F.near_items[ci].p, ni,
+                         64 >> ci, T.xoff.p, T.xsrc.p, mu4, L);
   } else if (var == 4) {
     ensure_clusters(S);
     const dim3 g(ceil_div(F.n_clusters, block / 64)), b(block);
@@ -1108,7 +1142,7 @@ void launch_near(System& S, const NearArgs& a, const MU4* mu4, const R* L) {
 }
 
 template <int P, typename R, int MODE>
-void fmm_near_pass(System& S, const double* mu, double damping, const R* L, double* out) {
+void fmm_near_pass(System& S, const double* mu, double damping, const R* L, double* out, bool with_l2p = true) {
   FmmState& F     = *S.fmm;
   const Octree& T = S.tree;
   hipStream_t st  = S.stream;
@@ -1121,37 +1155,58 @@ void fmm_near_pass(System& S, const double* mu, double damping, const R* L, doub
     hipLaunchKernelGGL(k_pack_mu4<double4>, g, b, 0, st, S.n, mu, F.mu4.p);
     if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[3], st));
     if (damping > 0.0)
-      launch_near<P, R, MODE, true>(S, a, F.mu4.p, L);
+      launch_near<P, R, MODE, true>(S, a, F.mu4.p, L, with_l2p);
     else
-      launch_near<P, R, MODE, false>(S, a, F.mu4.p, L);
+      launch_near<P, R, MODE, false>(S, a, F.mu4.p, L, with_l2p);
   } else {
     ensure_buf(F.mu4f, S.n);
     hipLaunchKernelGGL(k_pack_mu4<float4>, g, b, 0, st, S.n, mu, F.mu4f.p);
     if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[3], st));
     if (damping > 0.0)
-      launch_near<P, R, MODE, true>(S, a, F.mu4f.p, L);
+      launch_near<P, R, MODE, true>(S, a, F.mu4f.p, L, with_l2p);
     else
-      launch_near<P, R, MODE, false>(S, a, F.mu4f.p, L);
+      launch_near<P, R, MODE, false>(S, a, F.mu4f.p, L, with_l2p);
   }
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[4], st));
 }
 
+// Far field then near field (fused L2P), or -- when tuned so and the near variant is a staged
+// one -- the far field on a second stream beside the near field, joined before an L2P pass.
+template <int P, typename R>
+void fmm_apply_b_prec(System& S, const double* x, double* out, R* M, R* L) {
+  FmmState& F      = *S.fmm;
+  const int prec   = sizeof(R) == 4;
+  const double dmp = S.opts.damping;
+  const bool ovl   = F.tune.overlap[prec] && F.tune.near_variant[prec] >= 5 && !S.prof && F.far_stream;
+  if (!ovl) {
+    fmm_far_pass<KIND_M1, P, R>(S, x, M, L, S.stream);
+    fmm_near_pass<P, R, MODE_BAPPLY>(S, x, dmp, L, out);
+    return;
+  }
+  CPPE_HIP_CHECK(hipEventRecord(F.ev_fork, S.stream));
+  CPPE_HIP_CHECK(hipStreamWaitEvent(F.far_stream, F.ev_fork, 0));
+  fmm_far_pass<KIND_M1, P, R>(S, x, M, L, F.far_stream);
+  CPPE_HIP_CHECK(hipEventRecord(F.ev_join, F.far_stream));
+  fmm_near_pass<P, R, MODE_BAPPLY>(S, x, dmp, L, out, false);
+  CPPE_HIP_CHECK(hipStreamWaitEvent(S.stream, F.ev_join, 0));
+  const Octree& T = S.tree;
+  hipLaunchKernelGGL((k_fmm_l2p_add<P, R>), dim3(ceil_div(S.n, 256)), dim3(256), 0, S.stream, S.n, T.leaf_of.p,
+                     F.pos4.p, T.cx.p, T.cy.p, T.cz.p, L, out);
+}
+
 template <int P>
 void fmm_apply_b_t(System& S, const double* x, double* out, bool fp32) {
-  FmmState& F      = *S.fmm;
-  const size_t nc  = S.tree.ncells;
-  using O64        = Ops<KIND_M1, P, double>;
-  const double dmp = S.opts.damping;
+  FmmState& F     = *S.fmm;
+  const size_t nc = S.tree.ncells;
+  using O64       = Ops<KIND_M1, P, double>;
   if (fp32) {
     ensure_buf(F.M32, nc * O64::MS);
     ensure_buf(F.L32, nc * O64::LS);
-    fmm_far_pass<KIND_M1, P, float>(S, x, F.M32.p, F.L32.p);
-    fmm_near_pass<P, float, MODE_BAPPLY>(S, x, dmp, F.L32.p, out);
+    fmm_apply_b_prec<P, float>(S, x, out, F.M32.p, F.L32.p);
   } else {
     ensure_buf(F.M64, nc * O64::MS);
     ensure_buf(F.L64, nc * O64::LS);
-    fmm_far_pass<KIND_M1, P, double>(S, x, F.M64.p, F.L64.p);
-    fmm_near_pass<P, double, MODE_BAPPLY>(S, x, dmp, F.L64.p, out);
+    fmm_apply_b_prec<P, double>(S, x, out, F.M64.p, F.L64.p);
   }
 }
 
@@ -1162,7 +1217,7 @@ void fmm_field1_t(System& S, const double* src, double* out, double damping) {
   using O64       = Ops<KIND_M1, P, double>;
   ensure_buf(F.M64, nc * O64::MS);
   ensure_buf(F.L64, nc * O64::LS);
-  fmm_far_pass<KIND_M1, P, double>(S, src, F.M64.p, F.L64.p);
+  fmm_far_pass<KIND_M1, P, double>(S, src, F.M64.p, F.L64.p, S.stream);
   fmm_near_pass<P, double, MODE_FIELD1>(S, src, damping, F.L64.p, out);
 }
 
@@ -1174,7 +1229,7 @@ void fmm_static_t(System& S, const double* src10, double* out, double damping) {
   using OC        = Ops<KIND_MC, P, double>;
   ensure_buf(F.Mc, nc * OC::MS);
   ensure_buf(F.Lc, nc * OC::LS);
-  fmm_far_pass<KIND_MC, P, double>(S, src10, F.Mc.p, F.Lc.p);
+  fmm_far_pass<KIND_MC, P, double>(S, src10, F.Mc.p, F.Lc.p, S.stream);
   fmm_p2p_pass<KIND_MC, P, double, MODE_STATIC>(S, src10, damping, F.Lc.p, out);
 }
 

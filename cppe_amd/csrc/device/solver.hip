@@ -12,8 +12,8 @@
 // FP64 form: the reference recurrence with every B apply in FP64.
 // Mixed form: the same recurrence with FP32 field applies, made exact by reliable updates
 // (residual replacement; Sleijpen & van der Vorst, Computing 56, 141 (1996)): whenever a
-// segment's recursively updated residual has fallen by kReliableDrop since its last
-// replacement, or below the threshold, its true residual rhs - B x is recomputed with an FP64
+// segment's recursively updated residual has fallen by the system's reliable_drop factor
+// (default 1e-5) since its last replacement, or below the threshold, its true residual rhs - B x is recomputed with an FP64
 // apply and replaces it. Convergence is only ever declared on an FP64 true residual, so both
 // forms stop on the same FP64 criterion ||rhs - B x||_2 < thresh, per segment.
 //
@@ -34,8 +34,7 @@ void apply_b_direct(System& S, const double* xin, double* out, bool fp32);
 void apply_b_fmm(System& S, const double* xin, double* out, bool fp32);
 void fmm_reserve(System& S);
 
-constexpr int kChunk          = 1024;
-constexpr double kReliableDrop = 1e-4;
+constexpr int kChunk = 1024;
 
 // per-segment doubles (SoA, each nseg long) and ints
 enum SegD { D_RZ = 0, D_ALPHA, D_BETA, D_RNORM, D_RLAST, D_COUNT };
@@ -490,14 +489,14 @@ int device_cg(System& S, double thresh, int maxiter, bool mixed, CgResult& res) 
     cg_graphs_release(S);  // the graphs hold the history buffer
   }
   if (S.opts.summation == CPPE_HIP_SUM_FMM) fmm_reserve(S);  // no allocation inside a graph
-  const double params[2] = {thresh, kReliableDrop};
+  const double params[2] = {thresh, S.reliable_drop};
   CPPE_HIP_CHECK(hipMemcpyAsync(S.cg_params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
   // r0 = rhs - B x0 (FP64), z0 = alpha r0, p = z0
   apply_b(S, S.v_mu.p, S.v_t.p, false);
   hipLaunchKernelGGL(k_sg_init, dim3(S.nchunk), dim3(256), 0, st, vecs(S));
   launch_final<F_INIT>(S, mixed);
   read_status(S);
-  int k = 0;
+  int k = 0, nrel = 0;
   const int m = mixed ? 1 : 0;
   while (S.h_status[ST_ACTIVE] > 0 && k < maxiter) {
     run_graph(S, S.g_step[m], [&] { seq_step(S, mixed); });
@@ -505,6 +504,7 @@ int device_cg(System& S, double thresh, int maxiter, bool mixed, CgResult& res) 
     if (S.h_status[ST_REL] > 0) {
       run_graph(S, S.g_rel, [&] { seq_reliable(S); });
       read_status(S);
+      ++nrel;
     }
     ++k;
     if (S.h_status[ST_ACTIVE] == 0) break;
@@ -533,7 +533,8 @@ int device_cg(System& S, double thresh, int maxiter, bool mixed, CgResult& res) 
     res.all_converged = res.all_converged && res.converged[g];
   }
   for (int it = 0; it < std::min(res.iters[0], nh); ++it) res.norms0.push_back(hist[(size_t)it * ns]);
-  res.loop_iters = k;
+  res.loop_iters       = k;
+  res.reliable_updates = nrel;
   return k;
 }
 

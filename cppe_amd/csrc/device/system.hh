@@ -63,6 +63,7 @@ struct CgResult {
   std::vector<double> rnorm;
   std::vector<double> norms0;  // residual norm per iteration of segment 0
   int loop_iters = 0;          // iterations of the shared loop (max over segments)
+  int reliable_updates = 0;    // FP64 true-residual applies of the mixed form
   bool all_converged = false;
 };
 
@@ -102,6 +103,7 @@ struct System {
   int hist_cap = 0;
   int* h_status = nullptr;  // pinned host copy of the status block
   bool use_graphs = true;
+  double reliable_drop = 1e-5;  // mixed form: residual drop that triggers an FP64 true residual
   CgResult last;            // outcome of the last solve
   hipGraphExec_t g_step[2] = {nullptr, nullptr}, g_rel = nullptr, g_dir[2] = {nullptr, nullptr};
   dbuf<double> io_a, io_b;  // AoS3 staging in caller order

@@ -145,6 +145,11 @@ int cppe_hip_batch_create(cppe_hip_system** out, int nseg, const int* seg_off, c
 int cppe_hip_segment_results(cppe_hip_system* sys, int* iters, double* rnorm, int* converged);
 /* CG iterations replayed as captured HIP graphs (default 1) or launched kernel by kernel (0). */
 int cppe_hip_set_graphs(cppe_hip_system* sys, int enable);
+/* Mixed form: factor by which a residual must fall before the FP64 true residual replaces the
+ * recursive one (default 1e-5; convergence is always declared on an FP64 true residual). */
+int cppe_hip_set_reliable_drop(cppe_hip_system* sys, double drop);
+/* Last solve: info[0] iterations of the shared loop, info[1] FP64 true-residual applies. */
+int cppe_hip_last_solve_info(cppe_hip_system* sys, int* info);
 
 /*
  * Autotuner of the tree path's B-apply launch shapes (no counterpart in the reference). For the
@@ -158,11 +163,15 @@ int cppe_hip_set_graphs(cppe_hip_system* sys, int enable);
  *      list (built on first use: one int per near source site of each leaf), block 64/128/256;
  *   4: one wavefront per target cluster (whole consecutive leaves, <= 64 sites) walking the union
  *      of their near lists with per-source-leaf lane masks, block 64/128/256;
- *   5, 6, 7: one wavefront per leaf chunk of at most 64, 32, 16 targets, several per block, the
+ *   5, 6, 7, 8: one wavefront per leaf chunk of at most 64, 32, 16, 8 targets, several per block, the
  *      expanded near list staged 64 sources at a time through wavefront-private LDS, lanes =
  *      (target, source slot), block 64/128/256.
- * choice[6]: (near variant, near block, m2l group) for FP64 then FP32.
- * ms[50]: per precision, the 22 near timings (order above) then the 3 M2L timings (ms/launch).
+ * Then, for a staged near variant (5-8), the whole apply is timed with the far field (P2M, M2M,
+ * M2L, L2L) in series and on a second stream beside the near field (joined before an L2P pass),
+ * and the faster is kept.
+ * choice[8]: (near variant, near block, m2l group, overlap 0/1) for FP64 then FP32.
+ * ms[60]: per precision, the 25 near timings (order above), the 3 M2L timings (ms/launch), then
+ *         the apply in series and overlapped (0 when not timed).
  * Launch shapes never change results beyond FP summation order.
  */
 int cppe_hip_autotune(cppe_hip_system* sys, int reps, int* choice, double* ms);

@@ -1,7 +1,7 @@
 """GPU checks of the launch-shape autotuner and of every launch shape it can pick.
 
-Each near-field variant x block size and each M2L group width must give the same B apply as the
-default shape (FP64: 1e-13 relative, summation order only; FP32: 1e-6 relative), and a solve with
+Each near-field variant x block size, each M2L group width and the far/near overlap must give the
+same B apply as the default shape (FP64: 1e-13 relative, summation order only; FP32: 1e-6 relative), and a solve with
 the tuned shapes must stop on the FP64 residual with dipoles equal to the default's to 1e-6 (both stop at 1e-8).
 """
 import itertools
@@ -35,13 +35,13 @@ def test_every_launch_shape_same_apply(precision):
     ref = S.apply_b(x)
     tol = 1e-13 if precision == "fp64" else 1e-6
     prec = 0 if precision == "fp64" else 1
-    for (v, b), g in itertools.product(capi.System.NEAR_CANDIDATES, capi.System.M2L_CANDIDATES):
-        choice = np.array([0, 256, 64, 0, 256, 64], dtype=np.int32)
-        choice[3 * prec:3 * prec + 3] = (v, b, g)
-        say("shape", precision, v, b, g)
+    for (v, b), g, o in itertools.product(capi.System.NEAR_CANDIDATES, capi.System.M2L_CANDIDATES, (0, 1)):
+        choice = np.array([0, 256, 64, 0, 0, 256, 64, 0], dtype=np.int32)
+        choice[4 * prec:4 * prec + 4] = (v, b, g, o)
+        say("shape", precision, v, b, g, o)
         S.set_tune(choice)
         y = S.apply_b(x)
-        assert np.abs(y - ref).max() < tol * np.abs(ref).max(), (v, b, g)
+        assert np.abs(y - ref).max() < tol * np.abs(ref).max(), (v, b, g, o)
 
 
 def test_autotune_then_solve():
@@ -57,7 +57,7 @@ def test_autotune_then_solve():
     for prec in ("fp64", "fp32"):
         r = rep[prec]
         assert (r["near_variant"], r["near_block"]) in capi.System.NEAR_CANDIDATES
-        assert r["m2l_group"] in capi.System.M2L_CANDIDATES
+        assert r["m2l_group"] in capi.System.M2L_CANDIDATES and r["overlap"] in (0, 1)
         assert all(t > 0 for t in r["near_ms"].values()) and all(t > 0 for t in r["m2l_ms"].values())
     it, norms, _ = S.solve_resident(0, 1e-8, 100)
     assert norms[-1] < 1e-8
@@ -68,6 +68,8 @@ def test_autotune_then_solve():
 def test_set_tune_rejects_unknown_shapes():
     _, S = system("fp64")
     with pytest.raises(capi.CppeHipError):
-        S.set_tune([2, 256, 64, 0, 256, 64])
+        S.set_tune([9, 256, 64, 0, 0, 256, 64, 0])
     with pytest.raises(capi.CppeHipError):
-        S.set_tune([0, 96, 64, 0, 256, 64])
+        S.set_tune([0, 96, 64, 0, 0, 256, 64, 0])
+    with pytest.raises(capi.CppeHipError):
+        S.set_tune([5, 256, 64, 2, 0, 256, 64, 0])
