@@ -835,15 +835,6 @@ __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __r
   const R yi = sizeof(R) == 8 ? (R)pi.y : (R)(pi.y - cay);
   const R zi = sizeof(R) == 8 ? (R)pi.z : (R)(pi.z - caz);
   const R di = DAMP ? (R)(a.damping * pi.w) : R(0);
-  int lo = i, hi = i;
-  if (a.eoff)
-    for (int k = a.eoff[i]; k < a.eoff[i + 1]; ++k) {
-      const int e = a.eidx[k];
-      if (e >= 0) {
-        lo = min(lo, e);
-        hi = max(hi, e);
-      }
-    }
   StagedRec<R>* sr = rec[wl];
   R fx = R(0), fy = R(0), fz = R(0);
   const long long q1 = xoff[A + 1];
@@ -865,14 +856,15 @@ __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __r
       sr[lane] = r;
     }
     wave_lds_barrier();
+    // every pair of the near list, the self pair contributing zero; excluded near pairs are
+    // taken out afterwards by k_fmm_excl_sub with the same arithmetic
     if (act)
       for (int u = sl; u < cn; u += S) {
         const StagedRec<R> r = sr[u];
-        if (r.j >= lo && r.j <= hi && excluded(i, r.j, a.eoff, a.eidx)) continue;
         if constexpr (sizeof(R) == 8)
-          pair_m1_f64<DAMP>(xi - r.x, yi - r.y, zi - r.z, r.mx, r.my, r.mz, di * r.s, fx, fy, fz);
+          pair_m1_f64<DAMP, true>(xi - r.x, yi - r.y, zi - r.z, r.mx, r.my, r.mz, di * r.s, fx, fy, fz);
         else
-          pair_m1_f32<DAMP>(xi - r.x, yi - r.y, zi - r.z, r.mx, r.my, r.mz, di * r.s, fx, fy, fz);
+          pair_m1_f32<DAMP, true>(xi - r.x, yi - r.y, zi - r.z, r.mx, r.my, r.mz, di * r.s, fx, fy, fz);
       }
     wave_lds_barrier();
   }
@@ -939,6 +931,53 @@ __global__ void k_pack_mu4(int n, const double* __restrict__ mu, MU4* __restrict
   v.z    = mu[2 * (size_t)n + i];
   v.w    = 0;
   out[i] = v;
+}
+
+// Takes the excluded near-field pairs out of a near-field sum that included them: for each site
+// i and each distinct exclusion partner e != i whose leaf is in the near list of i's leaf, out[i]
+// -= pair(i, e), evaluated as the staged kernel does (FP32: offsets from i's leaf centre).
+template <typename R, bool DAMP, typename MU4>
+__global__ __launch_bounds__(256) void k_fmm_excl_sub(NearArgs a, const MU4* __restrict__ mu4) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int k0 = a.eoff[i], k1 = a.eoff[i + 1];
+  if (k0 == k1) return;
+  const int A      = a.leaf_of[i];
+  const int o0 = a.p2p_off[A], o1 = a.p2p_off[A + 1];
+  const double4 pi = a.pos4[i];
+  const double cax = a.cx[A], cay = a.cy[A], caz = a.cz[A];
+  const R di       = DAMP ? (R)(a.damping * pi.w) : R(0);
+  R fx = R(0), fy = R(0), fz = R(0);
+  for (int k = k0; k < k1; ++k) {
+    const int e = a.eidx[k];
+    if (e < 0 || e >= a.n || e == i) continue;
+    bool dup = false;
+    for (int q = k0; q < k; ++q) dup |= (a.eidx[q] == e);
+    if (dup) continue;
+    const int B = a.leaf_of[e];
+    int lo = o0, hi = o1;  // near list of A is sorted by source leaf
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (a.p2p_src[mid] < B)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    if (lo == o1 || a.p2p_src[lo] != B) continue;  // a far-field pair: it stays in
+    const double4 pj = a.pos4[e];
+    const MU4 mj     = mu4[e];
+    if constexpr (sizeof(R) == 8) {
+      pair_m1_f64<DAMP>(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, mj.x, mj.y, mj.z, di * (R)pj.w, fx, fy, fz);
+    } else {
+      const R tx = (R)(pi.x - cax), ty = (R)(pi.y - cay), tz = (R)(pi.z - caz);
+      const R sx = (R)(pj.x - cax), sy = (R)(pj.y - cay), sz = (R)(pj.z - caz);
+      pair_m1_f32<DAMP>(tx - sx, ty - sy, tz - sz, (R)mj.x, (R)mj.y, (R)mj.z, di * (R)pj.w, fx, fy, fz);
+    }
+  }
+  const int n = a.n;
+  a.out[i] -= (double)fx;
+  a.out[n + i] -= (double)fy;
+  a.out[2 * (size_t)n + i] -= (double)fz;
 }
 
 // out += L2P of each site's leaf expansion (the far-field term when it runs beside the near field)
@@ -1104,6 +1143,8 @@ void launch_near(System& S, const NearArgs& a, const MU4* mu4, const R* L, bool 
     else
       hipLaunchKernelGGL((k_fmm_near_st<P, R, DAMP, MODE, MU4, false>), g, b, 0, st, a, F.near_items[ci].p, ni,
                          64 >> ci, T.xoff.p, T.xsrc.p, mu4, L);
+    if (a.eoff)
+      hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a, mu4);
   } else if (var == 4) {
     ensure_clusters(S);
     const dim3 g(ceil_div(F.n_clusters, block / 64)), b(block);

@@ -8,11 +8,12 @@ namespace cppe_dev {
 // Pair kernels. d = r_target - r_source.
 // ---------------------------------------------------------------------------------------------
 // -T2 mu with Thole damping: fE mu / R^3 - 3 fT (d.mu) d / R^5  (P2P<1,3>, field_m1.cc:127-193)
-template <bool DAMP>
+// SELF0: a zero displacement (a site with itself) contributes exactly zero.
+template <bool DAMP, bool SELF0 = false>
 __device__ __forceinline__ void pair_m1_f64(double dx, double dy, double dz, double mx, double my,
                                             double mz, double a, double& fx, double& fy, double& fz) {
   const double r2   = dx * dx + dy * dy + dz * dz;
-  const double rinv = rsqrt_f64(r2);
+  const double rinv = SELF0 ? (r2 > 0.0 ? rsqrt_f64(r2) : 0.0) : rsqrt_f64(r2);
   const double ri2  = rinv * rinv;
   const double ri3  = rinv * ri2;
   double fE = 1.0, fT = 1.0;
@@ -28,11 +29,11 @@ __device__ __forceinline__ void pair_m1_f64(double dx, double dy, double dz, dou
   fz += c3 * mz - c5 * dz;
 }
 
-template <bool DAMP>
+template <bool DAMP, bool SELF0 = false>
 __device__ __forceinline__ void pair_m1_f32(float dx, float dy, float dz, float mx, float my, float mz,
                                             float a, float& fx, float& fy, float& fz) {
   const float r2   = dx * dx + dy * dy + dz * dz;
-  const float rinv = rsqrt_f32(r2);
+  const float rinv = SELF0 ? (r2 > 0.0f ? rsqrt_f32(r2) : 0.0f) : rsqrt_f32(r2);
   const float ri2  = rinv * rinv;
   const float ri3  = rinv * ri2;
   float fE = 1.0f, fT = 1.0f;

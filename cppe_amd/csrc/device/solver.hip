@@ -22,6 +22,7 @@
 // asks for it) and direction (preconditioner, beta, p update). Each is captured once into a HIP
 // graph and replayed; all scalars live on the device, and the host reads a 4-int status block
 // after the step and reliable graphs to decide what to launch next.
+#include <algorithm>
 #include <cmath>
 #include <functional>
 #include <vector>
@@ -39,8 +40,9 @@ constexpr int kChunk = 1024;
 // per-segment doubles (SoA, each nseg long) and ints
 enum SegD { D_RZ = 0, D_ALPHA, D_BETA, D_RNORM, D_RLAST, D_COUNT };
 enum SegI { I_ACTIVE = 0, I_NEEDREL, I_ITERS, I_BAD, I_COUNT };
-// status block: [iteration, active segments, segments asking for a reliable update, bad]
-enum Status { ST_IT = 0, ST_ACTIVE, ST_REL, ST_BAD, ST_COUNT };
+// status block: [iteration, active segments, segments asking for a reliable update, bad,
+// step taken in the current iteration]
+enum Status { ST_IT = 0, ST_ACTIVE, ST_REL, ST_BAD, ST_STEPPED, ST_COUNT };
 enum FinalMode { F_INIT = 0, F_ALPHA, F_RNORM, F_REL, F_BETA };
 
 void apply_b(System& S, const double* x, double* out, bool fp32) {
@@ -269,11 +271,18 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
       sd[D_RLAST * ns + g] = rn;
       sd[D_ALPHA * ns + g] = 0.0;
       sd[D_BETA * ns + g]  = 0.0;
-      const bool bad       = !isfinite(rn);
-      si[I_BAD * ns + g]     = bad;
-      si[I_ACTIVE * ns + g]  = !bad && !(rn < thresh);  // an exact guess needs no iteration
-      si[I_NEEDREL * ns + g] = 0;
-      si[I_ITERS * ns + g]   = 0;
+      const bool bad = !isfinite(rn);
+      si[I_BAD * ns + g] = bad;
+      si[I_ITERS * ns + g] = 0;
+      if (MIXED) {
+        // an FP32 residual within one reliable-update drop of the threshold is replaced by the FP64
+        // true residual before any step (a guess that already converges stops at 0 iterations)
+        si[I_ACTIVE * ns + g]  = !bad;
+        si[I_NEEDREL * ns + g] = !bad && rn < thresh / drop;
+      } else {  // an exact guess needs no iteration
+        si[I_ACTIVE * ns + g]  = !bad && !(rn < thresh);
+        si[I_NEEDREL * ns + g] = 0;
+      }
     } else if (MODE == F_ALPHA) {
       if (si[I_ACTIVE * ns + g]) sd[D_ALPHA * ns + g] = sd[D_RZ * ns + g] / s1;
     } else if (MODE == F_RNORM) {
@@ -308,7 +317,7 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
           si[I_ITERS * ns + g]  = it + 1;
         } else if (rn < thresh) {
           si[I_ACTIVE * ns + g] = 0;
-          si[I_ITERS * ns + g]  = it + 1;
+          si[I_ITERS * ns + g]  = it + f.status[ST_STEPPED];  // 0 when confirmed before any step
         }
       }
     } else {  // F_BETA
@@ -329,8 +338,15 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
     f.status[ST_ACTIVE] = cnt[0];
     f.status[ST_REL]    = cnt[1];
     f.status[ST_BAD]    = cnt[2];
-    if (MODE == F_INIT) f.status[ST_IT] = 0;
-    if (MODE == F_BETA) f.status[ST_IT] = it + 1;
+    if (MODE == F_INIT) {
+      f.status[ST_IT]      = 0;
+      f.status[ST_STEPPED] = 0;
+    }
+    if (MODE == F_RNORM) f.status[ST_STEPPED] = 1;
+    if (MODE == F_BETA) {
+      f.status[ST_IT]      = it + 1;
+      f.status[ST_STEPPED] = 0;
+    }
   }
 }
 
@@ -491,13 +507,19 @@ int device_cg(System& S, double thresh, int maxiter, bool mixed, CgResult& res) 
   if (S.opts.summation == CPPE_HIP_SUM_FMM) fmm_reserve(S);  // no allocation inside a graph
   const double params[2] = {thresh, S.reliable_drop};
   CPPE_HIP_CHECK(hipMemcpyAsync(S.cg_params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
-  // r0 = rhs - B x0 (FP64), z0 = alpha r0, p = z0
-  apply_b(S, S.v_mu.p, S.v_t.p, false);
+  // r0 = rhs - B x0, z0 = alpha r0, p = z0. The mixed form takes r0 from an FP32 apply: it is
+  // replaced by an FP64 true residual at the first reliable update, before any convergence test.
+  apply_b(S, S.v_mu.p, S.v_t.p, mixed);
   hipLaunchKernelGGL(k_sg_init, dim3(S.nchunk), dim3(256), 0, st, vecs(S));
   launch_final<F_INIT>(S, mixed);
   read_status(S);
   int k = 0, nrel = 0;
   const int m = mixed ? 1 : 0;
+  if (S.h_status[ST_REL] > 0) {
+    run_graph(S, S.g_rel, [&] { seq_reliable(S); });
+    read_status(S);
+    ++nrel;
+  }
   while (S.h_status[ST_ACTIVE] > 0 && k < maxiter) {
     run_graph(S, S.g_step[m], [&] { seq_step(S, mixed); });
     read_status(S);

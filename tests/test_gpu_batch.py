@@ -173,3 +173,22 @@ def test_batch_rejects_cross_snapshot_exclusions():
     bad.excl[0] = [bad.n + 1]  # points into the next snapshot once concatenated
     with pytest.raises(capi.CppeHipError):
         capi.Batch([bad, snaps[1]], [True, True], damping=DAMP)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_converged_guess_needs_no_iteration(precision):
+    """A guess that already meets the threshold stops at 0 iterations (mixed form: confirmed by an
+    FP64 true residual), and a zero right-hand side with the alpha F guess converges immediately."""
+    w = synthetic.water_box(300, 7300, anisotropic=True)
+    F = rhs_of(w, "fmm", 64)
+    S = capi.System(w.pos, w.pol, w.alpha_iso(), w.excl_csr(), summation="fmm", damping=DAMP, precision=precision)
+    S.upload_rhs(F)
+    it0, _, _ = S.solve_resident(0, 1e-10, 100)
+    assert it0 > 0
+    it1, norms, _ = S.solve_resident(1, 1e-8, 100)  # restart from the converged moments
+    assert it1 == 0 and len(norms) == 0
+    info = S.last_solve_info()
+    assert info["reliable_updates"] == (1 if precision == "mixed" else 0)
+    S.upload_rhs(np.zeros_like(F))
+    it2, _, _ = S.solve_resident(0, 1e-8, 100)
+    assert it2 == 0 and not S.download_mu().any()
