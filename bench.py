@@ -40,7 +40,9 @@ WORKLOADS = {
     "300k": dict(n_waters=100000, seed=1003, anisotropic=True),
     "60k": dict(n_waters=20000, seed=1002, anisotropic=True),
     "3k": dict(n_waters=1000, seed=1001, anisotropic=False),
+    "batch256": dict(n_snapshots=256, seed=1004),
 }
+BATCH_DIRECT_BELOW = 5000  # sites: such snapshots are summed directly (one leaf), the others by the tree
 
 
 def log(*a):
@@ -67,6 +69,76 @@ def time_solves(S, steps, warmup, guess_mode, thresh=1e-8, maxiter=200):
         dev.append(ms)
         its.append(it)
     return float(np.mean(wall)), float(np.mean(dev)), its
+
+
+def run_batch(args, capi, synthetic):
+    """256 snapshot environments (lognormal water counts), one handle, one CG recurrence each."""
+    spec = WORKLOADS["batch256"]
+    sizes = synthetic.batch_sizes(spec["n_snapshots"], spec["seed"])
+    t0 = time.perf_counter()
+    snaps = [synthetic.water_box(int(nw), spec["seed"] * 1000 + k, anisotropic=True) for k, nw in enumerate(sizes)]
+    direct = [s.n < BATCH_DIRECT_BELOW for s in snaps]
+    n = sum(s.n for s in snaps)
+    log("batch: %d snapshots, %d sites, %d direct (%.1f s to generate)" % (len(snaps), n, sum(direct),
+                                                                           time.perf_counter() - t0))
+    common = dict(ncrit=64, order=5, theta=0.5)
+    Bs = capi.Batch(snaps, direct, **common)
+    Fn = np.concatenate([capi.nuclear_fields(s.pos, s.atoms_z.astype(float), s.atoms_pos) for s in snaps])
+    rhs = Fn + Bs.multipole_fields(np.concatenate([s.charges for s in snaps]),
+                                   np.concatenate([s.dipoles for s in snaps]),
+                                   np.concatenate([s.quadrupoles for s in snaps]))
+    Bs.close()
+    out = {}
+    for prec in ("fp64", "mixed"):
+        t0 = time.perf_counter()
+        B = capi.Batch(snaps, direct, damping=2.1304, precision=prec, **common)
+        setup = (time.perf_counter() - t0) * 1e3
+        B.upload_rhs(rhs)
+        tune = None if args.no_tune else B.autotune(reps=2)
+        res = {}
+        for graphs in (True, False):
+            B.set_graphs(graphs)
+            for _ in range(args.warmup):
+                B.solve_resident(0, 1e-8, 200)
+            wall, dev = [], []
+            for _ in range(args.steps):
+                t1 = time.perf_counter()
+                it, _, ms = B.solve_resident(0, 1e-8, 200)
+                wall.append((time.perf_counter() - t1) * 1e3)
+                dev.append(ms)
+            iters, rn, conv = B.segment_results()
+            res["graphs" if graphs else "no_graphs"] = dict(ms=round(float(np.mean(wall)), 3),
+                                                           device_ms=round(float(np.mean(dev)), 3))
+        assert conv.all()
+        mu = B.download_mu()
+        epol = [-0.5 * float(m @ f) for m, f in zip(B.split(mu), B.split(rhs))]
+        out[prec] = dict(res, setup_ms=round(setup, 1), loop_iterations=int(it),
+                         iterations={"min": int(iters.min()), "median": float(np.median(iters)), "max": int(iters.max())},
+                         max_final_residual=float(rn.max()), info=B.last_solve_info(),
+                         tune={p: {k: v for k, v in t.items() if k in ("near_variant", "near_block", "m2l_group",
+                                                                        "overlap")} for p, t in tune.items()}
+                         if tune else None, epol=epol)
+        log(prec, {k: v for k, v in out[prec].items() if k != "epol"})
+        B.close()
+    e64, e32 = np.array(out["fp64"].pop("epol")), np.array(out["mixed"].pop("epol"))
+    value = out["mixed"]["graphs"]["ms"]
+    line = {
+        "metric": "batched induced-dipole solve ms, 256 snapshots to 1e-8 resid each: FP32 refined vs plain FP64",
+        "value": value, "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": value,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32 applies in f64 refinement",
+        "data": "synthetic (seeded water boxes, seeds %d000 + k)" % spec["seed"],
+        "config": {"workload": "256 snapshots, waters lognormal(ln 2000, 0.8) clipped to 500..8000, direct below "
+                               "%d sites else tree (theta 0.5, order 5, ncrit 64), Thole 2.1304" % BATCH_DIRECT_BELOW,
+                   "sites": int(n), "snapshots": len(snaps), "direct_snapshots": int(sum(direct)),
+                   "parallelism": "single-gpu"},
+        "plain_fp64_ms": out["fp64"]["graphs"]["ms"],
+        "speedup_vs_fp64": round(out["fp64"]["graphs"]["ms"] / value, 3),
+        "sites_per_s": round(n / (value * 1e-3), 1),
+        "epol_max_rel_diff": float(np.max(np.abs(e32 - e64) / np.abs(e64))),
+        "forms": out,
+    }
+    print(json.dumps(line))
+    return 0
 
 
 def cpu_baseline(w, iters, ncrit, order, theta, damping):
@@ -98,6 +170,7 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=5)
     ap.add_argument("--no-ml", action="store_true", help="headline form without the learned warm start")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuned ones")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the 2,000-site direct-summation check")
     ap.add_argument("--reliable-drop", type=float, default=None,
                     help="mixed form: residual drop that triggers an FP64 true residual (library default 1e-5)")
     ap.add_argument("--apply-damping", type=float, default=2.1304, help="Thole factor of --apply-only runs")
@@ -116,6 +189,8 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         return 1
     ncrit, order, theta, damping = 64, 5, 0.5, 2.1304
+    if args.workload == "batch256":
+        return run_batch(args, capi, synthetic)
     spec = WORKLOADS[args.workload]
     log("generating workload", args.workload)
     w = synthetic.water_box(**spec)
@@ -188,6 +263,24 @@ def main():
     e64 = -0.5 * float(mu64 @ rhs)
     e32 = -0.5 * float(mu32 @ rhs)
 
+    acc = None
+    if args.summation == "fmm" and not args.no_accuracy:
+        # tree path against FP64 direct summation at 2,000 seeded sites, for the headline moments
+        tg = np.sort(np.random.default_rng(2000).choice(w.n, min(2000, w.n), replace=False))
+        Fd = S64.field_targets(tg, mu=mu32) + S64.apply_alpha_inv(mu32).reshape(-1, 3)[tg]
+        B64 = S64.apply_b(mu32).reshape(-1, 3)[tg]
+        B32 = S32.apply_b(mu32).reshape(-1, 3)[tg]
+        nd = np.linalg.norm(Fd, axis=1)
+        acc = {"targets": int(tg.size), "seed": 2000,
+               "bmu_rel_err_fp64_tree_vs_direct": {"max": float((np.linalg.norm(B64 - Fd, axis=1) / nd).max()),
+                                                   "rms": float(np.linalg.norm(B64 - Fd) / np.linalg.norm(Fd))},
+               "bmu_rel_err_fp32_tree_vs_direct": {"max": float((np.linalg.norm(B32 - Fd, axis=1) / nd).max()),
+                                                   "rms": float(np.linalg.norm(B32 - Fd) / np.linalg.norm(Fd))},
+               "bmu_rel_diff_fp32_vs_fp64_tree": float(np.linalg.norm(B32 - B64) / np.linalg.norm(B64)),
+               "residual_direct_operator_at_targets": float(np.linalg.norm(rhs.reshape(-1, 3)[tg] - Fd)),
+               "residual_tree_operator_at_targets": float(np.linalg.norm(rhs.reshape(-1, 3)[tg] - B64))}
+        log("accuracy", acc)
+
     st = S64.tree_stats()
     n_excl = int(w.excl_csr()[0][-1])
     pairs = st["p2p_site_pairs"] - w.n - n_excl if args.summation == "fmm" else w.n * (w.n - 1) - n_excl
@@ -196,22 +289,29 @@ def main():
     log("phases fp32", prof32)
     log("phases fp64", prof64)
 
-    def roofline(prof, peak):
+    near_names = {0: "k_fmm_near", 1: "k_fmm_near_lw", 2: "k_fmm_p2p", 3: "k_fmm_near_sl", 4: "k_fmm_near_cl",
+                  5: "k_fmm_near_st", 6: "k_fmm_near_st", 7: "k_fmm_near_st", 8: "k_fmm_near_st"}
+
+    def roofline(prof, peak, prec):
         near_flop = FLOP_PAIR_DAMPED * pairs
         m2l_flop = FLOP_M2L_M1_P5 * st["n_m2l"]
         if args.summation == "fmm" and prof["m2l"] > prof["near"]:
             name, flop, ms = "k_fmm_m2l (dipole M2L, order 5)", m2l_flop, prof["m2l"]
         else:
-            name = "k_fmm_p2p (near field P2P+L2P)" if args.summation == "fmm" else "k_apply_b (direct pairs)"
+            if args.summation == "fmm":
+                v = tune[prec]["near_variant"] if tune else 0
+                name = "%s (near field P2P + alpha^-1 mu)" % near_names[v]
+            else:
+                name = "k_apply_b (direct pairs)"
             flop, ms = near_flop, prof["near"]
         ach = flop / (ms * 1e-3)
         return {"bound": "mfma" if False else "valu", "kernel": name, "achieved": round(ach / 1e12, 3),
                 "peak": peak / 1e12, "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": None,
                 "flop_per_launch": flop, "ms_per_launch": round(ms, 4)}
 
-    roof = roofline(prof32, PEAK_FP32)
+    roof = roofline(prof32, PEAK_FP32, "fp32")
     roof["bound"] = "fp32-valu"
-    roof64 = roofline(prof64, PEAK_FP64)
+    roof64 = roofline(prof64, PEAK_FP64, "fp64")
     roof64["bound"] = "fp64-valu"
 
     cpu = None
@@ -248,6 +348,7 @@ def main():
         "ml_guess_ms": round(float(np.mean(guess_ms)), 3) if guess_ms else None,
         "mixed_alphaF_ms": round(wallm, 3),
         "mixed_alphaF_iterations": itsm,
+        "accuracy": acc,
         "epol_fp64": e64,
         "epol_rel_diff": abs(e32 - e64) / abs(e64),
         "sites_per_s": round(w.n / (wall32 * 1e-3), 1),

@@ -22,6 +22,7 @@ EXPORTS = [
     "cppe_hip_nuclear_multipole_energy", "cppe_hip_dot", "cppe_hip_tree_stats", "cppe_hip_tree_export",
     "cppe_hip_profile_apply", "cppe_hip_batch_create", "cppe_hip_segment_results", "cppe_hip_set_graphs",
     "cppe_hip_autotune", "cppe_hip_set_tune", "cppe_hip_set_reliable_drop", "cppe_hip_last_solve_info",
+    "cppe_hip_field_targets",
     "cppe_hip_mlp_create", "cppe_hip_mlp_destroy", "cppe_hip_mlp_set", "cppe_hip_mlp_get", "cppe_hip_ml_features",
     "cppe_hip_mlp_forward", "cppe_hip_mlp_loss_grad", "cppe_hip_mlp_reserve", "cppe_hip_mlp_collect",
     "cppe_hip_mlp_prepare", "cppe_hip_mlp_train", "cppe_hip_ml_guess",
@@ -83,6 +84,7 @@ def lib():
         L.cppe_hip_set_tune.argtypes = [vp, _i]
         L.cppe_hip_set_reliable_drop.argtypes = [vp, ctypes.c_double]
         L.cppe_hip_last_solve_info.argtypes = [vp, _i]
+        L.cppe_hip_field_targets.argtypes = [vp, _d, ctypes.c_int, _i, _d]
         _f = ctypes.POINTER(ctypes.c_float)
         L.cppe_hip_mlp_create.argtypes = [ctypes.POINTER(vp)]
         L.cppe_hip_mlp_destroy.argtypes = [vp]
@@ -239,6 +241,14 @@ class System:
     def set_tune(self, choice):
         c = np.ascontiguousarray(choice, dtype=np.int32)
         _check(lib().cppe_hip_set_tune(self.h, _p(c, _i)))
+
+    def field_targets(self, targets, mu=None):
+        """FP64 direct dipole field (P2P<1,3> convention) at the given sites, for mu or the resident moments."""
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        m = None if mu is None else np.ascontiguousarray(mu, dtype=np.float64)
+        out = np.zeros(3 * t.size)
+        _check(lib().cppe_hip_field_targets(self.h, _p(m), t.size, _p(t, _i), _p(out)))
+        return out.reshape(-1, 3)
 
     def set_reliable_drop(self, drop):
         _check(lib().cppe_hip_set_reliable_drop(self.h, float(drop)))

@@ -14,6 +14,7 @@ namespace cppe_dev {
 
 void dense_inverse(hipStream_t st, int N, double* A, double* X);
 void fmm_autotune(System& S, int reps, int* choice, double* ms);
+void field1_targets(System& S, const double* mu_soa, int nt, const int* tdev, double* out);
 void fmm_set_tune(System& S, const int* choice);
 
 __global__ void k_scatter_xyz(int n, const double* __restrict__ aos, double* x, double* y, double* z) {
@@ -541,6 +542,34 @@ int cppe_hip_solve_resident(cppe_hip_system* sys, int guess_mode, double thresh,
   CPPE_API_BEGIN
   if (!sys || guess_mode < 0 || guess_mode > 2) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
   return run_solve(sys->S, guess_mode, nullptr, thresh, maxiter, iters, norms, device_ms);
+  CPPE_API_END
+}
+
+int cppe_hip_field_targets(cppe_hip_system* sys, const double* mu, int nt, const int* targets, double* F) {
+  CPPE_API_BEGIN
+  if (!sys || nt < 0 || (nt > 0 && (!targets || !F))) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
+  System& S = sys->S;
+  if (nt == 0) return CPPE_HIP_OK;
+  std::vector<int> td(nt);
+  for (int t = 0; t < nt; ++t) {
+    if (targets[t] < 0 || targets[t] >= S.n) return fail(CPPE_HIP_ERR_ARG, "target index out of range");
+    td[t] = S.iperm_host(targets[t]);
+  }
+  const double* src = S.v_mu.p;
+  if (mu) {
+    host_to_soa(S, mu, 3, S.v_d.p);
+    src = S.v_d.p;
+  } else if (!S.mu_valid) {
+    return fail(CPPE_HIP_ERR_ARG, "system has no resident moments");
+  }
+  dbuf<int> dt;
+  dbuf<double> dout;
+  upload(dt, td.data(), (size_t)nt, S.stream);
+  dout.alloc(3 * (size_t)nt);
+  field1_targets(S, src, nt, dt.p, dout.p);
+  CPPE_HIP_CHECK(hipMemcpyAsync(F, dout.p, sizeof(double) * 3 * (size_t)nt, hipMemcpyDeviceToHost, S.stream));
+  CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
+  return CPPE_HIP_OK;
   CPPE_API_END
 }
 

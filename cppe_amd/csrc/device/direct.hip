@@ -286,6 +286,55 @@ void apply_b_direct(System& S, const double* xin, double* out, bool fp32) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// Dipole field (P2P<1,3> convention, Thole-damped with the system's factor, exclusions as in the
+// system) at selected targets by FP64 direct summation over every site of the target's snapshot:
+// one block per target, sources strided over the threads, fixed-order block reduction. Used to
+// measure the tree path's error against direct summation at sampled sites.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_field1_targets(int nt, const int* __restrict__ tdev, int n,
+                                                        const double* __restrict__ x, const double* __restrict__ y,
+                                                        const double* __restrict__ z, const double* __restrict__ s,
+                                                        double damping, const int* __restrict__ eoff,
+                                                        const int* __restrict__ eidx,
+                                                        const int* __restrict__ seg_of,
+                                                        const int* __restrict__ seg_off,
+                                                        const double* __restrict__ mu, double* __restrict__ out) {
+  __shared__ double red[3][256];
+  const int t = blockIdx.x;
+  if (t >= nt) return;
+  const int i  = tdev[t];
+  const int g  = seg_of[i];
+  const int j0 = seg_off[g], j1 = seg_off[g + 1];
+  const double xi = x[i], yi = y[i], zi = z[i];
+  const double di = damping * s[i];
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  for (int j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+    if (excluded(i, j, eoff, eidx)) continue;
+    if (damping > 0.0)
+      pair_m1_f64<true>(xi - x[j], yi - y[j], zi - z[j], mu[j], mu[n + j], mu[2 * (size_t)n + j], di * s[j], fx, fy,
+                        fz);
+    else
+      pair_m1_f64<false>(xi - x[j], yi - y[j], zi - z[j], mu[j], mu[n + j], mu[2 * (size_t)n + j], 0.0, fx, fy,
+                         fz);
+  }
+  red[0][threadIdx.x] = fx;
+  red[1][threadIdx.x] = fy;
+  red[2][threadIdx.x] = fz;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) out[3 * (size_t)t + threadIdx.x] = red[threadIdx.x][0];
+}
+
+void field1_targets(System& S, const double* mu_soa, int nt, const int* tdev, double* out) {
+  hipLaunchKernelGGL(k_field1_targets, dim3(nt), dim3(256), 0, S.stream, nt, tdev, S.n, S.x.p, S.y.p, S.z.p, S.s.p,
+                     S.opts.damping, S.excl_off.p, S.excl_idx.p, S.seg_of.p, S.seg_off.p, mu_soa, out);
+}
+
+// ---------------------------------------------------------------------------------------------
 // Nuclear fields (electric_fields.cc:58-75) and the nuclear-multipole energy
 // (multipole_expansion.cc:10-33), caller order, atoms staged in LDS.
 // ---------------------------------------------------------------------------------------------

@@ -39,10 +39,14 @@ constexpr int kChunk = 1024;
 
 // per-segment doubles (SoA, each nseg long) and ints
 enum SegD { D_RZ = 0, D_ALPHA, D_BETA, D_RNORM, D_RLAST, D_COUNT };
-enum SegI { I_ACTIVE = 0, I_NEEDREL, I_ITERS, I_BAD, I_COUNT };
+// I_PENDING (mixed form): recursive residual below the threshold, frozen until an FP64 true
+// residual confirms convergence or resumes the segment
+// I_ELIGIBLE (mixed form): residual within 100x of its replacement trigger; replaced whenever a
+// replacement runs for another segment, so that segments share FP64 applies
+enum SegI { I_ACTIVE = 0, I_NEEDREL, I_ITERS, I_BAD, I_PENDING, I_ELIGIBLE, I_COUNT };
 // status block: [iteration, active segments, segments asking for a reliable update, bad,
-// step taken in the current iteration]
-enum Status { ST_IT = 0, ST_ACTIVE, ST_REL, ST_BAD, ST_STEPPED, ST_COUNT };
+// step taken in the current iteration, segments pending an FP64 confirmation]
+enum Status { ST_IT = 0, ST_ACTIVE, ST_REL, ST_BAD, ST_STEPPED, ST_PEND, ST_COUNT };
 enum FinalMode { F_INIT = 0, F_ALPHA, F_RNORM, F_REL, F_BETA };
 
 void apply_b(System& S, const double* x, double* out, bool fp32) {
@@ -174,12 +178,12 @@ __global__ __launch_bounds__(256) void k_sg_xr(CgVecs v) {
   if (threadIdx.x == 0) v.part[c] = s;
 }
 
-// r = rhs - t for the segments that asked for a reliable update; partial r.r
+// r = rhs - t for the segments that asked for a reliable update or await confirmation; partial r.r
 __global__ __launch_bounds__(256) void k_sg_resid(CgVecs v) {
   __shared__ double red[4];
   const int c = blockIdx.x, n = v.n, g = v.cseg[c];
   double s = 0.0;
-  if (v.si[I_NEEDREL * v.nseg + g])
+  if (v.si[I_NEEDREL * v.nseg + g] || v.si[I_PENDING * v.nseg + g] || v.si[I_ELIGIBLE * v.nseg + g])
     for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x)
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
@@ -242,14 +246,14 @@ struct CgFinal {
 // the segment's scalars; the status block is refreshed at the end.
 template <int MODE, bool MIXED>
 __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
-  __shared__ int cnt[3];
-  if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
+  __shared__ int cnt[4];
+  if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
   __syncthreads();
   const int ns = f.nseg;
   const int it = f.status[ST_IT];
   const double thresh = f.params[0], drop = f.params[1];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  int na = 0, nr = 0, nb = 0;
+  int na = 0, nr = 0, nb = 0, np = 0;
   for (int g = wv; g < ns; g += 4) {
     double s1 = 0.0, s2 = 0.0;
     for (int c = f.seg_chunk[g] + lane; c < f.seg_chunk[g + 1]; c += 64) {
@@ -272,8 +276,10 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
       sd[D_ALPHA * ns + g] = 0.0;
       sd[D_BETA * ns + g]  = 0.0;
       const bool bad = !isfinite(rn);
-      si[I_BAD * ns + g] = bad;
-      si[I_ITERS * ns + g] = 0;
+      si[I_BAD * ns + g]      = bad;
+      si[I_ITERS * ns + g]    = 0;
+      si[I_PENDING * ns + g]  = 0;
+      si[I_ELIGIBLE * ns + g] = 0;
       if (MIXED) {
         // an FP32 residual within one reliable-update drop of the threshold is replaced by the FP64
         // true residual before any step (a guess that already converges stops at 0 iterations)
@@ -289,12 +295,18 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
       if (si[I_ACTIVE * ns + g]) {
         const double rn      = sqrt(s1);
         sd[D_RNORM * ns + g] = rn;
+        si[I_ELIGIBLE * ns + g] = MIXED && isfinite(rn) && rn < 100.0 * drop * sd[D_RLAST * ns + g];
         if (!isfinite(rn)) {
           si[I_BAD * ns + g]    = 1;
           si[I_ACTIVE * ns + g] = 0;
           si[I_ITERS * ns + g]  = it + 1;
           if (it < f.hist_cap) f.hist[(size_t)it * ns + g] = rn;
-        } else if (MIXED && (rn < thresh || rn < drop * sd[D_RLAST * ns + g])) {
+        } else if (MIXED && rn < thresh) {
+          // frozen until an FP64 true residual confirms it; confirmations are batched
+          si[I_PENDING * ns + g] = 1;
+          si[I_ACTIVE * ns + g]  = 0;
+          si[I_ITERS * ns + g]   = it + 1;
+        } else if (MIXED && rn < drop * sd[D_RLAST * ns + g]) {
           si[I_NEEDREL * ns + g] = 1;  // recorded after the replacement
         } else {
           if (it < f.hist_cap) f.hist[(size_t)it * ns + g] = rn;
@@ -305,7 +317,8 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
         }
       }
     } else if (MODE == F_REL) {
-      if (si[I_NEEDREL * ns + g]) {
+      if (si[I_NEEDREL * ns + g] || (si[I_ELIGIBLE * ns + g] && si[I_ACTIVE * ns + g])) {
+        si[I_ELIGIBLE * ns + g] = 0;
         const double rn        = sqrt(s1);
         sd[D_RNORM * ns + g]   = rn;
         sd[D_RLAST * ns + g]   = rn;
@@ -319,6 +332,18 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
           si[I_ACTIVE * ns + g] = 0;
           si[I_ITERS * ns + g]  = it + f.status[ST_STEPPED];  // 0 when confirmed before any step
         }
+      } else if (si[I_PENDING * ns + g]) {
+        const double rn        = sqrt(s1);
+        const int k0           = si[I_ITERS * ns + g] - 1;  // iteration at which it froze
+        sd[D_RNORM * ns + g]   = rn;
+        sd[D_RLAST * ns + g]   = rn;
+        si[I_PENDING * ns + g] = 0;
+        if (k0 >= 0 && k0 < f.hist_cap) f.hist[(size_t)k0 * ns + g] = rn;
+        if (!isfinite(rn)) {
+          si[I_BAD * ns + g] = 1;
+        } else if (!(rn < thresh)) {
+          si[I_ACTIVE * ns + g] = 1;  // resumes from the true residual
+        }
       }
     } else {  // F_BETA
       if (si[I_ACTIVE * ns + g]) {
@@ -329,22 +354,25 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
     na += si[I_ACTIVE * ns + g];
     nr += si[I_NEEDREL * ns + g];
     nb += si[I_BAD * ns + g];
+    np += si[I_PENDING * ns + g];
   }
   if (na) atomicAdd(&cnt[0], na);
   if (nr) atomicAdd(&cnt[1], nr);
   if (nb) atomicAdd(&cnt[2], nb);
+  if (np) atomicAdd(&cnt[3], np);
   __syncthreads();
   if (threadIdx.x == 0) {
     f.status[ST_ACTIVE] = cnt[0];
     f.status[ST_REL]    = cnt[1];
     f.status[ST_BAD]    = cnt[2];
+    f.status[ST_PEND]   = cnt[3];
     if (MODE == F_INIT) {
       f.status[ST_IT]      = 0;
       f.status[ST_STEPPED] = 0;
     }
     if (MODE == F_RNORM) f.status[ST_STEPPED] = 1;
-    if (MODE == F_BETA) {
-      f.status[ST_IT]      = it + 1;
+    if (MODE == F_BETA) {  // counts iterations that took a step (not a resumption after confirmation)
+      f.status[ST_IT]      = it + f.status[ST_STEPPED];
       f.status[ST_STEPPED] = 0;
     }
   }
@@ -520,18 +548,29 @@ int device_cg(System& S, double thresh, int maxiter, bool mixed, CgResult& res) 
     read_status(S);
     ++nrel;
   }
-  while (S.h_status[ST_ACTIVE] > 0 && k < maxiter) {
+  // Loop invariant at the top: every active segment has completed step, replacement and
+  // direction of its last iteration. Pending segments (mixed form) are confirmed together: inside a
+  // replacement that runs anyway, or once no segment is active.
+  auto reliable = [&] {
+    run_graph(S, S.g_rel, [&] { seq_reliable(S); });
+    read_status(S);
+    ++nrel;
+  };
+  while (k < maxiter) {
+    if (S.h_status[ST_ACTIVE] == 0) {
+      if (S.h_status[ST_PEND] == 0) break;
+      reliable();
+      if (S.h_status[ST_ACTIVE] == 0) break;
+      run_graph(S, S.g_dir[m], [&] { seq_direction(S, mixed); });
+      continue;
+    }
     run_graph(S, S.g_step[m], [&] { seq_step(S, mixed); });
     read_status(S);
-    if (S.h_status[ST_REL] > 0) {
-      run_graph(S, S.g_rel, [&] { seq_reliable(S); });
-      read_status(S);
-      ++nrel;
-    }
     ++k;
-    if (S.h_status[ST_ACTIVE] == 0) break;
-    run_graph(S, S.g_dir[m], [&] { seq_direction(S, mixed); });
+    if (S.h_status[ST_REL] > 0 || (S.h_status[ST_ACTIVE] == 0 && S.h_status[ST_PEND] > 0)) reliable();
+    if (S.h_status[ST_ACTIVE] > 0) run_graph(S, S.g_dir[m], [&] { seq_direction(S, mixed); });
   }
+  if (S.h_status[ST_PEND] > 0) reliable();  // maxiter reached with unconfirmed segments
   // per-segment results
   std::vector<int> si((size_t)I_COUNT * ns);
   std::vector<double> sd((size_t)D_COUNT * ns);
@@ -548,7 +587,7 @@ int device_cg(System& S, double thresh, int maxiter, bool mixed, CgResult& res) 
   res.norms0.clear();
   res.all_converged = true;
   for (int g = 0; g < ns; ++g) {
-    const bool active = si[I_ACTIVE * ns + g] != 0;
+    const bool active = si[I_ACTIVE * ns + g] != 0 || si[I_PENDING * ns + g] != 0;
     res.iters[g]      = active ? k : si[I_ITERS * ns + g];
     res.rnorm[g]      = sd[D_RNORM * ns + g];
     res.converged[g]  = !active && !si[I_BAD * ns + g] && res.rnorm[g] < thresh;

@@ -81,6 +81,14 @@ struct System {
   dbuf<int> perm;                // device order -> caller index
   dbuf<int> iperm;               // caller index -> device order
   std::vector<int> h_perm;
+  std::vector<int> h_iperm;  // caller index -> device order (filled on first use)
+  int iperm_host(int i) {
+    if (h_iperm.empty()) {
+      h_iperm.resize(n);
+      for (int k = 0; k < n; ++k) h_iperm[h_perm[k]] = k;
+    }
+    return h_iperm[i];
+  }
   std::vector<int> h_excl_off, h_excl_idx;  // caller order, values as given
 
   Octree tree;

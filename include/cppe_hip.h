@@ -141,6 +141,15 @@ int cppe_hip_download_mu(cppe_hip_system* sys, double* mu);
 int cppe_hip_batch_create(cppe_hip_system** out, int nseg, const int* seg_off, const int* seg_ncrit,
                           const double* pos, const double* alpha6, const double* alpha_iso, const int* excl_off,
                           const int* excl_idx, const cppe_hip_options* opts);
+/*
+ * Dipole field (the P2P<1,3> convention of cppe_hip_field, m = 1: Thole-damped with the system's
+ * factor, exclusions as in the system) at nt target sites (caller indices) by FP64 direct
+ * summation over all sites of the target's snapshot, for mu (AoS3, caller order) or, with
+ * mu == NULL, the system's resident moments. A measuring aid: it samples the tree path's error
+ * against direct summation (Tree::compute_field_exact, tree.cc:387-394, restricted to targets).
+ */
+int cppe_hip_field_targets(cppe_hip_system* sys, const double* mu, int nt, const int* targets, double* F);
+
 /* Per-snapshot outcome of the last solve (arrays of nseg; any may be NULL). */
 int cppe_hip_segment_results(cppe_hip_system* sys, int* iters, double* rnorm, int* converged);
 /* CG iterations replayed as captured HIP graphs (default 1) or launched kernel by kernel (0). */

@@ -204,3 +204,16 @@ def test_resident_solve_matches():
     it2, _, ms = S.solve_resident(0)
     assert it2 == it and ms > 0
     np.testing.assert_array_equal(S.download_mu(), mu)  # deterministic reductions
+
+
+@pytest.mark.parametrize("damping", [0.0, 2.1304])
+def test_field_targets_vs_oracle(damping):
+    """FP64 direct dipole field at sampled targets (the measuring aid of the tree path's error)."""
+    w = water(700, 1007, True)
+    rng = np.random.default_rng(17)
+    mu = rng.normal(size=(w.n, 3)) * 0.1
+    targets = rng.choice(w.n, 97, replace=False)
+    S = capi.System(w.pos, w.pol, w.alpha_iso(), w.excl_csr(), summation="fmm", damping=damping)
+    F = S.field_targets(targets, mu=mu)
+    ref = O.direct_field_targets(1, w.pos, mu, targets, w.alpha_iso() if damping else None, w.excl_csr(), damping)
+    assert relerr(F.reshape(-1), ref) < 1e-12
