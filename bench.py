@@ -35,6 +35,9 @@ PEAK_FP64 = 78.6e12
 PEAK_FP32 = 157.3e12
 FLOP_PAIR_DAMPED = 47
 FLOP_M2L_M1_P5 = 950
+# HBM bytes per launch of the staged near-field kernel on the 300k workload, measured with rocprofv3
+# FETCH_SIZE and WRITE_SIZE runs calibrated on a copy kernel (profiles/near_field_300k_hbm_traffic.md)
+MEASURED_TRAFFIC_300K = {"fp32": 724.7e6, "fp64": 1099.6e6}
 
 WORKLOADS = {
     "300k": dict(n_waters=100000, seed=1003, anisotropic=True),
@@ -305,8 +308,11 @@ def main():
                 name = "k_apply_b (direct pairs)"
             flop, ms = near_flop, prof["near"]
         ach = flop / (ms * 1e-3)
-        return {"bound": "mfma" if False else "valu", "kernel": name, "achieved": round(ach / 1e12, 3),
-                "peak": peak / 1e12, "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": None,
+        traffic = None
+        if args.workload == "300k" and "k_fmm_near_st" in name:
+            traffic = MEASURED_TRAFFIC_300K["fp32" if prec == "fp32" else "fp64"]
+        return {"bound": "valu", "kernel": name, "achieved": round(ach / 1e12, 3),
+                "peak": peak / 1e12, "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": traffic,
                 "flop_per_launch": flop, "ms_per_launch": round(ms, 4)}
 
     roof = roofline(prof32, PEAK_FP32, "fp32")
