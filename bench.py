@@ -259,8 +259,11 @@ def main():
     info32 = S32.last_solve_info()
     log("headline solve ms", wall32, "iters", its32, "ml", ml_used, "fp64 true-residual applies",
         info32["reliable_updates"])
-    # the same mixed solve from alpha F, for the ML gain
+    # the same mixed solve from alpha F, for the ML gain; then once more launched kernel by kernel
     wallm, devm, itsm = time_solves(S32, max(1, args.steps // 2), 0, 0)
+    S32.set_graphs(False)
+    walln, _, _ = time_solves(S32, max(1, args.steps // 2), 1, 0)
+    S32.set_graphs(True)
     mu64 = S64.download_mu()
     mu32 = S32.download_mu()
     e64 = -0.5 * float(mu64 @ rhs)
@@ -354,6 +357,7 @@ def main():
         "ml_guess_ms": round(float(np.mean(guess_ms)), 3) if guess_ms else None,
         "mixed_alphaF_ms": round(wallm, 3),
         "mixed_alphaF_iterations": itsm,
+        "mixed_alphaF_no_graphs_ms": round(walln, 3),
         "accuracy": acc,
         "epol_fp64": e64,
         "epol_rel_diff": abs(e32 - e64) / abs(e64),
