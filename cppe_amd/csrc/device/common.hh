@@ -52,8 +52,10 @@ __device__ __forceinline__ void thole_ET(R v, R e, R& fE, R& fT) {
   fT = fE - (v2 * v * R(1.0 / 6.0)) * e;
 }
 
-// Above this scaled distance e^-v * v^4 is below 2^-64 relative; the pair is treated as
-// undamped (screening factors equal 1 to FP64 precision).
-constexpr double kTholeCutoff = 50.0;
+// Above these scaled distances the pair is treated as undamped: the screening terms
+// e^-v (1 + v + v^2/2 + v^3/6) are below 5e-14 (FP64, v >= 40) and 1e-7 (FP32, v >= 25) relative,
+// far below the rounding of the forms that use them.
+constexpr double kTholeCutoff  = 40.0;
+constexpr float kTholeCutoff32 = 25.0f;
 
 }  // namespace cppe_dev

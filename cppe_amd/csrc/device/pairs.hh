@@ -39,8 +39,7 @@ __device__ __forceinline__ void pair_m1_f32(float dx, float dy, float dz, float 
   float fE = 1.0f, fT = 1.0f;
   if (DAMP) {
     const float u = a * (r2 * rinv);
-    const float e = (u < 50.0f && a > 0.0f) ? expm_f32(u) : 0.0f;
-    thole_ET(u, e, fE, fT);
+    if (u < kTholeCutoff32 && a > 0.0f) thole_ET(u, expm_f32(u), fE, fT);
   }
   const float dmu = dx * mx + dy * my + dz * mz;
   const float c3  = fE * ri3;

@@ -144,15 +144,16 @@ def gen_m2l(K, prec):
     lines = [f"__device__ __forceinline__ void m2l_{K.name}_p{K.p}_{prec}({R} dx, {R} dy, {R} dz, const {R}* __restrict__ M, {R}* __restrict__ L) {{",
              f"  using R = {R};"]
     gen_derivs(K.p, lines, rinv)
-    for n in K.llist:
+    # contraction ordered by source coefficient: each M_j is loaded once and scattered into every
+    # L_n it reaches, so only the derivatives and the L accumulators stay live
+    for j in K.mlist:
         terms = []
-        for j in K.mlist:
+        for n in K.llist:
             s = (n[0] + j[0], n[1] + j[1], n[2] + j[2])
-            if sum(s) > K.p:
-                continue
-            terms.append(f"M[{K.midx[j]}] * h0_{s[0]}_{s[1]}_{s[2]}")
+            if sum(s) <= K.p:
+                terms.append(f"L[{K.lidx[n]}] += m * h0_{s[0]}_{s[1]}_{s[2]};")
         if terms:
-            lines.append(f"  L[{K.lidx[n]}] += " + " + ".join(terms) + ";")
+            lines.append(f"  {{ const R m = M[{K.midx[j]}]; " + " ".join(terms) + " }")
     lines.append("}")
     return "\n".join(lines)
 
