@@ -225,14 +225,17 @@ def main():
         S32.set_reliable_drop(args.reliable_drop)
     log("setup (octree + lists) ms", round(t_setup, 1))
     tune = None
-    if args.summation == "fmm" and not args.no_tune:
+    if not args.no_tune:
+        log("autotuning launch shapes ...")
         t0 = time.perf_counter()
         tune = {"fp64": S64.autotune(reps=3)["fp64"], "fp32": S32.autotune(reps=3)["fp32"]}
         tune["seconds"] = round(time.perf_counter() - t0, 2)
         log("autotune", tune)
 
     from cppe_amd import warmstart
-    model = None if args.no_ml else warmstart.load()
+    # the shipped network was trained on tree-summation solutions of anisotropic boxes; direct
+    # summation (every pair damped) is outside that domain, where alpha F is the better guess
+    model = None if (args.no_ml or args.summation != "fmm") else warmstart.load()
     ml_used = model is not None
     wall64, dev64, its64 = time_solves(S64, args.steps, args.warmup, 0)
     log("fp64 solve ms", wall64, "iters", its64)
@@ -304,11 +307,9 @@ def main():
         if args.summation == "fmm" and prof["m2l"] > prof["near"]:
             name, flop, ms = "k_fmm_m2l (dipole M2L, order 5)", m2l_flop, prof["m2l"]
         else:
-            if args.summation == "fmm":
-                v = tune[prec]["near_variant"] if tune else 0
-                name = "%s (near field P2P + alpha^-1 mu)" % near_names[v]
-            else:
-                name = "k_apply_b (direct pairs)"
+            v = tune[prec]["near_variant"] if tune else 0
+            name = "%s (%s P2P + alpha^-1 mu)" % (near_names[v], "near field" if args.summation == "fmm"
+                                                   else "all pairs, one-leaf tree")
             flop, ms = near_flop, prof["near"]
         ach = flop / (ms * 1e-3)
         traffic = None

@@ -220,7 +220,7 @@ class System:
     M2L_CANDIDATES = [16, 32, 64]
 
     def autotune(self, reps=3):
-        """Times the B-apply launch candidates on this system and keeps the fastest (tree path)."""
+        """Times the B-apply launch candidates on this system and keeps the fastest."""
         choice = np.zeros(8, dtype=np.int32)
         nn, nm = len(self.NEAR_CANDIDATES), len(self.M2L_CANDIDATES)
         w = nn + nm + 2

@@ -16,6 +16,7 @@ void dense_inverse(hipStream_t st, int N, double* A, double* X);
 void fmm_autotune(System& S, int reps, int* choice, double* ms);
 void field1_targets(System& S, const double* mu_soa, int nt, const int* tdev, double* out);
 void fmm_set_tune(System& S, const int* choice);
+bool near_variant_allowed(System& S, int variant);
 
 __global__ void k_scatter_xyz(int n, const double* __restrict__ aos, double* x, double* y, double* z) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -190,7 +191,7 @@ static cppe_hip_system* create_system(int n, const double* pos, const double* al
                          idx_h.p, S.excl_off.p, S.excl_idx.p);
       CPPE_HIP_CHECK(hipStreamSynchronize(st));
     }
-    if (opts->summation == CPPE_HIP_SUM_FMM) build_lists(S);
+    build_lists(S);  // direct summation: one leaf whose near list is itself
     const size_t n3 = 3 * (size_t)n;
     S.v_mu.alloc(n3);
     S.v_rhs.alloc(n3);
@@ -230,7 +231,12 @@ int cppe_hip_system_create(cppe_hip_system** out, int n, const double* pos, cons
   if (!out) return fail(CPPE_HIP_ERR_ARG, "invalid system arguments");
   const int rc = check_create_args(n, pos, opts);
   if (rc != CPPE_HIP_OK) return rc;
-  *out = create_system(n, pos, alpha6, alpha_iso, excl_off, excl_idx, opts, {0, n}, {opts->ncrit});
+  // direct summation is the tree path with a single leaf (ncrit above n): its near list is every
+  // pair, and the far field is empty; the expansion order then only sizes unused buffers
+  cppe_hip_options o = *opts;
+  const bool direct  = o.summation != CPPE_HIP_SUM_FMM;
+  if (direct && (o.order < 2 || o.order > 8)) o.order = 2;
+  *out = create_system(n, pos, alpha6, alpha_iso, excl_off, excl_idx, &o, {0, n}, {direct ? n + 1 : o.ncrit});
   return CPPE_HIP_OK;
   CPPE_API_END
 }
@@ -591,7 +597,6 @@ int cppe_hip_autotune(cppe_hip_system* sys, int reps, int* choice, double* ms) {
   CPPE_API_BEGIN
   if (!sys || reps <= 0 || !choice || !ms) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
   System& S = sys->S;
-  if (S.opts.summation != CPPE_HIP_SUM_FMM) return fail(CPPE_HIP_ERR_UNSUPPORTED, "autotuning covers the tree path only");
   if (!S.has_alpha) return fail(CPPE_HIP_ERR_ARG, "system has no polarizabilities");
   fmm_autotune(S, reps, choice, ms);
   return CPPE_HIP_OK;
@@ -601,12 +606,13 @@ int cppe_hip_autotune(cppe_hip_system* sys, int reps, int* choice, double* ms) {
 int cppe_hip_set_tune(cppe_hip_system* sys, const int* choice) {
   CPPE_API_BEGIN
   if (!sys || !choice) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
-  if (sys->S.opts.summation != CPPE_HIP_SUM_FMM) return fail(CPPE_HIP_ERR_UNSUPPORTED, "autotuning covers the tree path only");
   for (int prec = 0; prec < 2; ++prec) {
     const int v = choice[4 * prec], b = choice[4 * prec + 1], g = choice[4 * prec + 2], o = choice[4 * prec + 3];
     if (v < 0 || v > 8 || (b != 64 && b != 128 && b != 256) || (v == 2 && b != 64) ||
         (g != 16 && g != 32 && g != 64) || (o != 0 && o != 1))
       return fail(CPPE_HIP_ERR_ARG, "invalid launch shape");
+    if (!near_variant_allowed(sys->S, v))
+      return fail(CPPE_HIP_ERR_UNSUPPORTED, "near-field variant 2 needs leaves of at most 1024 sites");
   }
   fmm_set_tune(sys->S, choice);
   return CPPE_HIP_OK;
@@ -664,7 +670,7 @@ int cppe_hip_profile_apply(cppe_hip_system* sys, int fp32, int reps, double* pha
   hipEvent_t ev[6];
   for (auto& e : ev) CPPE_HIP_CHECK(hipEventCreate(&e));
   double acc[5] = {0, 0, 0, 0, 0};
-  const bool fmm = S.opts.summation == CPPE_HIP_SUM_FMM;
+  const bool fmm = true;  // both summations apply B through the tree path
   for (int r = 0; r < reps; ++r) {
     if (fmm) S.prof = ev;
     CPPE_HIP_CHECK(hipEventRecord(ev[5], st));

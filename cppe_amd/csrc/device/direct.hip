@@ -7,8 +7,9 @@
 // Exclusions: a tile is checked per lane against its own index and CSR exclusion list; only
 // wavefronts with a hit in the tile run the masked inner loop (rare with Morton order).
 // Thole damping (pair parameter a = damping * s_i * s_j, s = alpha_iso^(-1/6), s = 0 for
-// undamped sites): e^-aR is evaluated only when aR < 50; beyond, the screening factors equal
-// 1 to FP64 precision.
+// undamped sites): e^-aR is evaluated only below the screening cutoff (common.hh).
+// The response operator B of a direct-summation system does not use these kernels: it runs on the
+// tree path with a single leaf (every pair in the near field), see solver.hip's apply_b.
 #include "pairs.hh"
 #include "system.hh"
 
@@ -141,148 +142,6 @@ void direct_static(System& S, const double* q, const double* mu, const double* t
   const double* s = damping > 0.0 ? S.s.p : nullptr;
   hipLaunchKernelGGL(k_direct_generic<3>, dim3(ceil_div(S.n, kTile)), dim3(kTile), 0, S.stream, S.n, S.x.p,
                      S.y.p, S.z.p, q, s, damping, S.excl_off.p, S.excl_idx.p, F);
-}
-
-// ---------------------------------------------------------------------------------------------
-// B apply, the hot kernel of the direct path: out = -T2 mu (damped) + alpha^-1 mu.
-// FP64 form: everything FP64. FP32 form: sources staged in LDS as FP32 offsets from a per-block
-// FP64 anchor (the block's first target), dipoles and damping parameters in FP32, FP32 pair
-// arithmetic with hardware rsqrt / exp2, per-tile FP32 partial sums promoted to FP64.
-// ---------------------------------------------------------------------------------------------
-template <bool DAMP>
-__global__ __launch_bounds__(256) void k_apply_b_f64(int n, const double* __restrict__ x,
-                                                     const double* __restrict__ y,
-                                                     const double* __restrict__ z,
-                                                     const double* __restrict__ s, double damping,
-                                                     const int* __restrict__ eoff, const int* __restrict__ eidx,
-                                                     const double* __restrict__ mu, const double* __restrict__ ainv,
-                                                     double* __restrict__ out) {
-  __shared__ double sx[kTile], sy[kTile], sz[kTile], sm[3][kTile], ss[kTile];
-  const int t    = threadIdx.x;
-  const int i    = blockIdx.x * kTile + t;
-  const bool act = i < n;
-  const double xi = act ? x[i] : 0.0, yi = act ? y[i] : 0.0, zi = act ? z[i] : 0.0;
-  const double di = (DAMP && act) ? damping * s[i] : 0.0;
-  double fx = 0.0, fy = 0.0, fz = 0.0;
-  for (int j0 = 0; j0 < n; j0 += kTile) {
-    const int tn = min(kTile, n - j0);
-    __syncthreads();
-    if (t < tn) {
-      const int j = j0 + t;
-      sx[t]    = x[j];
-      sy[t]    = y[j];
-      sz[t]    = z[j];
-      sm[0][t] = mu[j];
-      sm[1][t] = mu[n + j];
-      sm[2][t] = mu[2 * n + j];
-      ss[t]    = DAMP ? s[j] : 0.0;
-    }
-    __syncthreads();
-    const bool hit    = act && tile_hits(i, j0, j0 + tn, eoff, eidx);
-    const bool masked = __ballot(hit) != 0;
-    if (!masked) {
-#pragma unroll 4
-      for (int jj = 0; jj < tn; ++jj)
-        pair_m1_f64<DAMP>(xi - sx[jj], yi - sy[jj], zi - sz[jj], sm[0][jj], sm[1][jj], sm[2][jj],
-                          di * ss[jj], fx, fy, fz);
-    } else {
-      for (int jj = 0; jj < tn; ++jj) {
-        if (!act || (hit && excluded(i, j0 + jj, eoff, eidx))) continue;
-        pair_m1_f64<DAMP>(xi - sx[jj], yi - sy[jj], zi - sz[jj], sm[0][jj], sm[1][jj], sm[2][jj],
-                          di * ss[jj], fx, fy, fz);
-      }
-    }
-  }
-  if (act) {
-    const double m0 = mu[i], m1 = mu[n + i], m2 = mu[2 * n + i];
-    const double a0 = ainv[i], a1 = ainv[n + i], a2 = ainv[2 * n + i], a3 = ainv[3 * n + i],
-                 a4 = ainv[4 * n + i], a5 = ainv[5 * n + i];
-    out[i]         = fx + (a0 * m0 + a1 * m1 + a2 * m2);
-    out[n + i]     = fy + (a1 * m0 + a3 * m1 + a4 * m2);
-    out[2 * n + i] = fz + (a2 * m0 + a4 * m1 + a5 * m2);
-  }
-}
-
-template <bool DAMP>
-__global__ __launch_bounds__(256) void k_apply_b_f32(int n, const double* __restrict__ x,
-                                                     const double* __restrict__ y,
-                                                     const double* __restrict__ z,
-                                                     const double* __restrict__ s, double damping,
-                                                     const int* __restrict__ eoff, const int* __restrict__ eidx,
-                                                     const double* __restrict__ mu, const double* __restrict__ ainv,
-                                                     double* __restrict__ out) {
-  __shared__ float sx[kTile], sy[kTile], sz[kTile], sm[3][kTile], ss[kTile];
-  const int t    = threadIdx.x;
-  const int i    = blockIdx.x * kTile + t;
-  const bool act = i < n;
-  const int i0   = blockIdx.x * kTile;
-  const double ax = x[i0], ay = y[i0], az = z[i0];  // FP64 anchor of this block
-  const float xi = act ? (float)(x[i] - ax) : 0.f, yi = act ? (float)(y[i] - ay) : 0.f,
-              zi = act ? (float)(z[i] - az) : 0.f;
-  const float di = (DAMP && act) ? (float)(damping * s[i]) : 0.f;
-  double fx = 0.0, fy = 0.0, fz = 0.0;
-  for (int j0 = 0; j0 < n; j0 += kTile) {
-    const int tn = min(kTile, n - j0);
-    __syncthreads();
-    if (t < tn) {
-      const int j = j0 + t;
-      sx[t]    = (float)(x[j] - ax);
-      sy[t]    = (float)(y[j] - ay);
-      sz[t]    = (float)(z[j] - az);
-      sm[0][t] = (float)mu[j];
-      sm[1][t] = (float)mu[n + j];
-      sm[2][t] = (float)mu[2 * n + j];
-      ss[t]    = DAMP ? (float)s[j] : 0.f;
-    }
-    __syncthreads();
-    const bool hit    = act && tile_hits(i, j0, j0 + tn, eoff, eidx);
-    const bool masked = __ballot(hit) != 0;
-    float gx = 0.f, gy = 0.f, gz = 0.f;
-    if (!masked) {
-#pragma unroll 4
-      for (int jj = 0; jj < tn; ++jj)
-        pair_m1_f32<DAMP>(xi - sx[jj], yi - sy[jj], zi - sz[jj], sm[0][jj], sm[1][jj], sm[2][jj],
-                          di * ss[jj], gx, gy, gz);
-    } else {
-      for (int jj = 0; jj < tn; ++jj) {
-        if (!act || (hit && excluded(i, j0 + jj, eoff, eidx))) continue;
-        pair_m1_f32<DAMP>(xi - sx[jj], yi - sy[jj], zi - sz[jj], sm[0][jj], sm[1][jj], sm[2][jj],
-                          di * ss[jj], gx, gy, gz);
-      }
-    }
-    fx += gx;
-    fy += gy;
-    fz += gz;
-  }
-  if (act) {
-    const double m0 = mu[i], m1 = mu[n + i], m2 = mu[2 * n + i];
-    const double a0 = ainv[i], a1 = ainv[n + i], a2 = ainv[2 * n + i], a3 = ainv[3 * n + i],
-                 a4 = ainv[4 * n + i], a5 = ainv[5 * n + i];
-    out[i]         = fx + (a0 * m0 + a1 * m1 + a2 * m2);
-    out[n + i]     = fy + (a1 * m0 + a3 * m1 + a4 * m2);
-    out[2 * n + i] = fz + (a2 * m0 + a4 * m1 + a5 * m2);
-  }
-}
-
-void apply_b_direct(System& S, const double* xin, double* out, bool fp32) {
-  const dim3 g(ceil_div(S.n, kTile)), b(kTile);
-  const bool damp = S.opts.damping > 0.0;
-  const double d  = S.opts.damping;
-  if (fp32) {
-    if (damp)
-      hipLaunchKernelGGL(k_apply_b_f32<true>, g, b, 0, S.stream, S.n, S.x.p, S.y.p, S.z.p, S.s.p, d,
-                         S.excl_off.p, S.excl_idx.p, xin, S.ainv6.p, out);
-    else
-      hipLaunchKernelGGL(k_apply_b_f32<false>, g, b, 0, S.stream, S.n, S.x.p, S.y.p, S.z.p, S.s.p, d,
-                         S.excl_off.p, S.excl_idx.p, xin, S.ainv6.p, out);
-  } else {
-    if (damp)
-      hipLaunchKernelGGL(k_apply_b_f64<true>, g, b, 0, S.stream, S.n, S.x.p, S.y.p, S.z.p, S.s.p, d,
-                         S.excl_off.p, S.excl_idx.p, xin, S.ainv6.p, out);
-    else
-      hipLaunchKernelGGL(k_apply_b_f64<false>, g, b, 0, S.stream, S.n, S.x.p, S.y.p, S.z.p, S.s.p, d,
-                         S.excl_off.p, S.excl_idx.p, xin, S.ainv6.p, out);
-  }
 }
 
 // ---------------------------------------------------------------------------------------------

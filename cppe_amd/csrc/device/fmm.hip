@@ -222,6 +222,27 @@ static const int kNearCand[][2] = {{0, 64}, {0, 128}, {0, 256}, {1, 64}, {1, 128
 static const int kM2lCand[]     = {16, 32, 64};
 constexpr int kNNear = 25, kNM2l = 3;
 
+// The LDS-staged wavefront-per-leaf kernel (variant 2) gives one wavefront all targets of a leaf:
+// its time grows with the square of the leaf size, so it is only a candidate for small leaves.
+constexpr int kMaxLeafVariant2 = 1024;
+
+int max_leaf_size(System& S) {
+  Octree& T = S.tree;
+  if (T.max_leaf > 0) return T.max_leaf;
+  std::vector<int> ids(T.nleaves), cnt(T.ncells);
+  CPPE_HIP_CHECK(hipMemcpyAsync(ids.data(), T.leaf_ids.p, sizeof(int) * T.nleaves, hipMemcpyDeviceToHost, S.stream));
+  CPPE_HIP_CHECK(hipMemcpyAsync(cnt.data(), T.count.p, sizeof(int) * T.ncells, hipMemcpyDeviceToHost, S.stream));
+  CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
+  int m = 0;
+  for (int c : ids) m = std::max(m, cnt[c]);
+  T.max_leaf = m;
+  return m;
+}
+
+bool near_variant_allowed(System& S, int variant) {
+  return variant != 2 || max_leaf_size(S) <= kMaxLeafVariant2;
+}
+
 void fmm_autotune(System& S, int reps, int* choice, double* ms) {
   fmm_reserve(S);
   FmmState& F    = *S.fmm;
@@ -263,10 +284,14 @@ void fmm_autotune(System& S, int reps, int* choice, double* ms) {
       F.tune.overlap[prec] = 0;
       int best        = 0;
       for (int c = 0; c < kNNear; ++c) {
+        if (!near_variant_allowed(S, kNearCand[c][0])) {
+          m[c] = 0.0;  // not timed
+          continue;
+        }
         F.tune.near_variant[prec] = kNearCand[c][0];
         F.tune.near_block[prec]   = kNearCand[c][1];
         m[c]                      = timed(fp32, 0);
-        if (m[c] < m[best]) best = c;
+        if (m[best] <= 0.0 || m[c] < m[best]) best = c;
       }
       F.tune.near_variant[prec] = kNearCand[best][0];
       F.tune.near_block[prec]   = kNearCand[best][1];

@@ -31,7 +31,6 @@
 
 namespace cppe_dev {
 
-void apply_b_direct(System& S, const double* xin, double* out, bool fp32);
 void apply_b_fmm(System& S, const double* xin, double* out, bool fp32);
 void fmm_reserve(System& S);
 
@@ -49,12 +48,9 @@ enum SegI { I_ACTIVE = 0, I_NEEDREL, I_ITERS, I_BAD, I_PENDING, I_ELIGIBLE, I_CO
 enum Status { ST_IT = 0, ST_ACTIVE, ST_REL, ST_BAD, ST_STEPPED, ST_PEND, ST_COUNT };
 enum FinalMode { F_INIT = 0, F_ALPHA, F_RNORM, F_REL, F_BETA };
 
-void apply_b(System& S, const double* x, double* out, bool fp32) {
-  if (S.opts.summation == CPPE_HIP_SUM_FMM)
-    apply_b_fmm(S, x, out, fp32);
-  else
-    apply_b_direct(S, x, out, fp32);
-}
+// Both summations apply B through the tree path: a direct-summation system is a tree of one leaf
+// whose near list is itself (every pair, exclusions and Thole damping as in the direct sum).
+void apply_b(System& S, const double* x, double* out, bool fp32) { apply_b_fmm(S, x, out, fp32); }
 
 // out = alpha in (3x3 packed SoA per site)
 __global__ __launch_bounds__(256) void k_alpha_apply(int n, const double* __restrict__ a6,
@@ -532,7 +528,7 @@ int device_cg(System& S, double thresh, int maxiter, bool mixed, CgResult& res) 
     S.cg_hist.alloc((size_t)S.hist_cap * ns);
     cg_graphs_release(S);  // the graphs hold the history buffer
   }
-  if (S.opts.summation == CPPE_HIP_SUM_FMM) fmm_reserve(S);  // no allocation inside a graph
+  fmm_reserve(S);  // no allocation inside a graph
   const double params[2] = {thresh, S.reliable_drop};
   CPPE_HIP_CHECK(hipMemcpyAsync(S.cg_params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
   // r0 = rhs - B x0, z0 = alpha r0, p = z0. The mixed form takes r0 from an FP32 apply: it is

@@ -38,6 +38,7 @@ struct dbuf {
 // every cell covers the contiguous device-order range [start, start + count).
 struct Octree {
   int ncells = 0, nleaves = 0, max_level = 0;
+  int max_leaf = 0;  // sites in the largest leaf
   std::vector<int> level_offset;  // host copy: first cell index of each level (+ end)
   dbuf<double> cx, cy, cz, cr, crmax;
   dbuf<int> start, count, parent, level, child_first, nchild, octmask;

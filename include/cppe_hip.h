@@ -167,7 +167,8 @@ int cppe_hip_last_solve_info(cppe_hip_system* sys, int* info);
  * target cell), and keeps the fastest of each. Near-field candidates (variant, block):
  *   0: one thread per target site, block 64/128/256;
  *   1: one wavefront per 64-target leaf chunk, uniform (scalar) source walk, block 64/128/256;
- *   2: one wavefront per leaf, sources staged through LDS, block 64;
+ *   2: one wavefront per leaf, sources staged through LDS, block 64 (only when every leaf has at
+ *      most 1024 sites: its time grows with the square of the leaf size; otherwise not timed, 0 ms);
  *   3: one wavefront per leaf chunk, lanes = (target, source slot) over the leaf's expanded near
  *      list (built on first use: one int per near source site of each leaf), block 64/128/256;
  *   4: one wavefront per target cluster (whole consecutive leaves, <= 64 sites) walking the union
