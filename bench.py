@@ -14,8 +14,9 @@ One step = one full solve with the inputs resident in HBM (cppe_hip_solve_reside
 Both stop on the same FP64 residual norm. The JSON line reports the headline form's ms per
 solve as `value` (lower is better) and the plain form beside it.
 
-roofline: the dominant kernel of the headline apply, timed live with HIP events on the
-system's stream (cppe_hip_profile_apply); algorithmic flops per launch = 47 per damped
+roofline: the dominant kernel of the headline apply, timed live with HIP events recorded on the
+system's stream right before and after its launch (cppe_hip_profile_apply, phases run one
+after another); algorithmic flops per launch = 47 per damped
 near-field site pair (self and excluded pairs not counted) + nothing else for the near-field
 kernel, 950 per dipole M2L cell pair for the M2L kernel.
 cpu_baseline: the CPU oracle (a C++ restatement of the reference's FMM path, OpenMP) timed on a
@@ -310,7 +311,7 @@ def main():
             v = tune[prec]["near_variant"] if tune else 0
             name = "%s (%s P2P + alpha^-1 mu)" % (near_names[v], "near field" if args.summation == "fmm"
                                                    else "all pairs, one-leaf tree")
-            flop, ms = near_flop, prof["near"]
+            flop, ms = near_flop, (prof["near_kernel"] if prof["near_kernel"] > 0 else prof["near"])
         ach = flop / (ms * 1e-3)
         traffic = None
         if args.workload == "300k" and "k_fmm_near_st" in name:

@@ -22,7 +22,7 @@ EXPORTS = [
     "cppe_hip_nuclear_multipole_energy", "cppe_hip_dot", "cppe_hip_tree_stats", "cppe_hip_tree_export",
     "cppe_hip_profile_apply", "cppe_hip_batch_create", "cppe_hip_segment_results", "cppe_hip_set_graphs",
     "cppe_hip_autotune", "cppe_hip_set_tune", "cppe_hip_set_reliable_drop", "cppe_hip_last_solve_info",
-    "cppe_hip_field_targets",
+    "cppe_hip_field_targets", "cppe_hip_nuclear_field_gradient", "cppe_hip_nuclear_multipole_gradient",
     "cppe_hip_mlp_create", "cppe_hip_mlp_destroy", "cppe_hip_mlp_set", "cppe_hip_mlp_get", "cppe_hip_ml_features",
     "cppe_hip_mlp_forward", "cppe_hip_mlp_loss_grad", "cppe_hip_mlp_reserve", "cppe_hip_mlp_collect",
     "cppe_hip_mlp_prepare", "cppe_hip_mlp_train", "cppe_hip_ml_guess",
@@ -72,6 +72,8 @@ def lib():
         L.cppe_hip_download_mu.argtypes = [vp, _d]
         L.cppe_hip_nuclear_fields.argtypes = [ctypes.c_int, _d, ctypes.c_int, _d, _d, _d]
         L.cppe_hip_nuclear_multipole_energy.argtypes = [ctypes.c_int, _d, _i, _d, ctypes.c_int, _d, _d, _d]
+        L.cppe_hip_nuclear_field_gradient.argtypes = [ctypes.c_int, _d, ctypes.c_int, _d, _d, _d]
+        L.cppe_hip_nuclear_multipole_gradient.argtypes = [ctypes.c_int, _d, _i, _d, ctypes.c_int, _d, _d, _d]
         L.cppe_hip_dot.argtypes = [ctypes.c_longlong, _d, _d, _d]
         L.cppe_hip_tree_stats.argtypes = [vp, _ll]
         L.cppe_hip_tree_export.argtypes = [vp, _d, _ll, _ll, _i, _ll, _ll]
@@ -267,10 +269,11 @@ class System:
         return mu
 
     def profile_apply(self, fp32=False, reps=5):
-        """Mean ms per apply: upward, m2l, l2l, near (P2P+L2P or direct), total."""
-        out = np.zeros(5)
+        """Mean ms per apply: upward, m2l, l2l, near (near kernel + exclusion pass), total, and
+        near_kernel (the staged near-field kernel alone; -1 for the other variants)."""
+        out = np.zeros(6)
         _check(lib().cppe_hip_profile_apply(self.h, int(fp32), int(reps), _p(out)))
-        return dict(upward=out[0], m2l=out[1], l2l=out[2], near=out[3], total=out[4])
+        return dict(upward=out[0], m2l=out[1], l2l=out[2], near=out[3], total=out[4], near_kernel=out[5])
 
     def tree_stats(self):
         s = np.zeros(6, dtype=np.int64)
@@ -353,6 +356,29 @@ def nuclear_multipole_energy(pos, kmax, mul, charges, atom_pos):
     _check(lib().cppe_hip_nuclear_multipole_energy(pos.size // 3, _p(pos), _p(km, _i), _p(mul), ch.size, _p(ch),
                                                    _p(ap), ctypes.byref(e)))
     return e.value
+
+
+def nuclear_field_gradient(pos, charges, atom_pos):
+    """(3 natoms) x (3 n) derivative of the nuclear fields with respect to the atom positions."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+    ch = np.ascontiguousarray(charges, dtype=np.float64)
+    ap = np.ascontiguousarray(atom_pos, dtype=np.float64).reshape(-1)
+    G = np.zeros((3 * ch.size, pos.size))
+    _check(lib().cppe_hip_nuclear_field_gradient(pos.size // 3, _p(pos), ch.size, _p(ch), _p(ap), _p(G)))
+    return G
+
+
+def nuclear_multipole_gradient(pos, kmax, mul, charges, atom_pos):
+    """natoms x 3 derivative of the nuclear-multipole energy with respect to the atom positions."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+    km = np.ascontiguousarray(kmax, dtype=np.int32)
+    mul = np.ascontiguousarray(mul, dtype=np.float64).reshape(-1)
+    ch = np.ascontiguousarray(charges, dtype=np.float64)
+    ap = np.ascontiguousarray(atom_pos, dtype=np.float64).reshape(-1)
+    g = np.zeros((ch.size, 3))
+    _check(lib().cppe_hip_nuclear_multipole_gradient(pos.size // 3, _p(pos), _p(km, _i), _p(mul), ch.size, _p(ch),
+                                                     _p(ap), _p(g)))
+    return g
 
 
 def dot(a, b):

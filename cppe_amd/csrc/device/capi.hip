@@ -160,6 +160,7 @@ static cppe_hip_system* create_system(int n, const double* pos, const double* al
     if (alpha6) upload(a6h, alpha6, 6 * (size_t)n, st);
     if (alpha_iso) upload(isoh, alpha_iso, (size_t)n, st);
     S.has_alpha = alpha6 != nullptr;
+    S.all_damped = alpha_iso && std::all_of(alpha_iso, alpha_iso + n, [](double v) { return v > 0.0; });
     S.alpha6.alloc(6 * (size_t)n);
     S.ainv6.alloc(6 * (size_t)n);
     S.s.alloc(n);
@@ -408,6 +409,14 @@ __global__ __launch_bounds__(256) void k_nuc_mul_energy(int n, const double* __r
                                                         const double* __restrict__ apos,
                                                         double* __restrict__ part);
 __global__ void k_sum_partials(int np, const double* __restrict__ part, double* __restrict__ out);
+__global__ __launch_bounds__(256) void k_nuc_field_grad(int n, const double* __restrict__ pos, int na,
+                                                        const double* __restrict__ ch,
+                                                        const double* __restrict__ apos, double* __restrict__ G);
+__global__ __launch_bounds__(256) void k_nuc_mul_grad(int n, const double* __restrict__ pos,
+                                                      const int* __restrict__ kmax, const double* __restrict__ mul,
+                                                      const double* __restrict__ ch,
+                                                      const double* __restrict__ apos, double* __restrict__ part);
+__global__ __launch_bounds__(256) void k_grad_sum(int nb, const double* __restrict__ part, double* __restrict__ out);
 __global__ __launch_bounds__(256) void k_dot_part(long long n, const double* __restrict__ a,
                                                   const double* __restrict__ b, double* __restrict__ part);
 __global__ void k_soa6_to_aos9(int n, const int* __restrict__ perm, const double* __restrict__ a6,
@@ -593,11 +602,18 @@ int cppe_hip_segment_results(cppe_hip_system* sys, int* iters, double* rnorm, in
   CPPE_API_END
 }
 
+__global__ void k_fill_pattern(int n3, double* v) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n3) v[k] = 0.1 * sin(0.37 * (double)k + 1.0);  // fixed pseudo-random dipoles
+}
+
 int cppe_hip_autotune(cppe_hip_system* sys, int reps, int* choice, double* ms) {
   CPPE_API_BEGIN
   if (!sys || reps <= 0 || !choice || !ms) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
   System& S = sys->S;
   if (!S.has_alpha) return fail(CPPE_HIP_ERR_ARG, "system has no polarizabilities");
+  // time on dipoles of realistic bit activity (the clock under load depends on the data)
+  hipLaunchKernelGGL(k_fill_pattern, dim3(ceil_div(3 * S.n, 256)), dim3(256), 0, S.stream, 3 * S.n, S.v_d.p);
   fmm_autotune(S, reps, choice, ms);
   return CPPE_HIP_OK;
   CPPE_API_END
@@ -652,11 +668,6 @@ int cppe_hip_download_mu(cppe_hip_system* sys, double* mu) {
   CPPE_API_END
 }
 
-__global__ void k_fill_pattern(int n3, double* v) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n3) v[k] = 0.1 * sin(0.37 * (double)k + 1.0);  // fixed pseudo-random dipoles
-}
-
 int cppe_hip_profile_apply(cppe_hip_system* sys, int fp32, int reps, double* phase_ms) {
   CPPE_API_BEGIN
   if (!sys || reps <= 0 || !phase_ms) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
@@ -667,9 +678,10 @@ int cppe_hip_profile_apply(cppe_hip_system* sys, int fp32, int reps, double* pha
   hipLaunchKernelGGL(k_fill_pattern, dim3(ceil_div(n3, 256)), dim3(256), 0, st, n3, S.v_d.p);
   apply_b(S, S.v_d.p, S.v_t.p, fp32 != 0);  // warm (allocations)
   CPPE_HIP_CHECK(hipStreamSynchronize(st));
-  hipEvent_t ev[6];
+  hipEvent_t ev[8];
   for (auto& e : ev) CPPE_HIP_CHECK(hipEventCreate(&e));
-  double acc[5] = {0, 0, 0, 0, 0};
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const int var = fmm_near_variant(S, fp32 != 0);
   const bool fmm = true;  // both summations apply B through the tree path
   for (int r = 0; r < reps; ++r) {
     if (fmm) S.prof = ev;
@@ -692,9 +704,14 @@ int cppe_hip_profile_apply(cppe_hip_system* sys, int fp32, int reps, double* pha
     acc[3] += t;
     CPPE_HIP_CHECK(hipEventElapsedTime(&t, ev[5], ev[4]));
     acc[4] += t;
+    if (var >= 5) {
+      CPPE_HIP_CHECK(hipEventElapsedTime(&t, ev[6], ev[7]));
+      acc[5] += t;
+    }
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   for (int k = 0; k < 5; ++k) phase_ms[k] = acc[k] / reps;
+  phase_ms[5] = var >= 5 ? acc[5] / reps : -1.0;
   return CPPE_HIP_OK;
   CPPE_API_END
 }
@@ -740,6 +757,51 @@ int cppe_hip_nuclear_multipole_energy(int n, const double* pos, const int* kmax,
   hipLaunchKernelGGL(k_nuc_mul_energy, dim3(nb), dim3(256), 0, st, n, dp.p, dk.p, dm.p, natoms, dc.p, da.p, part.p);
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, nb, part.p, res.p);
   CPPE_HIP_CHECK(hipMemcpy(energy, res.p, sizeof(double), hipMemcpyDeviceToHost));
+  return CPPE_HIP_OK;
+  CPPE_API_END
+}
+
+int cppe_hip_nuclear_field_gradient(int n, const double* pos, int natoms, const double* charges,
+                                    const double* atom_pos, double* grad) {
+  CPPE_API_BEGIN
+  if (n < 0 || natoms < 0 || (n > 0 && natoms > 0 && (!pos || !charges || !atom_pos || !grad)))
+    return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
+  if (n == 0 || natoms == 0) return CPPE_HIP_OK;
+  hipStream_t st = nullptr;
+  dbuf<double> dp, dc, da, dG;
+  upload(dp, pos, 3 * (size_t)n, st);
+  upload(dc, charges, (size_t)natoms, st);
+  upload(da, atom_pos, 3 * (size_t)natoms, st);
+  const size_t total = 9 * (size_t)natoms * (size_t)n;
+  dG.alloc(total);
+  hipLaunchKernelGGL(k_nuc_field_grad, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, dp.p, natoms, dc.p, da.p, dG.p);
+  CPPE_HIP_CHECK(hipMemcpy(grad, dG.p, sizeof(double) * total, hipMemcpyDeviceToHost));
+  return CPPE_HIP_OK;
+  CPPE_API_END
+}
+
+int cppe_hip_nuclear_multipole_gradient(int n, const double* pos, const int* kmax, const double* mul, int natoms,
+                                        const double* charges, const double* atom_pos, double* grad) {
+  CPPE_API_BEGIN
+  if (n < 0 || natoms < 0 || (natoms > 0 && !grad)) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
+  if (natoms == 0) return CPPE_HIP_OK;
+  std::fill(grad, grad + 3 * (size_t)natoms, 0.0);
+  if (n == 0) return CPPE_HIP_OK;
+  if (!pos || !kmax || !mul || !charges || !atom_pos) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
+  hipStream_t st = nullptr;
+  dbuf<double> dp, dm, dc, da, part, res;
+  dbuf<int> dk;
+  upload(dp, pos, 3 * (size_t)n, st);
+  upload(dm, mul, 10 * (size_t)n, st);
+  upload(dk, kmax, (size_t)n, st);
+  upload(dc, charges, (size_t)natoms, st);
+  upload(da, atom_pos, 3 * (size_t)natoms, st);
+  const int nb = ceil_div(n, 256);
+  part.alloc(3 * (size_t)nb * natoms);
+  res.alloc(3 * (size_t)natoms);
+  hipLaunchKernelGGL(k_nuc_mul_grad, dim3(nb, natoms), dim3(256), 0, st, n, dp.p, dk.p, dm.p, dc.p, da.p, part.p);
+  hipLaunchKernelGGL(k_grad_sum, dim3(natoms), dim3(256), 0, st, nb, part.p, res.p);
+  CPPE_HIP_CHECK(hipMemcpy(grad, res.p, sizeof(double) * 3 * natoms, hipMemcpyDeviceToHost));
   return CPPE_HIP_OK;
   CPPE_API_END
 }

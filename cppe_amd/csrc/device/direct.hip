@@ -257,6 +257,88 @@ __global__ __launch_bounds__(256) void k_nuc_mul_energy(int n, const double* __r
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// Nuclear derivative of the nuclear fields (electric_fields.cc:77-104):
+// G[(3a + p) * 3n + 3i + q] = Z_a T2(r_i - R_a)_pq,  T2_pq(d) = (3 d_p d_q - |d|^2 delta_pq) / |d|^5
+__global__ __launch_bounds__(256) void k_nuc_field_grad(int n, const double* __restrict__ pos, int na,
+                                                        const double* __restrict__ ch,
+                                                        const double* __restrict__ apos, double* __restrict__ G) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t row = 3 * (size_t)n;
+  for (int a = 0; a < na; ++a) {
+    const double d[3] = {pos[3 * i] - apos[3 * a], pos[3 * i + 1] - apos[3 * a + 1], pos[3 * i + 2] - apos[3 * a + 2]};
+    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double r  = sqrt(r2);
+    const double r3 = 1.0 / (r * r2), r5 = r3 / r2;
+    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 3; ++q)
+        G[(3 * (size_t)a + p) * row + 3 * (size_t)i + q] = ch[a] * (3.0 * d[p] * d[q] * r5 - (p == q ? r3 : 0.0));
+  }
+}
+
+// Nuclear gradient of the nuclear-multipole energy (multipole_expansion.cc:35-62). With
+// d = R_a - r_i, E_ai = Z_a sum_k (-1)^k/k! M_k . T_k(d), so dE/dR_a = Z_a sum_k (-1)^k/k! M_k . T_(k+1)(d):
+//   k = 0: q T1 = -q d/r^3
+//   k = 1: -(T2 mu) = mu/r^3 - 3 (d.mu) d/r^5
+//   k = 2: 1/2 Theta : T3 = 1/2 (-15 (d.Theta.d) d/r^7 + 3 tr(Theta) d/r^5 + 6 Theta d/r^5)
+// Grid (site blocks, atoms); per-block partials, summed in fixed order by k_grad_sum.
+__global__ __launch_bounds__(256) void k_nuc_mul_grad(int n, const double* __restrict__ pos,
+                                                      const int* __restrict__ kmax, const double* __restrict__ mul,
+                                                      const double* __restrict__ ch,
+                                                      const double* __restrict__ apos, double* __restrict__ part) {
+  __shared__ double red[3][256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, a = blockIdx.y;
+  double g[3] = {0.0, 0.0, 0.0};
+  if (i < n) {
+    const int km    = kmax[i];
+    const double* M = &mul[10 * (size_t)i];
+    const double d[3] = {apos[3 * a] - pos[3 * i], apos[3 * a + 1] - pos[3 * i + 1], apos[3 * a + 2] - pos[3 * i + 2]};
+    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double r  = sqrt(r2);
+    const double r3 = 1.0 / (r * r2), r5 = r3 / r2, r7 = r5 / r2;
+    if (km >= 0)
+      for (int p = 0; p < 3; ++p) g[p] -= M[0] * d[p] * r3;
+    if (km >= 1) {
+      const double dm = d[0] * M[1] + d[1] * M[2] + d[2] * M[3];
+      for (int p = 0; p < 3; ++p) g[p] += M[1 + p] * r3 - 3.0 * dm * d[p] * r5;
+    }
+    if (km >= 2) {
+      const double* S = M + 4;  // xx xy xz yy yz zz
+      const double td[3] = {S[0] * d[0] + S[1] * d[1] + S[2] * d[2], S[1] * d[0] + S[3] * d[1] + S[4] * d[2],
+                            S[2] * d[0] + S[4] * d[1] + S[5] * d[2]};
+      const double dtd = d[0] * td[0] + d[1] * td[1] + d[2] * td[2];
+      const double tr  = S[0] + S[3] + S[5];
+      for (int p = 0; p < 3; ++p) g[p] += 0.5 * (-15.0 * dtd * d[p] * r7 + 3.0 * tr * d[p] * r5 + 6.0 * td[p] * r5);
+    }
+    for (int p = 0; p < 3; ++p) g[p] *= ch[a];
+  }
+  for (int p = 0; p < 3; ++p) red[p][threadIdx.x] = g[p];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int p = 0; p < 3; ++p) red[p][threadIdx.x] += red[p][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[((size_t)a * gridDim.x + blockIdx.x) * 3 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// out[a * 3 + p] = sum_b part[(a * nb + b) * 3 + p], b in a fixed order; one block per atom.
+__global__ __launch_bounds__(256) void k_grad_sum(int nb, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double red[3][256];
+  const int a = blockIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    for (int p = 0; p < 3; ++p) v[p] += part[((size_t)a * nb + b) * 3 + p];
+  for (int p = 0; p < 3; ++p) red[p][threadIdx.x] = v[p];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int p = 0; p < 3; ++p) red[p][threadIdx.x] += red[p][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) out[3 * a + threadIdx.x] = red[threadIdx.x][0];
+}
+
 __global__ void k_sum_partials(int np, const double* __restrict__ part, double* __restrict__ out) {
   __shared__ double red[256];
   double v = 0.0;

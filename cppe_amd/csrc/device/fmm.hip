@@ -247,7 +247,7 @@ void fmm_autotune(System& S, int reps, int* choice, double* ms) {
   fmm_reserve(S);
   FmmState& F    = *S.fmm;
   hipStream_t st = S.stream;
-  hipEvent_t ev[6];
+  hipEvent_t ev[8];
   for (auto& e : ev) CPPE_HIP_CHECK(hipEventCreate(&e));
   auto timed = [&](bool fp32, int phase) {  // phase 0: near (events 3-4), 1: m2l (events 1-2)
     apply_b_fmm(S, S.v_d.p, S.v_t.p, fp32);  // warm-up
@@ -276,8 +276,7 @@ void fmm_autotune(System& S, int reps, int* choice, double* ms) {
     CPPE_HIP_CHECK(hipEventElapsedTime(&t, ev[0], ev[1]));
     return (double)t / reps;
   };
-  try {
-    fill_f64(S, S.v_d.p, 3 * (size_t)S.n, 0.01);
+  try {  // S.v_d holds the caller's test dipoles
     for (int prec = 0; prec < 2; ++prec) {
       const bool fp32 = prec == 1;
       double* m       = ms + prec * (kNNear + kNM2l + 2);
@@ -338,5 +337,7 @@ void fmm_destroy(System& S) {
   delete S.fmm;
   S.fmm = nullptr;
 }
+
+int fmm_near_variant(const System& S, bool fp32) { return S.fmm ? S.fmm->tune.near_variant[fp32 ? 1 : 0] : 0; }
 
 }  // namespace cppe_dev

@@ -18,6 +18,8 @@
 // FP32 form: M, L and pair arithmetic in FP32 with coordinates taken relative to cell / leaf
 // centres in FP64 before rounding, so FP32 sees only local offsets.
 #pragma once
+#include <type_traits>
+
 #include "fmm_ops.gen.hh"
 #include "pairs.hh"
 #include "system.hh"
@@ -794,22 +796,42 @@ __device__ __forceinline__ void wave_lds_barrier() {
 }
 
 // LDS source record of the staged near-field kernel: position (FP32: offset from the target
-// leaf centre; FP64: absolute), alpha_iso^(-1/6), dipole, site index.
+// leaf centre; FP64: absolute), alpha_iso^(-1/6), dipole.
 template <typename R>
-struct StagedRec {
+struct alignas(4 * sizeof(R)) StagedRec {  // two 16-byte (FP32) / 32-byte (FP64) LDS reads
   R x, y, z, s;
   R mx, my, mz;
-  int j;
 };
 
-// Near field, one wavefront per leaf chunk of at most `chunk` (64, 32 or 16) targets, several
+// One near-field pair of the staged kernel and of the exclusion pass (identical arithmetic, so an
+// excluded pair is taken out exactly as it was added). B2: the branch-free FP32 form for systems
+// whose sites are all damped; di then carries the factor log2(e).
+template <typename R, bool DAMP, bool B2>
+__device__ __forceinline__ void staged_pair(R dx, R dy, R dz, R mx, R my, R mz, R a, R& fx, R& fy, R& fz) {
+  if constexpr (sizeof(R) == 8)
+    pair_m1_f64<DAMP, true>(dx, dy, dz, mx, my, mz, a, fx, fy, fz);
+  else if constexpr (B2)
+    pair_m1_f32_b2(dx, dy, dz, mx, my, mz, a, fx, fy, fz);
+  else
+    pair_m1_f32<DAMP, true>(dx, dy, dz, mx, my, mz, a, fx, fy, fz);
+}
+
+template <typename R, bool B2>
+__device__ __forceinline__ R staged_di(double damping, double s) {
+  constexpr double kLog2e = 1.44269504088896340736;
+  return (R)(B2 ? damping * s * kLog2e : damping * s);
+}
+
+// Near field, one wavefront per leaf chunk of at most `chunk` (64, 32, 16 or 8) targets, several
 // wavefronts per block, each with its own LDS region and no block-wide barrier. Lanes are (target
 // t, source slot s): T = targets of the chunk, S = 64 / T; smaller chunks spread a large leaf over
 // several wavefronts with more slots each. The leaf's expanded near list is consumed 64 sources at a time: each
 // lane gathers one source record into LDS (coalesced index load), then slot s evaluates sources
 // s, s + S, ... of the chunk against its target. FP32 form: positions staged as offsets from the
 // target leaf centre, formed in FP64 and rounded. Slot partials are summed through LDS.
-template <int P, typename R, bool DAMP, int MODE, typename MU4, bool WITH_L2P = true>
+// CHT = 8: built for chunks of 8 -- a full chunk against a full stage of 64 sources runs as a
+// straight 8-pair sequence per lane with constant LDS offsets (no per-pair loop arithmetic).
+template <int P, typename R, bool DAMP, int MODE, typename MU4, bool WITH_L2P = true, int CHT = 0, bool B2 = false>
 __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __restrict__ items, int nitems,
                                                      int chunk, const long long* __restrict__ xoff,
                                                      const int* __restrict__ xsrc, const MU4* __restrict__ mu4,
@@ -821,6 +843,7 @@ __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __r
   const int lane = threadIdx.x & 63;
   const int w    = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + wl);
   if (w >= nitems) return;  // uniform per wavefront; no block-wide barrier follows
+  if (CHT) chunk = CHT;
   const int2 item = items[w];
   const int A     = __builtin_amdgcn_readfirstlane(item.x);
   const int t0    = __builtin_amdgcn_readfirstlane(item.y);
@@ -834,7 +857,7 @@ __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __r
   const R xi = sizeof(R) == 8 ? (R)pi.x : (R)(pi.x - cax);
   const R yi = sizeof(R) == 8 ? (R)pi.y : (R)(pi.y - cay);
   const R zi = sizeof(R) == 8 ? (R)pi.z : (R)(pi.z - caz);
-  const R di = DAMP ? (R)(a.damping * pi.w) : R(0);
+  const R di = DAMP ? staged_di<R, B2>(a.damping, pi.w) : R(0);
   StagedRec<R>* sr = rec[wl];
   R fx = R(0), fy = R(0), fz = R(0);
   const long long q1 = xoff[A + 1];
@@ -852,20 +875,31 @@ __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __r
       r.mx = (R)mj.x;
       r.my = (R)mj.y;
       r.mz = (R)mj.z;
-      r.j  = j;
       sr[lane] = r;
     }
     wave_lds_barrier();
     // every pair of the near list, the self pair contributing zero; excluded near pairs are
     // taken out afterwards by k_fmm_excl_sub with the same arithmetic
-    if (act)
+    if (CHT && T == CHT && cn == 64) {  // uniform: all 64 lanes active, 64 / CHT slots
+      if constexpr (B2 || !DAMP) {  // branch-free pair: straight-line code, registers stay low
+#pragma unroll
+        for (int k = 0; k < CHT; ++k) {
+          const StagedRec<R> r = sr[sl + k * (64 / CHT)];
+          staged_pair<R, DAMP, B2>(xi - r.x, yi - r.y, zi - r.z, r.mx, r.my, r.mz, di * r.s, fx, fy, fz);
+        }
+      } else {  // a branch per pair: unrolling would hoist the records into registers
+#pragma unroll 1
+        for (int k = 0; k < CHT; ++k) {
+          const StagedRec<R> r = sr[sl + k * (64 / CHT)];
+          staged_pair<R, DAMP, B2>(xi - r.x, yi - r.y, zi - r.z, r.mx, r.my, r.mz, di * r.s, fx, fy, fz);
+        }
+      }
+    } else if (act) {
       for (int u = sl; u < cn; u += S) {
         const StagedRec<R> r = sr[u];
-        if constexpr (sizeof(R) == 8)
-          pair_m1_f64<DAMP, true>(xi - r.x, yi - r.y, zi - r.z, r.mx, r.my, r.mz, di * r.s, fx, fy, fz);
-        else
-          pair_m1_f32<DAMP, true>(xi - r.x, yi - r.y, zi - r.z, r.mx, r.my, r.mz, di * r.s, fx, fy, fz);
+        staged_pair<R, DAMP, B2>(xi - r.x, yi - r.y, zi - r.z, r.mx, r.my, r.mz, di * r.s, fx, fy, fz);
       }
+    }
     wave_lds_barrier();
   }
   red[0][wl][lane] = act ? fx : R(0);
@@ -936,7 +970,7 @@ __global__ void k_pack_mu4(int n, const double* __restrict__ mu, MU4* __restrict
 // Takes the excluded near-field pairs out of a near-field sum that included them: for each site
 // i and each distinct exclusion partner e != i whose leaf is in the near list of i's leaf, out[i]
 // -= pair(i, e), evaluated as the staged kernel does (FP32: offsets from i's leaf centre).
-template <typename R, bool DAMP, typename MU4>
+template <typename R, bool DAMP, typename MU4, bool B2 = false>
 __global__ __launch_bounds__(256) void k_fmm_excl_sub(NearArgs a, const MU4* __restrict__ mu4) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
@@ -946,7 +980,7 @@ __global__ __launch_bounds__(256) void k_fmm_excl_sub(NearArgs a, const MU4* __r
   const int o0 = a.p2p_off[A], o1 = a.p2p_off[A + 1];
   const double4 pi = a.pos4[i];
   const double cax = a.cx[A], cay = a.cy[A], caz = a.cz[A];
-  const R di       = DAMP ? (R)(a.damping * pi.w) : R(0);
+  const R di       = DAMP ? staged_di<R, B2>(a.damping, pi.w) : R(0);
   R fx = R(0), fy = R(0), fz = R(0);
   for (int k = k0; k < k1; ++k) {
     const int e = a.eidx[k];
@@ -967,11 +1001,12 @@ __global__ __launch_bounds__(256) void k_fmm_excl_sub(NearArgs a, const MU4* __r
     const double4 pj = a.pos4[e];
     const MU4 mj     = mu4[e];
     if constexpr (sizeof(R) == 8) {
-      pair_m1_f64<DAMP>(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, mj.x, mj.y, mj.z, di * (R)pj.w, fx, fy, fz);
+      staged_pair<R, DAMP, false>(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, mj.x, mj.y, mj.z, di * (R)pj.w, fx, fy,
+                                  fz);
     } else {
       const R tx = (R)(pi.x - cax), ty = (R)(pi.y - cay), tz = (R)(pi.z - caz);
       const R sx = (R)(pj.x - cax), sy = (R)(pj.y - cay), sz = (R)(pj.z - caz);
-      pair_m1_f32<DAMP>(tx - sx, ty - sy, tz - sz, (R)mj.x, (R)mj.y, (R)mj.z, di * (R)pj.w, fx, fy, fz);
+      staged_pair<R, DAMP, B2>(tx - sx, ty - sy, tz - sz, (R)mj.x, (R)mj.y, (R)mj.z, di * (R)pj.w, fx, fy, fz);
     }
   }
   const int n = a.n;
@@ -1137,14 +1172,45 @@ void launch_near(System& S, const NearArgs& a, const MU4* mu4, const R* L, bool 
     build_near_expanded(S);
     const Octree& T = S.tree;
     const dim3 g(ceil_div(ni, block / 64)), b(block);
-    if (with_l2p)
-      hipLaunchKernelGGL((k_fmm_near_st<P, R, DAMP, MODE, MU4, true>), g, b, 0, st, a, F.near_items[ci].p, ni,
-                         64 >> ci, T.xoff.p, T.xsrc.p, mu4, L);
+    const int chunk = 64 >> ci;
+    // the branch-free FP32 pair needs every site damped
+    constexpr bool kB2 = sizeof(R) == 4 && DAMP;
+    const bool b2      = kB2 && S.all_damped;
+    auto go = [&](auto l2p, auto cht, auto bb) {
+      hipLaunchKernelGGL((k_fmm_near_st<P, R, DAMP, MODE, MU4, decltype(l2p)::value, decltype(cht)::value,
+                                        decltype(bb)::value>),
+                         g, b, 0, st, a, F.near_items[ci].p, ni, chunk, T.xoff.p, T.xsrc.p, mu4, L);
+    };
+    auto go2 = [&](auto l2p, auto bb) {
+      // the unrolled full-chunk path pays for branch-free pairs only (a damped FP64 pair branches
+      // around its exponential, and unrolling it costs occupancy)
+      if constexpr (decltype(bb)::value || !DAMP) {
+        if (chunk == 8) {
+          go(l2p, std::integral_constant<int, 8>{}, bb);
+          return;
+        }
+      }
+      go(l2p, std::integral_constant<int, 0>{}, bb);
+    };
+    auto go1 = [&](auto bb) {
+      if (with_l2p)
+        go2(std::true_type{}, bb);
+      else
+        go2(std::false_type{}, bb);
+    };
+    if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[6], st));
+    if (b2)
+      go1(std::integral_constant<bool, kB2>{});
     else
-      hipLaunchKernelGGL((k_fmm_near_st<P, R, DAMP, MODE, MU4, false>), g, b, 0, st, a, F.near_items[ci].p, ni,
-                         64 >> ci, T.xoff.p, T.xsrc.p, mu4, L);
-    if (a.eoff)
-      hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a, mu4);
+      go1(std::false_type{});
+    if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[7], st));
+    if (a.eoff) {
+      if (b2)
+        hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, kB2>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a, mu4);
+      else
+        hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, false>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a,
+                           mu4);
+    }
   } else if (var == 4) {
     ensure_clusters(S);
     const dim3 g(ceil_div(F.n_clusters, block / 64)), b(block);

@@ -24,9 +24,12 @@ __device__ __forceinline__ void pair_m1_f64(double dx, double dy, double dz, dou
   const double dmu = dx * mx + dy * my + dz * mz;
   const double c3  = fE * ri3;
   const double c5  = 3.0 * fT * dmu * ri3 * ri2;
-  fx += c3 * mx - c5 * dx;
-  fy += c3 * my - c5 * dy;
-  fz += c3 * mz - c5 * dz;
+  fx = __builtin_fma(c3, mx, fx);
+  fx = __builtin_fma(-c5, dx, fx);
+  fy = __builtin_fma(c3, my, fy);
+  fy = __builtin_fma(-c5, dy, fy);
+  fz = __builtin_fma(c3, mz, fz);
+  fz = __builtin_fma(-c5, dz, fz);
 }
 
 template <bool DAMP, bool SELF0 = false>
@@ -44,9 +47,46 @@ __device__ __forceinline__ void pair_m1_f32(float dx, float dy, float dz, float 
   const float dmu = dx * mx + dy * my + dz * mz;
   const float c3  = fE * ri3;
   const float c5  = 3.0f * fT * dmu * ri3 * ri2;
-  fx += c3 * mx - c5 * dx;
-  fy += c3 * my - c5 * dy;
-  fz += c3 * mz - c5 * dz;
+  fx = __builtin_fmaf(c3, mx, fx);
+  fx = __builtin_fmaf(-c5, dx, fx);
+  fy = __builtin_fmaf(c3, my, fy);
+  fy = __builtin_fmaf(-c5, dy, fy);
+  fz = __builtin_fmaf(c3, mz, fz);
+  fz = __builtin_fmaf(-c5, dz, fz);
+}
+
+// FP32 damped pair for systems whose sites are all damped (alpha_iso > 0 everywhere), without
+// branches. a2 = damping (s_i s_j) log2(e), so u2 = a2 R is the Thole argument v in base-2 units
+// (v = u2 ln 2) and e^-v = 2^-u2 comes straight from v_exp_f32. With v = u2 ln2 the factors are
+//   f_E = 1 - (1 + ln2 u2 + (ln2)^2/2 u2^2) 2^-u2,   f_T = f_E - (ln2)^3/6 u2^3 2^-u2.
+// A zero displacement (the site itself) contributes exactly zero: 1/R is clamped to 2^40, so
+// R = 0, u2 = 0, 2^-u2 = 1 and f_E = f_T = 0 exactly (this needs a2 finite, i.e. damped sites).
+// Large u2 (undamped in effect) gives 2^-u2 = 0 and f_E = f_T = 1.
+__device__ __forceinline__ void pair_m1_f32_b2(float dx, float dy, float dz, float mx, float my, float mz,
+                                               float a2, float& fx, float& fy, float& fz) {
+  constexpr float kC1 = 0.693147180559945309f;   // ln 2
+  constexpr float kC2 = 0.240226506959100712f;   // (ln 2)^2 / 2
+  constexpr float kC3 = 0.0555041086648215800f;  // (ln 2)^3 / 6
+  const float r2   = dx * dx + dy * dy + dz * dz;
+  const float rinv = __builtin_fminf(rsqrt_f32(r2), 1099511627776.0f);  // 2^40
+  const float ri2  = rinv * rinv;
+  const float ri3  = ri2 * rinv;
+  const float u    = a2 * (r2 * rinv);
+  const float e    = __builtin_amdgcn_exp2f(-u);
+  const float u2   = u * u;
+  const float p    = __builtin_fmaf(u2, kC2, __builtin_fmaf(u, kC1, 1.0f));
+  const float fE   = __builtin_fmaf(-e, p, 1.0f);
+  const float q    = (u2 * u) * e;
+  const float fT   = __builtin_fmaf(-kC3, q, fE);
+  const float dmu  = dx * mx + dy * my + dz * mz;
+  const float c3   = fE * ri3;
+  const float c5   = ((fT * dmu) * ri3) * (3.0f * ri2);
+  fx = __builtin_fmaf(c3, mx, fx);
+  fx = __builtin_fmaf(-c5, dx, fx);
+  fy = __builtin_fmaf(c3, my, fy);
+  fy = __builtin_fmaf(-c5, dy, fy);
+  fz = __builtin_fmaf(c3, mz, fz);
+  fz = __builtin_fmaf(-c5, dz, fz);
 }
 
 // Static field of charge + dipole + packed quadrupole S (half diagonal), combined as

@@ -73,6 +73,7 @@ struct System {
   cppe_hip_options opts{};
   hipStream_t stream = nullptr;
   bool has_alpha = false;
+  bool all_damped = false;  // every site has alpha_iso > 0 (Thole damping applies to every pair)
 
   // site data in device order (Morton-sorted)
   dbuf<double> x, y, z;          // positions
@@ -118,7 +119,8 @@ struct System {
   dbuf<double> io_a, io_b;  // AoS3 staging in caller order
   dbuf<float> f32_src;      // FP32 staging for mixed-precision applies
   bool mu_valid = false;
-  hipEvent_t* prof = nullptr;  // when set: events [0..4] recorded at the FMM phase boundaries
+  hipEvent_t* prof = nullptr;  // when set (8 events): [0..4] FMM phase boundaries, [6..7] around the
+                               // staged near-field kernel alone
 
   ~System();
 };
@@ -145,6 +147,7 @@ void cg_graphs_release(System& S);
 void fmm_field(System& S, int m, const double* src, double* F, double damping, bool fp32);
 void fmm_static(System& S, const double* q, const double* mu, const double* th, double* F, double damping);
 void fmm_destroy(System& S);
+int fmm_near_variant(const System& S, bool fp32);  // tuned near-field variant (0 before the first apply)
 void ml_grid_destroy(System& S);
 
 // layout conversions between caller AoS (caller order) and device SoA (device order)

@@ -83,6 +83,14 @@ void CppeState::update_induced_moments(Vector elec_fields, bool elec_only) {  //
   }
 }
 
+Matrix CppeState::nuclear_interaction_energy_gradient() {  // cppe_state.cc:73-76
+  return MultipoleExpansion(m_mol, m_potentials).nuclear_gradient();
+}
+
+Matrix CppeState::nuclear_field_gradient() {  // cppe_state.cc:78-81
+  return NuclearFields(m_mol, m_potentials).nuclear_gradient();
+}
+
 Matrix CppeState::induced_moments_eef() {  // cppe_state.cc:111-128
   const size_t n = m_polarizable_sites;
   Matrix ret(3 * n, 3);

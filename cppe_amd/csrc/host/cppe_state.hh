@@ -39,6 +39,8 @@ class CppeState {
   Vector get_induced_moments_vec() const { return m_induced_moments; }
   void update_induced_moments(Vector elec_fields, bool elec_only = false);
   Matrix induced_moments_eef();
+  Matrix nuclear_interaction_energy_gradient();  // cppe_state.cc:73-76, natoms x 3
+  Matrix nuclear_field_gradient();               // cppe_state.cc:78-81, (3 natoms) x (3 n_polsites)
 
   size_t get_polarizable_site_number() const { return m_polarizable_sites; }
   Vector get_static_fields() const { return m_nuc_fields + m_multipole_fields; }

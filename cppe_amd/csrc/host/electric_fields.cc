@@ -29,6 +29,24 @@ Vector NuclearFields::compute() {  // electric_fields.cc:58-75
   return F;
 }
 
+Matrix NuclearFields::nuclear_gradient() {  // electric_fields.cc:77-104
+  const int n = (int)m_polsites.size();
+  Matrix g(3 * m_mol.size(), 3 * (size_t)n);
+  if (n == 0 || m_mol.empty()) return g;
+  std::vector<double> pos(3 * (size_t)n), ch, apos;
+  for (int i = 0; i < n; ++i) {
+    pos[3 * i]     = m_polsites[i].m_x;
+    pos[3 * i + 1] = m_polsites[i].m_y;
+    pos[3 * i + 2] = m_polsites[i].m_z;
+  }
+  for (const Atom& a : m_mol) {
+    ch.push_back((double)a.charge);
+    apos.insert(apos.end(), {a.m_x, a.m_y, a.m_z});
+  }
+  hip_check(cppe_hip_nuclear_field_gradient(n, pos.data(), (int)m_mol.size(), ch.data(), apos.data(), g.data.data()));
+  return g;
+}
+
 MultipoleFields::MultipoleFields(std::vector<Potential> potentials, const PeOptions& options)
       : m_potentials(std::move(potentials)), m_options(options) {}
 

@@ -19,6 +19,7 @@ class NuclearFields {
  public:
   NuclearFields(Molecule mol, std::vector<Potential> potentials);
   Vector compute();  // 3 * n_polsites, AoS
+  Matrix nuclear_gradient();  // electric_fields.cc:77-104, (3 natoms) x (3 n_polsites)
 
  private:
   Molecule m_mol;

@@ -4,10 +4,24 @@
 
 namespace libcppe {
 
-double MultipoleExpansion::interaction_energy() {
-  const int n = (int)m_potentials.size();
-  std::vector<double> pos(3 * (size_t)n), mul(10 * (size_t)n, 0.0), ch, apos;
-  std::vector<int> kmax(n, -1);
+namespace {
+// Site and atom arrays of the device calls: positions, highest order per site, 10 values per site
+// (q, mu, packed quadrupole), atom charges and positions.
+struct PackedSites {
+  std::vector<double> pos, mul, ch, apos;
+  std::vector<int> kmax;
+};
+
+PackedSites pack(const std::vector<Potential>& pots, const Molecule& mol) {
+  const int n = (int)pots.size();
+  PackedSites P;
+  P.pos.resize(3 * (size_t)n);
+  P.mul.assign(10 * (size_t)n, 0.0);
+  P.kmax.assign(n, -1);
+  std::vector<double>& pos = P.pos;
+  std::vector<double>& mul = P.mul;
+  std::vector<int>& kmax   = P.kmax;
+  const std::vector<Potential>& m_potentials = pots;
   for (int i = 0; i < n; ++i) {
     const Potential& p = m_potentials[i];
     pos[3 * i]         = p.m_x;
@@ -21,15 +35,32 @@ double MultipoleExpansion::interaction_energy() {
       if ((int)m.m_k > kmax[i]) kmax[i] = (int)m.m_k;
     }
   }
-  for (const Atom& a : m_mol) {
-    ch.push_back((double)a.charge);
-    apos.insert(apos.end(), {a.m_x, a.m_y, a.m_z});
+  for (const Atom& a : mol) {
+    P.ch.push_back((double)a.charge);
+    P.apos.insert(P.apos.end(), {a.m_x, a.m_y, a.m_z});
   }
-  double e = 0.0;
+  return P;
+}
+}  // namespace
+
+double MultipoleExpansion::interaction_energy() {  // multipole_expansion.cc:10-33
+  const int n = (int)m_potentials.size();
+  double e    = 0.0;
   if (n == 0 || m_mol.empty()) return e;
-  hip_check(cppe_hip_nuclear_multipole_energy(n, pos.data(), kmax.data(), mul.data(), (int)m_mol.size(), ch.data(),
-                                              apos.data(), &e));
+  PackedSites P = pack(m_potentials, m_mol);
+  hip_check(cppe_hip_nuclear_multipole_energy(n, P.pos.data(), P.kmax.data(), P.mul.data(), (int)m_mol.size(),
+                                              P.ch.data(), P.apos.data(), &e));
   return e;
+}
+
+Matrix MultipoleExpansion::nuclear_gradient() {  // multipole_expansion.cc:35-62
+  const int n = (int)m_potentials.size();
+  Matrix g(m_mol.size(), 3);
+  if (n == 0 || m_mol.empty()) return g;
+  PackedSites P = pack(m_potentials, m_mol);
+  hip_check(cppe_hip_nuclear_multipole_gradient(n, P.pos.data(), P.kmax.data(), P.mul.data(), (int)m_mol.size(),
+                                                P.ch.data(), P.apos.data(), g.data.data()));
+  return g;
 }
 
 }  // namespace libcppe

@@ -2,6 +2,7 @@
 #pragma once
 #include <vector>
 
+#include "linalg.hh"
 #include "molecule.hh"
 #include "potential.hh"
 
@@ -12,6 +13,7 @@ class MultipoleExpansion {
   MultipoleExpansion(Molecule core, std::vector<Potential> potentials)
         : m_mol(std::move(core)), m_potentials(std::move(potentials)) {}
   double interaction_energy();  // multipole_expansion.cc:10-33
+  Matrix nuclear_gradient();    // multipole_expansion.cc:35-62, natoms x 3
 
  private:
   Molecule m_mol;

@@ -165,6 +165,9 @@ PYBIND11_MODULE(pycppe, m) {
         .def("calculate_static_energies_and_fields", &CppeState::calculate_static_energies_and_fields)
         .def("get_induced_moments", [](const CppeState& s) { return to_np(s.get_induced_moments()); })
         .def("induced_moments_eef", [](CppeState& s) { return to_np(s.induced_moments_eef()); })
+        .def("nuclear_interaction_energy_gradient",
+             [](CppeState& s) { return to_np(s.nuclear_interaction_energy_gradient()); })
+        .def("nuclear_field_gradient", [](CppeState& s) { return to_np(s.nuclear_field_gradient()); })
         .def_readwrite("energies", &CppeState::m_pe_energy)
         .def_property_readonly("total_energy", &CppeState::get_total_energy)
         .def(
@@ -187,7 +190,8 @@ PYBIND11_MODULE(pycppe, m) {
   // fields (export_fields.cc)
   py::class_<NuclearFields>(m, "NuclearFields", "Electric fields created by nuclei")
         .def(py::init<Molecule, std::vector<Potential>>())
-        .def("compute", [](NuclearFields& f) { return to_np(f.compute()); });
+        .def("compute", [](NuclearFields& f) { return to_np(f.compute()); })
+        .def("nuclear_gradient", [](NuclearFields& f) { return to_np(f.nuclear_gradient()); });
   py::class_<MultipoleFields, std::shared_ptr<MultipoleFields>>(m, "MultipoleFields",
                                                                 "Electric fields created by multipoles")
         .def(py::init([](std::vector<Potential> p, py::dict o) {
@@ -225,5 +229,6 @@ PYBIND11_MODULE(pycppe, m) {
              });
   py::class_<MultipoleExpansion, std::shared_ptr<MultipoleExpansion>>(m, "MultipoleExpansion")
         .def(py::init<Molecule, std::vector<Potential>>())
-        .def("interaction_energy", &MultipoleExpansion::interaction_energy);
+        .def("interaction_energy", &MultipoleExpansion::interaction_energy)
+        .def("nuclear_gradient", [](MultipoleExpansion& e) { return to_np(e.nuclear_gradient()); });
 }

@@ -190,8 +190,9 @@ int cppe_hip_set_tune(cppe_hip_system* sys, const int* choice);
 
 /*
  * Times `reps` B applies of a fixed pseudo-random vector on the system's stream, with HIP
- * events around each phase. phase_ms (5 values, mean per apply): [0] upward pass (P2M+M2M),
- * [1] M2L, [2] L2L, [3] near field (P2P+L2P, or the direct pair kernel), [4] whole apply.
+ * events around each phase. phase_ms (6 values, mean per apply): [0] upward pass (P2M+M2M),
+ * [1] M2L, [2] L2L, [3] near field (near-field kernel with fused L2P, plus the exclusion pass),
+ * [4] whole apply, [5] the staged near-field kernel alone (-1 when another variant is tuned).
  * No counterpart in the reference (measurement support).
  */
 int cppe_hip_profile_apply(cppe_hip_system* sys, int fp32, int reps, double* phase_ms);
@@ -207,6 +208,23 @@ int cppe_hip_nuclear_fields(int n, const double* pos, int natoms, const double* 
 int cppe_hip_nuclear_multipole_energy(int n, const double* pos, const int* kmax, const double* mul,
                                       int natoms, const double* charges, const double* atom_pos,
                                       double* energy);
+
+/*
+ * NuclearFields::nuclear_gradient (electric_fields.cc:77-104): derivative of the nuclear fields at
+ * the n polarizable sites with respect to the atom positions, row-major (3 natoms) x (3 n):
+ * grad[(3a + p) * 3n + 3i + q] = Z_a T2(r_i - R_a)_pq.
+ */
+int cppe_hip_nuclear_field_gradient(int n, const double* pos, int natoms, const double* charges,
+                                    const double* atom_pos, double* grad);
+
+/*
+ * MultipoleExpansion::nuclear_gradient (multipole_expansion.cc:35-62): derivative of the
+ * nuclear-multipole energy with respect to each atom position, natoms x 3 row-major. Same site
+ * arguments as cppe_hip_nuclear_multipole_energy.
+ */
+int cppe_hip_nuclear_multipole_gradient(int n, const double* pos, const int* kmax, const double* mul,
+                                        int natoms, const double* charges, const double* atom_pos,
+                                        double* grad);
 
 /* out = sum_i a_i b_i in FP64 on the device (polarization energies, cppe_state.cc:97-108) */
 int cppe_hip_dot(long long n, const double* a, const double* b, double* out);

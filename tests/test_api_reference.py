@@ -256,3 +256,68 @@ def test_fmm_fields_error_columns(theta, order):
     err = np.linalg.norm(fs - F, axis=1) / np.linalg.norm(fs, axis=1)
     col = list(ref["err_columns"]).index(f"err_F_{theta}_{order}")
     np.testing.assert_allclose(ref["errors"][:, col], err, atol=1e-8)
+
+
+# ---- test_gradients.py ---------------------------------------------------------------------------
+STENCIL_5P = ([-2, -1, 1, 2], np.array([1.0, -8.0, 8.0, -1.0]) / 12.0)
+
+
+def _molecule(zs, coords):
+    mol = cppe.Molecule()
+    for z, c in zip(zs, coords):
+        mol.append(cppe.Atom(int(z), *c))
+    return mol
+
+
+def _fdiff(zs, coords, fn, step=1e-3):
+    """Five-point central differences of fn(molecule) in every atom coordinate (test_gradients.py:48-97)."""
+    out = None
+    for i in range(len(zs)):
+        for c in range(3):
+            acc = 0.0
+            for f, w in zip(*STENCIL_5P):
+                cp = coords.copy()
+                cp[i, c] += f * step
+                acc = acc + w * np.asarray(fn(_molecule(zs, cp))) / step
+            if out is None:
+                out = np.zeros((len(zs), 3) + np.shape(acc))
+            out[i, c] = acc
+    return out
+
+
+@pytest.mark.gpu
+def test_nuclear_gradients_vs_finite_differences():
+    """tests/test_gradients.py:32-46: energy and field gradients against 5-point differences, atol 1e-10."""
+    mol = pna_molecule()
+    zs = [a.atomic_number for a in mol]
+    coords = np.array([a.position for a in mol])
+    potentials = cppe.PotfileReader(POT).read()
+    grad_nuc = cppe.MultipoleExpansion(mol, potentials).nuclear_gradient()
+    fd = _fdiff(zs, coords, lambda m: cppe.MultipoleExpansion(m, potentials).interaction_energy())
+    assert grad_nuc.shape == (len(zs), 3)
+    np.testing.assert_allclose(fd, grad_nuc, atol=1e-10)
+    npol = len(cppe.get_polarizable_sites(potentials))
+    grad_field = cppe.NuclearFields(mol, potentials).nuclear_gradient()
+    assert grad_field.shape == (3 * len(zs), 3 * npol)
+    fdf = _fdiff(zs, coords, lambda m: cppe.NuclearFields(m, potentials).compute().reshape(npol, 3))
+    np.testing.assert_allclose(fdf, grad_field.reshape(len(zs), 3, npol, 3), atol=1e-10)
+    # the state's entry points return the same matrices (cppe_state.cc:73-81)
+    s = cppe.CppeState({"potfile": POT}, mol, lambda x: None)
+    np.testing.assert_array_equal(s.nuclear_interaction_energy_gradient(), grad_nuc)
+    np.testing.assert_array_equal(s.nuclear_field_gradient(), grad_field)
+
+
+@pytest.mark.gpu
+def test_nuclear_gradients_no_atoms_and_symmetry():
+    """An empty molecule gives an empty gradient; each 3x3 block Z_a T2(r_i - R_a) of the field
+    gradient is symmetric and traceless (the potential of a point charge is harmonic)."""
+    potentials = cppe.PotfileReader(POT).read()
+    assert cppe.MultipoleExpansion(cppe.Molecule(), potentials).nuclear_gradient().shape == (0, 3)
+    mol = pna_molecule()
+    npol = len(cppe.get_polarizable_sites(potentials))
+    G = cppe.NuclearFields(mol, potentials).nuclear_gradient().reshape(len(mol), 3, npol, 3)
+    # T2 is symmetric and traceless at every pair
+    for a in range(len(mol)):
+        blk = G[a].transpose(1, 0, 2)  # (site, p, q)
+        np.testing.assert_allclose(blk, blk.transpose(0, 2, 1), atol=1e-14)
+        np.testing.assert_allclose(np.trace(blk, axis1=1, axis2=2), 0.0, atol=1e-12)

@@ -73,3 +73,24 @@ def test_set_tune_rejects_unknown_shapes():
         S.set_tune([0, 96, 64, 0, 0, 256, 64, 0])
     with pytest.raises(capi.CppeHipError):
         S.set_tune([5, 256, 64, 2, 0, 256, 64, 0])
+
+
+def test_fp32_staged_with_undamped_sites():
+    """Sites with alpha_iso = 0 are undamped: the FP32 near field then takes the branching pair
+    (the branch-free form assumes every site damped). Both forms, on the same system, against
+    the FP64 apply of the same shape."""
+    w = synthetic.water_box(1500, 7201, anisotropic=True)
+    iso = w.alpha_iso().copy()
+    iso[::7] = 0.0
+    x = np.random.default_rng(12).normal(size=3 * w.n)
+    out = {}
+    for precision in ("fp64", "mixed"):
+        S = capi.System(w.pos, w.pol, iso, w.excl_csr(), summation="fmm", ncrit=64, damping=DAMP,
+                        precision=precision)
+        S.set_tune(np.array([8, 256, 16, 0, 8, 256, 16, 0], dtype=np.int32))
+        out[precision] = S.apply_b(x)
+    ref = out["fp64"]
+    assert np.abs(out["mixed"] - ref).max() < 1e-5 * np.abs(ref).max()
+    # and the undamped sites really are undamped: the all-damped system differs
+    S = capi.System(w.pos, w.pol, w.alpha_iso(), w.excl_csr(), summation="fmm", ncrit=64, damping=DAMP)
+    assert np.abs(S.apply_b(x) - ref).max() > 1e-3 * np.abs(ref).max()
