@@ -26,7 +26,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(ROOT, "include"),
 ]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wno-unused-function",
               "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(CSRC, "host")]
 
 LIB_HIP = os.path.join(PKG, "libcppe_hip.so")
@@ -77,7 +77,7 @@ def build_host(jobs=8):
     objs = [os.path.join(BUILD, "host_" + os.path.basename(s) + ".o") for s in host_srcs]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda so: _run(["g++", "-c", *HOST_FLAGS, so[0], "-o", so[1]]), zip(host_srcs, objs)))
-    _run(["g++", "-shared", "-fPIC", *objs, "-o", LIB_HOST, "-L" + PKG, "-lcppe_hip",
+    _run(["g++", "-shared", "-fPIC", "-pthread", *objs, "-o", LIB_HOST, "-L" + PKG, "-lcppe_hip",
           "-Wl,-rpath,$ORIGIN"])
     py_srcs = sorted(glob.glob(os.path.join(CSRC, "python", "*.cc")))
     if py_srcs:
