@@ -23,6 +23,7 @@ EXPORTS = [
     "cppe_hip_profile_apply", "cppe_hip_batch_create", "cppe_hip_segment_results", "cppe_hip_set_graphs",
     "cppe_hip_autotune", "cppe_hip_set_tune", "cppe_hip_set_reliable_drop", "cppe_hip_last_solve_info",
     "cppe_hip_field_targets", "cppe_hip_nuclear_field_gradient", "cppe_hip_nuclear_multipole_gradient",
+    "cppe_hip_solve_multi", "cppe_hip_apply_b_multi",
     "cppe_hip_mlp_create", "cppe_hip_mlp_destroy", "cppe_hip_mlp_set", "cppe_hip_mlp_get", "cppe_hip_ml_features",
     "cppe_hip_mlp_forward", "cppe_hip_mlp_loss_grad", "cppe_hip_mlp_reserve", "cppe_hip_mlp_collect",
     "cppe_hip_mlp_prepare", "cppe_hip_mlp_train", "cppe_hip_ml_guess",
@@ -73,6 +74,9 @@ def lib():
         L.cppe_hip_nuclear_fields.argtypes = [ctypes.c_int, _d, ctypes.c_int, _d, _d, _d]
         L.cppe_hip_nuclear_multipole_energy.argtypes = [ctypes.c_int, _d, _i, _d, ctypes.c_int, _d, _d, _d]
         L.cppe_hip_nuclear_field_gradient.argtypes = [ctypes.c_int, _d, ctypes.c_int, _d, _d, _d]
+        L.cppe_hip_apply_b_multi.argtypes = [vp, ctypes.c_int, _d, _d]
+        L.cppe_hip_solve_multi.argtypes = [vp, ctypes.c_int, _d, _d, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                           _d, _i, _d]
         L.cppe_hip_nuclear_multipole_gradient.argtypes = [ctypes.c_int, _d, _i, _d, ctypes.c_int, _d, _d, _d]
         L.cppe_hip_dot.argtypes = [ctypes.c_longlong, _d, _d, _d]
         L.cppe_hip_tree_stats.argtypes = [vp, _ll]
@@ -158,6 +162,12 @@ class System:
     def apply_b(self, x):
         return self._vec_op("cppe_hip_apply_b", x)
 
+    def apply_b_multi(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 3 * self.n)
+        out = np.zeros_like(X)
+        _check(lib().cppe_hip_apply_b_multi(self.h, X.shape[0], _p(X), _p(out)))
+        return out
+
     def apply_alpha(self, x):
         return self._vec_op("cppe_hip_apply_alpha", x)
 
@@ -201,6 +211,18 @@ class System:
         if rc != OK and (raise_on_noconv or rc != ERR_NOCONV):
             _check(rc)
         return mu, it.value, norms[:it.value]
+
+    def solve_multi(self, rhs, guess=None, make_guess=True, thresh=1e-8, maxiter=50):
+        """Solves B mu_k = rhs_k for the rows of rhs (nrhs x 3n) together; returns (mu, iters, norms)."""
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64).reshape(-1, 3 * self.n)
+        k = rhs.shape[0]
+        g = None if guess is None else np.ascontiguousarray(guess, dtype=np.float64).reshape(k, 3 * self.n)
+        mu = np.zeros((k, 3 * self.n))
+        it = np.zeros(k, dtype=np.int32)
+        norms = np.zeros((k, max(maxiter, 1)))
+        _check(lib().cppe_hip_solve_multi(self.h, k, _p(rhs), _p(g), int(make_guess), float(thresh), int(maxiter),
+                                          _p(mu), _p(it, _i), _p(norms)))
+        return mu, it, [norms[j, :it[j]] for j in range(k)]
 
     def upload_rhs(self, rhs):
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)

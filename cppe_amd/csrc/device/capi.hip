@@ -78,7 +78,6 @@ __global__ void k_excl_fill(int n, const int* __restrict__ perm, const int* __re
 
 System::~System() {
   cg_graphs_release(*this);
-  if (h_status) (void)hipHostFree(h_status);
   if (fmm) fmm_destroy(*this);
   if (ml_grid) ml_grid_destroy(*this);
   if (stream) (void)hipStreamDestroy(stream);
@@ -356,6 +355,23 @@ int cppe_hip_apply_b(cppe_hip_system* sys, const double* x, double* out) {
   CPPE_API_END
 }
 
+int cppe_hip_apply_b_multi(cppe_hip_system* sys, int nvec, const double* x, double* out) {
+  CPPE_API_BEGIN
+  if (!sys || nvec < 1 || !x || !out) return fail(CPPE_HIP_ERR_ARG, "invalid apply arguments");
+  System& S = sys->S;
+  if (!S.has_alpha) return fail(CPPE_HIP_ERR_ARG, "system has no polarizabilities");
+  if (S.nseg > 1) return fail(CPPE_HIP_ERR_UNSUPPORTED, "several right-hand sides need a one-snapshot system");
+  setup_multi_rhs(S, nvec);
+  fmm_reserve_multi(S);
+  CgState& C      = S.cgm;
+  const size_t n3 = 3 * (size_t)S.n;
+  for (int k = 0; k < nvec; ++k) host_to_soa(S, x + k * n3, 3, C.p + k * n3);
+  apply_b_multi(S, C.p, C.Ap, nvec, S.opts.precision == CPPE_HIP_PREC_MIXED);
+  for (int k = 0; k < nvec; ++k) soa_to_host(S, C.Ap + k * n3, out + k * n3);
+  return CPPE_HIP_OK;
+  CPPE_API_END
+}
+
 __global__ void k_alpha_inv_apply(int n, const double* __restrict__ a6, const double* __restrict__ in,
                                   double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -513,7 +529,7 @@ static int run_solve(System& S, int guess_mode, const double* guess_host, double
   else if (guess_mode == 2 || (guess_mode == 1 && !S.mu_valid))
     fill_f64(S, S.v_mu.p, n3, 0.0);
   CgResult& res = S.last;
-  device_cg(S, thresh, maxiter, S.opts.precision == CPPE_HIP_PREC_MIXED, res);
+  device_cg(S, S.cg, thresh, maxiter, S.opts.precision == CPPE_HIP_PREC_MIXED, res);
   CPPE_HIP_CHECK(hipEventRecord(e1, st));
   CPPE_HIP_CHECK(hipEventSynchronize(e1));
   float ms = 0.f;
@@ -540,6 +556,39 @@ int cppe_hip_solve(cppe_hip_system* sys, const double* rhs, const double* guess,
   int rc   = run_solve(S, mode, guess, thresh, maxiter, iters, norms, nullptr);
   soa_to_host(S, S.v_mu.p, mu);
   return rc;
+  CPPE_API_END
+}
+
+int cppe_hip_solve_multi(cppe_hip_system* sys, int nrhs, const double* rhs, const double* guess, int make_guess,
+                         double thresh, int maxiter, double* mu, int* iters, double* norms) {
+  CPPE_API_BEGIN
+  if (!sys || nrhs < 1 || !rhs || !mu || maxiter < 0) return fail(CPPE_HIP_ERR_ARG, "invalid solve arguments");
+  System& S = sys->S;
+  if (!S.has_alpha) return fail(CPPE_HIP_ERR_ARG, "system has no polarizabilities");
+  if (S.nseg > 1) return fail(CPPE_HIP_ERR_UNSUPPORTED, "several right-hand sides need a one-snapshot system");
+  setup_multi_rhs(S, nrhs);
+  CgState& C      = S.cgm;
+  const size_t n3 = 3 * (size_t)S.n;
+  for (int k = 0; k < nrhs; ++k) {
+    host_to_soa(S, rhs + k * n3, 3, C.rhs + k * n3);
+    if (make_guess)
+      precondition(S, C.rhs + k * n3, C.x + k * n3);
+    else if (guess)
+      host_to_soa(S, guess + k * n3, 3, C.x + k * n3);
+    else
+      fill_f64(S, C.x + k * n3, n3, 0.0);
+  }
+  CgResult res;
+  device_cg(S, C, thresh, maxiter, S.opts.precision == CPPE_HIP_PREC_MIXED, res);
+  for (int k = 0; k < nrhs; ++k) {
+    soa_to_host(S, C.x + k * n3, mu + k * n3);
+    if (iters) iters[k] = res.iters[k];
+    if (norms)
+      for (size_t it = 0; it < res.norms[k].size() && (int)it < maxiter; ++it)
+        norms[(size_t)k * maxiter + it] = res.norms[k][it];
+  }
+  if (!res.all_converged) return fail(CPPE_HIP_ERR_NOCONV, "Failed to converge induced dipole moments.");
+  return CPPE_HIP_OK;
   CPPE_API_END
 }
 

@@ -10,6 +10,7 @@ namespace cppe_dev {
 // instantiated per order in fmm_order.hip
 #define CPPE_EXTERN_FMM(P)                                                                \
   extern template void fmm_apply_b_t<P>(System&, const double*, double*, bool);           \
+  extern template void fmm_apply_b_multi_t<P>(System&, const double*, double*, bool);     \
   extern template void fmm_field1_t<P>(System&, const double*, double*, double);          \
   extern template void fmm_static_t<P>(System&, const double*, double*, double);
 CPPE_EXTERN_FMM(2)
@@ -56,6 +57,34 @@ void apply_b_fmm(System& S, const double* x, double* out, bool fp32) {
 #define CALL(P) fmm_apply_b_t<P>(S, x, out, fp32)
   CPPE_FMM_DISPATCH(CALL)
 #undef CALL
+}
+
+// Groups of kMultiRhs right-hand sides share one near-field launch (staged near variants); the
+// rest, and every right-hand side under another variant, are applied one at a time.
+void apply_b_multi(System& S, const double* x, double* out, int nrhs, bool fp32) {
+  FmmState& F     = state(S);
+  const size_t n3 = 3 * (size_t)S.n;
+  int k           = 0;
+  if (F.tune.near_variant[fp32 ? 1 : 0] >= 5)
+    for (; k + kMultiRhs <= nrhs; k += kMultiRhs) {
+#define CALL(P) fmm_apply_b_multi_t<P>(S, x + k * n3, out + k * n3, fp32)
+      CPPE_FMM_DISPATCH(CALL)
+#undef CALL
+    }
+  for (; k < nrhs; ++k) apply_b_fmm(S, x + k * n3, out + k * n3, fp32);
+}
+
+void fmm_reserve_multi(System& S) {
+  FmmState& F     = state(S);
+  const size_t nc = S.tree.ncells;
+  const int p     = S.opts.order;
+  ensure_buf(F.M64k, kMultiRhs * nc * fmm_msize(0, p));
+  ensure_buf(F.L64k, kMultiRhs * nc * fmm_lsize(0, p));
+  ensure_buf(F.M32k, kMultiRhs * nc * fmm_msize(0, p));
+  ensure_buf(F.L32k, kMultiRhs * nc * fmm_lsize(0, p));
+  ensure_buf(F.mu4k, kMultiRhs * (size_t)S.n);
+  ensure_buf(F.mu4fk, kMultiRhs * (size_t)S.n);
+  CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
 }
 
 __global__ void k_pack10(int n, int m, const double* __restrict__ src, double* __restrict__ out) {

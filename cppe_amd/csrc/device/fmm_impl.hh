@@ -66,7 +66,19 @@ struct FmmState {
   dbuf<double4> pos4;     // near-field source records: x, y, z, alpha_iso^(-1/6)
   dbuf<double4> mu4;      // near-field dipole records (FP64 form)
   dbuf<float4> mu4f;      // near-field dipole records (FP32 form)
+  // several right-hand sides per apply (kMultiRhs at a time): their expansions and dipole records
+  dbuf<double> M64k, L64k;
+  dbuf<float> M32k, L32k;
+  dbuf<double4> mu4k;
+  dbuf<float4> mu4fk;
 };
+
+// right-hand sides whose near fields one staged launch evaluates together
+constexpr int kMultiRhs = 3;
+
+// near-field dipole record type of each precision
+template <typename R>
+using MU4Of = typename std::conditional<sizeof(R) == 8, double4, float4>::type;
 
 template <int KIND, int P, typename R>
 struct Ops;
@@ -796,24 +808,35 @@ __device__ __forceinline__ void wave_lds_barrier() {
 }
 
 // LDS source record of the staged near-field kernel: position (FP32: offset from the target
-// leaf centre; FP64: absolute), alpha_iso^(-1/6), dipole.
-template <typename R>
-struct alignas(4 * sizeof(R)) StagedRec {  // two 16-byte (FP32) / 32-byte (FP64) LDS reads
+// leaf centre; FP64: absolute), alpha_iso^(-1/6), and the dipole of each of K right-hand sides.
+template <typename R, int K>
+struct alignas(K == 1 ? 4 * sizeof(R) : 16) StagedRec {  // K = 1: two 16-byte (FP32) / 32-byte (FP64) reads
   R x, y, z, s;
-  R mx, my, mz;
+  R m[3 * K];
 };
 
-// One near-field pair of the staged kernel and of the exclusion pass (identical arithmetic, so an
-// excluded pair is taken out exactly as it was added). B2: the branch-free FP32 form for systems
-// whose sites are all damped; di then carries the factor log2(e).
+// Geometry of one near-field pair of the staged kernel and of the exclusion pass (identical
+// arithmetic, so an excluded pair is taken out exactly as it was added). B2: the branch-free FP32
+// form for systems whose sites are all damped; di then carries the factor log2(e).
 template <typename R, bool DAMP, bool B2>
-__device__ __forceinline__ void staged_pair(R dx, R dy, R dz, R mx, R my, R mz, R a, R& fx, R& fy, R& fz) {
+__device__ __forceinline__ void staged_geo(R dx, R dy, R dz, R a, R& c3, R& cf) {
   if constexpr (sizeof(R) == 8)
-    pair_m1_f64<DAMP, true>(dx, dy, dz, mx, my, mz, a, fx, fy, fz);
+    geo_m1_f64<DAMP, true>(dx, dy, dz, a, c3, cf);
   else if constexpr (B2)
-    pair_m1_f32_b2(dx, dy, dz, mx, my, mz, a, fx, fy, fz);
+    geo_m1_f32_b2(dx, dy, dz, a, c3, cf);
   else
-    pair_m1_f32<DAMP, true>(dx, dy, dz, mx, my, mz, a, fx, fy, fz);
+    geo_m1_f32<DAMP, true>(dx, dy, dz, a, c3, cf);
+}
+
+// one pair against the K dipoles of a staged record; f holds 3 K accumulators
+template <typename R, bool DAMP, bool B2, int K>
+__device__ __forceinline__ void staged_pair(R xi, R yi, R zi, R di, const StagedRec<R, K>& r, R* f) {
+  const R dx = xi - r.x, dy = yi - r.y, dz = zi - r.z;
+  R c3, cf;
+  staged_geo<R, DAMP, B2>(dx, dy, dz, di * r.s, c3, cf);
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    acc_m1(c3, cf, dx, dy, dz, r.m[3 * k], r.m[3 * k + 1], r.m[3 * k + 2], f[3 * k], f[3 * k + 1], f[3 * k + 2]);
 }
 
 template <typename R, bool B2>
@@ -831,14 +854,18 @@ __device__ __forceinline__ R staged_di(double damping, double s) {
 // target leaf centre, formed in FP64 and rounded. Slot partials are summed through LDS.
 // CHT = 8: built for chunks of 8 -- a full chunk against a full stage of 64 sources runs as a
 // straight 8-pair sequence per lane with constant LDS offsets (no per-pair loop arithmetic).
-template <int P, typename R, bool DAMP, int MODE, typename MU4, bool WITH_L2P = true, int CHT = 0, bool B2 = false>
+// K right-hand sides (B applies only): each pair's geometry is evaluated once and applied to K
+// dipoles; dipole records of right-hand side k at mu4 + k n, its L at L + k lstride, and
+// src / out at + 3 k n.
+template <int P, typename R, bool DAMP, int MODE, typename MU4, bool WITH_L2P = true, int CHT = 0, bool B2 = false,
+          int K = 1>
 __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __restrict__ items, int nitems,
                                                      int chunk, const long long* __restrict__ xoff,
                                                      const int* __restrict__ xsrc, const MU4* __restrict__ mu4,
-                                                     const R* __restrict__ L) {
+                                                     const R* __restrict__ L, size_t lstride = 0) {
   using O = Ops<KIND_M1, P, R>;
-  __shared__ StagedRec<R> rec[4][64];
-  __shared__ R red[3][4][64];
+  static_assert(3 * K * 64 * sizeof(R) <= 64 * sizeof(StagedRec<R, K>), "slot sums reuse the record LDS");
+  __shared__ StagedRec<R, K> rec[4][64];
   const int wl   = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int w    = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + wl);
@@ -852,29 +879,35 @@ __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __r
   const int t = lane % T, sl = lane / T;
   const bool act = sl < S;
   const int i    = a.start[A] + t0 + t;
+  const int n    = a.n;
   const double4 pi = a.pos4[i];
   const double cax = a.cx[A], cay = a.cy[A], caz = a.cz[A];
   const R xi = sizeof(R) == 8 ? (R)pi.x : (R)(pi.x - cax);
   const R yi = sizeof(R) == 8 ? (R)pi.y : (R)(pi.y - cay);
   const R zi = sizeof(R) == 8 ? (R)pi.z : (R)(pi.z - caz);
   const R di = DAMP ? staged_di<R, B2>(a.damping, pi.w) : R(0);
-  StagedRec<R>* sr = rec[wl];
-  R fx = R(0), fy = R(0), fz = R(0);
+  StagedRec<R, K>* sr = rec[wl];
+  R f[3 * K];
+#pragma unroll
+  for (int c = 0; c < 3 * K; ++c) f[c] = R(0);
   const long long q1 = xoff[A + 1];
   for (long long q0 = xoff[A]; q0 < q1; q0 += 64) {
     const int cn = (int)min<long long>(64, q1 - q0);
     if (lane < cn) {
       const int j      = xsrc[q0 + lane];
       const double4 pj = a.pos4[j];
-      const MU4 mj     = mu4[j];
-      StagedRec<R> r;
-      r.x  = sizeof(R) == 8 ? (R)pj.x : (R)(pj.x - cax);
-      r.y  = sizeof(R) == 8 ? (R)pj.y : (R)(pj.y - cay);
-      r.z  = sizeof(R) == 8 ? (R)pj.z : (R)(pj.z - caz);
-      r.s  = (R)pj.w;
-      r.mx = (R)mj.x;
-      r.my = (R)mj.y;
-      r.mz = (R)mj.z;
+      StagedRec<R, K> r;
+      r.x = sizeof(R) == 8 ? (R)pj.x : (R)(pj.x - cax);
+      r.y = sizeof(R) == 8 ? (R)pj.y : (R)(pj.y - cay);
+      r.z = sizeof(R) == 8 ? (R)pj.z : (R)(pj.z - caz);
+      r.s = (R)pj.w;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const MU4 mj     = mu4[(size_t)k * n + j];
+        r.m[3 * k]     = (R)mj.x;
+        r.m[3 * k + 1] = (R)mj.y;
+        r.m[3 * k + 2] = (R)mj.z;
+      }
       sr[lane] = r;
     }
     wave_lds_barrier();
@@ -883,58 +916,54 @@ __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __r
     if (CHT && T == CHT && cn == 64) {  // uniform: all 64 lanes active, 64 / CHT slots
       if constexpr (B2 || !DAMP) {  // branch-free pair: straight-line code, registers stay low
 #pragma unroll
-        for (int k = 0; k < CHT; ++k) {
-          const StagedRec<R> r = sr[sl + k * (64 / CHT)];
-          staged_pair<R, DAMP, B2>(xi - r.x, yi - r.y, zi - r.z, r.mx, r.my, r.mz, di * r.s, fx, fy, fz);
-        }
+        for (int k = 0; k < CHT; ++k) staged_pair<R, DAMP, B2, K>(xi, yi, zi, di, sr[sl + k * (64 / CHT)], f);
       } else {  // a branch per pair: unrolling would hoist the records into registers
 #pragma unroll 1
-        for (int k = 0; k < CHT; ++k) {
-          const StagedRec<R> r = sr[sl + k * (64 / CHT)];
-          staged_pair<R, DAMP, B2>(xi - r.x, yi - r.y, zi - r.z, r.mx, r.my, r.mz, di * r.s, fx, fy, fz);
-        }
+        for (int k = 0; k < CHT; ++k) staged_pair<R, DAMP, B2, K>(xi, yi, zi, di, sr[sl + k * (64 / CHT)], f);
       }
     } else if (act) {
-      for (int u = sl; u < cn; u += S) {
-        const StagedRec<R> r = sr[u];
-        staged_pair<R, DAMP, B2>(xi - r.x, yi - r.y, zi - r.z, r.mx, r.my, r.mz, di * r.s, fx, fy, fz);
-      }
+      for (int u = sl; u < cn; u += S) staged_pair<R, DAMP, B2, K>(xi, yi, zi, di, sr[u], f);
     }
     wave_lds_barrier();
   }
-  red[0][wl][lane] = act ? fx : R(0);
-  red[1][wl][lane] = act ? fy : R(0);
-  red[2][wl][lane] = act ? fz : R(0);
+  // slot sums through the (now unused) record LDS of this wavefront: red[c][lane]
+  R* red = reinterpret_cast<R*>(sr);
+#pragma unroll
+  for (int c = 0; c < 3 * K; ++c) red[c * 64 + lane] = act ? f[c] : R(0);
   wave_lds_barrier();
   if (lane >= T) return;
-  double Fx = 0.0, Fy = 0.0, Fz = 0.0;
-  for (int s2 = 0; s2 < S; ++s2) {
-    Fx += (double)red[0][wl][lane + T * s2];
-    Fy += (double)red[1][wl][lane + T * s2];
-    Fz += (double)red[2][wl][lane + T * s2];
+  const double* ai = a.ainv;
+  double a6[6]     = {0, 0, 0, 0, 0, 0};
+  if (MODE == MODE_BAPPLY)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) a6[c] = ai[(size_t)c * n + i];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double F[3] = {0.0, 0.0, 0.0};
+    for (int s2 = 0; s2 < S; ++s2)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) F[c] += (double)red[(3 * k + c) * 64 + lane + T * s2];
+    if constexpr (WITH_L2P) {
+      R lx = R(0), ly = R(0), lz = R(0);
+      O::l2p((R)(pi.x - cax), (R)(pi.y - cay), (R)(pi.z - caz), &L[k * lstride + (size_t)A * O::LS], lx, ly, lz);
+      F[0] += (double)lx;
+      F[1] += (double)ly;
+      F[2] += (double)lz;
+    } else {
+      (void)L;
+      (void)lstride;
+    }
+    const size_t o = 3 * (size_t)n * k;
+    if (MODE == MODE_BAPPLY) {
+      const double m0 = a.src[o + i], m1 = a.src[o + n + i], m2 = a.src[o + 2 * (size_t)n + i];
+      F[0] += a6[0] * m0 + a6[1] * m1 + a6[2] * m2;
+      F[1] += a6[1] * m0 + a6[3] * m1 + a6[4] * m2;
+      F[2] += a6[2] * m0 + a6[4] * m1 + a6[5] * m2;
+    }
+    a.out[o + i]                 = F[0];
+    a.out[o + n + i]             = F[1];
+    a.out[o + 2 * (size_t)n + i] = F[2];
   }
-  if constexpr (WITH_L2P) {
-    R lx = R(0), ly = R(0), lz = R(0);
-    O::l2p((R)(pi.x - cax), (R)(pi.y - cay), (R)(pi.z - caz), &L[(size_t)A * O::LS], lx, ly, lz);
-    Fx += (double)lx;
-    Fy += (double)ly;
-    Fz += (double)lz;
-  } else {
-    (void)L;
-  }
-  const int n = a.n;
-  if (MODE == MODE_BAPPLY) {
-    const double m0 = a.src[i], m1 = a.src[n + i], m2 = a.src[2 * (size_t)n + i];
-    const double* ai = a.ainv;
-    const double a0 = ai[i], a1 = ai[n + i], a2 = ai[2 * (size_t)n + i], a3 = ai[3 * (size_t)n + i],
-                 a4 = ai[4 * (size_t)n + i], a5 = ai[5 * (size_t)n + i];
-    Fx += a0 * m0 + a1 * m1 + a2 * m2;
-    Fy += a1 * m0 + a3 * m1 + a4 * m2;
-    Fz += a2 * m0 + a4 * m1 + a5 * m2;
-  }
-  a.out[i]                 = Fx;
-  a.out[n + i]             = Fy;
-  a.out[2 * (size_t)n + i] = Fz;
 }
 
 // FP32 source records: offset of each site from its leaf centre, and alpha_iso^(-1/6).
@@ -969,22 +998,29 @@ __global__ void k_pack_mu4(int n, const double* __restrict__ mu, MU4* __restrict
 
 // Takes the excluded near-field pairs out of a near-field sum that included them: for each site
 // i and each distinct exclusion partner e != i whose leaf is in the near list of i's leaf, out[i]
-// -= pair(i, e), evaluated as the staged kernel does (FP32: offsets from i's leaf centre).
-template <typename R, bool DAMP, typename MU4, bool B2 = false>
+// -= pair(i, e), evaluated as the staged kernel does (same record, same arithmetic; FP32: offsets
+// from i's leaf centre). K right-hand sides as in the staged kernel.
+template <typename R, bool DAMP, typename MU4, bool B2 = false, int K = 1>
 __global__ __launch_bounds__(256) void k_fmm_excl_sub(NearArgs a, const MU4* __restrict__ mu4) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const int k0 = a.eoff[i], k1 = a.eoff[i + 1];
   if (k0 == k1) return;
+  const int n      = a.n;
   const int A      = a.leaf_of[i];
   const int o0 = a.p2p_off[A], o1 = a.p2p_off[A + 1];
   const double4 pi = a.pos4[i];
   const double cax = a.cx[A], cay = a.cy[A], caz = a.cz[A];
-  const R di       = DAMP ? staged_di<R, B2>(a.damping, pi.w) : R(0);
-  R fx = R(0), fy = R(0), fz = R(0);
+  const R xi = sizeof(R) == 8 ? (R)pi.x : (R)(pi.x - cax);
+  const R yi = sizeof(R) == 8 ? (R)pi.y : (R)(pi.y - cay);
+  const R zi = sizeof(R) == 8 ? (R)pi.z : (R)(pi.z - caz);
+  const R di = DAMP ? staged_di<R, B2>(a.damping, pi.w) : R(0);
+  R f[3 * K];
+#pragma unroll
+  for (int c = 0; c < 3 * K; ++c) f[c] = R(0);
   for (int k = k0; k < k1; ++k) {
     const int e = a.eidx[k];
-    if (e < 0 || e >= a.n || e == i) continue;
+    if (e < 0 || e >= n || e == i) continue;
     bool dup = false;
     for (int q = k0; q < k; ++q) dup |= (a.eidx[q] == e);
     if (dup) continue;
@@ -999,31 +1035,42 @@ __global__ __launch_bounds__(256) void k_fmm_excl_sub(NearArgs a, const MU4* __r
     }
     if (lo == o1 || a.p2p_src[lo] != B) continue;  // a far-field pair: it stays in
     const double4 pj = a.pos4[e];
-    const MU4 mj     = mu4[e];
-    if constexpr (sizeof(R) == 8) {
-      staged_pair<R, DAMP, false>(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, mj.x, mj.y, mj.z, di * (R)pj.w, fx, fy,
-                                  fz);
-    } else {
-      const R tx = (R)(pi.x - cax), ty = (R)(pi.y - cay), tz = (R)(pi.z - caz);
-      const R sx = (R)(pj.x - cax), sy = (R)(pj.y - cay), sz = (R)(pj.z - caz);
-      staged_pair<R, DAMP, B2>(tx - sx, ty - sy, tz - sz, (R)mj.x, (R)mj.y, (R)mj.z, di * (R)pj.w, fx, fy, fz);
+    StagedRec<R, K> r;
+    r.x = sizeof(R) == 8 ? (R)pj.x : (R)(pj.x - cax);
+    r.y = sizeof(R) == 8 ? (R)pj.y : (R)(pj.y - cay);
+    r.z = sizeof(R) == 8 ? (R)pj.z : (R)(pj.z - caz);
+    r.s = (R)pj.w;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const MU4 mj   = mu4[(size_t)q * n + e];
+      r.m[3 * q]     = (R)mj.x;
+      r.m[3 * q + 1] = (R)mj.y;
+      r.m[3 * q + 2] = (R)mj.z;
     }
+    staged_pair<R, DAMP, (sizeof(R) == 4 && B2), K>(xi, yi, zi, di, r, f);
   }
-  const int n = a.n;
-  a.out[i] -= (double)fx;
-  a.out[n + i] -= (double)fy;
-  a.out[2 * (size_t)n + i] -= (double)fz;
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    const size_t o = 3 * (size_t)n * q;
+    a.out[o + i] -= (double)f[3 * q];
+    a.out[o + n + i] -= (double)f[3 * q + 1];
+    a.out[o + 2 * (size_t)n + i] -= (double)f[3 * q + 2];
+  }
 }
 
 // out += L2P of each site's leaf expansion (the far-field term when it runs beside the near field)
+// (grid.y = right-hand side: L + y lstride, out + 3 y n)
 template <int P, typename R>
 __global__ __launch_bounds__(256) void k_fmm_l2p_add(int n, const int* __restrict__ leaf_of,
                                                      const double4* __restrict__ pos4, const double* __restrict__ cx,
                                                      const double* __restrict__ cy, const double* __restrict__ cz,
-                                                     const R* __restrict__ L, double* __restrict__ out) {
+                                                     const R* __restrict__ L, double* __restrict__ out,
+                                                     size_t lstride = 0) {
   using O     = Ops<KIND_M1, P, R>;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  L += blockIdx.y * lstride;
+  out += 3 * (size_t)n * blockIdx.y;
   const int A      = leaf_of[i];
   const double4 pi = pos4[i];
   R lx = R(0), ly = R(0), lz = R(0);
@@ -1159,8 +1206,8 @@ inline void ensure_posf4(System& S) {
 
 void ensure_clusters(System& S);  // fmm.hip
 
-template <int P, typename R, int MODE, bool DAMP, typename MU4>
-void launch_near(System& S, const NearArgs& a, const MU4* mu4, const R* L, bool with_l2p) {
+template <int P, typename R, int MODE, bool DAMP, typename MU4, int K = 1>
+void launch_near(System& S, const NearArgs& a, const MU4* mu4, const R* L, bool with_l2p, size_t lstride = 0) {
   FmmState& F     = *S.fmm;
   const int prec  = sizeof(R) == 4;
   const int block = F.tune.near_block[prec];
@@ -1178,8 +1225,8 @@ void launch_near(System& S, const NearArgs& a, const MU4* mu4, const R* L, bool 
     const bool b2      = kB2 && S.all_damped;
     auto go = [&](auto l2p, auto cht, auto bb) {
       hipLaunchKernelGGL((k_fmm_near_st<P, R, DAMP, MODE, MU4, decltype(l2p)::value, decltype(cht)::value,
-                                        decltype(bb)::value>),
-                         g, b, 0, st, a, F.near_items[ci].p, ni, chunk, T.xoff.p, T.xsrc.p, mu4, L);
+                                        decltype(bb)::value, K>),
+                         g, b, 0, st, a, F.near_items[ci].p, ni, chunk, T.xoff.p, T.xsrc.p, mu4, L, lstride);
     };
     auto go2 = [&](auto l2p, auto bb) {
       // the unrolled full-chunk path pays for branch-free pairs only (a damped FP64 pair branches
@@ -1206,11 +1253,14 @@ void launch_near(System& S, const NearArgs& a, const MU4* mu4, const R* L, bool 
     if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[7], st));
     if (a.eoff) {
       if (b2)
-        hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, kB2>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a, mu4);
+        hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, kB2, K>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a,
+                           mu4);
       else
-        hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, false>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a,
+        hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, false, K>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a,
                            mu4);
     }
+  } else if constexpr (K > 1) {
+    throw hip_error("several right-hand sides per launch need a staged near-field variant");
   } else if (var == 4) {
     ensure_clusters(S);
     const dim3 g(ceil_div(F.n_clusters, block / 64)), b(block);
@@ -1317,6 +1367,52 @@ void fmm_apply_b_t(System& S, const double* x, double* out, bool fp32) {
   }
 }
 
+// kMultiRhs right-hand sides ([rhs][component][site]) through one near-field launch: the far
+// passes run one right-hand side after another into their own expansions, the staged near-field
+// kernel evaluates each pair's geometry once for all of them. Staged near variants only.
+template <int P, typename R>
+void fmm_apply_b_multi_prec(System& S, const double* x, double* out, R* M, R* L, MU4Of<R>* mu4) {
+  constexpr int K  = kMultiRhs;
+  FmmState& F      = *S.fmm;
+  const int prec   = sizeof(R) == 4;
+  const double dmp = S.opts.damping;
+  const Octree& T  = S.tree;
+  using O          = Ops<KIND_M1, P, R>;
+  const size_t ms = T.ncells * (size_t)O::MS, ls = T.ncells * (size_t)O::LS, n3 = 3 * (size_t)S.n;
+  const bool ovl  = F.tune.overlap[prec] && !S.prof && F.far_stream;
+  hipStream_t fst = ovl ? F.far_stream : S.stream;
+  if (ovl) {
+    CPPE_HIP_CHECK(hipEventRecord(F.ev_fork, S.stream));
+    CPPE_HIP_CHECK(hipStreamWaitEvent(F.far_stream, F.ev_fork, 0));
+  }
+  for (int k = 0; k < K; ++k) fmm_far_pass<KIND_M1, P, R>(S, x + k * n3, M + k * ms, L + k * ls, fst);
+  if (ovl) CPPE_HIP_CHECK(hipEventRecord(F.ev_join, F.far_stream));
+  ensure_pos4(S);
+  NearArgs a{T.leaf_of.p, T.start.p, T.count.p, T.p2p_off.p, T.p2p_src.p, T.cx.p, T.cy.p, T.cz.p, F.pos4.p,
+             S.excl_off.p, S.excl_idx.p, x, S.ainv6.p, dmp, S.n, out};
+  const dim3 g(ceil_div(S.n, 256)), b(256);
+  for (int k = 0; k < K; ++k)
+    hipLaunchKernelGGL(k_pack_mu4<MU4Of<R>>, g, b, 0, S.stream, S.n, x + k * n3, mu4 + (size_t)k * S.n);
+  if (dmp > 0.0)
+    launch_near<P, R, MODE_BAPPLY, true, MU4Of<R>, K>(S, a, mu4, L, !ovl, ls);
+  else
+    launch_near<P, R, MODE_BAPPLY, false, MU4Of<R>, K>(S, a, mu4, L, !ovl, ls);
+  if (ovl) {
+    CPPE_HIP_CHECK(hipStreamWaitEvent(S.stream, F.ev_join, 0));
+    hipLaunchKernelGGL((k_fmm_l2p_add<P, R>), dim3(ceil_div(S.n, 256), K), dim3(256), 0, S.stream, S.n, T.leaf_of.p,
+                       F.pos4.p, T.cx.p, T.cy.p, T.cz.p, L, out, ls);
+  }
+}
+
+template <int P>
+void fmm_apply_b_multi_t(System& S, const double* x, double* out, bool fp32) {
+  FmmState& F = *S.fmm;
+  if (fp32)
+    fmm_apply_b_multi_prec<P, float>(S, x, out, F.M32k.p, F.L32k.p, F.mu4fk.p);
+  else
+    fmm_apply_b_multi_prec<P, double>(S, x, out, F.M64k.p, F.L64k.p, F.mu4k.p);
+}
+
 template <int P>
 void fmm_field1_t(System& S, const double* src, double* out, double damping) {
   FmmState& F     = *S.fmm;
@@ -1342,6 +1438,7 @@ void fmm_static_t(System& S, const double* src10, double* out, double damping) {
 
 #define CPPE_INSTANTIATE_FMM(P)                                                          \
   template void fmm_apply_b_t<P>(System&, const double*, double*, bool);                 \
+  template void fmm_apply_b_multi_t<P>(System&, const double*, double*, bool);           \
   template void fmm_field1_t<P>(System&, const double*, double*, double);                \
   template void fmm_static_t<P>(System&, const double*, double*, double);
 

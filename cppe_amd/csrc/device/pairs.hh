@@ -7,11 +7,12 @@ namespace cppe_dev {
 // ---------------------------------------------------------------------------------------------
 // Pair kernels. d = r_target - r_source.
 // ---------------------------------------------------------------------------------------------
-// -T2 mu with Thole damping: fE mu / R^3 - 3 fT (d.mu) d / R^5  (P2P<1,3>, field_m1.cc:127-193)
+// -T2 mu with Thole damping: fE mu / R^3 - 3 fT (d.mu) d / R^5  (P2P<1,3>, field_m1.cc:127-193),
+// split into the pair's geometry (c3 = fE / R^3, cf = 3 fT / R^5) and the accumulation of one
+// dipole, so that several dipole vectors can share one geometry evaluation.
 // SELF0: a zero displacement (a site with itself) contributes exactly zero.
 template <bool DAMP, bool SELF0 = false>
-__device__ __forceinline__ void pair_m1_f64(double dx, double dy, double dz, double mx, double my,
-                                            double mz, double a, double& fx, double& fy, double& fz) {
+__device__ __forceinline__ void geo_m1_f64(double dx, double dy, double dz, double a, double& c3, double& cf) {
   const double r2   = dx * dx + dy * dy + dz * dz;
   const double rinv = SELF0 ? (r2 > 0.0 ? rsqrt_f64(r2) : 0.0) : rsqrt_f64(r2);
   const double ri2  = rinv * rinv;
@@ -21,20 +22,12 @@ __device__ __forceinline__ void pair_m1_f64(double dx, double dy, double dz, dou
     const double u = a * (r2 * rinv);
     if (u < kTholeCutoff && a > 0.0) thole_ET(u, exp(-u), fE, fT);
   }
-  const double dmu = dx * mx + dy * my + dz * mz;
-  const double c3  = fE * ri3;
-  const double c5  = 3.0 * fT * dmu * ri3 * ri2;
-  fx = __builtin_fma(c3, mx, fx);
-  fx = __builtin_fma(-c5, dx, fx);
-  fy = __builtin_fma(c3, my, fy);
-  fy = __builtin_fma(-c5, dy, fy);
-  fz = __builtin_fma(c3, mz, fz);
-  fz = __builtin_fma(-c5, dz, fz);
+  c3 = fE * ri3;
+  cf = (3.0 * fT) * (ri3 * ri2);
 }
 
 template <bool DAMP, bool SELF0 = false>
-__device__ __forceinline__ void pair_m1_f32(float dx, float dy, float dz, float mx, float my, float mz,
-                                            float a, float& fx, float& fy, float& fz) {
+__device__ __forceinline__ void geo_m1_f32(float dx, float dy, float dz, float a, float& c3, float& cf) {
   const float r2   = dx * dx + dy * dy + dz * dz;
   const float rinv = SELF0 ? (r2 > 0.0f ? rsqrt_f32(r2) : 0.0f) : rsqrt_f32(r2);
   const float ri2  = rinv * rinv;
@@ -44,26 +37,18 @@ __device__ __forceinline__ void pair_m1_f32(float dx, float dy, float dz, float 
     const float u = a * (r2 * rinv);
     if (u < kTholeCutoff32 && a > 0.0f) thole_ET(u, expm_f32(u), fE, fT);
   }
-  const float dmu = dx * mx + dy * my + dz * mz;
-  const float c3  = fE * ri3;
-  const float c5  = 3.0f * fT * dmu * ri3 * ri2;
-  fx = __builtin_fmaf(c3, mx, fx);
-  fx = __builtin_fmaf(-c5, dx, fx);
-  fy = __builtin_fmaf(c3, my, fy);
-  fy = __builtin_fmaf(-c5, dy, fy);
-  fz = __builtin_fmaf(c3, mz, fz);
-  fz = __builtin_fmaf(-c5, dz, fz);
+  c3 = fE * ri3;
+  cf = (3.0f * fT) * (ri3 * ri2);
 }
 
-// FP32 damped pair for systems whose sites are all damped (alpha_iso > 0 everywhere), without
-// branches. a2 = damping (s_i s_j) log2(e), so u2 = a2 R is the Thole argument v in base-2 units
-// (v = u2 ln 2) and e^-v = 2^-u2 comes straight from v_exp_f32. With v = u2 ln2 the factors are
+// FP32 damped pair geometry for systems whose sites are all damped (alpha_iso > 0 everywhere),
+// without branches. a2 = damping (s_i s_j) log2(e), so u2 = a2 R is the Thole argument v in
+// base-2 units (v = u2 ln 2) and e^-v = 2^-u2 comes straight from v_exp_f32. With v = u2 ln2
 //   f_E = 1 - (1 + ln2 u2 + (ln2)^2/2 u2^2) 2^-u2,   f_T = f_E - (ln2)^3/6 u2^3 2^-u2.
 // A zero displacement (the site itself) contributes exactly zero: 1/R is clamped to 2^40, so
 // R = 0, u2 = 0, 2^-u2 = 1 and f_E = f_T = 0 exactly (this needs a2 finite, i.e. damped sites).
 // Large u2 (undamped in effect) gives 2^-u2 = 0 and f_E = f_T = 1.
-__device__ __forceinline__ void pair_m1_f32_b2(float dx, float dy, float dz, float mx, float my, float mz,
-                                               float a2, float& fx, float& fy, float& fz) {
+__device__ __forceinline__ void geo_m1_f32_b2(float dx, float dy, float dz, float a2, float& c3, float& cf) {
   constexpr float kC1 = 0.693147180559945309f;   // ln 2
   constexpr float kC2 = 0.240226506959100712f;   // (ln 2)^2 / 2
   constexpr float kC3 = 0.0555041086648215800f;  // (ln 2)^3 / 6
@@ -78,15 +63,47 @@ __device__ __forceinline__ void pair_m1_f32_b2(float dx, float dy, float dz, flo
   const float fE   = __builtin_fmaf(-e, p, 1.0f);
   const float q    = (u2 * u) * e;
   const float fT   = __builtin_fmaf(-kC3, q, fE);
-  const float dmu  = dx * mx + dy * my + dz * mz;
-  const float c3   = fE * ri3;
-  const float c5   = ((fT * dmu) * ri3) * (3.0f * ri2);
+  c3 = fE * ri3;
+  cf = (fT * ri3) * (3.0f * ri2);  // in this order: at R = 0, ri3 ri2 = 2^200 would overflow
+}
+
+// field += c3 mu - cf (d.mu) d
+__device__ __forceinline__ void acc_m1(double c3, double cf, double dx, double dy, double dz, double mx, double my,
+                                       double mz, double& fx, double& fy, double& fz) {
+  const double c5 = cf * (dx * mx + dy * my + dz * mz);
+  fx = __builtin_fma(c3, mx, fx);
+  fx = __builtin_fma(-c5, dx, fx);
+  fy = __builtin_fma(c3, my, fy);
+  fy = __builtin_fma(-c5, dy, fy);
+  fz = __builtin_fma(c3, mz, fz);
+  fz = __builtin_fma(-c5, dz, fz);
+}
+
+__device__ __forceinline__ void acc_m1(float c3, float cf, float dx, float dy, float dz, float mx, float my,
+                                       float mz, float& fx, float& fy, float& fz) {
+  const float c5 = cf * (dx * mx + dy * my + dz * mz);
   fx = __builtin_fmaf(c3, mx, fx);
   fx = __builtin_fmaf(-c5, dx, fx);
   fy = __builtin_fmaf(c3, my, fy);
   fy = __builtin_fmaf(-c5, dy, fy);
   fz = __builtin_fmaf(c3, mz, fz);
   fz = __builtin_fmaf(-c5, dz, fz);
+}
+
+template <bool DAMP, bool SELF0 = false>
+__device__ __forceinline__ void pair_m1_f64(double dx, double dy, double dz, double mx, double my, double mz, double a,
+                                            double& fx, double& fy, double& fz) {
+  double c3, cf;
+  geo_m1_f64<DAMP, SELF0>(dx, dy, dz, a, c3, cf);
+  acc_m1(c3, cf, dx, dy, dz, mx, my, mz, fx, fy, fz);
+}
+
+template <bool DAMP, bool SELF0 = false>
+__device__ __forceinline__ void pair_m1_f32(float dx, float dy, float dz, float mx, float my, float mz, float a,
+                                            float& fx, float& fy, float& fz) {
+  float c3, cf;
+  geo_m1_f32<DAMP, SELF0>(dx, dy, dz, a, c3, cf);
+  acc_m1(c3, cf, dx, dy, dz, mx, my, mz, fx, fy, fz);
 }
 
 // Static field of charge + dipole + packed quadrupole S (half diagonal), combined as

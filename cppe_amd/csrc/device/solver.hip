@@ -101,8 +101,8 @@ __device__ __forceinline__ void z_of(const double* __restrict__ a6, int n, int i
 }
 
 struct CgVecs {
-  int n, nseg, nchunk;
-  const int* clo;   // nchunk + 1 site offsets of the chunks
+  int na, spr, nseg, nchunk;  // sites per right-hand side, segments per right-hand side
+  const int* clo;   // nchunk + 1 offsets of the chunks (sites of the stacked right-hand sides)
   const int* cseg;  // segment of each chunk
   const double* a6;
   const double* rhs;
@@ -112,24 +112,38 @@ struct CgVecs {
   int* si;          // I_COUNT * nseg
 };
 
+// Vectors are [rhs][component][site]; site i of chunk c (stacked numbering) has component q at
+// base(c) + q * na + i and its polarizability at i - rhs(c) * na.
+struct ChunkIdx {
+  size_t b;
+  int ia;
+  __device__ ChunkIdx(const CgVecs& v, int c) {
+    const int kr = v.cseg[c] / v.spr;
+    b            = 2 * (size_t)kr * v.na;
+    ia           = -kr * v.na;
+  }
+};
+
 // r = rhs - t ; z = alpha r ; p = z ; partials (r.z, r.r)
 __global__ __launch_bounds__(256) void k_sg_init(CgVecs v) {
   __shared__ double red[4];
-  const int c = blockIdx.x, n = v.n;
+  const int c = blockIdx.x, n = v.na;
+  const ChunkIdx ci(v, c);
   double rz = 0.0, rr = 0.0;
   for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x) {
-    const double r0 = v.rhs[i] - v.t[i], r1 = v.rhs[n + i] - v.t[n + i], r2 = v.rhs[2 * n + i] - v.t[2 * n + i];
+    const size_t k0 = ci.b + i, k1 = k0 + n, k2 = k1 + n;
+    const double r0 = v.rhs[k0] - v.t[k0], r1 = v.rhs[k1] - v.t[k1], r2 = v.rhs[k2] - v.t[k2];
     double z0, z1, z2;
-    z_of(v.a6, n, i, r0, r1, r2, z0, z1, z2);
-    v.r[i] = r0;
-    v.r[n + i] = r1;
-    v.r[2 * n + i] = r2;
-    v.z[i] = z0;
-    v.z[n + i] = z1;
-    v.z[2 * n + i] = z2;
-    v.p[i] = z0;
-    v.p[n + i] = z1;
-    v.p[2 * n + i] = z2;
+    z_of(v.a6, n, i + ci.ia, r0, r1, r2, z0, z1, z2);
+    v.r[k0] = r0;
+    v.r[k1] = r1;
+    v.r[k2] = r2;
+    v.z[k0] = z0;
+    v.z[k1] = z1;
+    v.z[k2] = z2;
+    v.p[k0] = z0;
+    v.p[k1] = z1;
+    v.p[k2] = z2;
     rz += r0 * z0 + r1 * z1 + r2 * z2;
     rr += r0 * r0 + r1 * r1 + r2 * r2;
   }
@@ -144,11 +158,14 @@ __global__ __launch_bounds__(256) void k_sg_init(CgVecs v) {
 // partial p.Ap (active segments)
 __global__ __launch_bounds__(256) void k_sg_pap(CgVecs v) {
   __shared__ double red[4];
-  const int c = blockIdx.x, n = v.n;
+  const int c = blockIdx.x, n = v.na;
+  const ChunkIdx ci(v, c);
   double s = 0.0;
   if (v.si[I_ACTIVE * v.nseg + v.cseg[c]])
-    for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x)
-      s += v.p[i] * v.Ap[i] + v.p[n + i] * v.Ap[n + i] + v.p[2 * n + i] * v.Ap[2 * n + i];
+    for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x) {
+      const size_t k0 = ci.b + i, k1 = k0 + n, k2 = k1 + n;
+      s += v.p[k0] * v.Ap[k0] + v.p[k1] * v.Ap[k1] + v.p[k2] * v.Ap[k2];
+    }
   s = block_sum(s, red);
   if (threadIdx.x == 0) v.part[c] = s;
 }
@@ -156,14 +173,15 @@ __global__ __launch_bounds__(256) void k_sg_pap(CgVecs v) {
 // x += alpha p ; r -= alpha Ap ; partial r.r (active segments)
 __global__ __launch_bounds__(256) void k_sg_xr(CgVecs v) {
   __shared__ double red[4];
-  const int c = blockIdx.x, n = v.n, g = v.cseg[c];
+  const int c = blockIdx.x, n = v.na, g = v.cseg[c];
+  const ChunkIdx ci(v, c);
   double s = 0.0;
   if (v.si[I_ACTIVE * v.nseg + g]) {
     const double al = v.sd[D_ALPHA * v.nseg + g];
     for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x)
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const size_t k = (size_t)q * n + i;
+        const size_t k = ci.b + (size_t)q * n + i;
         v.x[k] += al * v.p[k];
         const double rk = v.r[k] - al * v.Ap[k];
         v.r[k] = rk;
@@ -177,13 +195,14 @@ __global__ __launch_bounds__(256) void k_sg_xr(CgVecs v) {
 // r = rhs - t for the segments that asked for a reliable update or await confirmation; partial r.r
 __global__ __launch_bounds__(256) void k_sg_resid(CgVecs v) {
   __shared__ double red[4];
-  const int c = blockIdx.x, n = v.n, g = v.cseg[c];
+  const int c = blockIdx.x, n = v.na, g = v.cseg[c];
+  const ChunkIdx ci(v, c);
   double s = 0.0;
   if (v.si[I_NEEDREL * v.nseg + g] || v.si[I_PENDING * v.nseg + g] || v.si[I_ELIGIBLE * v.nseg + g])
     for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x)
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const size_t k  = (size_t)q * n + i;
+        const size_t k  = ci.b + (size_t)q * n + i;
         const double rk = v.rhs[k] - v.t[k];
         v.r[k] = rk;
         s += rk * rk;
@@ -195,16 +214,18 @@ __global__ __launch_bounds__(256) void k_sg_resid(CgVecs v) {
 // z = alpha r ; partial z.r (active segments)
 __global__ __launch_bounds__(256) void k_sg_z(CgVecs v) {
   __shared__ double red[4];
-  const int c = blockIdx.x, n = v.n;
+  const int c = blockIdx.x, n = v.na;
+  const ChunkIdx ci(v, c);
   double s = 0.0;
   if (v.si[I_ACTIVE * v.nseg + v.cseg[c]])
     for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x) {
-      const double r0 = v.r[i], r1 = v.r[n + i], r2 = v.r[2 * n + i];
+      const size_t k0 = ci.b + i, k1 = k0 + n, k2 = k1 + n;
+      const double r0 = v.r[k0], r1 = v.r[k1], r2 = v.r[k2];
       double z0, z1, z2;
-      z_of(v.a6, n, i, r0, r1, r2, z0, z1, z2);
-      v.z[i] = z0;
-      v.z[n + i] = z1;
-      v.z[2 * n + i] = z2;
+      z_of(v.a6, n, i + ci.ia, r0, r1, r2, z0, z1, z2);
+      v.z[k0] = z0;
+      v.z[k1] = z1;
+      v.z[k2] = z2;
       s += r0 * z0 + r1 * z1 + r2 * z2;
     }
   s = block_sum(s, red);
@@ -213,13 +234,14 @@ __global__ __launch_bounds__(256) void k_sg_z(CgVecs v) {
 
 // p = z + beta p (active segments)
 __global__ __launch_bounds__(256) void k_sg_p(CgVecs v) {
-  const int c = blockIdx.x, n = v.n, g = v.cseg[c];
+  const int c = blockIdx.x, n = v.na, g = v.cseg[c];
   if (!v.si[I_ACTIVE * v.nseg + g]) return;
+  const ChunkIdx ci(v, c);
   const double be = v.sd[D_BETA * v.nseg + g];
   for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x)
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const size_t k = (size_t)q * n + i;
+      const size_t k = ci.b + (size_t)q * n + i;
       v.p[k]         = v.z[k] + be * v.p[k];
     }
 }
@@ -375,13 +397,25 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
 }
 
 // ---- host side -------------------------------------------------------------------------------
-void setup_segments(System& S, const std::vector<int>& seg_off, const std::vector<int>& seg_ncrit) {
-  const int n = S.n, ns = (int)seg_off.size() - 1;
-  S.nseg      = ns;
-  S.h_seg_off = seg_off;
-  std::vector<int> of(n), clo, cseg, schunk(ns + 1, 0);
+void CgState::release_graphs() {
+  for (auto* g : {&g_step[0], &g_step[1], &g_rel, &g_dir[0], &g_dir[1]})
+    if (*g) {
+      (void)hipGraphExecDestroy(*g);
+      *g = nullptr;
+    }
+}
+
+CgState::~CgState() {
+  release_graphs();
+  if (h_status) (void)hipHostFree(h_status);
+}
+
+// Chunks of <= kChunk sites inside each segment (site ranges seg_off, stacked numbering) and the
+// per-segment scalar / status storage.
+static void cg_layout(System& S, CgState& C, const std::vector<int>& seg_off) {
+  const int ns = (int)seg_off.size() - 1;
+  std::vector<int> clo, cseg, schunk(ns + 1, 0);
   for (int g = 0; g < ns; ++g) {
-    for (int i = seg_off[g]; i < seg_off[g + 1]; ++i) of[i] = g;
     schunk[g] = (int)cseg.size();
     for (int i = seg_off[g]; i < seg_off[g + 1]; i += kChunk) {
       clo.push_back(i);
@@ -389,99 +423,159 @@ void setup_segments(System& S, const std::vector<int>& seg_off, const std::vecto
     }
   }
   schunk[ns] = (int)cseg.size();
-  clo.push_back(n);
-  S.nchunk = (int)cseg.size();
+  clo.push_back(seg_off[ns]);
+  C.release_graphs();
+  C.nseg   = ns;
+  C.nchunk = (int)cseg.size();
   auto up  = [&](dbuf<int>& b, const std::vector<int>& h) {
+    b.alloc(h.size());
+    CPPE_HIP_CHECK(hipMemcpyAsync(b.p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, S.stream));
+  };
+  up(C.chunk_lo, clo);
+  up(C.chunk_seg, cseg);
+  up(C.seg_chunk, schunk);
+  C.part.alloc(2 * (size_t)C.nchunk);
+  C.sd.alloc(D_COUNT * (size_t)ns);
+  C.si.alloc(I_COUNT * (size_t)ns);
+  C.status.alloc(ST_COUNT);
+  C.params.alloc(2);
+  C.hist_cap = 0;
+  C.hist.release();
+  if (!C.h_status) CPPE_HIP_CHECK(hipHostMalloc((void**)&C.h_status, sizeof(int) * ST_COUNT));
+  CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
+}
+
+void setup_segments(System& S, const std::vector<int>& seg_off, const std::vector<int>& seg_ncrit) {
+  const int n = S.n, ns = (int)seg_off.size() - 1;
+  S.nseg      = ns;
+  S.h_seg_off = seg_off;
+  std::vector<int> of(n);
+  for (int g = 0; g < ns; ++g)
+    for (int i = seg_off[g]; i < seg_off[g + 1]; ++i) of[i] = g;
+  auto up = [&](dbuf<int>& b, const std::vector<int>& h) {
     b.alloc(h.size());
     CPPE_HIP_CHECK(hipMemcpyAsync(b.p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, S.stream));
   };
   up(S.seg_off, seg_off);
   up(S.seg_ncrit, seg_ncrit);
   up(S.seg_of, of);
-  up(S.chunk_lo, clo);
-  up(S.chunk_seg, cseg);
-  up(S.seg_chunk, schunk);
-  S.cg_part.alloc(2 * (size_t)S.nchunk);
-  S.cg_sd.alloc(D_COUNT * (size_t)ns);
-  S.cg_si.alloc(I_COUNT * (size_t)ns);
-  S.cg_status.alloc(ST_COUNT);
-  S.cg_params.alloc(2);
-  if (!S.h_status) CPPE_HIP_CHECK(hipHostMalloc((void**)&S.h_status, sizeof(int) * ST_COUNT));
-  CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
+  S.cg.na   = n;
+  S.cg.nrhs = 1;
+  S.cg.spr  = ns;
+  cg_layout(S, S.cg, seg_off);
 }
 
-static CgVecs vecs(System& S) {
+void setup_multi_rhs(System& S, int nrhs) {
+  if (S.nseg != 1) throw hip_error("several right-hand sides need a one-snapshot system");
+  CgState& C = S.cgm;
+  if (C.nrhs == nrhs && C.na == S.n && C.x) return;
+  const int n = S.n;
+  std::vector<int> off(nrhs + 1);
+  for (int k = 0; k <= nrhs; ++k) off[k] = k * n;
+  C.na   = n;
+  C.nrhs = nrhs;
+  C.spr  = 1;
+  cg_layout(S, C, off);
+  const size_t nv = 3 * (size_t)n * nrhs;
+  C.store.alloc(7 * nv);
+  double* b = C.store.p;
+  C.x   = b;
+  C.rhs = b + nv;
+  C.r   = b + 2 * nv;
+  C.z   = b + 3 * nv;
+  C.p   = b + 4 * nv;
+  C.Ap  = b + 5 * nv;
+  C.t   = b + 6 * nv;
+}
+
+static void bind_single(System& S) {
+  CgState& C = S.cg;
+  C.x        = S.v_mu.p;
+  C.rhs      = S.v_rhs.p;
+  C.r        = S.v_r.p;
+  C.z        = S.v_z.p;
+  C.p        = S.v_p.p;
+  C.Ap       = S.v_Ap.p;
+  C.t        = S.v_t.p;
+}
+
+static CgVecs vecs(System& S, CgState& C) {
   CgVecs v;
-  v.n = S.n;
-  v.nseg = S.nseg;
-  v.nchunk = S.nchunk;
-  v.clo = S.chunk_lo.p;
-  v.cseg = S.chunk_seg.p;
-  v.a6 = S.alpha6.p;
-  v.rhs = S.v_rhs.p;
-  v.x = S.v_mu.p;
-  v.r = S.v_r.p;
-  v.z = S.v_z.p;
-  v.p = S.v_p.p;
-  v.Ap = S.v_Ap.p;
-  v.t = S.v_t.p;
-  v.part = S.cg_part.p;
-  v.sd = S.cg_sd.p;
-  v.si = S.cg_si.p;
+  v.na     = C.na;
+  v.spr    = C.spr;
+  v.nseg   = C.nseg;
+  v.nchunk = C.nchunk;
+  v.clo    = C.chunk_lo.p;
+  v.cseg   = C.chunk_seg.p;
+  v.a6     = S.alpha6.p;
+  v.rhs    = C.rhs;
+  v.x      = C.x;
+  v.r      = C.r;
+  v.z      = C.z;
+  v.p      = C.p;
+  v.Ap     = C.Ap;
+  v.t      = C.t;
+  v.part   = C.part.p;
+  v.sd     = C.sd.p;
+  v.si     = C.si.p;
   return v;
 }
 
-static CgFinal finals(System& S) {
+static CgFinal finals(CgState& C) {
   CgFinal f;
-  f.nseg = S.nseg;
-  f.nchunk = S.nchunk;
-  f.seg_chunk = S.seg_chunk.p;
-  f.part = S.cg_part.p;
-  f.sd = S.cg_sd.p;
-  f.si = S.cg_si.p;
-  f.status = S.cg_status.p;
-  f.params = S.cg_params.p;
-  f.hist = S.cg_hist.p;
-  f.hist_cap = S.hist_cap;
+  f.nseg      = C.nseg;
+  f.nchunk    = C.nchunk;
+  f.seg_chunk = C.seg_chunk.p;
+  f.part      = C.part.p;
+  f.sd        = C.sd.p;
+  f.si        = C.si.p;
+  f.status    = C.status.p;
+  f.params    = C.params.p;
+  f.hist      = C.hist.p;
+  f.hist_cap  = C.hist_cap;
   return f;
 }
 
 template <int MODE>
-static void launch_final(System& S, bool mixed) {
+static void launch_final(System& S, CgState& C, bool mixed) {
   if (mixed)
-    hipLaunchKernelGGL((k_sg_final<MODE, true>), dim3(1), dim3(256), 0, S.stream, finals(S));
+    hipLaunchKernelGGL((k_sg_final<MODE, true>), dim3(1), dim3(256), 0, S.stream, finals(C));
   else
-    hipLaunchKernelGGL((k_sg_final<MODE, false>), dim3(1), dim3(256), 0, S.stream, finals(S));
+    hipLaunchKernelGGL((k_sg_final<MODE, false>), dim3(1), dim3(256), 0, S.stream, finals(C));
 }
 
-static void seq_step(System& S, bool mixed) {
-  const dim3 g(S.nchunk), b(256);
-  apply_b(S, S.v_p.p, S.v_Ap.p, mixed);
-  hipLaunchKernelGGL(k_sg_pap, g, b, 0, S.stream, vecs(S));
-  launch_final<F_ALPHA>(S, mixed);
-  hipLaunchKernelGGL(k_sg_xr, g, b, 0, S.stream, vecs(S));
-  launch_final<F_RNORM>(S, mixed);
+static void apply(System& S, CgState& C, const double* x, double* out, bool fp32) {
+  if (C.nrhs == 1)
+    apply_b(S, x, out, fp32);
+  else
+    apply_b_multi(S, x, out, C.nrhs, fp32);
 }
 
-static void seq_reliable(System& S) {
-  apply_b(S, S.v_mu.p, S.v_t.p, false);
-  hipLaunchKernelGGL(k_sg_resid, dim3(S.nchunk), dim3(256), 0, S.stream, vecs(S));
-  launch_final<F_REL>(S, true);
+static void seq_step(System& S, CgState& C, bool mixed) {
+  const dim3 g(C.nchunk), b(256);
+  apply(S, C, C.p, C.Ap, mixed);
+  hipLaunchKernelGGL(k_sg_pap, g, b, 0, S.stream, vecs(S, C));
+  launch_final<F_ALPHA>(S, C, mixed);
+  hipLaunchKernelGGL(k_sg_xr, g, b, 0, S.stream, vecs(S, C));
+  launch_final<F_RNORM>(S, C, mixed);
 }
 
-static void seq_direction(System& S, bool mixed) {
-  const dim3 g(S.nchunk), b(256);
-  hipLaunchKernelGGL(k_sg_z, g, b, 0, S.stream, vecs(S));
-  launch_final<F_BETA>(S, mixed);
-  hipLaunchKernelGGL(k_sg_p, g, b, 0, S.stream, vecs(S));
+static void seq_reliable(System& S, CgState& C) {
+  apply(S, C, C.x, C.t, false);
+  hipLaunchKernelGGL(k_sg_resid, dim3(C.nchunk), dim3(256), 0, S.stream, vecs(S, C));
+  launch_final<F_REL>(S, C, true);
+}
+
+static void seq_direction(System& S, CgState& C, bool mixed) {
+  const dim3 g(C.nchunk), b(256);
+  hipLaunchKernelGGL(k_sg_z, g, b, 0, S.stream, vecs(S, C));
+  launch_final<F_BETA>(S, C, mixed);
+  hipLaunchKernelGGL(k_sg_p, g, b, 0, S.stream, vecs(S, C));
 }
 
 void cg_graphs_release(System& S) {
-  for (auto* g : {&S.g_step[0], &S.g_step[1], &S.g_rel, &S.g_dir[0], &S.g_dir[1]})
-    if (*g) {
-      (void)hipGraphExecDestroy(*g);
-      *g = nullptr;
-    }
+  S.cg.release_graphs();
+  S.cgm.release_graphs();
 }
 
 static void capture(System& S, hipGraphExec_t& exec, const std::function<void()>& body) {
@@ -513,74 +607,73 @@ static void run_graph(System& S, hipGraphExec_t& exec, const std::function<void(
   CPPE_HIP_CHECK(hipGraphLaunch(exec, S.stream));
 }
 
-static void read_status(System& S) {
-  CPPE_HIP_CHECK(hipMemcpyAsync(S.h_status, S.cg_status.p, sizeof(int) * ST_COUNT, hipMemcpyDeviceToHost, S.stream));
+static void read_status(System& S, CgState& C) {
+  CPPE_HIP_CHECK(hipMemcpyAsync(C.h_status, C.status.p, sizeof(int) * ST_COUNT, hipMemcpyDeviceToHost, S.stream));
   CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
 }
 
-// Solves B x = rhs for S.v_rhs into S.v_mu (which holds the initial guess). Results per segment
-// are left in the device state; norms0 receives the residual norms of segment 0.
-int device_cg(System& S, double thresh, int maxiter, bool mixed, CgResult& res) {
+// Solves B x = rhs for C.rhs into C.x (which holds the initial guess). Results per segment are
+// left in the device state; res.norms holds every segment's residual norms, norms0 segment 0's.
+int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgResult& res) {
   hipStream_t st = S.stream;
-  const int ns   = S.nseg;
-  if (S.hist_cap < maxiter || !S.cg_hist.p) {
-    S.hist_cap = std::max(maxiter, 64);
-    S.cg_hist.alloc((size_t)S.hist_cap * ns);
-    cg_graphs_release(S);  // the graphs hold the history buffer
+  if (&C == &S.cg) bind_single(S);
+  const int ns = C.nseg;
+  if (C.hist_cap < maxiter || !C.hist.p) {
+    C.hist_cap = std::max(maxiter, 64);
+    C.hist.alloc((size_t)C.hist_cap * ns);
+    C.release_graphs();  // the graphs hold the history buffer
   }
   fmm_reserve(S);  // no allocation inside a graph
+  if (C.nrhs > 1) fmm_reserve_multi(S);
   const double params[2] = {thresh, S.reliable_drop};
-  CPPE_HIP_CHECK(hipMemcpyAsync(S.cg_params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
+  CPPE_HIP_CHECK(hipMemcpyAsync(C.params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
   // r0 = rhs - B x0, z0 = alpha r0, p = z0. The mixed form takes r0 from an FP32 apply: it is
   // replaced by an FP64 true residual at the first reliable update, before any convergence test.
-  apply_b(S, S.v_mu.p, S.v_t.p, mixed);
-  hipLaunchKernelGGL(k_sg_init, dim3(S.nchunk), dim3(256), 0, st, vecs(S));
-  launch_final<F_INIT>(S, mixed);
-  read_status(S);
+  apply(S, C, C.x, C.t, mixed);
+  hipLaunchKernelGGL(k_sg_init, dim3(C.nchunk), dim3(256), 0, st, vecs(S, C));
+  launch_final<F_INIT>(S, C, mixed);
+  read_status(S, C);
   int k = 0, nrel = 0;
   const int m = mixed ? 1 : 0;
-  if (S.h_status[ST_REL] > 0) {
-    run_graph(S, S.g_rel, [&] { seq_reliable(S); });
-    read_status(S);
-    ++nrel;
-  }
   // Loop invariant at the top: every active segment has completed step, replacement and
   // direction of its last iteration. Pending segments (mixed form) are confirmed together: inside a
   // replacement that runs anyway, or once no segment is active.
   auto reliable = [&] {
-    run_graph(S, S.g_rel, [&] { seq_reliable(S); });
-    read_status(S);
+    run_graph(S, C.g_rel, [&] { seq_reliable(S, C); });
+    read_status(S, C);
     ++nrel;
   };
+  if (C.h_status[ST_REL] > 0) reliable();
   while (k < maxiter) {
-    if (S.h_status[ST_ACTIVE] == 0) {
-      if (S.h_status[ST_PEND] == 0) break;
+    if (C.h_status[ST_ACTIVE] == 0) {
+      if (C.h_status[ST_PEND] == 0) break;
       reliable();
-      if (S.h_status[ST_ACTIVE] == 0) break;
-      run_graph(S, S.g_dir[m], [&] { seq_direction(S, mixed); });
+      if (C.h_status[ST_ACTIVE] == 0) break;
+      run_graph(S, C.g_dir[m], [&] { seq_direction(S, C, mixed); });
       continue;
     }
-    run_graph(S, S.g_step[m], [&] { seq_step(S, mixed); });
-    read_status(S);
+    run_graph(S, C.g_step[m], [&] { seq_step(S, C, mixed); });
+    read_status(S, C);
     ++k;
-    if (S.h_status[ST_REL] > 0 || (S.h_status[ST_ACTIVE] == 0 && S.h_status[ST_PEND] > 0)) reliable();
-    if (S.h_status[ST_ACTIVE] > 0) run_graph(S, S.g_dir[m], [&] { seq_direction(S, mixed); });
+    if (C.h_status[ST_REL] > 0 || (C.h_status[ST_ACTIVE] == 0 && C.h_status[ST_PEND] > 0)) reliable();
+    if (C.h_status[ST_ACTIVE] > 0) run_graph(S, C.g_dir[m], [&] { seq_direction(S, C, mixed); });
   }
-  if (S.h_status[ST_PEND] > 0) reliable();  // maxiter reached with unconfirmed segments
+  if (C.h_status[ST_PEND] > 0) reliable();  // maxiter reached with unconfirmed segments
   // per-segment results
   std::vector<int> si((size_t)I_COUNT * ns);
   std::vector<double> sd((size_t)D_COUNT * ns);
-  CPPE_HIP_CHECK(hipMemcpyAsync(si.data(), S.cg_si.p, sizeof(int) * si.size(), hipMemcpyDeviceToHost, st));
-  CPPE_HIP_CHECK(hipMemcpyAsync(sd.data(), S.cg_sd.p, sizeof(double) * sd.size(), hipMemcpyDeviceToHost, st));
-  const int nh = std::min(k, S.hist_cap);
+  CPPE_HIP_CHECK(hipMemcpyAsync(si.data(), C.si.p, sizeof(int) * si.size(), hipMemcpyDeviceToHost, st));
+  CPPE_HIP_CHECK(hipMemcpyAsync(sd.data(), C.sd.p, sizeof(double) * sd.size(), hipMemcpyDeviceToHost, st));
+  const int nh = std::min(k, C.hist_cap);
   std::vector<double> hist((size_t)nh * ns);
   if (nh)
-    CPPE_HIP_CHECK(hipMemcpyAsync(hist.data(), S.cg_hist.p, sizeof(double) * hist.size(), hipMemcpyDeviceToHost, st));
+    CPPE_HIP_CHECK(hipMemcpyAsync(hist.data(), C.hist.p, sizeof(double) * hist.size(), hipMemcpyDeviceToHost, st));
   CPPE_HIP_CHECK(hipStreamSynchronize(st));
   res.iters.assign(ns, 0);
   res.rnorm.assign(ns, 0.0);
   res.converged.assign(ns, 0);
   res.norms0.clear();
+  res.norms.assign(ns, {});
   res.all_converged = true;
   for (int g = 0; g < ns; ++g) {
     const bool active = si[I_ACTIVE * ns + g] != 0 || si[I_PENDING * ns + g] != 0;
@@ -588,8 +681,9 @@ int device_cg(System& S, double thresh, int maxiter, bool mixed, CgResult& res) 
     res.rnorm[g]      = sd[D_RNORM * ns + g];
     res.converged[g]  = !active && !si[I_BAD * ns + g] && res.rnorm[g] < thresh;
     res.all_converged = res.all_converged && res.converged[g];
+    for (int it = 0; it < std::min(res.iters[g], nh); ++it) res.norms[g].push_back(hist[(size_t)it * ns + g]);
   }
-  for (int it = 0; it < std::min(res.iters[0], nh); ++it) res.norms0.push_back(hist[(size_t)it * ns]);
+  res.norms0           = res.norms[0];
   res.loop_iters       = k;
   res.reliable_updates = nrel;
   return k;

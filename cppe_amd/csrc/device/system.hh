@@ -63,9 +63,30 @@ struct CgResult {
   std::vector<int> iters, converged;
   std::vector<double> rnorm;
   std::vector<double> norms0;  // residual norm per iteration of segment 0
+  std::vector<std::vector<double>> norms;  // residual norms per iteration, every segment
   int loop_iters = 0;          // iterations of the shared loop (max over segments)
   int reliable_updates = 0;    // FP64 true-residual applies of the mixed form
   bool all_converged = false;
+};
+
+// Work split, scalars, status block and captured graphs of one device CG (solver.hip). Vectors
+// are [rhs][component][site]: nrhs blocks of 3 * na values, component-major; each segment lies
+// inside one right-hand side (segments per right-hand side: spr).
+struct CgState {
+  int nseg = 0, nchunk = 0;
+  int na = 0;    // sites per right-hand side
+  int nrhs = 1;  // right-hand sides
+  int spr = 1;   // segments per right-hand side
+  dbuf<int> chunk_lo, chunk_seg, seg_chunk;
+  dbuf<double> part, sd, params, hist;
+  dbuf<int> si, status;
+  int hist_cap = 0;
+  int* h_status = nullptr;  // pinned host copy of the status block
+  hipGraphExec_t g_step[2] = {nullptr, nullptr}, g_rel = nullptr, g_dir[2] = {nullptr, nullptr};
+  double *x = nullptr, *rhs = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr, *t = nullptr;
+  dbuf<double> store;  // backing of the vectors when this state owns them
+  void release_graphs();
+  ~CgState();
 };
 
 struct System {
@@ -105,17 +126,12 @@ struct System {
 
   // solver work vectors: 3n SoA (component-major) in device order
   dbuf<double> v_mu, v_rhs, v_r, v_z, v_p, v_Ap, v_t, v_d;
-  // CG work split and state (solver.hip)
-  int nchunk = 0;
-  dbuf<int> chunk_lo, chunk_seg, seg_chunk;
-  dbuf<double> cg_part, cg_sd, cg_params, cg_hist;
-  dbuf<int> cg_si, cg_status;
-  int hist_cap = 0;
-  int* h_status = nullptr;  // pinned host copy of the status block
+  // CG state: cg for the system's own solve (one right-hand side, one segment per snapshot),
+  // cgm for solves of several right-hand sides at once (built on first use)
+  CgState cg, cgm;
   bool use_graphs = true;
   double reliable_drop = 1e-5;  // mixed form: residual drop that triggers an FP64 true residual
   CgResult last;            // outcome of the last solve
-  hipGraphExec_t g_step[2] = {nullptr, nullptr}, g_rel = nullptr, g_dir[2] = {nullptr, nullptr};
   dbuf<double> io_a, io_b;  // AoS3 staging in caller order
   dbuf<float> f32_src;      // FP32 staging for mixed-precision applies
   bool mu_valid = false;
@@ -140,9 +156,15 @@ void precondition(System& S, const double* in, double* out);
 void fill_f64(System& S, double* a, size_t n, double v);
 // segment tables and CG state; seg_off has nseg + 1 entries
 void setup_segments(System& S, const std::vector<int>& seg_off, const std::vector<int>& seg_ncrit);
-// CG on S.v_rhs from the guess in S.v_mu; returns the shared loop's iteration count
-int device_cg(System& S, double thresh, int maxiter, bool mixed, CgResult& res);
+// CG on C.rhs from the guess in C.x (S.cg: the system's vectors S.v_rhs / S.v_mu); returns the
+// shared loop's iteration count
+int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgResult& res);
+// S.cgm for nrhs right-hand sides of a one-snapshot system, its vectors allocated
+void setup_multi_rhs(System& S, int nrhs);
 void cg_graphs_release(System& S);
+// out = B x for nrhs right-hand sides ([rhs][component][site] layout), one launch sequence
+void apply_b_multi(System& S, const double* x, double* out, int nrhs, bool fp32);
+void fmm_reserve_multi(System& S);
 
 void fmm_field(System& S, int m, const double* src, double* F, double damping, bool fp32);
 void fmm_static(System& S, const double* q, const double* mu, const double* th, double* F, double damping);

@@ -92,15 +92,15 @@ Matrix CppeState::nuclear_field_gradient() {  // cppe_state.cc:78-81
 }
 
 Matrix CppeState::induced_moments_eef() {  // cppe_state.cc:111-128
+  // the three unit-field solves as one device solve of three right-hand sides
   const size_t n = m_polarizable_sites;
   Matrix ret(3 * n, 3);
-  InducedMoments& ind = solver();
-  for (int a = 0; a < 3; ++a) {
-    Vector Fdn(3 * n, 0.0);
-    for (size_t s = 0; s < n; ++s) Fdn[3 * s + a] = 1.0;
-    Vector mu = ind.compute(Fdn, Vector(3 * n, 0.0), true);
-    for (size_t r = 0; r < 3 * n; ++r) ret(r, a) = mu[r];
-  }
+  std::vector<Vector> F(3, Vector(3 * n, 0.0));
+  for (int a = 0; a < 3; ++a)
+    for (size_t s = 0; s < n; ++s) F[a][3 * s + a] = 1.0;
+  const std::vector<Vector> mu = solver().compute_multi(F, true);
+  for (int a = 0; a < 3; ++a)
+    for (size_t r = 0; r < 3 * n; ++r) ret(r, a) = mu[a][r];
   return ret;
 }
 

@@ -1,6 +1,7 @@
 #include "electric_fields.hh"
 
 #include <iomanip>
+#include <algorithm>
 #include <sstream>
 #include <stdexcept>
 
@@ -108,6 +109,37 @@ Vector InducedMoments::compute(const Vector& rhs, Vector guess, bool make_guess)
   if (rc == CPPE_HIP_ERR_NOCONV) throw std::runtime_error("Failed to converge induced dipole moments.");
   hip_check(rc);
   return mu;
+}
+
+std::vector<Vector> InducedMoments::compute_multi(const std::vector<Vector>& rhs, bool make_guess) {
+  const size_t n3 = 3 * m_polsites.size();
+  const int k     = (int)rhs.size();
+  if (k == 0) return {};
+  std::vector<double> r((size_t)k * n3), mu((size_t)k * n3, 0.0);
+  for (int a = 0; a < k; ++a) {
+    if (rhs[a].size() != n3) throw std::invalid_argument("rhs size does not match 3 * number of polarizable sites");
+    std::copy(rhs[a].begin(), rhs[a].end(), r.begin() + (size_t)a * n3);
+  }
+  const int maxit = std::max(m_options.maxiter, 1);
+  std::vector<int> iters(k, 0);
+  std::vector<double> norms((size_t)k * maxit, 0.0);
+  int rc = cppe_hip_solve_multi(bmatrix().device().get(), k, r.data(), nullptr, make_guess ? 1 : 0,
+                                m_options.induced_thresh, m_options.maxiter, mu.data(), iters.data(), norms.data());
+  for (int a = 0; a < k; ++a) {
+    m_last_iters = iters[a];
+    m_last_norms.assign(norms.begin() + (size_t)a * maxit, norms.begin() + (size_t)a * maxit + iters[a]);
+    for (int it = 0; it < iters[a]; ++it) {
+      std::stringstream ss;
+      ss.precision(12);
+      ss << std::fixed << m_last_norms[it];
+      m_printer(std::to_string(it) + " --- Norm: " + ss.str());
+    }
+  }
+  if (rc == CPPE_HIP_ERR_NOCONV) throw std::runtime_error("Failed to converge induced dipole moments.");
+  hip_check(rc);
+  std::vector<Vector> out(k);
+  for (int a = 0; a < k; ++a) out[a].assign(mu.begin() + (size_t)a * n3, mu.begin() + (size_t)(a + 1) * n3);
+  return out;
 }
 
 }  // namespace libcppe

@@ -47,6 +47,9 @@ class InducedMoments {
   Vector compute(const Vector& total_fields, bool make_guess) {
     return compute(total_fields, Vector(total_fields.size(), 0.0), make_guess);
   }
+  // several right-hand sides solved together (one device CG sharing every B apply); the printer
+  // receives each right-hand side's lines in turn, as consecutive compute calls would print them
+  std::vector<Vector> compute_multi(const std::vector<Vector>& rhs, bool make_guess);
   // iterations and residual norms of the last solve
   int last_iterations() const { return m_last_iters; }
   const std::vector<double>& last_norms() const { return m_last_norms; }

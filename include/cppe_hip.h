@@ -87,6 +87,10 @@ int cppe_hip_multipole_fields(cppe_hip_system* sys, const double* q, const doubl
 
 /* out = B x = field_m1(x) + alpha^-1 x  (BMatrix::apply, bmatrix.cc:9-40) */
 int cppe_hip_apply_b(cppe_hip_system* sys, const double* x, double* out);
+/* out_k = B x_k for nvec vectors (nvec consecutive 3n AoS blocks), applied together as in
+ * cppe_hip_solve_multi: one near-field launch per group of three. One-snapshot systems only.
+ * No counterpart in the reference (several BMatrix::apply calls). */
+int cppe_hip_apply_b_multi(cppe_hip_system* sys, int nvec, const double* x, double* out);
 /* out = alpha x (BMatrix::apply_diagonal_inverse, bmatrix.cc:42-51) */
 int cppe_hip_apply_alpha(cppe_hip_system* sys, const double* x, double* out);
 /* out = alpha^-1 x (BMatrix::apply_diagonal, bmatrix.cc:53-62) */
@@ -113,6 +117,18 @@ int cppe_hip_dense_b_inverse(cppe_hip_system* sys, const int* index, double* out
  */
 int cppe_hip_solve(cppe_hip_system* sys, const double* rhs, const double* guess, int make_guess,
                    double thresh, int maxiter, double* mu, int* iters, double* norms);
+
+/*
+ * Several right-hand sides solved together: nrhs independent CG recurrences (each with its own step
+ * lengths and stopping test, exactly the single-solve recurrence) that share every B apply, so
+ * the near-field pair geometry is evaluated once for all right-hand sides.
+ * CppeState::induced_moments_eef (cppe_state.cc:111-128) solves its three unit fields this way
+ * instead of with three InducedMoments::compute calls. rhs, guess and mu are nrhs consecutive
+ * 3n AoS blocks (caller order); iters receives nrhs counts, norms (optional) nrhs x maxiter
+ * residual norms. One-snapshot systems only.
+ */
+int cppe_hip_solve_multi(cppe_hip_system* sys, int nrhs, const double* rhs, const double* guess, int make_guess,
+                         double thresh, int maxiter, double* mu, int* iters, double* norms);
 
 /*
  * Device-resident variants (SCF-loop residency): rhs/guess are uploaded once and mu stays in

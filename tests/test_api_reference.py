@@ -225,16 +225,27 @@ def test_polarization_energies():
 
 
 @pytest.mark.gpu
-def test_induced_moments_eef():
-    s = cppe.CppeState({"potfile": POT, "induced_thresh": 1e-12}, pna_molecule(), lambda x: None)
+@pytest.mark.parametrize("summation", ["direct", "fmm"])
+def test_induced_moments_eef(summation):
+    lines = []
+    opts = {"potfile": POT, "induced_thresh": 1e-12, "summation_induced_fields": summation}
+    s = cppe.CppeState(opts, pna_molecule(), lines.append)
     eef = s.induced_moments_eef()
-    b = cppe.BMatrix(cppe.get_polarizable_sites(s.potentials), {"potfile": POT})
-    binv = np.linalg.inv(b.to_dense_matrix())
+    b = cppe.BMatrix(cppe.get_polarizable_sites(s.potentials), opts)
     n = s.get_polarizable_site_number()
+    binv = np.linalg.inv(b.to_dense_matrix())
     for a in range(3):
         F = np.zeros(3 * n)
         F[a::3] = 1.0
-        np.testing.assert_allclose(eef[:, a], binv @ F, atol=1e-10)
+        if summation == "direct":
+            np.testing.assert_allclose(eef[:, a], binv @ F, atol=1e-10)
+        else:  # the tree operator's own residual
+            assert np.linalg.norm(F - b.apply(eef[:, a])) < 1e-12
+        # the same dipoles as one InducedMoments solve of that unit field
+        im = cppe.InducedMoments(s.potentials, opts)
+        np.testing.assert_allclose(eef[:, a], im.compute(F, True), rtol=0, atol=1e-13)
+    # three blocks of printer lines, one per unit field, each starting at iteration 0
+    assert sum(1 for x in lines if x.startswith("0 --- Norm:")) == 3
 
 
 @pytest.mark.gpu
