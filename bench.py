@@ -175,6 +175,7 @@ def main():
     ap.add_argument("--no-ml", action="store_true", help="headline form without the learned warm start")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuned ones")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the 2,000-site direct-summation check")
+    ap.add_argument("--no-eef", action="store_true", help="skip the three-unit-field (induced_moments_eef) timing")
     ap.add_argument("--reliable-drop", type=float, default=None,
                     help="mixed form: residual drop that triggers an FP64 true residual (library default 1e-5)")
     ap.add_argument("--apply-damping", type=float, default=2.1304, help="Thole factor of --apply-only runs")
@@ -325,6 +326,26 @@ def main():
     roof64 = roofline(prof64, PEAK_FP64, "fp64")
     roof64["bound"] = "fp64-valu"
 
+    eef = None
+    if args.summation == "fmm" and not args.no_eef:
+        # CppeState::induced_moments_eef: three unit-field solves, one multi-right-hand-side device
+        # solve against three single solves (host buffers in and out in both, FP64 and mixed form)
+        R = np.zeros((3, 3 * w.n))
+        for a in range(3):
+            R[a, a::3] = 1.0
+        eef = {}
+        for name, S in (("fp64", S64), ("mixed", S32)):
+            S.solve_multi(R, thresh=1e-8, maxiter=200)
+            t0 = time.perf_counter()
+            _, itm, _ = S.solve_multi(R, thresh=1e-8, maxiter=200)
+            tm = (time.perf_counter() - t0) * 1e3
+            t0 = time.perf_counter()
+            its = [S.solve(R[a], thresh=1e-8, maxiter=200)[1] for a in range(3)]
+            ts = (time.perf_counter() - t0) * 1e3
+            eef[name] = {"multi_rhs_ms": round(tm, 3), "three_solves_ms": round(ts, 3),
+                         "iterations": [int(v) for v in itm], "three_solves_iterations": its}
+        log("eef", eef)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
@@ -371,6 +392,7 @@ def main():
                       "fp64": {k: round(v, 4) for k, v in prof64.items()}},
         "roofline": roof,
         "roofline_fp64": roof64,
+        "eef_unit_fields": eef,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
