@@ -38,7 +38,7 @@ FLOP_PAIR_DAMPED = 47
 FLOP_M2L_M1_P5 = 950
 # HBM bytes per launch of the staged near-field kernel on the 300k workload, measured with rocprofv3
 # FETCH_SIZE and WRITE_SIZE runs calibrated on a copy kernel (profiles/near_field_300k_hbm_traffic.md)
-MEASURED_TRAFFIC_300K = {"fp32": 724.7e6, "fp64": 1099.6e6}
+MEASURED_TRAFFIC_300K = {"fp32": 638.6e6, "fp64": 938.6e6}
 
 WORKLOADS = {
     "300k": dict(n_waters=100000, seed=1003, anisotropic=True),
