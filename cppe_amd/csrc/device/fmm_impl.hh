@@ -818,10 +818,14 @@ struct alignas(K == 1 ? 4 * sizeof(R) : 16) StagedRec {  // K = 1: two 16-byte (
 // Geometry of one near-field pair of the staged kernel and of the exclusion pass (identical
 // arithmetic, so an excluded pair is taken out exactly as it was added). B2: the branch-free FP32
 // form for systems whose sites are all damped; di then carries the factor log2(e).
-template <typename R, bool DAMP, bool B2>
+// FAR: a source whose damping is below the precision's cutoff for every target of the leaf
+// (never the target itself): the undamped geometry, exactly what the damped forms give there.
+template <typename R, bool DAMP, bool B2, bool FAR = false>
 __device__ __forceinline__ void staged_geo(R dx, R dy, R dz, R a, R& c3, R& cf) {
   if constexpr (sizeof(R) == 8)
-    geo_m1_f64<DAMP, true>(dx, dy, dz, a, c3, cf);
+    geo_m1_f64<DAMP && !FAR, !FAR>(dx, dy, dz, a, c3, cf);
+  else if constexpr (FAR)
+    geo_m1_f32<false, false>(dx, dy, dz, a, c3, cf);
   else if constexpr (B2)
     geo_m1_f32_b2(dx, dy, dz, a, c3, cf);
   else
@@ -829,11 +833,11 @@ __device__ __forceinline__ void staged_geo(R dx, R dy, R dz, R a, R& c3, R& cf) 
 }
 
 // one pair against the K dipoles of a staged record; f holds 3 K accumulators
-template <typename R, bool DAMP, bool B2, int K>
+template <typename R, bool DAMP, bool B2, int K, bool FAR = false>
 __device__ __forceinline__ void staged_pair(R xi, R yi, R zi, R di, const StagedRec<R, K>& r, R* f) {
   const R dx = xi - r.x, dy = yi - r.y, dz = zi - r.z;
   R c3, cf;
-  staged_geo<R, DAMP, B2>(dx, dy, dz, di * r.s, c3, cf);
+  staged_geo<R, DAMP, B2, FAR>(dx, dy, dz, di * r.s, c3, cf);
 #pragma unroll
   for (int k = 0; k < K; ++k)
     acc_m1(c3, cf, dx, dy, dz, r.m[3 * k], r.m[3 * k + 1], r.m[3 * k + 2], f[3 * k], f[3 * k + 1], f[3 * k + 2]);
@@ -862,7 +866,8 @@ template <int P, typename R, bool DAMP, int MODE, typename MU4, bool WITH_L2P = 
 __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __restrict__ items, int nitems,
                                                      int chunk, const long long* __restrict__ xoff,
                                                      const int* __restrict__ xsrc, const MU4* __restrict__ mu4,
-                                                     const R* __restrict__ L, size_t lstride = 0) {
+                                                     const R* __restrict__ L, size_t lstride = 0,
+                                                     const long long* __restrict__ xsplit = nullptr) {
   using O = Ops<KIND_M1, P, R>;
   static_assert(3 * K * 64 * sizeof(R) <= 64 * sizeof(StagedRec<R, K>), "slot sums reuse the record LDS");
   __shared__ StagedRec<R, K> rec[4][64];
@@ -891,6 +896,9 @@ __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __r
 #pragma unroll
   for (int c = 0; c < 3 * K; ++c) f[c] = R(0);
   const long long q1 = xoff[A + 1];
+  // stages from qf on hold far sources only (FP32: about half of the near pairs at 300k sites; FP64,
+  // with its higher cutoff, only a few percent, which would not pay for the extra code path)
+  const long long qf = (DAMP && sizeof(R) == 4 && xsplit) ? xsplit[A] : q1;
   for (long long q0 = xoff[A]; q0 < q1; q0 += 64) {
     const int cn = (int)min<long long>(64, q1 - q0);
     if (lane < cn) {
@@ -913,13 +921,23 @@ __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __r
     wave_lds_barrier();
     // every pair of the near list, the self pair contributing zero; excluded near pairs are
     // taken out afterwards by k_fmm_excl_sub with the same arithmetic
-    if (CHT && T == CHT && cn == 64) {  // uniform: all 64 lanes active, 64 / CHT slots
+    if (DAMP && sizeof(R) == 4 && q0 >= qf) {  // far stage (uniform): undamped pairs, branch-free
+      if (CHT && T == CHT && cn == 64) {
+#pragma unroll
+        for (int k = 0; k < CHT; ++k)
+          staged_pair<R, DAMP, B2, K, true>(xi, yi, zi, di, sr[sl + k * (64 / (CHT ? CHT : 1))], f);
+      } else if (act) {
+        for (int u = sl; u < cn; u += S) staged_pair<R, DAMP, B2, K, true>(xi, yi, zi, di, sr[u], f);
+      }
+    } else if (CHT && T == CHT && cn == 64) {  // uniform: all 64 lanes active, 64 / CHT slots
       if constexpr (B2 || !DAMP) {  // branch-free pair: straight-line code, registers stay low
 #pragma unroll
-        for (int k = 0; k < CHT; ++k) staged_pair<R, DAMP, B2, K>(xi, yi, zi, di, sr[sl + k * (64 / CHT)], f);
+        for (int k = 0; k < CHT; ++k)
+          staged_pair<R, DAMP, B2, K>(xi, yi, zi, di, sr[sl + k * (64 / (CHT ? CHT : 1))], f);
       } else {  // a branch per pair: unrolling would hoist the records into registers
 #pragma unroll 1
-        for (int k = 0; k < CHT; ++k) staged_pair<R, DAMP, B2, K>(xi, yi, zi, di, sr[sl + k * (64 / CHT)], f);
+        for (int k = 0; k < CHT; ++k)
+          staged_pair<R, DAMP, B2, K>(xi, yi, zi, di, sr[sl + k * (64 / (CHT ? CHT : 1))], f);
       }
     } else if (act) {
       for (int u = sl; u < cn; u += S) staged_pair<R, DAMP, B2, K>(xi, yi, zi, di, sr[u], f);
@@ -933,10 +951,6 @@ __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __r
   wave_lds_barrier();
   if (lane >= T) return;
   const double* ai = a.ainv;
-  double a6[6]     = {0, 0, 0, 0, 0, 0};
-  if (MODE == MODE_BAPPLY)
-#pragma unroll
-    for (int c = 0; c < 6; ++c) a6[c] = ai[(size_t)c * n + i];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     double F[3] = {0.0, 0.0, 0.0};
@@ -954,7 +968,10 @@ __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __r
       (void)lstride;
     }
     const size_t o = 3 * (size_t)n * k;
-    if (MODE == MODE_BAPPLY) {
+    if (MODE == MODE_BAPPLY) {  // alpha^-1 loaded after the L2P: fewer registers live across it
+      double a6[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) a6[c] = ai[(size_t)c * n + i];
       const double m0 = a.src[o + i], m1 = a.src[o + n + i], m2 = a.src[o + 2 * (size_t)n + i];
       F[0] += a6[0] * m0 + a6[1] * m1 + a6[2] * m2;
       F[1] += a6[1] * m0 + a6[3] * m1 + a6[4] * m2;
@@ -1223,10 +1240,12 @@ void launch_near(System& S, const NearArgs& a, const MU4* mu4, const R* L, bool 
     // the branch-free FP32 pair needs every site damped
     constexpr bool kB2 = sizeof(R) == 4 && DAMP;
     const bool b2      = kB2 && S.all_damped;
+    // far-source stages: valid for damping factors at least the classification's
+    const long long* xs = (DAMP && a.damping >= T.xdamp) ? T.xsplit[prec].p : nullptr;
     auto go = [&](auto l2p, auto cht, auto bb) {
       hipLaunchKernelGGL((k_fmm_near_st<P, R, DAMP, MODE, MU4, decltype(l2p)::value, decltype(cht)::value,
                                         decltype(bb)::value, K>),
-                         g, b, 0, st, a, F.near_items[ci].p, ni, chunk, T.xoff.p, T.xsrc.p, mu4, L, lstride);
+                         g, b, 0, st, a, F.near_items[ci].p, ni, chunk, T.xoff.p, T.xsrc.p, mu4, L, lstride, xs);
     };
     auto go2 = [&](auto l2p, auto bb) {
       // the unrolled full-chunk path pays for branch-free pairs only (a damped FP64 pair branches

@@ -561,15 +561,49 @@ __global__ void k_xcount(int ncells, const int* __restrict__ off, const int* __r
   xcnt[c] = t;
 }
 
+// One wavefront per cell: the bounding box and smallest s of the cell's own sites, then its near
+// sources written class by class (near_class), each class in the order of the near list (stable
+// compaction with a ballot per 64 sources).
 __global__ __launch_bounds__(64) void k_xfill(const int* __restrict__ off, const int* __restrict__ src,
                                               const int* __restrict__ start, const int* __restrict__ count,
-                                              const long long* __restrict__ xoff, int* __restrict__ xsrc) {
-  const int c  = blockIdx.x;
+                                              const double* __restrict__ x, const double* __restrict__ y,
+                                              const double* __restrict__ z, const double* __restrict__ s,
+                                              double damping, const long long* __restrict__ xoff,
+                                              int* __restrict__ xsrc, long long* __restrict__ xs64,
+                                              long long* __restrict__ xs32, double* __restrict__ xbox) {
+  const int c = blockIdx.x, lane = threadIdx.x;
   long long pos = xoff[c];
-  for (int k = off[c]; k < off[c + 1]; ++k) {
-    const int B = src[k], j0 = start[B], nb = count[B];
-    for (int u = threadIdx.x; u < nb; u += 64) xsrc[pos + u] = j0 + u;
-    pos += nb;
+  double b[8] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 0.0};
+  for (int i = start[c] + lane; i < start[c] + count[c]; i += 64) {
+    b[0] = fmin(b[0], x[i]);
+    b[1] = fmin(b[1], y[i]);
+    b[2] = fmin(b[2], z[i]);
+    b[3] = fmax(b[3], x[i]);
+    b[4] = fmax(b[4], y[i]);
+    b[5] = fmax(b[5], z[i]);
+    b[6] = fmin(b[6], s ? s[i] : 0.0);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    for (int q = 0; q < 3; ++q) b[q] = fmin(b[q], __shfl_xor(b[q], o));
+    for (int q = 3; q < 6; ++q) b[q] = fmax(b[q], __shfl_xor(b[q], o));
+    b[6] = fmin(b[6], __shfl_xor(b[6], o));
+  }
+  if (count[c] == 0 || !s || !(damping > 0.0)) b[6] = 0.0;  // no bound: every source in class 0
+  if (lane < 8) xbox[8 * (size_t)c + lane] = b[lane];
+  for (int pass = 0; pass < 3; ++pass) {
+    for (int k = off[c]; k < off[c + 1]; ++k) {
+      const int B = src[k], j0 = start[B], nb = count[B];
+      for (int u0 = 0; u0 < nb; u0 += 64) {
+        const int u  = u0 + lane;
+        const int j  = j0 + u;
+        const int cl = u < nb ? near_class(b, damping, x[j], y[j], z[j], s ? s[j] : 0.0) : -1;
+        const unsigned long long m = __ballot(cl == pass);
+        if (cl == pass) xsrc[pos + __popcll(m & ((1ull << lane) - 1ull))] = j;
+        pos += __popcll(m);
+      }
+    }
+    if (lane == 0 && pass == 0) xs32[c] = pos;
+    if (lane == 0 && pass == 1) xs64[c] = pos;
   }
 }
 
@@ -591,10 +625,15 @@ void build_near_expanded(System& S) {
   CPPE_HIP_CHECK(hipMemcpyAsync(&total, T.xoff.p + nc, sizeof(long long), hipMemcpyDeviceToHost, st));
   CPPE_HIP_CHECK(hipStreamSynchronize(st));
   T.xsrc.alloc(std::max<long long>(total, 1));
-  hipLaunchKernelGGL(k_xfill, dim3(nc), dim3(64), 0, st, T.p2p_off.p, T.p2p_src.p, T.start.p, T.count.p, T.xoff.p,
-                     T.xsrc.p);
+  T.xsplit[0].alloc(nc);
+  T.xsplit[1].alloc(nc);
+  T.xbox.alloc(8 * (size_t)nc);
+  hipLaunchKernelGGL(k_xfill, dim3(nc), dim3(64), 0, st, T.p2p_off.p, T.p2p_src.p, T.start.p, T.count.p, S.x.p,
+                     S.y.p, S.z.p, S.s.p, S.opts.damping, T.xoff.p, T.xsrc.p, T.xsplit[0].p, T.xsplit[1].p,
+                     T.xbox.p);
   CPPE_HIP_CHECK(hipStreamSynchronize(st));
-  T.nx = total;
+  T.nx    = total;
+  T.xdamp = S.opts.damping;
 }
 
 void build_lists(System& S) {
