@@ -36,9 +36,11 @@ PEAK_FP64 = 78.6e12
 PEAK_FP32 = 157.3e12
 FLOP_PAIR_DAMPED = 47
 FLOP_M2L_M1_P5 = 950
-# HBM bytes per launch of the staged near-field kernel on the 300k workload, measured with rocprofv3
-# FETCH_SIZE and WRITE_SIZE runs calibrated on a copy kernel (profiles/near_field_300k_hbm_traffic.md)
-MEASURED_TRAFFIC_300K = {"fp32": 638.6e6, "fp64": 938.6e6}
+# HBM bytes per launch of the staged near-field kernel on the 300k workload in the launch shape
+# (variant 8 = 8-target chunks, block 256) the rocprofv3 FETCH_SIZE and WRITE_SIZE runs used,
+# calibrated on a copy kernel (profiles/near_field_300k_hbm_traffic.md); other shapes: not measured
+MEASURED_TRAFFIC_300K = {"fp32": 912.3e6, "fp64": 1536.6e6}
+MEASURED_TRAFFIC_SHAPE = (8, 256)
 
 WORKLOADS = {
     "300k": dict(n_waters=100000, seed=1003, anisotropic=True),
@@ -315,7 +317,8 @@ def main():
             flop, ms = near_flop, (prof["near_kernel"] if prof["near_kernel"] > 0 else prof["near"])
         ach = flop / (ms * 1e-3)
         traffic = None
-        if args.workload == "300k" and "k_fmm_near_st" in name:
+        shape = (tune[prec]["near_variant"], tune[prec]["near_block"]) if tune else None
+        if args.workload == "300k" and "k_fmm_near_st" in name and shape == MEASURED_TRAFFIC_SHAPE:
             traffic = MEASURED_TRAFFIC_300K["fp32" if prec == "fp32" else "fp64"]
         return {"bound": "valu", "kernel": name, "achieved": round(ach / 1e12, 3),
                 "peak": peak / 1e12, "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": traffic,
