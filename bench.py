@@ -36,11 +36,12 @@ PEAK_FP64 = 78.6e12
 PEAK_FP32 = 157.3e12
 FLOP_PAIR_DAMPED = 47
 FLOP_M2L_M1_P5 = 950
-# HBM bytes per launch of the staged near-field kernel on the 300k workload in the launch shape
-# (variant 8 = 8-target chunks, block 256) the rocprofv3 FETCH_SIZE and WRITE_SIZE runs used,
-# calibrated on a copy kernel (profiles/near_field_300k_hbm_traffic.md); other shapes: not measured
-MEASURED_TRAFFIC_300K = {"fp32": 912.3e6, "fp64": 1536.6e6}
-MEASURED_TRAFFIC_SHAPE = (8, 256)
+# HBM bytes per launch of the staged near-field kernel (fused L2P) on the 300k workload, per
+# precision and launch shape (near variant, block) the autotuner picks there, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE runs calibrated on a copy kernel (profiles/near_field_300k_hbm_traffic.md);
+# other shapes: not measured (null)
+MEASURED_TRAFFIC_300K = {("fp32", 8, 256): 912.3e6, ("fp64", 8, 256): 1536.6e6,
+                         ("fp32", 7, 128): 1292.6e6, ("fp64", 7, 128): 1465.6e6}
 
 WORKLOADS = {
     "300k": dict(n_waters=100000, seed=1003, anisotropic=True),
@@ -317,9 +318,9 @@ def main():
             flop, ms = near_flop, (prof["near_kernel"] if prof["near_kernel"] > 0 else prof["near"])
         ach = flop / (ms * 1e-3)
         traffic = None
-        shape = (tune[prec]["near_variant"], tune[prec]["near_block"]) if tune else None
-        if args.workload == "300k" and "k_fmm_near_st" in name and shape == MEASURED_TRAFFIC_SHAPE:
-            traffic = MEASURED_TRAFFIC_300K["fp32" if prec == "fp32" else "fp64"]
+        if args.workload == "300k" and "k_fmm_near_st" in name and tune:
+            key = ("fp32" if prec == "fp32" else "fp64", tune[prec]["near_variant"], tune[prec]["near_block"])
+            traffic = MEASURED_TRAFFIC_300K.get(key)
         return {"bound": "valu", "kernel": name, "achieved": round(ach / 1e12, 3),
                 "peak": peak / 1e12, "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": traffic,
                 "flop_per_launch": flop, "ms_per_launch": round(ms, 4)}
