@@ -303,8 +303,11 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
         // true residual before any step (a guess that already converges stops at 0 iterations)
         si[I_ACTIVE * ns + g]  = !bad;
         si[I_NEEDREL * ns + g] = !bad && rn < thresh / drop;
-      } else {  // an exact guess needs no iteration
-        si[I_ACTIVE * ns + g]  = !bad && !(rn < thresh);
+      } else {
+        // the reference recurrence takes one step before its first test (electric_fields.cc:213-229),
+        // also from a guess that already meets the threshold; only an exactly zero residual (where
+        // that step would divide 0 by 0) stops at 0 iterations
+        si[I_ACTIVE * ns + g]  = !bad && rn > 0.0;
         si[I_NEEDREL * ns + g] = 0;
       }
     } else if (MODE == F_ALPHA) {
