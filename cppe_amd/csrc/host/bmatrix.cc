@@ -8,7 +8,8 @@ namespace libcppe {
 BMatrix::BMatrix(std::vector<Potential> polsites, const PeOptions& options)
       : m_polsites(std::move(polsites)), m_options(options) {
   double damping = m_options.damp_induced ? m_options.damping_factor_induced : 0.0;  // bmatrix.cc:18-21
-  m_dev          = std::make_unique<DeviceSystem>(m_polsites, m_options, damping, true);
+  // no polarizable sites: B is the empty operator, nothing goes to the device
+  if (!m_polsites.empty()) m_dev = std::make_unique<DeviceSystem>(m_polsites, m_options, damping, true);
 }
 
 static void check_size(const Vector& v, size_t n) {
@@ -18,6 +19,7 @@ static void check_size(const Vector& v, size_t n) {
 Vector BMatrix::apply(const Vector& x) {
   check_size(x, size());
   Vector out(x.size());
+  if (out.empty()) return out;
   hip_check(cppe_hip_apply_b(m_dev->get(), x.data(), out.data()));
   return out;
 }
@@ -25,6 +27,7 @@ Vector BMatrix::apply(const Vector& x) {
 Vector BMatrix::apply_diagonal_inverse(const Vector& x) {
   check_size(x, size());
   Vector out(x.size());
+  if (out.empty()) return out;
   hip_check(cppe_hip_apply_alpha(m_dev->get(), x.data(), out.data()));
   return out;
 }
@@ -32,6 +35,7 @@ Vector BMatrix::apply_diagonal_inverse(const Vector& x) {
 Vector BMatrix::apply_diagonal(const Vector& x) {
   check_size(x, size());
   Vector out(x.size());
+  if (out.empty()) return out;
   hip_check(cppe_hip_apply_alpha_inv(m_dev->get(), x.data(), out.data()));
   return out;
 }
@@ -41,6 +45,7 @@ Matrix BMatrix::to_dense_matrix() {
   std::vector<int> index(n);
   for (size_t i = 0; i < n; ++i) index[i] = m_polsites[i].index;
   Matrix B(3 * n, 3 * n);
+  if (n == 0) return B;
   hip_check(cppe_hip_dense_b(m_dev->get(), index.data(), B.data.data()));
   return B;
 }
@@ -50,6 +55,7 @@ Matrix BMatrix::direct_inverse() {
   std::vector<int> index(n);
   for (size_t i = 0; i < n; ++i) index[i] = m_polsites[i].index;
   Matrix B(3 * n, 3 * n);
+  if (n == 0) return B;
   hip_check(cppe_hip_dense_b_inverse(m_dev->get(), index.data(), B.data.data()));
   return B;
 }

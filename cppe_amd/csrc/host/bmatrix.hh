@@ -1,5 +1,6 @@
 // Classical response operator B = alpha^-1 - T2 over polarizable sites (src/core/bmatrix.hh:10-43).
 #pragma once
+#include <stdexcept>
 #include <memory>
 #include <vector>
 
@@ -18,7 +19,10 @@ class BMatrix {
   Vector apply_diagonal(const Vector& in);            // alpha^-1 x (bmatrix.cc:53-62)
   Matrix to_dense_matrix();                            // bmatrix.cc:64-92
   Matrix direct_inverse();                             // bmatrix.cc:94-97
-  DeviceSystem& device() { return *m_dev; }
+  DeviceSystem& device() {
+    if (!m_dev) throw std::runtime_error("B matrix has no polarizable sites");
+    return *m_dev;
+  }
   size_t size() const { return m_polsites.size(); }
 
  private:

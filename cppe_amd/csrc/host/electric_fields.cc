@@ -91,6 +91,12 @@ Vector InducedMoments::compute(const Vector& rhs, Vector guess, bool make_guess)
   if (rhs.size() != n3) throw std::invalid_argument("rhs size does not match 3 * number of polarizable sites");
   if (!make_guess && guess.size() != n3) throw std::invalid_argument("guess size does not match");
   Vector mu(n3, 0.0);
+  if (n3 == 0) {  // the reference's loop takes one step on empty vectors: residual norm 0
+    m_last_iters = 1;
+    m_last_norms.assign(1, 0.0);
+    m_printer("0 --- Norm: 0.000000000000");
+    return mu;
+  }
   m_last_norms.assign(std::max(m_options.maxiter, 1), 0.0);
   int iters = 0;
   int rc    = cppe_hip_solve(bmatrix().device().get(), rhs.data(), make_guess ? nullptr : guess.data(),
@@ -115,6 +121,11 @@ std::vector<Vector> InducedMoments::compute_multi(const std::vector<Vector>& rhs
   const size_t n3 = 3 * m_polsites.size();
   const int k     = (int)rhs.size();
   if (k == 0) return {};
+  if (n3 == 0) {  // as compute() on empty vectors, once per right-hand side
+    std::vector<Vector> out;
+    for (int a = 0; a < k; ++a) out.push_back(compute(rhs[a], Vector(), true));
+    return out;
+  }
   std::vector<double> r((size_t)k * n3), mu((size_t)k * n3, 0.0);
   for (int a = 0; a < k; ++a) {
     if (rhs[a].size() != n3) throw std::invalid_argument("rhs size does not match 3 * number of polarizable sites");

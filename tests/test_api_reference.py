@@ -332,3 +332,24 @@ def test_nuclear_gradients_no_atoms_and_symmetry():
         blk = G[a].transpose(1, 0, 2)  # (site, p, q)
         np.testing.assert_allclose(blk, blk.transpose(0, 2, 1), atol=1e-14)
         np.testing.assert_allclose(np.trace(blk, axis1=1, axis2=2), 0.0, atol=1e-12)
+
+
+@pytest.mark.gpu
+def test_no_polarizable_sites(tmp_path):
+    """A potential without polarizabilities: InducedMoments works on empty vectors and, as the
+    reference's loop does (electric_fields.cc:213-229), prints one iteration with residual norm 0;
+    B is the empty operator."""
+    text = "@COORDINATES\n2\nAU\nO 0 0 0 1\nH 0 0 1.8 2\n@MULTIPOLES\nORDER 0\n2\n1 -0.4\n2 0.4\n"
+    pot = tmp_path / "nopol.pot"
+    pot.write_text(text)
+    potentials = cppe.PotfileReader(str(pot)).read()
+    assert len(cppe.get_polarizable_sites(potentials)) == 0
+    lines = []
+    im = cppe.InducedMoments(potentials, {"potfile": str(pot)})
+    im.set_print_callback(lines.append)
+    assert im.compute(np.zeros(0), True).size == 0
+    assert lines == ["0 --- Norm: 0.000000000000"]
+    s = cppe.CppeState({"potfile": str(pot)}, pna_molecule(), lambda x: None)
+    assert s.induced_moments_eef().shape == (0, 3)
+    b = cppe.BMatrix(cppe.get_polarizable_sites(potentials), {"potfile": str(pot)})
+    assert b.apply(np.zeros(0)).size == 0 and b.to_dense_matrix().shape == (0, 0)
