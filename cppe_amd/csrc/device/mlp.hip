@@ -328,6 +328,51 @@ void mlp_refresh_bf16(Mlp& M) {
                      M.w1t.p, M.w1.p, M.w2t.p, M.w2.p);
 }
 
+// Forward pass only (the warm-start guess): the arithmetic of k_mlp_tile<false> -- the same MFMA
+// tiles, roundings and layer-3 summation order, so the same outputs -- without the training
+// copies (transposes, pre-activations), so a block needs ~30 KB of LDS instead of ~90 KB and
+// several blocks share a CU; SiLU of layer 2 is taken once per element.
+__global__ __launch_bounds__(256) void k_mlp_fwd(MlpTileArgs a) {
+  __shared__ __attribute__((aligned(16))) unsigned short Xs[TS * LX];
+  __shared__ __attribute__((aligned(16))) unsigned short H1s[TS * LH];
+  __shared__ float S2[TS * kMlpHid];
+  const int tid = threadIdx.x, wave = tid >> 6;
+  const int col = acc_col();
+  for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    const long long s0 = (long long)tile * TS;
+    for (int e = tid; e < TS * kMlpIn; e += 256) {
+      const int s = e / kMlpIn, f = e % kMlpIn;
+      unsigned short v = 0;
+      if (s0 + s < a.ns) v = f2bf((a.Xraw[(s0 + s) * kMlpIn + f] - a.mean[f]) * a.inv_std[f]);
+      Xs[s * LX + f] = v;
+    }
+    __syncthreads();
+    {
+      f32x16 acc = f32x16{};
+      mfma_tile(Xs, LX, a.w1t + (wave * 32) * kMlpIn, kMlpIn, kMlpIn, acc);
+      const int n = wave * 32 + col;
+      const float b = a.w[kOffB1 + n];
+      for (int q = 0; q < 16; ++q) H1s[acc_row(q) * LH + n] = f2bf(silu(acc[q] + b));
+    }
+    __syncthreads();
+    {
+      f32x16 acc = f32x16{};
+      mfma_tile(H1s, LH, a.w2t + (wave * 32) * kMlpHid, kMlpHid, kMlpHid, acc);
+      const int n = wave * 32 + col;
+      const float b = a.w[kOffB2 + n];
+      for (int q = 0; q < 16; ++q) S2[acc_row(q) * kMlpHid + n] = silu(acc[q] + b);
+    }
+    __syncthreads();
+    if (tid < TS * kMlpOut) {
+      const int s = tid / 3, o = tid % 3;
+      float acc = a.w[kOffB3 + o];
+      for (int h = 0; h < kMlpHid; ++h) acc += S2[s * kMlpHid + h] * a.w[kOffW3 + h * kMlpOut + o];
+      if (s0 + s < a.ns) a.y[(s0 + s) * 3 + o] = acc;
+    }
+    __syncthreads();
+  }
+}
+
 void mlp_forward(Mlp& M, const float* X, int ns, float* y, hipStream_t st) {
   MlpTileArgs a{};
   a.w = M.w.p;
@@ -342,7 +387,7 @@ void mlp_forward(Mlp& M, const float* X, int ns, float* y, hipStream_t st) {
   a.ntiles = ceil_div(ns, TS);
   a.y = y;
   const int grid = std::min(a.ntiles, 4096);
-  hipLaunchKernelGGL(k_mlp_tile<false>, dim3(std::max(grid, 1)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_mlp_fwd, dim3(std::max(grid, 1)), dim3(256), 0, st, a);
 }
 
 void mlp_loss_grad(Mlp& M, const unsigned short* Xb, const float* V, const float* T, const float* Wt, long long ns,

@@ -34,7 +34,7 @@ namespace cppe_dev {
 void apply_b_fmm(System& S, const double* xin, double* out, bool fp32);
 void fmm_reserve(System& S);
 
-constexpr int kChunk = 1024;
+constexpr int kChunk = 256;  // one site per thread: enough blocks in flight to stream the vectors
 
 // per-segment doubles (SoA, each nseg long) and ints
 enum SegD { D_RZ = 0, D_ALPHA, D_BETA, D_RNORM, D_RLAST, D_COUNT };
