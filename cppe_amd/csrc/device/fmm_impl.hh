@@ -86,13 +86,14 @@ struct Ops;
 #define CPPE_DEFINE_OPS(P)                                                                                    \
   template <typename R>                                                                                       \
   struct Ops<KIND_M1, P, R> {                                                                                 \
-    static constexpr int MS = fmm_msize(0, P), LS = fmm_lsize(0, P), NS = 3;                                  \
+    static constexpr int MS = fmm_msize(0, P), LS = fmm_lsize(0, P), FS = fmm_fsize(0, P), NS = 3;            \
     static __device__ __forceinline__ void p2m(R x, R y, R z, const R* S, R* M) {                             \
       fmmops::p2m_m1_p##P<R>(x, y, z, S, M);                                                                  \
     }                                                                                                         \
     static __device__ __forceinline__ void m2m(R x, R y, R z, const R* Mc, R* Mp) {                           \
       fmmops::m2m_m1_p##P<R>(x, y, z, Mc, Mp);                                                                \
     }                                                                                                         \
+    static __device__ __forceinline__ void fold(R* M) { fmmops::fold_m1_p##P<R>(M); }                         \
     static __device__ __forceinline__ void m2l(double x, double y, double z, const double* M, double* L) {    \
       fmmops::m2l_m1_p##P##_f64(x, y, z, M, L);                                                               \
     }                                                                                                         \
@@ -108,13 +109,14 @@ struct Ops;
   };                                                                                                          \
   template <typename R>                                                                                       \
   struct Ops<KIND_MC, P, R> {                                                                                 \
-    static constexpr int MS = fmm_msize(1, P), LS = fmm_lsize(1, P), NS = 10;                                 \
+    static constexpr int MS = fmm_msize(1, P), LS = fmm_lsize(1, P), FS = fmm_fsize(1, P), NS = 10;           \
     static __device__ __forceinline__ void p2m(R x, R y, R z, const R* S, R* M) {                             \
       fmmops::p2m_mc_p##P<R>(x, y, z, S, M);                                                                  \
     }                                                                                                         \
     static __device__ __forceinline__ void m2m(R x, R y, R z, const R* Mc, R* Mp) {                           \
       fmmops::m2m_mc_p##P<R>(x, y, z, Mc, Mp);                                                                \
     }                                                                                                         \
+    static __device__ __forceinline__ void fold(R* M) { fmmops::fold_mc_p##P<R>(M); }                         \
     static __device__ __forceinline__ void m2l(double x, double y, double z, const double* M, double* L) {    \
       fmmops::m2l_mc_p##P##_f64(x, y, z, M, L);                                                               \
     }                                                                                                         \
@@ -212,6 +214,16 @@ __device__ __forceinline__ R group_sum(R v) {
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
   return v;
+}
+
+// Folds each cell's multipole expansion in place onto its harmonic components (the first FS
+// entries of the row; see fmm_ops.gen.hh): M2L then contracts FS instead of MS coefficients.
+template <int KIND, int P, typename R>
+__global__ __launch_bounds__(64) void k_fmm_fold(int c0, int c1, R* __restrict__ M) {
+  using O     = Ops<KIND, P, R>;
+  const int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= c1) return;
+  O::fold(&M[(size_t)c * O::MS]);
 }
 
 template <int KIND, int P, typename R, int G>
@@ -1112,6 +1124,9 @@ void fmm_far_pass(System& S, const double* src, R* M, R* L, hipStream_t st) {
     hipLaunchKernelGGL((k_fmm_m2m<KIND, P, R>), dim3(ceil_div(c1 - c0, 64)), dim3(64), 0, st, c0, c1,
                        T.nchild.p, T.child_first.p, T.cx.p, T.cy.p, T.cz.p, M);
   }
+  if (T.ncells > lo[1])
+    hipLaunchKernelGGL((k_fmm_fold<KIND, P, R>), dim3(ceil_div(T.ncells - lo[1], 64)), dim3(64), 0, st, lo[1],
+                       T.ncells, M);
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[1], st));
   const int grp = S.fmm ? S.fmm->tune.m2l_group[sizeof(R) == 4] : 64;
   if constexpr (KIND == KIND_M1) {
@@ -1348,6 +1363,9 @@ void fmm_near_pass(System& S, const double* mu, double damping, const R* L, doub
 
 // Far field then near field (fused L2P), or -- when tuned so and the near variant is a staged
 // one -- the far field on a second stream beside the near field, joined before an L2P pass.
+// The near-field kernels are enqueued before the far pass: the launch order carries into a
+// captured graph, and the near field then starts together with the (latency-bound) upward pass
+// instead of behind it.
 template <int P, typename R>
 void fmm_apply_b_prec(System& S, const double* x, double* out, R* M, R* L) {
   FmmState& F      = *S.fmm;
@@ -1361,9 +1379,9 @@ void fmm_apply_b_prec(System& S, const double* x, double* out, R* M, R* L) {
   }
   CPPE_HIP_CHECK(hipEventRecord(F.ev_fork, S.stream));
   CPPE_HIP_CHECK(hipStreamWaitEvent(F.far_stream, F.ev_fork, 0));
+  fmm_near_pass<P, R, MODE_BAPPLY>(S, x, dmp, L, out, false);
   fmm_far_pass<KIND_M1, P, R>(S, x, M, L, F.far_stream);
   CPPE_HIP_CHECK(hipEventRecord(F.ev_join, F.far_stream));
-  fmm_near_pass<P, R, MODE_BAPPLY>(S, x, dmp, L, out, false);
   CPPE_HIP_CHECK(hipStreamWaitEvent(S.stream, F.ev_join, 0));
   const Octree& T = S.tree;
   hipLaunchKernelGGL((k_fmm_l2p_add<P, R>), dim3(ceil_div(S.n, 256)), dim3(256), 0, S.stream, S.n, T.leaf_of.p,
@@ -1400,12 +1418,15 @@ void fmm_apply_b_multi_prec(System& S, const double* x, double* out, R* M, R* L,
   const size_t ms = T.ncells * (size_t)O::MS, ls = T.ncells * (size_t)O::LS, n3 = 3 * (size_t)S.n;
   const bool ovl  = F.tune.overlap[prec] && !S.prof && F.far_stream;
   hipStream_t fst = ovl ? F.far_stream : S.stream;
+  auto far = [&] {
+    for (int k = 0; k < K; ++k) fmm_far_pass<KIND_M1, P, R>(S, x + k * n3, M + k * ms, L + k * ls, fst);
+  };
   if (ovl) {
     CPPE_HIP_CHECK(hipEventRecord(F.ev_fork, S.stream));
     CPPE_HIP_CHECK(hipStreamWaitEvent(F.far_stream, F.ev_fork, 0));
+  } else {
+    far();  // the fused L2P of the near-field kernel reads the local expansions
   }
-  for (int k = 0; k < K; ++k) fmm_far_pass<KIND_M1, P, R>(S, x + k * n3, M + k * ms, L + k * ls, fst);
-  if (ovl) CPPE_HIP_CHECK(hipEventRecord(F.ev_join, F.far_stream));
   ensure_pos4(S);
   NearArgs a{T.leaf_of.p, T.start.p, T.count.p, T.p2p_off.p, T.p2p_src.p, T.cx.p, T.cy.p, T.cz.p, F.pos4.p,
              S.excl_off.p, S.excl_idx.p, x, S.ainv6.p, dmp, S.n, out};
@@ -1417,6 +1438,8 @@ void fmm_apply_b_multi_prec(System& S, const double* x, double* out, R* M, R* L,
   else
     launch_near<P, R, MODE_BAPPLY, false, MU4Of<R>, K>(S, a, mu4, L, !ovl, ls);
   if (ovl) {
+    far();  // enqueued after the near field so that both start together (see fmm_apply_b_prec)
+    CPPE_HIP_CHECK(hipEventRecord(F.ev_join, F.far_stream));
     CPPE_HIP_CHECK(hipStreamWaitEvent(S.stream, F.ev_join, 0));
     hipLaunchKernelGGL((k_fmm_l2p_add<P, R>), dim3(ceil_div(S.n, 256), K), dim3(256), 0, S.stream, S.n, T.leaf_of.p,
                        F.pos4.p, T.cx.p, T.cy.p, T.cz.p, L, out, ls);

@@ -110,6 +110,7 @@ struct CgVecs {
   double* part;     // 2 * nchunk partials
   double* sd;       // D_COUNT * nseg
   int* si;          // I_COUNT * nseg
+  const int* status;  // ST_COUNT ints
 };
 
 // Vectors are [rhs][component][site]; site i of chunk c (stacked numbering) has component q at
@@ -211,9 +212,13 @@ __global__ __launch_bounds__(256) void k_sg_resid(CgVecs v) {
   if (threadIdx.x == 0) v.part[c] = s;
 }
 
+// The direction kernels (k_sg_z, F_BETA, k_sg_p) follow the step in the same launch sequence and
+// do nothing while a segment waits for a reliable update (the host runs them again after it).
+
 // z = alpha r ; partial z.r (active segments)
 __global__ __launch_bounds__(256) void k_sg_z(CgVecs v) {
   __shared__ double red[4];
+  if (v.status[ST_REL] > 0) return;
   const int c = blockIdx.x, n = v.na;
   const ChunkIdx ci(v, c);
   double s = 0.0;
@@ -234,6 +239,7 @@ __global__ __launch_bounds__(256) void k_sg_z(CgVecs v) {
 
 // p = z + beta p (active segments)
 __global__ __launch_bounds__(256) void k_sg_p(CgVecs v) {
+  if (v.status[ST_REL] > 0) return;
   const int c = blockIdx.x, n = v.na, g = v.cseg[c];
   if (!v.si[I_ACTIVE * v.nseg + g]) return;
   const ChunkIdx ci(v, c);
@@ -265,6 +271,7 @@ struct CgFinal {
 template <int MODE, bool MIXED>
 __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
   __shared__ int cnt[4];
+  if (MODE == F_BETA && f.status[ST_REL] > 0) return;
   if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
   __syncthreads();
   const int ns = f.nseg;
@@ -521,6 +528,7 @@ static CgVecs vecs(System& S, CgState& C) {
   v.part   = C.part.p;
   v.sd     = C.sd.p;
   v.si     = C.si.p;
+  v.status = C.status.p;
   return v;
 }
 
@@ -574,6 +582,14 @@ static void seq_direction(System& S, CgState& C, bool mixed) {
   hipLaunchKernelGGL(k_sg_z, g, b, 0, S.stream, vecs(S, C));
   launch_final<F_BETA>(S, C, mixed);
   hipLaunchKernelGGL(k_sg_p, g, b, 0, S.stream, vecs(S, C));
+}
+
+// One iteration: step, then the direction unless a segment asked for a reliable update (the
+// direction kernels test the status block on the device), so that one host round trip per
+// iteration suffices.
+static void seq_step_direction(System& S, CgState& C, bool mixed) {
+  seq_step(S, C, mixed);
+  seq_direction(S, C, mixed);
 }
 
 void cg_graphs_release(System& S) {
@@ -655,11 +671,16 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
       run_graph(S, C.g_dir[m], [&] { seq_direction(S, C, mixed); });
       continue;
     }
-    run_graph(S, C.g_step[m], [&] { seq_step(S, C, mixed); });
+    run_graph(S, C.g_step[m], [&] { seq_step_direction(S, C, mixed); });
     read_status(S, C);
     ++k;
-    if (C.h_status[ST_REL] > 0 || (C.h_status[ST_ACTIVE] == 0 && C.h_status[ST_PEND] > 0)) reliable();
-    if (C.h_status[ST_ACTIVE] > 0) run_graph(S, C.g_dir[m], [&] { seq_direction(S, C, mixed); });
+    // The direction already ran unless a reliable update was asked for (it is then taken after
+    // the update). Confirming pending segments when none is active leaves the direction pass a
+    // no-op on the vectors; segments that resume then take theirs after the confirmation.
+    if (C.h_status[ST_REL] > 0 || (C.h_status[ST_ACTIVE] == 0 && C.h_status[ST_PEND] > 0)) {
+      reliable();
+      if (C.h_status[ST_ACTIVE] > 0) run_graph(S, C.g_dir[m], [&] { seq_direction(S, C, mixed); });
+    }
   }
   if (C.h_status[ST_PEND] > 0) reliable();  // maxiter reached with unconfirmed segments
   // per-segment results
