@@ -78,9 +78,9 @@ void fmm_reserve_multi(System& S) {
   FmmState& F     = state(S);
   const size_t nc = S.tree.ncells;
   const int p     = S.opts.order;
-  ensure_buf(F.M64k, kMultiRhs * nc * fmm_msize(0, p));
+  ensure_buf(F.M64k, kMultiRhs * fmm_mbuf_size(nc, 0, p));
   ensure_buf(F.L64k, kMultiRhs * nc * fmm_lsize(0, p));
-  ensure_buf(F.M32k, kMultiRhs * nc * fmm_msize(0, p));
+  ensure_buf(F.M32k, kMultiRhs * fmm_mbuf_size(nc, 0, p));
   ensure_buf(F.L32k, kMultiRhs * nc * fmm_lsize(0, p));
   ensure_buf(F.mu4k, kMultiRhs * (size_t)S.n);
   ensure_buf(F.mu4fk, kMultiRhs * (size_t)S.n);
@@ -211,9 +211,9 @@ void fmm_reserve(System& S) {
   FmmState& F     = state(S);
   const size_t nc = S.tree.ncells;
   const int p     = S.opts.order;
-  ensure_buf(F.M64, nc * fmm_msize(0, p));
+  ensure_buf(F.M64, fmm_mbuf_size(nc, 0, p));
   ensure_buf(F.L64, nc * fmm_lsize(0, p));
-  ensure_buf(F.M32, nc * fmm_msize(0, p));
+  ensure_buf(F.M32, fmm_mbuf_size(nc, 0, p));
   ensure_buf(F.L32, nc * fmm_lsize(0, p));
   ensure_buf(F.mu4, S.n);
   ensure_buf(F.mu4f, S.n);

@@ -26,6 +26,16 @@
 
 namespace cppe_dev {
 
+// Multipole buffers hold, per right-hand side, the full rows (ncells x msize) followed, at a 16-byte
+// aligned offset, by the folded rows M2L reads (ncells x fstride: fsize padded to 4 values).
+constexpr int fmm_fstride(int kind_mc, int p) { return (fmm_fsize(kind_mc, p) + 3) & ~3; }
+inline size_t fmm_fold_offset(size_t ncells, int kind_mc, int p) {
+  return (ncells * fmm_msize(kind_mc, p) + 3) & ~(size_t)3;
+}
+inline size_t fmm_mbuf_size(size_t ncells, int kind_mc, int p) {
+  return fmm_fold_offset(ncells, kind_mc, p) + ncells * fmm_fstride(kind_mc, p);
+}
+
 enum FmmKind { KIND_M1 = 0, KIND_MC = 1 };
 enum P2PMode { MODE_BAPPLY = 0, MODE_FIELD1 = 1, MODE_STATIC = 2 };
 
@@ -87,6 +97,7 @@ struct Ops;
   template <typename R>                                                                                       \
   struct Ops<KIND_M1, P, R> {                                                                                 \
     static constexpr int MS = fmm_msize(0, P), LS = fmm_lsize(0, P), FS = fmm_fsize(0, P), NS = 3;            \
+    static constexpr int FP = fmm_fstride(0, P);                                                              \
     static __device__ __forceinline__ void p2m(R x, R y, R z, const R* S, R* M) {                             \
       fmmops::p2m_m1_p##P<R>(x, y, z, S, M);                                                                  \
     }                                                                                                         \
@@ -94,6 +105,8 @@ struct Ops;
       fmmops::m2m_m1_p##P<R>(x, y, z, Mc, Mp);                                                                \
     }                                                                                                         \
     static __device__ __forceinline__ void fold(R* M) { fmmops::fold_m1_p##P<R>(M); }                         \
+    static __device__ __forceinline__ void unfold(R* L) { fmmops::unfold_m1_p##P<R>(L); }                     \
+    static constexpr bool lkept(int k) { return fmmops::lkept_m1_p##P(k); }                                  \
     static __device__ __forceinline__ void m2l(double x, double y, double z, const double* M, double* L) {    \
       fmmops::m2l_m1_p##P##_f64(x, y, z, M, L);                                                               \
     }                                                                                                         \
@@ -110,6 +123,7 @@ struct Ops;
   template <typename R>                                                                                       \
   struct Ops<KIND_MC, P, R> {                                                                                 \
     static constexpr int MS = fmm_msize(1, P), LS = fmm_lsize(1, P), FS = fmm_fsize(1, P), NS = 10;           \
+    static constexpr int FP = fmm_fstride(1, P);                                                              \
     static __device__ __forceinline__ void p2m(R x, R y, R z, const R* S, R* M) {                             \
       fmmops::p2m_mc_p##P<R>(x, y, z, S, M);                                                                  \
     }                                                                                                         \
@@ -117,6 +131,8 @@ struct Ops;
       fmmops::m2m_mc_p##P<R>(x, y, z, Mc, Mp);                                                                \
     }                                                                                                         \
     static __device__ __forceinline__ void fold(R* M) { fmmops::fold_mc_p##P<R>(M); }                         \
+    static __device__ __forceinline__ void unfold(R* L) { fmmops::unfold_mc_p##P<R>(L); }                     \
+    static constexpr bool lkept(int k) { return fmmops::lkept_mc_p##P(k); }                                  \
     static __device__ __forceinline__ void m2l(double x, double y, double z, const double* M, double* L) {    \
       fmmops::m2l_mc_p##P##_f64(x, y, z, M, L);                                                               \
     }                                                                                                         \
@@ -216,21 +232,37 @@ __device__ __forceinline__ R group_sum(R v) {
   return v;
 }
 
-// Folds each cell's multipole expansion in place onto its harmonic components (the first FS
-// entries of the row; see fmm_ops.gen.hh): M2L then contracts FS instead of MS coefficients.
+// Folds each cell's multipole expansion onto its harmonic components (see fmm_ops.gen.hh) into
+// the compact rows Mf (stride FP) that M2L reads: FS instead of MS coefficients per source.
 template <int KIND, int P, typename R>
-__global__ __launch_bounds__(64) void k_fmm_fold(int c0, int c1, R* __restrict__ M) {
+__global__ __launch_bounds__(64) void k_fmm_fold(int c0, int c1, const R* __restrict__ M, R* __restrict__ Mf) {
   using O     = Ops<KIND, P, R>;
   const int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= c1) return;
-  O::fold(&M[(size_t)c * O::MS]);
+  R m[O::MS];
+#pragma unroll
+  for (int k = 0; k < O::MS; ++k) m[k] = M[(size_t)c * O::MS + k];
+  O::fold(m);
+#pragma unroll
+  for (int k = 0; k < O::FS; ++k) Mf[(size_t)c * O::FP + k] = m[k];
 }
 
+// M2L produces the local coefficients L_n with n_z < 2 only; the others follow from them (the
+// local expansion is harmonic) and are restored here, per cell, before L2L.
+template <int KIND, int P, typename R>
+__global__ __launch_bounds__(64) void k_fmm_unfold(int ncells, R* __restrict__ L) {
+  using O     = Ops<KIND, P, R>;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  O::unfold(&L[(size_t)c * O::LS]);
+}
+
+// At most four wavefronts per SIMD: more (the register count allows five in FP32) was measured
+// slower, the concurrent source rows then miss the caches more.
 template <int KIND, int P, typename R, int G>
-__global__ __launch_bounds__(64) void k_fmm_m2l(int ncells, const int* __restrict__ off, const int* __restrict__ srcs,
-                                                const double* __restrict__ cx, const double* __restrict__ cy,
-                                                const double* __restrict__ cz, const R* __restrict__ M,
-                                                R* __restrict__ L) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_fmm_m2l(
+    int ncells, const int* __restrict__ off, const int* __restrict__ srcs, const double* __restrict__ cx,
+    const double* __restrict__ cy, const double* __restrict__ cz, const R* __restrict__ Mf, R* __restrict__ L) {
   using O         = Ops<KIND, P, R>;
   const int lane  = threadIdx.x;
   const int gl    = lane % G;
@@ -244,7 +276,8 @@ __global__ __launch_bounds__(64) void k_fmm_m2l(int ncells, const int* __restric
     const double ax = cx[A], ay = cy[A], az = cz[A];
     for (int k = o0 + gl; k < o1; k += G) {
       const int B = srcs[k];
-      O::m2l((R)(ax - cx[B]), (R)(ay - cy[B]), (R)(az - cz[B]), &M[(size_t)B * O::MS], l);
+      const R* mrow = static_cast<const R*>(__builtin_assume_aligned(&Mf[(size_t)B * O::FP], 16));
+      O::m2l((R)(ax - cx[B]), (R)(ay - cy[B]), (R)(az - cz[B]), mrow, l);
     }
   }
   constexpr int NCH = (O::LS + G - 1) / G;
@@ -253,6 +286,7 @@ __global__ __launch_bounds__(64) void k_fmm_m2l(int ncells, const int* __restric
   for (int k = 0; k < NCH; ++k) mine[k] = R(0);
 #pragma unroll
   for (int k = 0; k < O::LS; ++k) {
+    if (!O::lkept(k)) continue;  // stored as 0 here, restored by k_fmm_unfold
     const R v = group_sum<G>(l[k]);
     if (gl == (k % G)) mine[k / G] = v;
   }
@@ -1124,26 +1158,28 @@ void fmm_far_pass(System& S, const double* src, R* M, R* L, hipStream_t st) {
     hipLaunchKernelGGL((k_fmm_m2m<KIND, P, R>), dim3(ceil_div(c1 - c0, 64)), dim3(64), 0, st, c0, c1,
                        T.nchild.p, T.child_first.p, T.cx.p, T.cy.p, T.cz.p, M);
   }
+  R* Mf = M + fmm_fold_offset(T.ncells, KIND, P);
   if (T.ncells > lo[1])
     hipLaunchKernelGGL((k_fmm_fold<KIND, P, R>), dim3(ceil_div(T.ncells - lo[1], 64)), dim3(64), 0, st, lo[1],
-                       T.ncells, M);
+                       T.ncells, M, Mf);
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[1], st));
   const int grp = S.fmm ? S.fmm->tune.m2l_group[sizeof(R) == 4] : 64;
   if constexpr (KIND == KIND_M1) {
     if (grp == 16)
       hipLaunchKernelGGL((k_fmm_m2l<KIND, P, R, 16>), dim3(ceil_div(T.ncells, 4)), dim3(64), 0, st, T.ncells,
-                         T.m2l_off.p, T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, M, L);
+                         T.m2l_off.p, T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, Mf, L);
     else if (grp == 32)
       hipLaunchKernelGGL((k_fmm_m2l<KIND, P, R, 32>), dim3(ceil_div(T.ncells, 2)), dim3(64), 0, st, T.ncells,
-                         T.m2l_off.p, T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, M, L);
+                         T.m2l_off.p, T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, Mf, L);
     else
       hipLaunchKernelGGL((k_fmm_m2l<KIND, P, R, 64>), dim3(T.ncells), dim3(64), 0, st, T.ncells, T.m2l_off.p,
-                         T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, M, L);
+                         T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, Mf, L);
   } else {
     (void)grp;
     hipLaunchKernelGGL((k_fmm_m2l<KIND, P, R, 64>), dim3(T.ncells), dim3(64), 0, st, T.ncells, T.m2l_off.p,
-                       T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, M, L);
+                       T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, Mf, L);
   }
+  hipLaunchKernelGGL((k_fmm_unfold<KIND, P, R>), dim3(ceil_div(T.ncells, 64)), dim3(64), 0, st, T.ncells, L);
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[2], st));
   for (int lev = 1; lev <= T.max_level; ++lev) {
     const int c0 = lo[lev], c1 = lo[lev + 1];
@@ -1394,11 +1430,11 @@ void fmm_apply_b_t(System& S, const double* x, double* out, bool fp32) {
   const size_t nc = S.tree.ncells;
   using O64       = Ops<KIND_M1, P, double>;
   if (fp32) {
-    ensure_buf(F.M32, nc * O64::MS);
+    ensure_buf(F.M32, fmm_mbuf_size(nc, KIND_M1, P));
     ensure_buf(F.L32, nc * O64::LS);
     fmm_apply_b_prec<P, float>(S, x, out, F.M32.p, F.L32.p);
   } else {
-    ensure_buf(F.M64, nc * O64::MS);
+    ensure_buf(F.M64, fmm_mbuf_size(nc, KIND_M1, P));
     ensure_buf(F.L64, nc * O64::LS);
     fmm_apply_b_prec<P, double>(S, x, out, F.M64.p, F.L64.p);
   }
@@ -1415,7 +1451,7 @@ void fmm_apply_b_multi_prec(System& S, const double* x, double* out, R* M, R* L,
   const double dmp = S.opts.damping;
   const Octree& T  = S.tree;
   using O          = Ops<KIND_M1, P, R>;
-  const size_t ms = T.ncells * (size_t)O::MS, ls = T.ncells * (size_t)O::LS, n3 = 3 * (size_t)S.n;
+  const size_t ms = fmm_mbuf_size(T.ncells, KIND_M1, P), ls = T.ncells * (size_t)O::LS, n3 = 3 * (size_t)S.n;
   const bool ovl  = F.tune.overlap[prec] && !S.prof && F.far_stream;
   hipStream_t fst = ovl ? F.far_stream : S.stream;
   auto far = [&] {
@@ -1460,7 +1496,7 @@ void fmm_field1_t(System& S, const double* src, double* out, double damping) {
   FmmState& F     = *S.fmm;
   const size_t nc = S.tree.ncells;
   using O64       = Ops<KIND_M1, P, double>;
-  ensure_buf(F.M64, nc * O64::MS);
+  ensure_buf(F.M64, fmm_mbuf_size(nc, KIND_M1, P));
   ensure_buf(F.L64, nc * O64::LS);
   fmm_far_pass<KIND_M1, P, double>(S, src, F.M64.p, F.L64.p, S.stream);
   fmm_near_pass<P, double, MODE_FIELD1>(S, src, damping, F.L64.p, out);
@@ -1472,7 +1508,7 @@ void fmm_static_t(System& S, const double* src10, double* out, double damping) {
   FmmState& F     = *S.fmm;
   const size_t nc = S.tree.ncells;
   using OC        = Ops<KIND_MC, P, double>;
-  ensure_buf(F.Mc, nc * OC::MS);
+  ensure_buf(F.Mc, fmm_mbuf_size(nc, KIND_MC, P));
   ensure_buf(F.Lc, nc * OC::LS);
   fmm_far_pass<KIND_MC, P, double>(S, src10, F.Mc.p, F.Lc.p, S.stream);
   fmm_p2p_pass<KIND_MC, P, double, MODE_STATIC>(S, src10, damping, F.Lc.p, out);

@@ -11,7 +11,11 @@ for every s. Applied repeatedly to the source index, every D_{n+j} with j_z >= 2
 integer combination of D_{n+j'} with j'_z < 2 and |j'| = |j|, whatever n. So before M2L each cell's
 M is folded onto the 2l+1 components of every order l with j_z < 2:
   fold                  Mf_{j'} = sum_j c(j -> j') M_j,   L_n += sum_{j'} Mf_{j'} D_{n+j'}
-which gives the same L with 322 instead of 406 multiply-adds per dipole M2L at p = 5.
+which gives the same L with 322 instead of 406 multiply-adds per dipole M2L at p = 5. The local
+coefficients are derivatives of a harmonic function as well, L_{n+2e_z} = -L_{n+2e_x} - L_{n+2e_y}
+(exactly, term by term of the truncated sum), so M2L computes only the L_n with n_z < 2 and
+  unfold                L_n = -L_{n-2e_z+2e_x} - L_{n-2e_z+2e_y}   (n_z >= 2, in increasing n_z)
+restores the rest per cell before L2L: 265 multiply-adds per dipole M2L at p = 5.
 with D_k(r) = d^k/dr^k (1/|r|) from the Hermite-Coulomb recursion of McMurchie & Davidson
 (J. Comput. Phys. 26, 218 (1978)) in its point-charge limit:
   R^(j)_000 = (-1)^j (2j-1)!! / |r|^(2j+1),  R^(j)_{t+1,u,v} = t R^(j+1)_{t-1,u,v} + x R^(j+1)_{t,u,v}.
@@ -62,6 +66,7 @@ class Kind:
         self.lidx = {m: i for i, m in enumerate(self.llist)}
         self.flist = [m for m in self.mlist if m[2] < 2]  # folded M components (stored first)
         self.fidx = {m: i for i, m in enumerate(self.flist)}
+        self.lkeep = [n for n in self.llist if n[2] < 2]  # L components computed by M2L
 
 
 def fold_coeffs(j):
@@ -191,12 +196,28 @@ def gen_m2l(K, prec):
     # accumulators stay live
     for j in K.flist:
         terms = []
-        for n in K.llist:
+        for n in K.lkeep:
             s = (n[0] + j[0], n[1] + j[1], n[2] + j[2])
             if sum(s) <= K.p:
                 terms.append(f"L[{K.lidx[n]}] += m * h0_{s[0]}_{s[1]}_{s[2]};")
         if terms:
             lines.append(f"  {{ const R m = M[{K.fidx[j]}]; " + " ".join(terms) + " }")
+    lines.append("}")
+    return "\n".join(lines)
+
+
+def gen_unfold(K):
+    """In place: the L_n with n_z >= 2 from those with smaller n_z (harmonic local expansion)."""
+    lines = [f"template <typename R>\n__device__ __forceinline__ void unfold_{K.name}_p{K.p}(R* __restrict__ L) {{"]
+    for n in sorted((n for n in K.llist if n[2] >= 2), key=lambda n: n[2]):
+        a = K.lidx[(n[0] + 2, n[1], n[2] - 2)]
+        b = K.lidx[(n[0], n[1] + 2, n[2] - 2)]
+        lines.append(f"  L[{K.lidx[n]}] = -(L[{a}] + L[{b}]);")
+    lines.append("}")
+    kept = ", ".join("true" if n[2] < 2 else "false" for n in K.llist)
+    lines.append(f"constexpr bool lkept_{K.name}_p{K.p}(int k) {{")
+    lines.append(f"  constexpr bool t[] = {{{kept}}};")
+    lines.append("  return t[k];")
     lines.append("}")
     return "\n".join(lines)
 
@@ -248,6 +269,7 @@ def main():
             out.append(gen_fold(K))
             out.append(gen_m2l(K, "f64"))
             out.append(gen_m2l(K, "f32"))
+            out.append(gen_unfold(K))
             out.append(gen_l2l(K))
             out.append(gen_l2p(K))
             out.append("#endif")

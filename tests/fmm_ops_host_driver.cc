@@ -13,6 +13,7 @@ inline float rsqrt_f32(float x) { return 1.0f / std::sqrt(x); }
 
 using namespace cppe_dev::fmmops;
 using cppe_dev::fmm_msize;
+using cppe_dev::fmm_lsize;
 
 #define P CPPE_FMMOPS_ONLY_P
 #define CAT2(a, b) a##b
@@ -26,30 +27,37 @@ void gen_m2m(int mc, double x, double y, double z, const double* Mc, double* Mp)
   if (mc) CAT(m2m_mc_p, P)<double>(x, y, z, Mc, Mp); else CAT(m2m_m1_p, P)<double>(x, y, z, Mc, Mp);
 }
 // M2L as the device runs it: the multipole row is folded onto its harmonic components first (on a
-// copy here, in place on the device), then contracted.
+// copy here, in place on the device), contracted into the local coefficients with n_z < 2, and the
+// others restored (per cell on the device, after all M2L pairs).
 void gen_m2l(int mc, double x, double y, double z, const double* M, double* L) {
-  double f[256];
-  const int ms = mc ? fmm_msize(1, P) : fmm_msize(0, P);
+  double f[256], l[256] = {};
+  const int ms = mc ? fmm_msize(1, P) : fmm_msize(0, P), ls = mc ? fmm_lsize(1, P) : fmm_lsize(0, P);
   for (int k = 0; k < ms; ++k) f[k] = M[k];
   if (mc) {
     CAT(fold_mc_p, P)<double>(f);
-    CAT(CAT(m2l_mc_p, P), _f64)(x, y, z, f, L);
+    CAT(CAT(m2l_mc_p, P), _f64)(x, y, z, f, l);
+    CAT(unfold_mc_p, P)<double>(l);
   } else {
     CAT(fold_m1_p, P)<double>(f);
-    CAT(CAT(m2l_m1_p, P), _f64)(x, y, z, f, L);
+    CAT(CAT(m2l_m1_p, P), _f64)(x, y, z, f, l);
+    CAT(unfold_m1_p, P)<double>(l);
   }
+  for (int k = 0; k < ls; ++k) L[k] += l[k];
 }
 void gen_m2l_f32(int mc, float x, float y, float z, const float* M, float* L) {
-  float f[256];
-  const int ms = mc ? fmm_msize(1, P) : fmm_msize(0, P);
+  float f[256], l[256] = {};
+  const int ms = mc ? fmm_msize(1, P) : fmm_msize(0, P), ls = mc ? fmm_lsize(1, P) : fmm_lsize(0, P);
   for (int k = 0; k < ms; ++k) f[k] = M[k];
   if (mc) {
     CAT(fold_mc_p, P)<float>(f);
-    CAT(CAT(m2l_mc_p, P), _f32)(x, y, z, f, L);
+    CAT(CAT(m2l_mc_p, P), _f32)(x, y, z, f, l);
+    CAT(unfold_mc_p, P)<float>(l);
   } else {
     CAT(fold_m1_p, P)<float>(f);
-    CAT(CAT(m2l_m1_p, P), _f32)(x, y, z, f, L);
+    CAT(CAT(m2l_m1_p, P), _f32)(x, y, z, f, l);
+    CAT(unfold_m1_p, P)<float>(l);
   }
+  for (int k = 0; k < ls; ++k) L[k] += l[k];
 }
 void gen_l2l(int mc, double x, double y, double z, const double* Lp, double* Lc) {
   if (mc) CAT(l2l_mc_p, P)<double>(x, y, z, Lp, Lc); else CAT(l2l_m1_p, P)<double>(x, y, z, Lp, Lc);
