@@ -168,53 +168,6 @@ __device__ __forceinline__ void load_source(const double* __restrict__ src, int 
   }
 }
 
-template <int KIND, int P, typename R>
-__global__ __launch_bounds__(64) void k_fmm_p2m(int nleaves, const int* __restrict__ leaf_ids,
-                                                const int* __restrict__ level,
-                                                const int* __restrict__ start, const int* __restrict__ count,
-                                                const double* __restrict__ cx, const double* __restrict__ cy,
-                                                const double* __restrict__ cz, const double* __restrict__ x,
-                                                const double* __restrict__ y, const double* __restrict__ z,
-                                                const double* __restrict__ src, int n, R* __restrict__ M) {
-  using O     = Ops<KIND, P, R>;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nleaves) return;
-  const int c = leaf_ids[t];
-  if (level[c] == 0) return;  // a root's expansion never enters an M2L pair
-  R m[O::MS];
-#pragma unroll
-  for (int k = 0; k < O::MS; ++k) m[k] = R(0);
-  const int i0 = start[c], i1 = i0 + count[c];
-  const double ccx = cx[c], ccy = cy[c], ccz = cz[c];
-  for (int i = i0; i < i1; ++i) {
-    R S[O::NS];
-    load_source<KIND, R>(src, n, i, S);
-    O::p2m((R)(ccx - x[i]), (R)(ccy - y[i]), (R)(ccz - z[i]), S, m);
-  }
-#pragma unroll
-  for (int k = 0; k < O::MS; ++k) M[(size_t)c * O::MS + k] = m[k];
-}
-
-template <int KIND, int P, typename R>
-__global__ __launch_bounds__(64) void k_fmm_m2m(int c0, int c1, const int* __restrict__ nchild,
-                                                const int* __restrict__ child_first, const double* __restrict__ cx,
-                                                const double* __restrict__ cy, const double* __restrict__ cz,
-                                                R* __restrict__ M) {
-  using O     = Ops<KIND, P, R>;
-  const int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= c1) return;
-  const int nc = nchild[c];
-  if (nc == 0) return;
-  R m[O::MS];
-#pragma unroll
-  for (int k = 0; k < O::MS; ++k) m[k] = R(0);
-  const int f = child_first[c];
-  for (int ch = f; ch < f + nc; ++ch)
-    O::m2m((R)(cx[c] - cx[ch]), (R)(cy[c] - cy[ch]), (R)(cz[c] - cz[ch]), &M[(size_t)ch * O::MS], m);
-#pragma unroll
-  for (int k = 0; k < O::MS; ++k) M[(size_t)c * O::MS + k] = m[k];
-}
-
 template <typename R>
 __device__ __forceinline__ R wave_sum(R v) {
 #pragma unroll
@@ -222,14 +175,99 @@ __device__ __forceinline__ R wave_sum(R v) {
   return v;
 }
 
-// G lanes per target cell (G = 16, 32, 64; 64/G cells per wavefront). Lanes of a group take the
-// sources of the target-grouped list in turn; the G partial expansions are summed by xor
-// shuffles inside the group and the owner lane of each coefficient stores it.
+// Sum over the G consecutive lanes of a group (G a power of two <= 64), by xor shuffles.
 template <int G, typename R>
 __device__ __forceinline__ R group_sum(R v) {
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
   return v;
+}
+
+// P2M: G lanes per leaf (64/G leaves per wavefront); lane g expands sites g, g + G, ... of the
+// leaf about its centre, the G partial expansions are summed by xor shuffles inside the group and
+// the owner lane of each coefficient stores it. Roots are skipped (their expansions never enter
+// an M2L pair).
+template <int KIND, int P, typename R, int G>
+__global__ __launch_bounds__(64) void k_fmm_p2m(int nleaves, const int* __restrict__ leaf_ids,
+                                                const int* __restrict__ level,
+                                                const int* __restrict__ start, const int* __restrict__ count,
+                                                const double* __restrict__ cx, const double* __restrict__ cy,
+                                                const double* __restrict__ cz, const double* __restrict__ x,
+                                                const double* __restrict__ y, const double* __restrict__ z,
+                                                const double* __restrict__ src, int n, R* __restrict__ M) {
+  using O          = Ops<KIND, P, R>;
+  const int lane   = threadIdx.x;
+  const int gl     = lane % G;
+  const int t      = blockIdx.x * (64 / G) + lane / G;
+  const int c      = t < nleaves ? leaf_ids[t] : 0;
+  const bool valid = t < nleaves && level[c] != 0;
+  R m[O::MS];
+#pragma unroll
+  for (int k = 0; k < O::MS; ++k) m[k] = R(0);
+  if (valid) {
+    const int i0 = start[c], i1 = i0 + count[c];
+    const double ccx = cx[c], ccy = cy[c], ccz = cz[c];
+    for (int i = i0 + gl; i < i1; i += G) {
+      R S[O::NS];
+      load_source<KIND, R>(src, n, i, S);
+      O::p2m((R)(ccx - x[i]), (R)(ccy - y[i]), (R)(ccz - z[i]), S, m);
+    }
+  }
+  constexpr int NCH = (O::MS + G - 1) / G;
+  R mine[NCH];
+#pragma unroll
+  for (int k = 0; k < NCH; ++k) mine[k] = R(0);
+#pragma unroll
+  for (int k = 0; k < O::MS; ++k) {
+    const R v = group_sum<G>(m[k]);
+    if (gl == (k % G)) mine[k / G] = v;
+  }
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) {
+      const int k = q * G + gl;
+      if (k < O::MS) M[(size_t)c * O::MS + k] = mine[q];
+    }
+  }
+}
+
+// M2M: 8 lanes per parent cell, lane g translates child g (if any); the partial expansions are
+// summed by xor shuffles inside the group. Cells without children (leaves) keep their P2M.
+template <int KIND, int P, typename R>
+__global__ __launch_bounds__(64) void k_fmm_m2m(int c0, int c1, const int* __restrict__ nchild,
+                                                const int* __restrict__ child_first, const double* __restrict__ cx,
+                                                const double* __restrict__ cy, const double* __restrict__ cz,
+                                                R* __restrict__ M) {
+  using O          = Ops<KIND, P, R>;
+  constexpr int G  = 8;
+  const int lane   = threadIdx.x;
+  const int gl     = lane % G;
+  const int c      = c0 + blockIdx.x * (64 / G) + lane / G;
+  const int nc     = c < c1 ? nchild[c] : 0;
+  const bool valid = nc > 0;
+  R m[O::MS];
+#pragma unroll
+  for (int k = 0; k < O::MS; ++k) m[k] = R(0);
+  if (gl < nc) {
+    const int ch = child_first[c] + gl;
+    O::m2m((R)(cx[c] - cx[ch]), (R)(cy[c] - cy[ch]), (R)(cz[c] - cz[ch]), &M[(size_t)ch * O::MS], m);
+  }
+  constexpr int NCH = (O::MS + G - 1) / G;
+  R mine[NCH];
+#pragma unroll
+  for (int k = 0; k < NCH; ++k) mine[k] = R(0);
+#pragma unroll
+  for (int k = 0; k < O::MS; ++k) {
+    const R v = group_sum<G>(m[k]);
+    if (gl == (k % G)) mine[k / G] = v;
+  }
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) {
+      const int k = q * G + gl;
+      if (k < O::MS) M[(size_t)c * O::MS + k] = mine[q];
+    }
+  }
 }
 
 // Folds each cell's multipole expansion onto its harmonic components (see fmm_ops.gen.hh) into
@@ -257,8 +295,11 @@ __global__ __launch_bounds__(64) void k_fmm_unfold(int ncells, R* __restrict__ L
   O::unfold(&L[(size_t)c * O::LS]);
 }
 
-// At most four wavefronts per SIMD: more (the register count allows five in FP32) was measured
-// slower, the concurrent source rows then miss the caches more.
+// M2L: G lanes per target cell (G = 16, 32, 64; 64/G cells per wavefront). Lanes of a group take
+// the sources of the target-grouped list in turn; the G partial expansions are summed by xor
+// shuffles inside the group and the owner lane of each coefficient stores it. At most four
+// wavefronts per SIMD: more (the register count allows five in FP32) was measured slower, the
+// concurrent source rows then miss the caches more.
 template <int KIND, int P, typename R, int G>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_fmm_m2l(
     int ncells, const int* __restrict__ off, const int* __restrict__ srcs, const double* __restrict__ cx,
@@ -1150,12 +1191,13 @@ void fmm_far_pass(System& S, const double* src, R* M, R* L, hipStream_t st) {
   const Octree& T = S.tree;
   const auto& lo = T.level_offset;
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[0], st));
-  hipLaunchKernelGGL((k_fmm_p2m<KIND, P, R>), dim3(ceil_div(T.nleaves, 64)), dim3(64), 0, st, T.nleaves,
+  constexpr int GP = 16;  // lanes per leaf in P2M
+  hipLaunchKernelGGL((k_fmm_p2m<KIND, P, R, GP>), dim3(ceil_div(T.nleaves, 64 / GP)), dim3(64), 0, st, T.nleaves,
                      T.leaf_ids.p, T.level.p, T.start.p, T.count.p, T.cx.p, T.cy.p, T.cz.p, S.x.p, S.y.p, S.z.p, src, S.n, M);
   // roots (level 0) are skipped: their expansions never enter an M2L pair
   for (int lev = T.max_level - 1; lev >= 1; --lev) {
     const int c0 = lo[lev], c1 = lo[lev + 1];
-    hipLaunchKernelGGL((k_fmm_m2m<KIND, P, R>), dim3(ceil_div(c1 - c0, 64)), dim3(64), 0, st, c0, c1,
+    hipLaunchKernelGGL((k_fmm_m2m<KIND, P, R>), dim3(ceil_div(c1 - c0, 8)), dim3(64), 0, st, c0, c1,
                        T.nchild.p, T.child_first.p, T.cx.p, T.cy.p, T.cz.p, M);
   }
   R* Mf = M + fmm_fold_offset(T.ncells, KIND, P);
