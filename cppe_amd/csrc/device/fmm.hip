@@ -74,6 +74,14 @@ void apply_b_multi(System& S, const double* x, double* out, int nrhs, bool fp32)
   for (; k < nrhs; ++k) apply_b_fmm(S, x + k * n3, out + k * n3, fp32);
 }
 
+void fmm_dipole_records(System& S, float4** rec32, double4** rec64) {
+  FmmState& F = state(S);
+  ensure_buf(F.mu4, S.n);
+  ensure_buf(F.mu4f, S.n);
+  *rec32 = F.mu4f.p;
+  *rec64 = F.mu4.p;
+}
+
 void fmm_reserve_multi(System& S) {
   FmmState& F     = state(S);
   const size_t nc = S.tree.ncells;

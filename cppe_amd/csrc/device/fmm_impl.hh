@@ -1421,7 +1421,7 @@ void fmm_near_pass(System& S, const double* mu, double damping, const R* L, doub
   const dim3 g(ceil_div(S.n, 256)), b(256);
   if constexpr (sizeof(R) == 8) {
     ensure_buf(F.mu4, S.n);
-    hipLaunchKernelGGL(k_pack_mu4<double4>, g, b, 0, st, S.n, mu, F.mu4.p);
+    if (mu != S.prepacked) hipLaunchKernelGGL(k_pack_mu4<double4>, g, b, 0, st, S.n, mu, F.mu4.p);
     if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[3], st));
     if (damping > 0.0)
       launch_near<P, R, MODE, true>(S, a, F.mu4.p, L, with_l2p);
@@ -1429,7 +1429,7 @@ void fmm_near_pass(System& S, const double* mu, double damping, const R* L, doub
       launch_near<P, R, MODE, false>(S, a, F.mu4.p, L, with_l2p);
   } else {
     ensure_buf(F.mu4f, S.n);
-    hipLaunchKernelGGL(k_pack_mu4<float4>, g, b, 0, st, S.n, mu, F.mu4f.p);
+    if (mu != S.prepacked) hipLaunchKernelGGL(k_pack_mu4<float4>, g, b, 0, st, S.n, mu, F.mu4f.p);
     if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[3], st));
     if (damping > 0.0)
       launch_near<P, R, MODE, true>(S, a, F.mu4f.p, L, with_l2p);

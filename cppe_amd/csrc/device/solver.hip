@@ -111,7 +111,15 @@ struct CgVecs {
   double* sd;       // D_COUNT * nseg
   int* si;          // I_COUNT * nseg
   const int* status;  // ST_COUNT ints
+  float4* rec32;      // when set: near-field dipole records of p for the next FP32 apply
+  double4* rec64;     // when set: the same for the next FP64 apply
 };
+
+// p of site i (components p0, p1, p2) into the near-field dipole record the next apply reads
+__device__ __forceinline__ void put_record(const CgVecs& v, int i, double p0, double p1, double p2) {
+  if (v.rec32) v.rec32[i] = make_float4((float)p0, (float)p1, (float)p2, 0.0f);
+  if (v.rec64) v.rec64[i] = make_double4(p0, p1, p2, 0.0);
+}
 
 // Vectors are [rhs][component][site]; site i of chunk c (stacked numbering) has component q at
 // base(c) + q * na + i and its polarizability at i - rhs(c) * na.
@@ -145,6 +153,7 @@ __global__ __launch_bounds__(256) void k_sg_init(CgVecs v) {
     v.p[k0] = z0;
     v.p[k1] = z1;
     v.p[k2] = z2;
+    put_record(v, i + ci.ia, z0, z1, z2);
     rz += r0 * z0 + r1 * z1 + r2 * z2;
     rr += r0 * r0 + r1 * r1 + r2 * r2;
   }
@@ -244,12 +253,16 @@ __global__ __launch_bounds__(256) void k_sg_p(CgVecs v) {
   if (!v.si[I_ACTIVE * v.nseg + g]) return;
   const ChunkIdx ci(v, c);
   const double be = v.sd[D_BETA * v.nseg + g];
-  for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x)
+  for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x) {
+    double pq[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const size_t k = ci.b + (size_t)q * n + i;
-      v.p[k]         = v.z[k] + be * v.p[k];
+      pq[q]          = v.z[k] + be * v.p[k];
+      v.p[k]         = pq[q];
     }
+    put_record(v, i + ci.ia, pq[0], pq[1], pq[2]);
+  }
 }
 
 struct CgFinal {
@@ -529,6 +542,8 @@ static CgVecs vecs(System& S, CgState& C) {
   v.sd     = C.sd.p;
   v.si     = C.si.p;
   v.status = C.status.p;
+  v.rec32  = C.rec32;
+  v.rec64  = C.rec64;
   return v;
 }
 
@@ -564,7 +579,9 @@ static void apply(System& S, CgState& C, const double* x, double* out, bool fp32
 
 static void seq_step(System& S, CgState& C, bool mixed) {
   const dim3 g(C.nchunk), b(256);
+  if (C.rec32 || C.rec64) S.prepacked = C.p;  // records written by k_sg_init / k_sg_p
   apply(S, C, C.p, C.Ap, mixed);
+  S.prepacked = nullptr;
   hipLaunchKernelGGL(k_sg_pap, g, b, 0, S.stream, vecs(S, C));
   launch_final<F_ALPHA>(S, C, mixed);
   hipLaunchKernelGGL(k_sg_xr, g, b, 0, S.stream, vecs(S, C));
@@ -644,6 +661,19 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
   }
   fmm_reserve(S);  // no allocation inside a graph
   if (C.nrhs > 1) fmm_reserve_multi(S);
+  // single right-hand side: the direction kernels also write p's near-field dipole records in
+  // the precision of the step's apply, which then skips its own packing
+  C.rec32 = nullptr;
+  C.rec64 = nullptr;
+  if (C.nrhs == 1) {
+    float4* r32;
+    double4* r64;
+    fmm_dipole_records(S, &r32, &r64);
+    if (mixed)
+      C.rec32 = r32;
+    else
+      C.rec64 = r64;
+  }
   const double params[2] = {thresh, S.reliable_drop};
   CPPE_HIP_CHECK(hipMemcpyAsync(C.params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
   // r0 = rhs - B x0, z0 = alpha r0, p = z0. The mixed form takes r0 from an FP32 apply: it is

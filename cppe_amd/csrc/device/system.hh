@@ -91,6 +91,8 @@ struct CgState {
   int* h_status = nullptr;  // pinned host copy of the status block
   hipGraphExec_t g_step[2] = {nullptr, nullptr}, g_rel = nullptr, g_dir[2] = {nullptr, nullptr};
   double *x = nullptr, *rhs = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr, *t = nullptr;
+  float4* rec32   = nullptr;  // near-field dipole records of p written by the direction kernels
+  double4* rec64 = nullptr;   // (one of them, in the precision of the step's apply; none: nrhs > 1)
   dbuf<double> store;  // backing of the vectors when this state owns them
   void release_graphs();
   ~CgState();
@@ -142,6 +144,9 @@ struct System {
   dbuf<double> io_a, io_b;  // AoS3 staging in caller order
   dbuf<float> f32_src;      // FP32 staging for mixed-precision applies
   bool mu_valid = false;
+  // When set, the near-field dipole records of the next B apply's precision already hold this
+  // vector (the CG direction kernels write them beside p), and the apply does not pack them.
+  const double* prepacked = nullptr;
   hipEvent_t* prof = nullptr;  // when set (8 events): [0..4] FMM phase boundaries, [6..7] around the
                                // staged near-field kernel alone
 
@@ -172,6 +177,8 @@ void cg_graphs_release(System& S);
 // out = B x for nrhs right-hand sides ([rhs][component][site] layout), one launch sequence
 void apply_b_multi(System& S, const double* x, double* out, int nrhs, bool fp32);
 void fmm_reserve_multi(System& S);
+// near-field dipole records of a single-right-hand-side B apply (allocated by fmm_reserve)
+void fmm_dipole_records(System& S, float4** rec32, double4** rec64);
 
 void fmm_field(System& S, int m, const double* src, double* F, double damping, bool fp32);
 void fmm_static(System& S, const double* q, const double* mu, const double* th, double* F, double damping);
