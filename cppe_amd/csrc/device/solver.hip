@@ -46,7 +46,8 @@ enum SegI { I_ACTIVE = 0, I_NEEDREL, I_ITERS, I_BAD, I_PENDING, I_ELIGIBLE, I_CO
 // status block: [iteration, active segments, segments asking for a reliable update, bad,
 // step taken in the current iteration, segments pending an FP64 confirmation]
 enum Status { ST_IT = 0, ST_ACTIVE, ST_REL, ST_BAD, ST_STEPPED, ST_PEND, ST_COUNT };
-enum FinalMode { F_INIT = 0, F_ALPHA, F_RNORM, F_REL, F_BETA };
+// F_RNB: F_RNORM, then -- unless a segment asked for a reliable update -- F_BETA, in one launch
+enum FinalMode { F_INIT = 0, F_ALPHA, F_RNORM, F_REL, F_BETA, F_RNB };
 
 // Both summations apply B through the tree path: a direct-summation system is a tree of one leaf
 // whose near list is itself (every pair, exclusions and Thole damping as in the direct sum).
@@ -180,26 +181,43 @@ __global__ __launch_bounds__(256) void k_sg_pap(CgVecs v) {
   if (threadIdx.x == 0) v.part[c] = s;
 }
 
-// x += alpha p ; r -= alpha Ap ; partial r.r (active segments)
-__global__ __launch_bounds__(256) void k_sg_xr(CgVecs v) {
+// Step update and preconditioner in one pass (active segments): x += alpha p ; r -= alpha Ap ; z = alpha r ;
+// partials r.r (part[c]) and r.z (part[nchunk + c]), each summed as in the separate kernels. The
+// z of a segment that converges or asks for a reliable update here is not used (the direction
+// after a reliable update recomputes it).
+__global__ __launch_bounds__(256) void k_sg_xrz(CgVecs v) {
   __shared__ double red[4];
   const int c = blockIdx.x, n = v.na, g = v.cseg[c];
   const ChunkIdx ci(v, c);
-  double s = 0.0;
+  double srr = 0.0, srz = 0.0;
   if (v.si[I_ACTIVE * v.nseg + g]) {
     const double al = v.sd[D_ALPHA * v.nseg + g];
-    for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x)
+    for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x) {
+      double rq[3];
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         const size_t k = ci.b + (size_t)q * n + i;
         v.x[k] += al * v.p[k];
         const double rk = v.r[k] - al * v.Ap[k];
         v.r[k] = rk;
-        s += rk * rk;
+        srr += rk * rk;
+        rq[q] = rk;
       }
+      double z0, z1, z2;
+      z_of(v.a6, n, i + ci.ia, rq[0], rq[1], rq[2], z0, z1, z2);
+      const size_t k0 = ci.b + i;
+      v.z[k0]         = z0;
+      v.z[k0 + n]     = z1;
+      v.z[k0 + 2 * (size_t)n] = z2;
+      srz += rq[0] * z0 + rq[1] * z1 + rq[2] * z2;
+    }
   }
-  s = block_sum(s, red);
-  if (threadIdx.x == 0) v.part[c] = s;
+  const double s1 = block_sum(srr, red);
+  const double s2 = block_sum(srz, red);
+  if (threadIdx.x == 0) {
+    v.part[c]            = s1;
+    v.part[v.nchunk + c] = s2;
+  }
 }
 
 // r = rhs - t for the segments that asked for a reliable update or await confirmation; partial r.r
@@ -221,8 +239,9 @@ __global__ __launch_bounds__(256) void k_sg_resid(CgVecs v) {
   if (threadIdx.x == 0) v.part[c] = s;
 }
 
-// The direction kernels (k_sg_z, F_BETA, k_sg_p) follow the step in the same launch sequence and
-// do nothing while a segment waits for a reliable update (the host runs them again after it).
+// The direction (beta in F_RNB, k_sg_p) follows the step in the same launch sequence and does
+// nothing while a segment waits for a reliable update; the host then runs the direction kernels
+// (k_sg_z, F_BETA, k_sg_p, which test the same condition) after the update.
 
 // z = alpha r ; partial z.r (active segments)
 __global__ __launch_bounds__(256) void k_sg_z(CgVecs v) {
@@ -332,7 +351,7 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
       }
     } else if (MODE == F_ALPHA) {
       if (si[I_ACTIVE * ns + g]) sd[D_ALPHA * ns + g] = sd[D_RZ * ns + g] / s1;
-    } else if (MODE == F_RNORM) {
+    } else if (MODE == F_RNORM || MODE == F_RNB) {
       if (si[I_ACTIVE * ns + g]) {
         const double rn      = sqrt(s1);
         sd[D_RNORM * ns + g] = rn;
@@ -402,6 +421,19 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
   if (nb) atomicAdd(&cnt[2], nb);
   if (np) atomicAdd(&cnt[3], np);
   __syncthreads();
+  const bool beta_now = MODE == F_RNB && cnt[1] == 0;  // no reliable update pending: direction now
+  if (beta_now) {
+    for (int g = wv; g < ns; g += 4) {
+      double s2 = 0.0;
+      for (int c = f.seg_chunk[g] + lane; c < f.seg_chunk[g + 1]; c += 64) s2 += f.part[f.nchunk + c];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) s2 += __shfl_xor(s2, off);
+      if (lane == 0 && f.si[I_ACTIVE * ns + g]) {
+        f.sd[D_BETA * ns + g] = s2 / f.sd[D_RZ * ns + g];
+        f.sd[D_RZ * ns + g]   = s2;
+      }
+    }
+  }
   if (threadIdx.x == 0) {
     f.status[ST_ACTIVE] = cnt[0];
     f.status[ST_REL]    = cnt[1];
@@ -411,7 +443,11 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
       f.status[ST_IT]      = 0;
       f.status[ST_STEPPED] = 0;
     }
-    if (MODE == F_RNORM) f.status[ST_STEPPED] = 1;
+    if (MODE == F_RNORM || (MODE == F_RNB && !beta_now)) f.status[ST_STEPPED] = 1;
+    if (beta_now) {  // the step and its direction: one more iteration counted (as F_BETA does)
+      f.status[ST_IT]      = it + 1;
+      f.status[ST_STEPPED] = 0;
+    }
     if (MODE == F_BETA) {  // counts iterations that took a step (not a resumption after confirmation)
       f.status[ST_IT]      = it + f.status[ST_STEPPED];
       f.status[ST_STEPPED] = 0;
@@ -577,17 +613,6 @@ static void apply(System& S, CgState& C, const double* x, double* out, bool fp32
     apply_b_multi(S, x, out, C.nrhs, fp32);
 }
 
-static void seq_step(System& S, CgState& C, bool mixed) {
-  const dim3 g(C.nchunk), b(256);
-  if (C.rec32 || C.rec64) S.prepacked = C.p;  // records written by k_sg_init / k_sg_p
-  apply(S, C, C.p, C.Ap, mixed);
-  S.prepacked = nullptr;
-  hipLaunchKernelGGL(k_sg_pap, g, b, 0, S.stream, vecs(S, C));
-  launch_final<F_ALPHA>(S, C, mixed);
-  hipLaunchKernelGGL(k_sg_xr, g, b, 0, S.stream, vecs(S, C));
-  launch_final<F_RNORM>(S, C, mixed);
-}
-
 static void seq_reliable(System& S, CgState& C) {
   apply(S, C, C.x, C.t, false);
   hipLaunchKernelGGL(k_sg_resid, dim3(C.nchunk), dim3(256), 0, S.stream, vecs(S, C));
@@ -601,12 +626,19 @@ static void seq_direction(System& S, CgState& C, bool mixed) {
   hipLaunchKernelGGL(k_sg_p, g, b, 0, S.stream, vecs(S, C));
 }
 
-// One iteration: step, then the direction unless a segment asked for a reliable update (the
-// direction kernels test the status block on the device), so that one host round trip per
-// iteration suffices.
+// One iteration: the step (apply, p.Ap, alpha, x / r / z update, residual norm) and -- unless a
+// segment asked for a reliable update, which the final kernel and k_sg_p test on the device --
+// beta and the new direction, so that one host round trip per iteration suffices.
 static void seq_step_direction(System& S, CgState& C, bool mixed) {
-  seq_step(S, C, mixed);
-  seq_direction(S, C, mixed);
+  const dim3 g(C.nchunk), b(256);
+  if (C.rec32 || C.rec64) S.prepacked = C.p;  // records written by k_sg_init / k_sg_p
+  apply(S, C, C.p, C.Ap, mixed);
+  S.prepacked = nullptr;
+  hipLaunchKernelGGL(k_sg_pap, g, b, 0, S.stream, vecs(S, C));
+  launch_final<F_ALPHA>(S, C, mixed);
+  hipLaunchKernelGGL(k_sg_xrz, g, b, 0, S.stream, vecs(S, C));
+  launch_final<F_RNB>(S, C, mixed);
+  hipLaunchKernelGGL(k_sg_p, g, b, 0, S.stream, vecs(S, C));
 }
 
 void cg_graphs_release(System& S) {
