@@ -47,8 +47,10 @@ struct MlGrid {
   dbuf<long long> noff;  // n + 1
   dbuf<int> nj;          // neighbour index
   dbuf<float4> ngeo;     // (r, unit vector i -> j)
-  dbuf<double> nk;       // 6 per pair: fc(r) T2d_ij packed xx,xy,xz,yy,yz,zz (0 for excluded pairs)
+  dbuf<float> nk;        // 6 per pair: fc(r) T2d_ij packed xx,xy,xz,yy,yz,zz (0 for excluded pairs)
   dbuf<float> iso;       // alpha_iso of each site
+  // per-site records gathered by the list kernels: (F / |F|, alpha_iso), and v1 / v2 in FP32
+  dbuf<float4> frec, w1, w2;
   dbuf<float> X, y;      // n x 64 features, n x 3 network outputs
   dbuf<double> V;        // 9n bases
 };

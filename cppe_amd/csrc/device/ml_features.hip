@@ -24,8 +24,10 @@
 // with each pair's distance, unit vector and damped tensor block fc T2d (geometry only, built
 // once per system like the octree). The descriptors are
 // evaluated in FP32 (the eight radial Gaussians by the recurrence g_{k+1} = g_k A B_k,
-// A = exp(2 eta dr (r - r_0)), B_k = exp(-eta dr^2 (2k + 1)), two exponentials per neighbour),
-// the bases in FP64 (Thole-damped dipole tensor as in the response operator).
+// A = exp(2 eta dr (r - r_0)), B_k = exp(-eta dr^2 (2k + 1)), two exponentials per neighbour).
+// The bases: tensor blocks formed in FP64 (Thole-damped dipole tensor as in the response
+// operator) and stored in FP32, neighbour vectors gathered as FP32 records, sums in FP64 -- the
+// list passes are bound by the bytes they move (a guess needs no more than FP32 accuracy).
 #include <hipcub/hipcub.hpp>
 
 #include "ml.hh"
@@ -149,7 +151,7 @@ __device__ __forceinline__ void t2_block(double dx, double dy, double dz, double
 }
 
 __global__ __launch_bounds__(128) void k_ml_nfill(FeatArgs a, const long long* __restrict__ noff, int* __restrict__ nj,
-                                                  float4* __restrict__ ngeo, double* __restrict__ nk,
+                                                  float4* __restrict__ ngeo, float* __restrict__ nk,
                                                   float* __restrict__ iso) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
@@ -165,12 +167,9 @@ __global__ __launch_bounds__(128) void k_ml_nfill(FeatArgs a, const long long* _
     for (int k = e0; k < e1; ++k) ex |= (a.eidx[k] == j);
     nj[p]   = j;
     ngeo[p] = make_float4((float)r, (float)(dx / r), (float)(dy / r), (float)(dz / r));
-    double* k = nk + 6 * p;
-    if (ex) {
-      for (int c = 0; c < 6; ++c) k[c] = 0.0;
-    } else {
-      t2_block(dx, dy, dz, r, di * a.s[j], fc, k);
-    }
+    double k[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (!ex) t2_block(dx, dy, dz, r, di * a.s[j], fc, k);
+    for (int c = 0; c < 6; ++c) nk[6 * p + c] = (float)k[c];
     ++p;
   });
 }
@@ -180,8 +179,11 @@ struct ListArgs {
   const long long* noff;
   const int* nj;
   const float4* ngeo;
-  const double* nk;
+  const float* nk;
   const float* iso;
+  float4* frec;  // per site: F / |F| and alpha_iso (written by k_ml_v1)
+  float4* w1;    // per site: v1 in FP32 (k_ml_v1)
+  float4* w2;    // per site: v2 in FP32 (k_ml_basis<1>)
   const double *x, *y, *z, *a6, *F;
   const int* eoff;
   const int* eidx;
@@ -232,13 +234,12 @@ __global__ __launch_bounds__(256) void k_ml_desc(ListArgs a) {
     const float r = gq.x, ux = gq.y, uy = gq.z, uz = gq.w;
     const float ri  = 1.0f / r;
     const float fc  = 0.5f * (cosf(kPiRc * r) + 1.0f);
-    const float Fjx = (float)a.F[j], Fjy = (float)a.F[n + j], Fjz = (float)a.F[2 * (size_t)n + j];
-    const float Fji = 1.0f / (sqrtf(Fjx * Fjx + Fjy * Fjy + Fjz * Fjz) + 1e-30f);
-    const float iso = a.iso[j];
+    const float4 fj = a.frec[j];  // F_j / |F_j|, alpha_iso_j
+    const float iso = fj.w;
     const float cth = ux * ffx + uy * ffy + uz * ffz;
     const float p2  = 1.5f * cth * cth - 0.5f;
-    const float al  = (Fjx * ffx + Fjy * ffy + Fjz * ffz) * Fji;
-    const float bj  = (ux * Fjx + uy * Fjy + uz * Fjz) * Fji;
+    const float al  = fj.x * ffx + fj.y * ffy + fj.z * ffz;
+    const float bj  = ux * fj.x + uy * fj.y + uz * fj.z;
     const float u   = r - (float)kRk0;
     float g         = __expf(-(float)kEta * u * u) * fc;
     const float A   = __expf(kA * u);
@@ -301,21 +302,25 @@ __global__ __launch_bounds__(256) void k_ml_desc(ListArgs a) {
   Xi[55] = f55;
 }
 
-// v1 = alpha F
+// v1 = alpha F, and the per-site FP32 records the list kernels gather
 __global__ void k_ml_v1(ListArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const int n = a.n;
+  const double Fx = a.F[i], Fy = a.F[n + i], Fz = a.F[2 * (size_t)n + i];
   double ai[6], ox, oy, oz;
   site_alpha(a.a6, n, i, ai);
-  amul(ai, a.F[i], a.F[n + i], a.F[2 * (size_t)n + i], ox, oy, oz);
+  amul(ai, Fx, Fy, Fz, ox, oy, oz);
   a.V[i]                 = ox;
   a.V[n + i]             = oy;
   a.V[2 * (size_t)n + i] = oz;
+  a.w1[i]                = make_float4((float)ox, (float)oy, (float)oz, 0.0f);
+  const double Fi = 1.0 / (sqrt(Fx * Fx + Fy * Fy + Fz * Fz) + 1e-30);
+  a.frec[i]       = make_float4((float)(Fx * Fi), (float)(Fy * Fi), (float)(Fz * Fi), a.iso[i]);
 }
 
 // PASS 1: v2 = alpha_i sum_j K_ij v1_j.  PASS 2: v3 = alpha_i sum_j K_ij v2_j, descriptors 56-63.
-// K_ij = fc(r) T2d_ij from the neighbour list (FP64).
+// K_ij = fc(r) T2d_ij from the neighbour list and v_j as FP32 records; products summed in FP64.
 template <int PASS>
 __global__ __launch_bounds__(256) void k_ml_basis(ListArgs a) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -324,16 +329,16 @@ __global__ __launch_bounds__(256) void k_ml_basis(ListArgs a) {
   const bool ok = i0 < a.n;
   const int i   = ok ? i0 : a.n - 1;  // trailing lanes shadow the last site and store nothing
   const int n = a.n;
-  const double* W = a.V + (PASS == 1 ? 0 : 3 * (size_t)n);
+  const float4* W = PASS == 1 ? a.w1 : a.w2;
   double sx = 0.0, sy = 0.0, sz = 0.0;
   const long long p1 = ok ? a.noff[i + 1] : 0;
   for (long long p = a.noff[i] + sub; p < p1; p += kBasisLanes) {
-    const int j     = a.nj[p];
-    const double* k = a.nk + 6 * p;
-    const double wx = W[j], wy = W[n + j], wz = W[2 * (size_t)n + j];
-    sx += k[0] * wx + k[1] * wy + k[2] * wz;
-    sy += k[1] * wx + k[3] * wy + k[4] * wz;
-    sz += k[2] * wx + k[4] * wy + k[5] * wz;
+    const int j    = a.nj[p];
+    const float* k = a.nk + 6 * p;
+    const float4 w = W[j];
+    sx += (double)(k[0] * w.x + k[1] * w.y + k[2] * w.z);
+    sy += (double)(k[1] * w.x + k[3] * w.y + k[4] * w.z);
+    sz += (double)(k[2] * w.x + k[4] * w.y + k[5] * w.z);
   }
   sx = lane_sum<kBasisLanes>(sx);
   sy = lane_sum<kBasisLanes>(sy);
@@ -346,6 +351,7 @@ __global__ __launch_bounds__(256) void k_ml_basis(ListArgs a) {
     a.V[3 * (size_t)n + i] = ox;
     a.V[4 * (size_t)n + i] = oy;
     a.V[5 * (size_t)n + i] = oz;
+    a.w2[i]                = make_float4((float)ox, (float)oy, (float)oz, 0.0f);
     return;
   }
   a.V[6 * (size_t)n + i] = ox;
@@ -484,10 +490,13 @@ void ml_features(System& S, float* X, double* V) {
   const int n    = S.n;
   hipStream_t st = S.stream;
   MlGrid& G      = ml_grid(S);
-  ListArgs a{n, G.noff.p, G.nj.p, G.ngeo.p, G.nk.p, G.iso.p, S.x.p, S.y.p, S.z.p, S.alpha6.p, S.v_rhs.p,
-             S.excl_off.p, S.excl_idx.p, X, V};
-  hipLaunchKernelGGL(k_ml_desc, dim3(ceil_div((long long)n * kDescLanes, 256)), dim3(256), 0, st, a);
+  G.frec.alloc(n);
+  G.w1.alloc(n);
+  G.w2.alloc(n);
+  ListArgs a{n, G.noff.p, G.nj.p, G.ngeo.p, G.nk.p, G.iso.p, G.frec.p, G.w1.p, G.w2.p, S.x.p, S.y.p, S.z.p,
+             S.alpha6.p, S.v_rhs.p, S.excl_off.p, S.excl_idx.p, X, V};
   hipLaunchKernelGGL(k_ml_v1, dim3(ceil_div(n, 256)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_ml_desc, dim3(ceil_div((long long)n * kDescLanes, 256)), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_ml_basis<1>, dim3(ceil_div((long long)n * kBasisLanes, 256)), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_ml_basis<2>, dim3(ceil_div((long long)n * kBasisLanes, 256)), dim3(256), 0, st, a);
 }
