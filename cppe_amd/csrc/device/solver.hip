@@ -37,7 +37,8 @@ void fmm_reserve(System& S);
 constexpr int kChunk = 256;  // one site per thread: enough blocks in flight to stream the vectors
 
 // per-segment doubles (SoA, each nseg long) and ints
-enum SegD { D_RZ = 0, D_ALPHA, D_BETA, D_RNORM, D_RLAST, D_COUNT };
+// D_RZN: the new r.z of an F_RNB launch, kept between its two passes
+enum SegD { D_RZ = 0, D_BETA, D_RNORM, D_RLAST, D_RZN, D_COUNT };
 // I_PENDING (mixed form): recursive residual below the threshold, frozen until an FP64 true
 // residual confirms convergence or resumes the segment
 // I_ELIGIBLE (mixed form): residual within 100x of its replacement trigger; replaced whenever a
@@ -46,8 +47,9 @@ enum SegI { I_ACTIVE = 0, I_NEEDREL, I_ITERS, I_BAD, I_PENDING, I_ELIGIBLE, I_CO
 // status block: [iteration, active segments, segments asking for a reliable update, bad,
 // step taken in the current iteration, segments pending an FP64 confirmation]
 enum Status { ST_IT = 0, ST_ACTIVE, ST_REL, ST_BAD, ST_STEPPED, ST_PEND, ST_COUNT };
-// F_RNB: F_RNORM, then -- unless a segment asked for a reliable update -- F_BETA, in one launch
-enum FinalMode { F_INIT = 0, F_ALPHA, F_RNORM, F_REL, F_BETA, F_RNB };
+// F_RNB: the residual norm and its tests, then -- unless a segment asked for a reliable update --
+// F_BETA's update, in one launch
+enum FinalMode { F_INIT = 0, F_REL, F_BETA, F_RNB };
 
 // Both summations apply B through the tree path: a direct-summation system is a tree of one leaf
 // whose near list is itself (every pair, exclusions and Thole damping as in the direct sum).
@@ -105,6 +107,7 @@ struct CgVecs {
   int na, spr, nseg, nchunk;  // sites per right-hand side, segments per right-hand side
   const int* clo;   // nchunk + 1 offsets of the chunks (sites of the stacked right-hand sides)
   const int* cseg;  // segment of each chunk
+  const int* seg_chunk;  // nseg + 1 chunk offsets of the segments
   const double* a6;
   const double* rhs;
   double *x, *r, *z, *p, *Ap, *t;
@@ -166,7 +169,7 @@ __global__ __launch_bounds__(256) void k_sg_init(CgVecs v) {
   }
 }
 
-// partial p.Ap (active segments)
+// partial p.Ap (active segments), into the third block of the partials (read by k_sg_xrz)
 __global__ __launch_bounds__(256) void k_sg_pap(CgVecs v) {
   __shared__ double red[4];
   const int c = blockIdx.x, n = v.na;
@@ -178,7 +181,7 @@ __global__ __launch_bounds__(256) void k_sg_pap(CgVecs v) {
       s += v.p[k0] * v.Ap[k0] + v.p[k1] * v.Ap[k1] + v.p[k2] * v.Ap[k2];
     }
   s = block_sum(s, red);
-  if (threadIdx.x == 0) v.part[c] = s;
+  if (threadIdx.x == 0) v.part[2 * (size_t)v.nchunk + c] = s;
 }
 
 // Step update and preconditioner in one pass (active segments): x += alpha p ; r -= alpha Ap ; z = alpha r ;
@@ -187,11 +190,25 @@ __global__ __launch_bounds__(256) void k_sg_pap(CgVecs v) {
 // after a reliable update recomputes it).
 __global__ __launch_bounds__(256) void k_sg_xrz(CgVecs v) {
   __shared__ double red[4];
+  __shared__ double alpha;
   const int c = blockIdx.x, n = v.na, g = v.cseg[c];
   const ChunkIdx ci(v, c);
+  const bool active = v.si[I_ACTIVE * v.nseg + g];
+  // alpha = (r.z) / (p.Ap) of the segment, from k_sg_pap's partials: every block sums them in the
+  // order the final kernels use (lane l: chunks l, l + 64, ...; then an xor-shuffle tree), so all
+  // blocks of a segment take the same alpha, without a launch of its own
+  if (active && threadIdx.x < 64) {
+    double s1 = 0.0;
+    for (int q = v.seg_chunk[g] + (int)threadIdx.x; q < v.seg_chunk[g + 1]; q += 64)
+      s1 += v.part[2 * (size_t)v.nchunk + q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s1 += __shfl_xor(s1, off);
+    if (threadIdx.x == 0) alpha = v.sd[D_RZ * v.nseg + g] / s1;
+  }
+  __syncthreads();
   double srr = 0.0, srz = 0.0;
-  if (v.si[I_ACTIVE * v.nseg + g]) {
-    const double al = v.sd[D_ALPHA * v.nseg + g];
+  if (active) {
+    const double al = alpha;
     for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x) {
       double rq[3];
 #pragma unroll
@@ -315,14 +332,15 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
     double s1 = 0.0, s2 = 0.0;
     for (int c = f.seg_chunk[g] + lane; c < f.seg_chunk[g + 1]; c += 64) {
       s1 += f.part[c];
-      if (MODE == F_INIT) s2 += f.part[f.nchunk + c];
+      if (MODE == F_INIT || MODE == F_RNB) s2 += f.part[f.nchunk + c];
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       s1 += __shfl_xor(s1, off);
-      if (MODE == F_INIT) s2 += __shfl_xor(s2, off);
+      if (MODE == F_INIT || MODE == F_RNB) s2 += __shfl_xor(s2, off);
     }
     if (lane != 0) continue;
+    if (MODE == F_RNB) f.sd[D_RZN * ns + g] = s2;  // read back by this thread after the barrier
     double* sd = f.sd;
     int* si    = f.si;
     if (MODE == F_INIT) {
@@ -330,7 +348,6 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
       sd[D_RZ * ns + g]    = s1;
       sd[D_RNORM * ns + g] = rn;
       sd[D_RLAST * ns + g] = rn;
-      sd[D_ALPHA * ns + g] = 0.0;
       sd[D_BETA * ns + g]  = 0.0;
       const bool bad = !isfinite(rn);
       si[I_BAD * ns + g]      = bad;
@@ -349,9 +366,7 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
         si[I_ACTIVE * ns + g]  = !bad && rn > 0.0;
         si[I_NEEDREL * ns + g] = 0;
       }
-    } else if (MODE == F_ALPHA) {
-      if (si[I_ACTIVE * ns + g]) sd[D_ALPHA * ns + g] = sd[D_RZ * ns + g] / s1;
-    } else if (MODE == F_RNORM || MODE == F_RNB) {
+    } else if (MODE == F_RNB) {
       if (si[I_ACTIVE * ns + g]) {
         const double rn      = sqrt(s1);
         sd[D_RNORM * ns + g] = rn;
@@ -422,13 +437,10 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
   if (np) atomicAdd(&cnt[3], np);
   __syncthreads();
   const bool beta_now = MODE == F_RNB && cnt[1] == 0;  // no reliable update pending: direction now
-  if (beta_now) {
+  if (beta_now && lane == 0) {
     for (int g = wv; g < ns; g += 4) {
-      double s2 = 0.0;
-      for (int c = f.seg_chunk[g] + lane; c < f.seg_chunk[g + 1]; c += 64) s2 += f.part[f.nchunk + c];
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) s2 += __shfl_xor(s2, off);
-      if (lane == 0 && f.si[I_ACTIVE * ns + g]) {
+      if (f.si[I_ACTIVE * ns + g]) {
+        const double s2       = f.sd[D_RZN * ns + g];
         f.sd[D_BETA * ns + g] = s2 / f.sd[D_RZ * ns + g];
         f.sd[D_RZ * ns + g]   = s2;
       }
@@ -443,7 +455,7 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
       f.status[ST_IT]      = 0;
       f.status[ST_STEPPED] = 0;
     }
-    if (MODE == F_RNORM || (MODE == F_RNB && !beta_now)) f.status[ST_STEPPED] = 1;
+    if (MODE == F_RNB && !beta_now) f.status[ST_STEPPED] = 1;
     if (beta_now) {  // the step and its direction: one more iteration counted (as F_BETA does)
       f.status[ST_IT]      = it + 1;
       f.status[ST_STEPPED] = 0;
@@ -493,7 +505,7 @@ static void cg_layout(System& S, CgState& C, const std::vector<int>& seg_off) {
   up(C.chunk_lo, clo);
   up(C.chunk_seg, cseg);
   up(C.seg_chunk, schunk);
-  C.part.alloc(2 * (size_t)C.nchunk);
+  C.part.alloc(3 * (size_t)C.nchunk);  // [r.r or p.Ap | r.z | p.Ap of the step]
   C.sd.alloc(D_COUNT * (size_t)ns);
   C.si.alloc(I_COUNT * (size_t)ns);
   C.status.alloc(ST_COUNT);
@@ -566,6 +578,7 @@ static CgVecs vecs(System& S, CgState& C) {
   v.nchunk = C.nchunk;
   v.clo    = C.chunk_lo.p;
   v.cseg   = C.chunk_seg.p;
+  v.seg_chunk = C.seg_chunk.p;
   v.a6     = S.alpha6.p;
   v.rhs    = C.rhs;
   v.x      = C.x;
@@ -635,7 +648,6 @@ static void seq_step_direction(System& S, CgState& C, bool mixed) {
   apply(S, C, C.p, C.Ap, mixed);
   S.prepacked = nullptr;
   hipLaunchKernelGGL(k_sg_pap, g, b, 0, S.stream, vecs(S, C));
-  launch_final<F_ALPHA>(S, C, mixed);
   hipLaunchKernelGGL(k_sg_xrz, g, b, 0, S.stream, vecs(S, C));
   launch_final<F_RNB>(S, C, mixed);
   hipLaunchKernelGGL(k_sg_p, g, b, 0, S.stream, vecs(S, C));
