@@ -69,6 +69,7 @@ struct FmmState {
   dbuf<int> cl_off, cl_src;          // CSR: n_clusters+1 offsets, source leaves
   dbuf<unsigned long long> cl_mask;  // lane masks
   dbuf<float4> posf4;     // FP32 source records: offset from the site's leaf centre, alpha_iso^(-1/6)
+  dbuf<float4> m2l_rec;   // FP32 M2L pairs in list order: target - source centre (formed in FP64), source
   dbuf<double> M64, L64;  // dipole kind
   dbuf<float> M32, L32;
   dbuf<double> Mc, Lc;    // combined kind (static fields, m = 0 / 2)
@@ -303,7 +304,8 @@ __global__ __launch_bounds__(64) void k_fmm_unfold(int ncells, R* __restrict__ L
 template <int KIND, int P, typename R, int G>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_fmm_m2l(
     int ncells, const int* __restrict__ off, const int* __restrict__ srcs, const double* __restrict__ cx,
-    const double* __restrict__ cy, const double* __restrict__ cz, const R* __restrict__ Mf, R* __restrict__ L) {
+    const double* __restrict__ cy, const double* __restrict__ cz, const R* __restrict__ Mf, R* __restrict__ L,
+    const float4* __restrict__ rec) {
   using O         = Ops<KIND, P, R>;
   const int lane  = threadIdx.x;
   const int gl    = lane % G;
@@ -314,11 +316,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void
   for (int k = 0; k < O::LS; ++k) l[k] = R(0);
   if (valid) {
     const int o0 = off[A], o1 = off[A + 1];
-    const double ax = cx[A], ay = cy[A], az = cz[A];
-    for (int k = o0 + gl; k < o1; k += G) {
-      const int B = srcs[k];
-      const R* mrow = static_cast<const R*>(__builtin_assume_aligned(&Mf[(size_t)B * O::FP], 16));
-      O::m2l((R)(ax - cx[B]), (R)(ay - cy[B]), (R)(az - cz[B]), mrow, l);
+    if constexpr (sizeof(R) == 4) {  // one coalesced record per pair: no dependent centre gathers
+      for (int k = o0 + gl; k < o1; k += G) {
+        const float4 q = rec[k];
+        const int B    = __float_as_int(q.w);
+        const R* mrow  = static_cast<const R*>(__builtin_assume_aligned(&Mf[(size_t)B * O::FP], 16));
+        O::m2l(q.x, q.y, q.z, mrow, l);
+      }
+    } else {
+      (void)rec;
+      const double ax = cx[A], ay = cy[A], az = cz[A];
+      for (int k = o0 + gl; k < o1; k += G) {
+        const int B = srcs[k];
+        const R* mrow = static_cast<const R*>(__builtin_assume_aligned(&Mf[(size_t)B * O::FP], 16));
+        O::m2l((R)(ax - cx[B]), (R)(ay - cy[B]), (R)(az - cz[B]), mrow, l);
+      }
     }
   }
   constexpr int NCH = (O::LS + G - 1) / G;
@@ -1185,6 +1197,31 @@ __global__ __launch_bounds__(256) void k_fmm_l2p_add(int n, const int* __restric
 }
 
 // ---- host drivers ----------------------------------------------------------------------------
+// FP32 M2L pair records (target-grouped list order): the displacement is formed in FP64 and
+// rounded, as the FP64-centre form of the kernel does, so both give the same FP32 operands.
+static __global__ void k_m2l_rec(int ncells, const int* __restrict__ off, const int* __restrict__ srcs,
+                                 const double* __restrict__ cx, const double* __restrict__ cy,
+                                 const double* __restrict__ cz, float4* __restrict__ rec) {
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= ncells) return;
+  const double ax = cx[A], ay = cy[A], az = cz[A];
+  for (int k = off[A]; k < off[A + 1]; ++k) {
+    const int B = srcs[k];
+    rec[k]      = make_float4((float)(ax - cx[B]), (float)(ay - cy[B]), (float)(az - cz[B]), __int_as_float(B));
+  }
+}
+
+// The records are filled on stream st: the stream of the M2L that reads them (the far field may
+// run on a second stream beside the near field).
+inline void ensure_m2l_rec(System& S, hipStream_t st) {
+  FmmState& F     = *S.fmm;
+  const Octree& T = S.tree;
+  if (F.m2l_rec.p && F.m2l_rec.n >= (size_t)std::max<long long>(T.n_m2l, 1)) return;
+  F.m2l_rec.alloc((size_t)std::max<long long>(T.n_m2l, 1));
+  hipLaunchKernelGGL(k_m2l_rec, dim3(ceil_div(T.ncells, 256)), dim3(256), 0, st, T.ncells, T.m2l_off.p,
+                     T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, F.m2l_rec.p);
+}
+
 template <int KIND, int P, typename R>
 void fmm_far_pass(System& S, const double* src, R* M, R* L, hipStream_t st) {
   using O         = Ops<KIND, P, R>;
@@ -1206,20 +1243,26 @@ void fmm_far_pass(System& S, const double* src, R* M, R* L, hipStream_t st) {
                        T.ncells, M, Mf);
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[1], st));
   const int grp = S.fmm ? S.fmm->tune.m2l_group[sizeof(R) == 4] : 64;
+  const float4* rec = nullptr;
+  if constexpr (sizeof(R) == 4) {
+    if (!S.fmm) throw std::runtime_error("FP32 far field without FMM state");
+    ensure_m2l_rec(S, st);  // allocated beforehand by fmm_reserve when the apply is captured
+    rec = S.fmm->m2l_rec.p;
+  }
   if constexpr (KIND == KIND_M1) {
     if (grp == 16)
       hipLaunchKernelGGL((k_fmm_m2l<KIND, P, R, 16>), dim3(ceil_div(T.ncells, 4)), dim3(64), 0, st, T.ncells,
-                         T.m2l_off.p, T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, Mf, L);
+                         T.m2l_off.p, T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, Mf, L, rec);
     else if (grp == 32)
       hipLaunchKernelGGL((k_fmm_m2l<KIND, P, R, 32>), dim3(ceil_div(T.ncells, 2)), dim3(64), 0, st, T.ncells,
-                         T.m2l_off.p, T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, Mf, L);
+                         T.m2l_off.p, T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, Mf, L, rec);
     else
       hipLaunchKernelGGL((k_fmm_m2l<KIND, P, R, 64>), dim3(T.ncells), dim3(64), 0, st, T.ncells, T.m2l_off.p,
-                         T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, Mf, L);
+                         T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, Mf, L, rec);
   } else {
     (void)grp;
     hipLaunchKernelGGL((k_fmm_m2l<KIND, P, R, 64>), dim3(T.ncells), dim3(64), 0, st, T.ncells, T.m2l_off.p,
-                       T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, Mf, L);
+                       T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, Mf, L, rec);
   }
   hipLaunchKernelGGL((k_fmm_unfold<KIND, P, R>), dim3(ceil_div(T.ncells, 64)), dim3(64), 0, st, T.ncells, L);
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[2], st));
