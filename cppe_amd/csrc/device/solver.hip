@@ -94,6 +94,22 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return s;
 }
 
+// Lane l's share of the chunk partials part[lo, hi): chunks l, l + 64, l + 128, ... summed into
+// four interleaved accumulators (a fixed order, four loads in flight), combined as
+// (a0 + a1) + (a2 + a3). The 64 lane shares are then combined by an xor-shuffle tree.
+__device__ __forceinline__ double lane_chunk_sum(const double* __restrict__ part, int lo, int hi, int lane) {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int c = lo + lane;
+  for (; c + 192 < hi; c += 256) {
+    a0 += part[c];
+    a1 += part[c + 64];
+    a2 += part[c + 128];
+    a3 += part[c + 192];
+  }
+  for (; c < hi; c += 64) a0 += part[c];
+  return (a0 + a1) + (a2 + a3);
+}
+
 __device__ __forceinline__ void z_of(const double* __restrict__ a6, int n, int i, double r0, double r1, double r2,
                                      double& z0, double& z1, double& z2) {
   const double a0 = a6[i], a1 = a6[n + i], a2 = a6[2 * n + i], a3 = a6[3 * n + i], a4 = a6[4 * n + i],
@@ -194,13 +210,11 @@ __global__ __launch_bounds__(256) void k_sg_xrz(CgVecs v) {
   const int c = blockIdx.x, n = v.na, g = v.cseg[c];
   const ChunkIdx ci(v, c);
   const bool active = v.si[I_ACTIVE * v.nseg + g];
-  // alpha = (r.z) / (p.Ap) of the segment, from k_sg_pap's partials: every block sums them in the
-  // order the final kernels use (lane l: chunks l, l + 64, ...; then an xor-shuffle tree), so all
-  // blocks of a segment take the same alpha, without a launch of its own
+  // alpha = (r.z) / (p.Ap) of the segment, from k_sg_pap's partials: every block sums them in one
+  // fixed order (lane_chunk_sum, then an xor-shuffle tree), so all blocks of a segment take the
+  // same alpha, without a launch of its own
   if (active && threadIdx.x < 64) {
-    double s1 = 0.0;
-    for (int q = v.seg_chunk[g] + (int)threadIdx.x; q < v.seg_chunk[g + 1]; q += 64)
-      s1 += v.part[2 * (size_t)v.nchunk + q];
+    double s1 = lane_chunk_sum(v.part + 2 * (size_t)v.nchunk, v.seg_chunk[g], v.seg_chunk[g + 1], threadIdx.x);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s1 += __shfl_xor(s1, off);
     if (threadIdx.x == 0) alpha = v.sd[D_RZ * v.nseg + g] / s1;
@@ -314,7 +328,7 @@ struct CgFinal {
 };
 
 // Per-segment finalisation of the chunk partials: one wavefront per segment (the block's four
-// wavefronts take segments in turn); lane l sums chunks l, l + 64, ... in order, then a fixed
+// wavefronts take segments in turn); each lane sums its chunks (lane_chunk_sum), then a fixed
 // shuffle tree combines the lanes -- deterministic for a given chunk split. Lane 0 then updates
 // the segment's scalars; the status block is refreshed at the end.
 template <int MODE, bool MIXED>
@@ -329,11 +343,8 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int na = 0, nr = 0, nb = 0, np = 0;
   for (int g = wv; g < ns; g += 4) {
-    double s1 = 0.0, s2 = 0.0;
-    for (int c = f.seg_chunk[g] + lane; c < f.seg_chunk[g + 1]; c += 64) {
-      s1 += f.part[c];
-      if (MODE == F_INIT || MODE == F_RNB) s2 += f.part[f.nchunk + c];
-    }
+    double s1 = lane_chunk_sum(f.part, f.seg_chunk[g], f.seg_chunk[g + 1], lane), s2 = 0.0;
+    if (MODE == F_INIT || MODE == F_RNB) s2 = lane_chunk_sum(f.part + f.nchunk, f.seg_chunk[g], f.seg_chunk[g + 1], lane);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       s1 += __shfl_xor(s1, off);
