@@ -193,7 +193,7 @@ struct ListArgs {
 
 // Lanes per site of the list kernels: consecutive lanes take consecutive pairs of a site's
 // neighbour list (coalesced), partial sums are combined by a fixed xor-shuffle tree.
-constexpr int kDescLanes = 4, kBasisLanes = 8;
+constexpr int kDescLanes = 8, kBasisLanes = 16;
 
 template <int G>
 __device__ __forceinline__ float lane_sum(float v) {
