@@ -24,7 +24,7 @@ def nterms(p):
     return 0 if p < 0 else (p + 1) * (p + 2) * (p + 3) // 6
 
 
-@pytest.fixture(scope="module", params=[3, 5])
+@pytest.fixture(scope="module", params=[2, 3, 4, 5, 6, 7, 8])
 def gen(request, tmp_path_factory):
     p = request.param
     out = tmp_path_factory.mktemp("fmmops") / f"libgen_p{p}.so"
@@ -53,6 +53,8 @@ def test_ops_vs_reference_kernels(gen):
     R = O.ref_kernels()
     if R is None:
         pytest.skip("oracle/_ref not built (reference checkout absent)")
+    if p > 7:
+        pytest.skip("the reference's generated operators cover orders 2-7 (fmm/field_m*.cc dispatchers)")
     rng = np.random.default_rng(p)
     for mc, refm in ((0, 1), (1, 0)):
         ms, ls = sizes(mc, p)
