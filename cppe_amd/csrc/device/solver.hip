@@ -23,6 +23,8 @@
 // graph and replayed; all scalars live on the device, and the host reads a 4-int status block
 // after the step and reliable graphs to decide what to launch next.
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <functional>
 #include <vector>
@@ -45,8 +47,9 @@ enum SegD { D_RZ = 0, D_BETA, D_RNORM, D_RLAST, D_RZN, D_COUNT };
 // replacement runs for another segment, so that segments share FP64 applies
 enum SegI { I_ACTIVE = 0, I_NEEDREL, I_ITERS, I_BAD, I_PENDING, I_ELIGIBLE, I_COUNT };
 // status block: [iteration, active segments, segments asking for a reliable update, bad,
-// step taken in the current iteration, segments pending an FP64 confirmation]
-enum Status { ST_IT = 0, ST_ACTIVE, ST_REL, ST_BAD, ST_STEPPED, ST_PEND, ST_COUNT };
+// step taken in the current iteration, segments pending an FP64 confirmation, number of final
+// kernels run (each final kernel publishes the block to a host-mapped mirror, this count last)]
+enum Status { ST_IT = 0, ST_ACTIVE, ST_REL, ST_BAD, ST_STEPPED, ST_PEND, ST_SEQ, ST_COUNT };
 // F_RNB: the residual norm and its tests, then -- unless a segment asked for a reliable update --
 // F_BETA's update, in one launch
 enum FinalMode { F_INIT = 0, F_REL, F_BETA, F_RNB };
@@ -322,6 +325,7 @@ struct CgFinal {
   double* sd;
   int* si;
   int* status;           // ST_COUNT ints
+  int* hstat;            // host-mapped mirror of the status block
   const double* params;  // [thresh, reliable drop]
   double* hist;          // residual norms, iteration-major [it * nseg + g]
   int hist_cap;          // iterations held by hist
@@ -334,10 +338,11 @@ struct CgFinal {
 template <int MODE, bool MIXED>
 __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
   __shared__ int cnt[4];
-  if (MODE == F_BETA && f.status[ST_REL] > 0) return;
+  // F_BETA while a reliable update is pending: no update, only the publication below
+  const bool skip = MODE == F_BETA && f.status[ST_REL] > 0;
   if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
   __syncthreads();
-  const int ns = f.nseg;
+  const int ns = skip ? 0 : f.nseg;
   const int it = f.status[ST_IT];
   const double thresh = f.params[0], drop = f.params[1];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -457,7 +462,7 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
       }
     }
   }
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && !skip) {
     f.status[ST_ACTIVE] = cnt[0];
     f.status[ST_REL]    = cnt[1];
     f.status[ST_BAD]    = cnt[2];
@@ -475,6 +480,13 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
       f.status[ST_IT]      = it + f.status[ST_STEPPED];
       f.status[ST_STEPPED] = 0;
     }
+  }
+  if (threadIdx.x == 0) {  // publish: the fields, a system-scope fence, then the count
+    const int seq    = f.status[ST_SEQ] + 1;
+    f.status[ST_SEQ] = seq;
+    for (int q = 0; q < ST_SEQ; ++q) f.hstat[q] = f.status[q];
+    __threadfence_system();
+    f.hstat[ST_SEQ] = seq;
   }
 }
 
@@ -523,8 +535,15 @@ static void cg_layout(System& S, CgState& C, const std::vector<int>& seg_off) {
   C.params.alloc(2);
   C.hist_cap = 0;
   C.hist.release();
-  if (!C.h_status) CPPE_HIP_CHECK(hipHostMalloc((void**)&C.h_status, sizeof(int) * ST_COUNT));
+  if (!C.h_status) {
+    CPPE_HIP_CHECK(hipHostMalloc((void**)&C.h_status, sizeof(int) * ST_COUNT,
+                                 hipHostMallocMapped | hipHostMallocCoherent));
+    CPPE_HIP_CHECK(hipHostGetDevicePointer((void**)&C.d_hstat, C.h_status, 0));
+  }
+  CPPE_HIP_CHECK(hipMemsetAsync(C.status.p, 0, sizeof(int) * ST_COUNT, S.stream));
   CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
+  for (int q = 0; q < ST_COUNT; ++q) C.h_status[q] = 0;
+  C.expect = 0;
 }
 
 void setup_segments(System& S, const std::vector<int>& seg_off, const std::vector<int>& seg_ncrit) {
@@ -616,6 +635,7 @@ static CgFinal finals(CgState& C) {
   f.sd        = C.sd.p;
   f.si        = C.si.p;
   f.status    = C.status.p;
+  f.hstat     = C.d_hstat;
   f.params    = C.params.p;
   f.hist      = C.hist.p;
   f.hist_cap  = C.hist_cap;
@@ -698,9 +718,24 @@ static void run_graph(System& S, hipGraphExec_t& exec, const std::function<void(
   CPPE_HIP_CHECK(hipGraphLaunch(exec, S.stream));
 }
 
+// Waits for the status block of the last final kernel launched (C.expect of them so far). The
+// final kernels publish it to the host-mapped mirror, so the host can decide and enqueue the next
+// launch sequence while the rest of the current one (the new direction) still runs. A mirror that
+// has not caught up within 20 ms (a long apply, or mapped writes that are late) is replaced by a
+// synchronous copy of the device block, which is authoritative.
 static void read_status(System& S, CgState& C) {
-  CPPE_HIP_CHECK(hipMemcpyAsync(C.h_status, C.status.p, sizeof(int) * ST_COUNT, hipMemcpyDeviceToHost, S.stream));
-  CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
+  volatile int* hs = C.h_status;
+  const auto t0    = std::chrono::steady_clock::now();
+  int spins        = 0;
+  while (hs[ST_SEQ] != C.expect) {
+    if (++spins % 1024 == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
+      CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
+      CPPE_HIP_CHECK(hipMemcpy(C.h_status, C.status.p, sizeof(int) * ST_COUNT, hipMemcpyDeviceToHost));
+      C.expect = C.h_status[ST_SEQ];  // every launched final has run: the device count is exact
+      break;
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
 }
 
 // Solves B x = rhs for C.rhs into C.x (which holds the initial guess). Results per segment are
@@ -736,6 +771,7 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
   apply(S, C, C.x, C.t, mixed);
   hipLaunchKernelGGL(k_sg_init, dim3(C.nchunk), dim3(256), 0, st, vecs(S, C));
   launch_final<F_INIT>(S, C, mixed);
+  ++C.expect;  // every launch sequence below runs exactly one final kernel
   read_status(S, C);
   int k = 0, nrel = 0;
   const int m = mixed ? 1 : 0;
@@ -744,6 +780,7 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
   // replacement that runs anyway, or once no segment is active.
   auto reliable = [&] {
     run_graph(S, C.g_rel, [&] { seq_reliable(S, C); });
+    ++C.expect;
     read_status(S, C);
     ++nrel;
   };
@@ -754,9 +791,11 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
       reliable();
       if (C.h_status[ST_ACTIVE] == 0) break;
       run_graph(S, C.g_dir[m], [&] { seq_direction(S, C, mixed); });
+      ++C.expect;
       continue;
     }
     run_graph(S, C.g_step[m], [&] { seq_step_direction(S, C, mixed); });
+    ++C.expect;
     read_status(S, C);
     ++k;
     // The direction already ran unless a reliable update was asked for (it is then taken after
@@ -764,7 +803,10 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
     // no-op on the vectors; segments that resume then take theirs after the confirmation.
     if (C.h_status[ST_REL] > 0 || (C.h_status[ST_ACTIVE] == 0 && C.h_status[ST_PEND] > 0)) {
       reliable();
-      if (C.h_status[ST_ACTIVE] > 0) run_graph(S, C.g_dir[m], [&] { seq_direction(S, C, mixed); });
+      if (C.h_status[ST_ACTIVE] > 0) {
+        run_graph(S, C.g_dir[m], [&] { seq_direction(S, C, mixed); });
+        ++C.expect;
+      }
     }
   }
   if (C.h_status[ST_PEND] > 0) reliable();  // maxiter reached with unconfirmed segments

@@ -88,7 +88,9 @@ struct CgState {
   dbuf<double> part, sd, params, hist;
   dbuf<int> si, status;
   int hist_cap = 0;
-  int* h_status = nullptr;  // pinned host copy of the status block
+  int* h_status = nullptr;  // host-mapped mirror of the status block (written by the final kernels)
+  int* d_hstat  = nullptr;  // its device address
+  int expect    = 0;        // final kernels launched so far: the mirror's count once they have run
   hipGraphExec_t g_step[2] = {nullptr, nullptr}, g_rel = nullptr, g_dir[2] = {nullptr, nullptr};
   double *x = nullptr, *rhs = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr, *t = nullptr;
   float4* rec32   = nullptr;  // near-field dipole records of p written by the direction kernels
