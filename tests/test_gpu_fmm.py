@@ -78,6 +78,21 @@ def test_apply_b_fmm_vs_oracle(nw, ncrit):
     assert relerr(Sm.apply_b(x), ref) < 1e-5
 
 
+@pytest.mark.parametrize("order", [2, 3, 4, 6, 7, 8])
+def test_apply_b_fmm_every_order(order):
+    """B apply at every supported expansion order (harmonically folded M2L, FP32 pair records)
+    against the oracle's FMM of the same order: FP64 to 1e-11, FP32 to its rounding."""
+    w = water(1000, 1001, True)
+    say("fmm apply order", order)
+    B = O.BSystem(w.pos, w.pol, w.excl_csr(), damping=2.1304, scheme="fmm", ncrit=16, order=order)
+    x = np.random.default_rng(order).normal(size=3 * w.n)
+    ref = B.apply(x)
+    for prec, tol in (("fp64", 1e-11), ("mixed", 1e-5)):
+        S = capi.System(w.pos, w.pol, w.alpha_iso(), w.excl_csr(), summation="fmm", ncrit=16, order=order,
+                        damping=2.1304, precision=prec)
+        assert relerr(S.apply_b(x), ref) < tol, prec
+
+
 def test_solve_fmm_vs_oracle():
     w = water(1000, 1001, True)
     rhs = O.nuclear_fields(w.pos, w.atoms_z.astype(float), w.atoms_pos)
