@@ -998,26 +998,46 @@ __global__ __launch_bounds__(256) void k_fmm_near_st(NearArgs a, const int2* __r
   // stages from qf on hold far sources only (FP32: about half of the near pairs at 300k sites; FP64,
   // with its higher cutoff, only a few percent, which would not pay for the extra code path)
   const long long qf = (DAMP && sizeof(R) == 4 && xsplit) ? xsplit[A] : q1;
-  for (long long q0 = xoff[A]; q0 < q1; q0 += 64) {
+  // FP32 records: software pipeline over the stages -- the source index of stage s + 2 and the
+  // position / dipole values of stage s + 1 are loaded while stage s is evaluated (lanes beyond a
+  // stage's count load nothing and keep values they never store). FP64 records are gathered just
+  // before their stage is stored: the pipeline's registers cost that kernel more than it hides.
+  constexpr bool kPipe = sizeof(R) == 4;
+  const long long qa = xoff[A];
+  int jnn = 0;  // source index of the stage after the next (pipelined form)
+  double4 pn = make_double4(0.0, 0.0, 0.0, 0.0);
+  MU4 mn[K];
+  auto gather = [&](int j) {
+    pn = a.pos4[j];
+#pragma unroll
+    for (int k = 0; k < K; ++k) mn[k] = mu4[(size_t)k * n + j];
+  };
+  if (kPipe) {
+    if (lane < q1 - qa) gather(xsrc[qa + lane]);
+    if (lane < q1 - qa - 64) jnn = xsrc[qa + 64 + lane];
+  }
+  for (long long q0 = qa; q0 < q1; q0 += 64) {
     const int cn = (int)min<long long>(64, q1 - q0);
+    if (!kPipe && lane < cn) gather(xsrc[q0 + lane]);
     if (lane < cn) {
-      const int j      = xsrc[q0 + lane];
-      const double4 pj = a.pos4[j];
       StagedRec<R, K> r;
-      r.x = sizeof(R) == 8 ? (R)pj.x : (R)(pj.x - cax);
-      r.y = sizeof(R) == 8 ? (R)pj.y : (R)(pj.y - cay);
-      r.z = sizeof(R) == 8 ? (R)pj.z : (R)(pj.z - caz);
-      r.s = (R)pj.w;
+      r.x = sizeof(R) == 8 ? (R)pn.x : (R)(pn.x - cax);
+      r.y = sizeof(R) == 8 ? (R)pn.y : (R)(pn.y - cay);
+      r.z = sizeof(R) == 8 ? (R)pn.z : (R)(pn.z - caz);
+      r.s = (R)pn.w;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const MU4 mj     = mu4[(size_t)k * n + j];
-        r.m[3 * k]     = (R)mj.x;
-        r.m[3 * k + 1] = (R)mj.y;
-        r.m[3 * k + 2] = (R)mj.z;
+        r.m[3 * k]     = (R)mn[k].x;
+        r.m[3 * k + 1] = (R)mn[k].y;
+        r.m[3 * k + 2] = (R)mn[k].z;
       }
       sr[lane] = r;
     }
     wave_lds_barrier();
+    if (kPipe) {
+      if (lane < q1 - q0 - 64) gather(jnn);                    // stage s + 1
+      if (lane < q1 - q0 - 128) jnn = xsrc[q0 + 128 + lane];  // stage s + 2: its source index
+    }
     // every pair of the near list, the self pair contributing zero; excluded near pairs are
     // taken out afterwards by k_fmm_excl_sub with the same arithmetic
     if (DAMP && sizeof(R) == 4 && q0 >= qf) {  // far stage (uniform): undamped pairs, branch-free
