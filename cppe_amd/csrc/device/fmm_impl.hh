@@ -45,7 +45,8 @@ struct FmmTune {
   int near_variant[2] = {0, 0};  // 0: one thread per target site, 1: one wavefront per leaf chunk
   int near_block[2]   = {256, 256};
   int m2l_group[2]    = {64, 64};  // lanes per target cell in M2L (16, 32 or 64)
-  int overlap[2]      = {0, 0};    // far field on a second stream beside the near field (staged variants)
+  int overlap[2]      = {0, 0};    // far field on a second stream beside the near field (staged variants):
+                                   // 0 none, 1 the whole far field, 2 M2L / L2L after the upward pass
 };
 
 struct FmmState {
@@ -1242,9 +1243,9 @@ inline void ensure_m2l_rec(System& S, hipStream_t st) {
                      T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, F.m2l_rec.p);
 }
 
+// Upward half of the far field: P2M, M2M level by level, fold into the M2L rows.
 template <int KIND, int P, typename R>
-void fmm_far_pass(System& S, const double* src, R* M, R* L, hipStream_t st) {
-  using O         = Ops<KIND, P, R>;
+void fmm_upward_pass(System& S, const double* src, R* M, hipStream_t st) {
   const Octree& T = S.tree;
   const auto& lo = T.level_offset;
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[0], st));
@@ -1262,6 +1263,15 @@ void fmm_far_pass(System& S, const double* src, R* M, R* L, hipStream_t st) {
     hipLaunchKernelGGL((k_fmm_fold<KIND, P, R>), dim3(ceil_div(T.ncells - lo[1], 64)), dim3(64), 0, st, lo[1],
                        T.ncells, M, Mf);
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[1], st));
+}
+
+// Downward half: M2L from the folded rows, unfold, L2L level by level.
+template <int KIND, int P, typename R>
+void fmm_down_pass(System& S, R* M, R* L, hipStream_t st) {
+  using O         = Ops<KIND, P, R>;
+  const Octree& T = S.tree;
+  const auto& lo = T.level_offset;
+  R* Mf = M + fmm_fold_offset(T.ncells, KIND, P);
   const int grp = S.fmm ? S.fmm->tune.m2l_group[sizeof(R) == 4] : 64;
   const float4* rec = nullptr;
   if constexpr (sizeof(R) == 4) {
@@ -1293,6 +1303,12 @@ void fmm_far_pass(System& S, const double* src, R* M, R* L, hipStream_t st) {
   }
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[3], st));
   (void)sizeof(O);
+}
+
+template <int KIND, int P, typename R>
+void fmm_far_pass(System& S, const double* src, R* M, R* L, hipStream_t st) {
+  fmm_upward_pass<KIND, P, R>(S, src, M, st);
+  fmm_down_pass<KIND, P, R>(S, M, L, st);
 }
 
 inline P2PArgs make_p2p_args(System& S, const double* src, double damping, double* out) {
@@ -1504,24 +1520,31 @@ void fmm_near_pass(System& S, const double* mu, double damping, const R* L, doub
 
 // Far field then near field (fused L2P), or -- when tuned so and the near variant is a staged
 // one -- the far field on a second stream beside the near field, joined before an L2P pass.
-// The near-field kernels are enqueued before the far pass: the launch order carries into a
-// captured graph, and the near field then starts together with the (latency-bound) upward pass
-// instead of behind it.
+// Overlap 1: the whole far field beside the near field. The near-field kernels are enqueued
+// before the far pass: the launch order carries into a captured graph, and the near field then
+// starts together with the (latency-bound) upward pass instead of behind it. Overlap 2: the
+// upward pass first, alone on the system's stream, then only M2L / L2L beside the near field --
+// the near field's workgroups otherwise occupy the device while the upward pass's small launches
+// wait for slots, and M2L then ends well after the near field.
 template <int P, typename R>
 void fmm_apply_b_prec(System& S, const double* x, double* out, R* M, R* L) {
   FmmState& F      = *S.fmm;
   const int prec   = sizeof(R) == 4;
   const double dmp = S.opts.damping;
-  const bool ovl   = F.tune.overlap[prec] && F.tune.near_variant[prec] >= 5 && !S.prof && F.far_stream;
+  const int ovl    = F.tune.near_variant[prec] >= 5 && !S.prof && F.far_stream ? F.tune.overlap[prec] : 0;
   if (!ovl) {
     fmm_far_pass<KIND_M1, P, R>(S, x, M, L, S.stream);
     fmm_near_pass<P, R, MODE_BAPPLY>(S, x, dmp, L, out);
     return;
   }
+  if (ovl == 2) fmm_upward_pass<KIND_M1, P, R>(S, x, M, S.stream);
   CPPE_HIP_CHECK(hipEventRecord(F.ev_fork, S.stream));
   CPPE_HIP_CHECK(hipStreamWaitEvent(F.far_stream, F.ev_fork, 0));
   fmm_near_pass<P, R, MODE_BAPPLY>(S, x, dmp, L, out, false);
-  fmm_far_pass<KIND_M1, P, R>(S, x, M, L, F.far_stream);
+  if (ovl == 2)
+    fmm_down_pass<KIND_M1, P, R>(S, M, L, F.far_stream);
+  else
+    fmm_far_pass<KIND_M1, P, R>(S, x, M, L, F.far_stream);
   CPPE_HIP_CHECK(hipEventRecord(F.ev_join, F.far_stream));
   CPPE_HIP_CHECK(hipStreamWaitEvent(S.stream, F.ev_join, 0));
   const Octree& T = S.tree;
@@ -1557,12 +1580,19 @@ void fmm_apply_b_multi_prec(System& S, const double* x, double* out, R* M, R* L,
   const Octree& T  = S.tree;
   using O          = Ops<KIND_M1, P, R>;
   const size_t ms = fmm_mbuf_size(T.ncells, KIND_M1, P), ls = T.ncells * (size_t)O::LS, n3 = 3 * (size_t)S.n;
-  const bool ovl  = F.tune.overlap[prec] && !S.prof && F.far_stream;
+  const int ovl   = !S.prof && F.far_stream ? F.tune.overlap[prec] : 0;
   hipStream_t fst = ovl ? F.far_stream : S.stream;
-  auto far = [&] {
-    for (int k = 0; k < K; ++k) fmm_far_pass<KIND_M1, P, R>(S, x + k * n3, M + k * ms, L + k * ls, fst);
+  auto far = [&] {  // (overlap 2: the upward passes already ran)
+    for (int k = 0; k < K; ++k) {
+      if (ovl == 2)
+        fmm_down_pass<KIND_M1, P, R>(S, M + k * ms, L + k * ls, fst);
+      else
+        fmm_far_pass<KIND_M1, P, R>(S, x + k * n3, M + k * ms, L + k * ls, fst);
+    }
   };
   if (ovl) {
+    if (ovl == 2)
+      for (int k = 0; k < K; ++k) fmm_upward_pass<KIND_M1, P, R>(S, x + k * n3, M + k * ms, S.stream);
     CPPE_HIP_CHECK(hipEventRecord(F.ev_fork, S.stream));
     CPPE_HIP_CHECK(hipStreamWaitEvent(F.far_stream, F.ev_fork, 0));
   } else {

@@ -193,11 +193,13 @@ int cppe_hip_last_solve_info(cppe_hip_system* sys, int* info);
  *      expanded near list staged 64 sources at a time through wavefront-private LDS, lanes =
  *      (target, source slot), block 64/128/256.
  * Then, for a staged near variant (5-8), the whole apply is timed with the far field (P2M, M2M,
- * M2L, L2L) in series and on a second stream beside the near field (joined before an L2P pass),
- * and the faster is kept.
- * choice[8]: (near variant, near block, m2l group, overlap 0/1) for FP64 then FP32.
+ * M2L, L2L) in series, on a second stream beside the near field (joined before an L2P pass), and
+ * with only M2L / L2L beside the near field after the upward pass (P2M, M2M) has run; the fastest
+ * is kept.
+ * choice[8]: (near variant, near block, m2l group, overlap 0/1/2) for FP64 then FP32; overlap 0 =
+ *            series, 1 = whole far field beside the near field, 2 = upward pass first.
  * ms[60]: per precision, the 25 near timings (order above), the 3 M2L timings (ms/launch), then
- *         the apply in series and overlapped (0 when not timed).
+ *         the apply in series and overlapped (the faster overlapped form; 0 when not timed).
  * Launch shapes never change results beyond FP summation order.
  */
 int cppe_hip_autotune(cppe_hip_system* sys, int reps, int* choice, double* ms);

@@ -91,12 +91,13 @@ def test_multi_rejects_batches():
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("damping", [DAMP, 0.0])
-def test_apply_multi_matches_single_applies(precision, damping):
+@pytest.mark.parametrize("overlap", [1, 2])
+def test_apply_multi_matches_single_applies(precision, damping, overlap):
     """B applied to 3 and 4 vectors together equals one apply per vector (same launch shape)."""
     w = box(1500, 7304)
     S = capi.System(w.pos, w.pol, w.alpha_iso(), w.excl_csr(), summation="fmm", damping=damping,
                     precision=precision)
-    S.set_tune(np.array([8, 256, 16, 1, 8, 256, 16, 1], dtype=np.int32))
+    S.set_tune(np.array([8, 256, 16, overlap, 8, 256, 16, overlap], dtype=np.int32))
     X = np.random.default_rng(21).normal(size=(4, 3 * w.n))
     single = np.array([S.apply_b(x) for x in X])
     tol = 1e-13 if precision == "fp64" else 1e-6

@@ -1,6 +1,6 @@
 """GPU checks of the launch-shape autotuner and of every launch shape it can pick.
 
-Each near-field variant x block size, each M2L group width and the far/near overlap must give the
+Each near-field variant x block size, each M2L group width and each far/near overlap form must give the
 same B apply as the default shape (FP64: 1e-13 relative, summation order only; FP32: 1e-6 relative), and a solve with
 the tuned shapes must stop on the FP64 residual with dipoles equal to the default's to 1e-6 (both stop at 1e-8).
 """
@@ -35,7 +35,7 @@ def test_every_launch_shape_same_apply(precision):
     ref = S.apply_b(x)
     tol = 1e-13 if precision == "fp64" else 1e-6
     prec = 0 if precision == "fp64" else 1
-    for (v, b), g, o in itertools.product(capi.System.NEAR_CANDIDATES, capi.System.M2L_CANDIDATES, (0, 1)):
+    for (v, b), g, o in itertools.product(capi.System.NEAR_CANDIDATES, capi.System.M2L_CANDIDATES, (0, 1, 2)):
         choice = np.array([0, 256, 64, 0, 0, 256, 64, 0], dtype=np.int32)
         choice[4 * prec:4 * prec + 4] = (v, b, g, o)
         say("shape", precision, v, b, g, o)
@@ -57,12 +57,33 @@ def test_autotune_then_solve():
     for prec in ("fp64", "fp32"):
         r = rep[prec]
         assert (r["near_variant"], r["near_block"]) in capi.System.NEAR_CANDIDATES
-        assert r["m2l_group"] in capi.System.M2L_CANDIDATES and r["overlap"] in (0, 1)
+        assert r["m2l_group"] in capi.System.M2L_CANDIDATES and r["overlap"] in (0, 1, 2)
         assert all(t > 0 for t in r["near_ms"].values()) and all(t > 0 for t in r["m2l_ms"].values())
     it, norms, _ = S.solve_resident(0, 1e-8, 100)
     assert norms[-1] < 1e-8
     mu = S.download_mu()
     assert np.abs(mu - mu0).max() < 1e-6 * np.abs(mu0).max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_with_each_overlap_form(precision):
+    """Graph-captured solves with the far field beside the near field (overlap 1) and with the
+    upward pass first (overlap 2) stop on the FP64 residual with the series form's dipoles."""
+    w, S = system(precision)
+    Fn = capi.nuclear_fields(w.pos, w.atoms_z.astype(float), w.atoms_pos)
+    F = Fn + capi.System(w.pos, excl=w.excl_csr(), summation="fmm").multipole_fields(w.charges, w.dipoles,
+                                                                                        w.quadrupoles)
+    S.upload_rhs(F)
+    S.set_tune(np.array([8, 256, 16, 0, 8, 256, 16, 0], dtype=np.int32))
+    S.solve_resident(0, 1e-8, 100)
+    mu0 = S.download_mu()
+    for o in (1, 2):
+        say("overlap", o, precision)
+        S.set_tune(np.array([8, 256, 16, o, 8, 256, 16, o], dtype=np.int32))
+        it, norms, _ = S.solve_resident(0, 1e-8, 100)
+        assert norms[-1] < 1e-8
+        mu = S.download_mu()
+        assert np.abs(mu - mu0).max() < 1e-6 * np.abs(mu0).max(), o
 
 
 def test_set_tune_rejects_unknown_shapes():
@@ -72,7 +93,7 @@ def test_set_tune_rejects_unknown_shapes():
     with pytest.raises(capi.CppeHipError):
         S.set_tune([0, 96, 64, 0, 0, 256, 64, 0])
     with pytest.raises(capi.CppeHipError):
-        S.set_tune([5, 256, 64, 2, 0, 256, 64, 0])
+        S.set_tune([5, 256, 64, 3, 0, 256, 64, 0])
 
 
 def test_fp32_staged_with_undamped_sites():
