@@ -1,6 +1,6 @@
 """GPU parity of the direct path and the octree, through the C ABI, against the CPU oracle.
 
-Bars: FP64 results within 1e-10 relative of the oracle, tree cells
+Bars: FP64 results within 1e-10 relative of the oracle (the plain form's parity tolerance), tree cells
 and interaction lists identical as sets, FP32 applies within 1e-5 relative.
 """
 import functools
