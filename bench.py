@@ -41,6 +41,7 @@ FLOP_M2L_M1_P5 = 950
 # FETCH_SIZE and WRITE_SIZE runs calibrated on a copy kernel (profiles/near_field_300k_hbm_traffic.md);
 # other shapes: not measured (null)
 MEASURED_TRAFFIC_300K = {("fp32", 8, 256): 941.3e6, ("fp64", 8, 256): 1542.7e6,
+                         ("fp32", 8, 128): 1184.0e6, ("fp64", 8, 128): 1381.2e6,
                          ("fp32", 7, 128): 1292.6e6, ("fp64", 7, 128): 1465.6e6}
 
 WORKLOADS = {
