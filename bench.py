@@ -268,15 +268,47 @@ def main():
     info32 = S32.last_solve_info()
     log("headline solve ms", wall32, "iters", its32, "ml", ml_used, "fp64 true-residual applies",
         info32["reliable_updates"])
-    # the same mixed solve from alpha F, for the ML gain; then once more launched kernel by kernel
+    # the same mixed solve from alpha F and from zero, for the ML gain; then once more launched
+    # kernel by kernel
     wallm, devm, itsm = time_solves(S32, max(1, args.steps // 2), 0, 0)
+    wallz, devz, itsz = time_solves(S32, max(1, args.steps // 2), 0, 2)
     S32.set_graphs(False)
     walln, _, _ = time_solves(S32, max(1, args.steps // 2), 1, 0)
     S32.set_graphs(True)
     mu64 = S64.download_mu()
+    S32.solve_resident(0, 1e-8, 200)  # the headline form's moments, from alpha F
     mu32 = S32.download_mu()
     e64 = -0.5 * float(mu64 @ rhs)
     e32 = -0.5 * float(mu32 @ rhs)
+    # the guesses' own dipole error against the FP64 solution
+    nrm = float(np.linalg.norm(mu64))
+    guess_err = {"alphaF": float(np.linalg.norm(S32.apply_alpha(rhs) - mu64)) / nrm, "zero": 1.0}
+    if ml_used:
+        capi.ml_guess(S32, model)
+        guess_err["ml"] = float(np.linalg.norm(S32.download_mu() - mu64)) / nrm
+    log("guess dipole error vs FP64 solution", guess_err)
+
+    # first-call cost of a new system (what CppeState construction and its first
+    # update_induced_moments pay): octree + lists, the lazily built near list and M2L records on the
+    # first apply, the learned guess's 6 A neighbour list on its first use; default launch shapes
+    first = None
+    if args.summation == "fmm":
+        t0 = time.perf_counter()
+        Sf = capi.System(w.pos, w.pol, w.alpha_iso(), w.excl_csr(), precision="mixed", **common)
+        t_create = (time.perf_counter() - t0) * 1e3
+        Sf.upload_rhs(rhs)
+        first = {"system_create_ms": round(t_create, 1)}
+        if ml_used:
+            t0 = time.perf_counter()
+            capi.ml_guess(Sf, model)
+            first["first_ml_guess_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        t0 = time.perf_counter()
+        itf, _, _ = Sf.solve_resident(1 if ml_used else 0, 1e-8, 200)
+        first["first_solve_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        first["first_call_total_ms"] = round(sum(first.values()), 1)
+        first["autotune_s"] = tune["seconds"] if tune else None
+        Sf.close()
+        log("first call", first)
 
     acc = None
     if args.summation == "fmm" and not args.no_accuracy:
@@ -386,6 +418,10 @@ def main():
         "mixed_alphaF_ms": round(wallm, 3),
         "mixed_alphaF_iterations": itsm,
         "mixed_alphaF_no_graphs_ms": round(walln, 3),
+        "mixed_zero_guess_ms": round(wallz, 3),
+        "mixed_zero_guess_iterations": itsz,
+        "guess_dipole_rel_err": guess_err,
+        "first_call": first,
         "accuracy": acc,
         "epol_fp64": e64,
         "epol_rel_diff": abs(e32 - e64) / abs(e64),

@@ -21,7 +21,8 @@ EXPORTS = [
     "cppe_hip_solve_resident", "cppe_hip_download_mu", "cppe_hip_nuclear_fields",
     "cppe_hip_nuclear_multipole_energy", "cppe_hip_dot", "cppe_hip_tree_stats", "cppe_hip_tree_export",
     "cppe_hip_profile_apply", "cppe_hip_batch_create", "cppe_hip_segment_results", "cppe_hip_set_graphs",
-    "cppe_hip_autotune", "cppe_hip_set_tune", "cppe_hip_set_reliable_drop", "cppe_hip_last_solve_info",
+    "cppe_hip_autotune", "cppe_hip_set_tune", "cppe_hip_set_reliable_drop", "cppe_hip_set_refinement",
+    "cppe_hip_last_solve_info",
     "cppe_hip_field_targets", "cppe_hip_nuclear_field_gradient", "cppe_hip_nuclear_multipole_gradient",
     "cppe_hip_solve_multi", "cppe_hip_apply_b_multi",
     "cppe_hip_mlp_create", "cppe_hip_mlp_destroy", "cppe_hip_mlp_set", "cppe_hip_mlp_get", "cppe_hip_ml_features",
@@ -89,6 +90,7 @@ def lib():
         L.cppe_hip_autotune.argtypes = [vp, ctypes.c_int, _i, _d]
         L.cppe_hip_set_tune.argtypes = [vp, _i]
         L.cppe_hip_set_reliable_drop.argtypes = [vp, ctypes.c_double]
+        L.cppe_hip_set_refinement.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double]
         L.cppe_hip_last_solve_info.argtypes = [vp, _i]
         L.cppe_hip_field_targets.argtypes = [vp, _d, ctypes.c_int, _i, _d]
         _f = ctypes.POINTER(ctypes.c_float)
@@ -277,10 +279,14 @@ class System:
     def set_reliable_drop(self, drop):
         _check(lib().cppe_hip_set_reliable_drop(self.h, float(drop)))
 
+    def set_refinement(self, drop=1e-5, stall=10, min_progress=0.5):
+        """Mixed form: reliable-update drop, stall steps and the stagnation test (cppe_hip_set_refinement)."""
+        _check(lib().cppe_hip_set_refinement(self.h, float(drop), int(stall), float(min_progress)))
+
     def last_solve_info(self):
-        info = np.zeros(2, dtype=np.int32)
+        info = np.zeros(3, dtype=np.int32)
         _check(lib().cppe_hip_last_solve_info(self.h, _p(info, _i)))
-        return dict(iterations=int(info[0]), reliable_updates=int(info[1]))
+        return dict(iterations=int(info[0]), reliable_updates=int(info[1]), fp64_fallback=int(info[2]))
 
     def set_graphs(self, enable=True):
         _check(lib().cppe_hip_set_graphs(self.h, int(bool(enable))))

@@ -692,11 +692,23 @@ int cppe_hip_set_reliable_drop(cppe_hip_system* sys, double drop) {
   CPPE_API_END
 }
 
+int cppe_hip_set_refinement(cppe_hip_system* sys, double drop, int stall, double min_progress) {
+  CPPE_API_BEGIN
+  if (!sys || !(drop > 0.0 && drop < 1.0) || stall < 1 || !(min_progress >= 0.0))
+    return fail(CPPE_HIP_ERR_ARG, "refinement: drop in (0, 1), stall >= 1, min_progress >= 0");
+  sys->S.reliable_drop       = drop;
+  sys->S.refine_stall        = stall;
+  sys->S.refine_min_progress = min_progress;
+  return CPPE_HIP_OK;
+  CPPE_API_END
+}
+
 int cppe_hip_last_solve_info(cppe_hip_system* sys, int* info) {
   CPPE_API_BEGIN
   if (!sys || !info) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
   info[0] = sys->S.last.loop_iters;
   info[1] = sys->S.last.reliable_updates;
+  info[2] = sys->S.last.fp64_fallback;
   return CPPE_HIP_OK;
   CPPE_API_END
 }

@@ -194,6 +194,7 @@ __global__ __launch_bounds__(256) void k_mlp_tile(MlpTileArgs a) {
     }
     if (tid < kMlpOut)
       for (int s = 0; s < TS; ++s) gb3 += dys[s * 4 + tid];
+    __syncthreads();  // every dW3 read of Z2 precedes its overwrite with dZ2 below
     for (int e = tid; e < TS * kMlpHid; e += 256) {
       const int s = e / kMlpHid, h = e % kMlpHid;
       const float dh = dys[s * 4] * a.w[kOffW3 + h * 3] + dys[s * 4 + 1] * a.w[kOffW3 + h * 3 + 1] +

@@ -29,6 +29,7 @@
 #include <functional>
 #include <vector>
 
+#include "cg_policy.hh"
 #include "system.hh"
 
 namespace cppe_dev {
@@ -38,18 +39,13 @@ void fmm_reserve(System& S);
 
 constexpr int kChunk = 256;  // one site per thread: enough blocks in flight to stream the vectors
 
-// per-segment doubles (SoA, each nseg long) and ints
-// D_RZN: the new r.z of an F_RNB launch, kept between its two passes
-enum SegD { D_RZ = 0, D_BETA, D_RNORM, D_RLAST, D_RZN, D_COUNT };
-// I_PENDING (mixed form): recursive residual below the threshold, frozen until an FP64 true
-// residual confirms convergence or resumes the segment
-// I_ELIGIBLE (mixed form): residual within 100x of its replacement trigger; replaced whenever a
-// replacement runs for another segment, so that segments share FP64 applies
-enum SegI { I_ACTIVE = 0, I_NEEDREL, I_ITERS, I_BAD, I_PENDING, I_ELIGIBLE, I_COUNT };
 // status block: [iteration, active segments, segments asking for a reliable update, bad,
-// step taken in the current iteration, segments pending an FP64 confirmation, number of final
-// kernels run (each final kernel publishes the block to a host-mapped mirror, this count last)]
-enum Status { ST_IT = 0, ST_ACTIVE, ST_REL, ST_BAD, ST_STEPPED, ST_PEND, ST_SEQ, ST_COUNT };
+// step taken in the current iteration, segments pending an FP64 confirmation, stagnating segments
+// (mixed form), number of final kernels run (each final kernel publishes the block to a
+// host-mapped mirror, this count last)]
+enum Status { ST_IT = 0, ST_ACTIVE, ST_REL, ST_BAD, ST_STEPPED, ST_PEND, ST_STAG, ST_SEQ, ST_COUNT };
+// solve parameters on the device: [thresh, reliable drop, min progress, stall steps, restart]
+enum Param { P_THRESH = 0, P_DROP, P_PROGRESS, P_STALL, P_RESTART, P_COUNT };
 // F_RNB: the residual norm and its tests, then -- unless a segment asked for a reliable update --
 // F_BETA's update, in one launch
 enum FinalMode { F_INIT = 0, F_REL, F_BETA, F_RNB };
@@ -326,27 +322,32 @@ struct CgFinal {
   int* si;
   int* status;           // ST_COUNT ints
   int* hstat;            // host-mapped mirror of the status block
-  const double* params;  // [thresh, reliable drop]
+  const double* params;  // P_COUNT values
   double* hist;          // residual norms, iteration-major [it * nseg + g]
   int hist_cap;          // iterations held by hist
 };
 
 // Per-segment finalisation of the chunk partials: one wavefront per segment (the block's four
 // wavefronts take segments in turn); each lane sums its chunks (lane_chunk_sum), then a fixed
-// shuffle tree combines the lanes -- deterministic for a given chunk split. Lane 0 then updates
-// the segment's scalars; the status block is refreshed at the end.
+// shuffle tree combines the lanes -- deterministic for a given chunk split. Lane 0 then applies
+// the segment's decision (cg_policy.hh); the status block is refreshed at the end.
+// F_INIT with params[P_RESTART] set: the plain-form restart of unfinished segments after the
+// mixed form stagnated (iteration count and history kept).
 template <int MODE, bool MIXED>
 __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
-  __shared__ int cnt[4];
+  __shared__ int cnt[5];
   // F_BETA while a reliable update is pending: no update, only the publication below
   const bool skip = MODE == F_BETA && f.status[ST_REL] > 0;
-  if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+  if (threadIdx.x < 5) cnt[threadIdx.x] = 0;
   __syncthreads();
   const int ns = skip ? 0 : f.nseg;
-  const int it = f.status[ST_IT];
-  const double thresh = f.params[0], drop = f.params[1];
+  const bool restart = MODE == F_INIT && f.params[P_RESTART] != 0.0;
+  // iterations so far; a restart also counts the step taken before the stagnating replacement
+  const int it = f.status[ST_IT] + (restart ? f.status[ST_STEPPED] : 0);
+  const Refine pr{f.params[P_THRESH], f.params[P_DROP], f.params[P_PROGRESS], (int)f.params[P_STALL]};
+  const Hist h{f.hist, f.hist_cap};
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  int na = 0, nr = 0, nb = 0, np = 0;
+  int na = 0, nr = 0, nb = 0, np = 0, nst = 0;
   for (int g = wv; g < ns; g += 4) {
     double s1 = lane_chunk_sum(f.part, f.seg_chunk[g], f.seg_chunk[g + 1], lane), s2 = 0.0;
     if (MODE == F_INIT || MODE == F_RNB) s2 = lane_chunk_sum(f.part + f.nchunk, f.seg_chunk[g], f.seg_chunk[g + 1], lane);
@@ -356,119 +357,46 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
       if (MODE == F_INIT || MODE == F_RNB) s2 += __shfl_xor(s2, off);
     }
     if (lane != 0) continue;
-    if (MODE == F_RNB) f.sd[D_RZN * ns + g] = s2;  // read back by this thread after the barrier
-    double* sd = f.sd;
-    int* si    = f.si;
+    const SegRef sg{f.sd, f.si, ns, g};
     if (MODE == F_INIT) {
-      const double rn = sqrt(s2);
-      sd[D_RZ * ns + g]    = s1;
-      sd[D_RNORM * ns + g] = rn;
-      sd[D_RLAST * ns + g] = rn;
-      sd[D_BETA * ns + g]  = 0.0;
-      const bool bad = !isfinite(rn);
-      si[I_BAD * ns + g]      = bad;
-      si[I_ITERS * ns + g]    = 0;
-      si[I_PENDING * ns + g]  = 0;
-      si[I_ELIGIBLE * ns + g] = 0;
-      if (MIXED) {
-        // an FP32 residual within one reliable-update drop of the threshold is replaced by the FP64
-        // true residual before any step (a guess that already converges stops at 0 iterations)
-        si[I_ACTIVE * ns + g]  = !bad;
-        si[I_NEEDREL * ns + g] = !bad && rn < thresh / drop;
-      } else {
-        // the reference recurrence takes one step before its first test (electric_fields.cc:213-229),
-        // also from a guess that already meets the threshold; only an exactly zero residual (where
-        // that step would divide 0 by 0) stops at 0 iterations
-        si[I_ACTIVE * ns + g]  = !bad && rn > 0.0;
-        si[I_NEEDREL * ns + g] = 0;
-      }
+      if (restart)
+        seg_restart(sg, s1, s2, it, pr);
+      else
+        seg_init(sg, s1, s2, MIXED, pr);
     } else if (MODE == F_RNB) {
-      if (si[I_ACTIVE * ns + g]) {
-        const double rn      = sqrt(s1);
-        sd[D_RNORM * ns + g] = rn;
-        si[I_ELIGIBLE * ns + g] = MIXED && isfinite(rn) && rn < 100.0 * drop * sd[D_RLAST * ns + g];
-        if (!isfinite(rn)) {
-          si[I_BAD * ns + g]    = 1;
-          si[I_ACTIVE * ns + g] = 0;
-          si[I_ITERS * ns + g]  = it + 1;
-          if (it < f.hist_cap) f.hist[(size_t)it * ns + g] = rn;
-        } else if (MIXED && rn < thresh) {
-          // frozen until an FP64 true residual confirms it; confirmations are batched
-          si[I_PENDING * ns + g] = 1;
-          si[I_ACTIVE * ns + g]  = 0;
-          si[I_ITERS * ns + g]   = it + 1;
-        } else if (MIXED && rn < drop * sd[D_RLAST * ns + g]) {
-          si[I_NEEDREL * ns + g] = 1;  // recorded after the replacement
-        } else {
-          if (it < f.hist_cap) f.hist[(size_t)it * ns + g] = rn;
-          if (rn < thresh) {
-            si[I_ACTIVE * ns + g] = 0;
-            si[I_ITERS * ns + g]  = it + 1;
-          }
-        }
-      }
+      sg.d(D_RZN) = s2;  // read back by this thread after the barrier
+      seg_step(sg, s1, it, MIXED, pr, h);
     } else if (MODE == F_REL) {
-      if (si[I_NEEDREL * ns + g] || (si[I_ELIGIBLE * ns + g] && si[I_ACTIVE * ns + g])) {
-        si[I_ELIGIBLE * ns + g] = 0;
-        const double rn        = sqrt(s1);
-        sd[D_RNORM * ns + g]   = rn;
-        sd[D_RLAST * ns + g]   = rn;
-        si[I_NEEDREL * ns + g] = 0;
-        if (it < f.hist_cap) f.hist[(size_t)it * ns + g] = rn;
-        if (!isfinite(rn)) {
-          si[I_BAD * ns + g]    = 1;
-          si[I_ACTIVE * ns + g] = 0;
-          si[I_ITERS * ns + g]  = it + 1;
-        } else if (rn < thresh) {
-          si[I_ACTIVE * ns + g] = 0;
-          si[I_ITERS * ns + g]  = it + f.status[ST_STEPPED];  // 0 when confirmed before any step
-        }
-      } else if (si[I_PENDING * ns + g]) {
-        const double rn        = sqrt(s1);
-        const int k0           = si[I_ITERS * ns + g] - 1;  // iteration at which it froze
-        sd[D_RNORM * ns + g]   = rn;
-        sd[D_RLAST * ns + g]   = rn;
-        si[I_PENDING * ns + g] = 0;
-        if (k0 >= 0 && k0 < f.hist_cap) f.hist[(size_t)k0 * ns + g] = rn;
-        if (!isfinite(rn)) {
-          si[I_BAD * ns + g] = 1;
-        } else if (!(rn < thresh)) {
-          si[I_ACTIVE * ns + g] = 1;  // resumes from the true residual
-        }
-      }
+      seg_true_residual(sg, s1, it, f.status[ST_STEPPED], pr, h);
     } else {  // F_BETA
-      if (si[I_ACTIVE * ns + g]) {
-        sd[D_BETA * ns + g] = s1 / sd[D_RZ * ns + g];
-        sd[D_RZ * ns + g]   = s1;
-      }
+      seg_beta(sg, s1);
     }
-    na += si[I_ACTIVE * ns + g];
-    nr += si[I_NEEDREL * ns + g];
-    nb += si[I_BAD * ns + g];
-    np += si[I_PENDING * ns + g];
+    na += sg.i(I_ACTIVE);
+    nr += sg.i(I_NEEDREL);
+    nb += sg.i(I_BAD);
+    np += sg.i(I_PENDING);
+    nst += sg.i(I_STAG);
   }
   if (na) atomicAdd(&cnt[0], na);
   if (nr) atomicAdd(&cnt[1], nr);
   if (nb) atomicAdd(&cnt[2], nb);
   if (np) atomicAdd(&cnt[3], np);
+  if (nst) atomicAdd(&cnt[4], nst);
   __syncthreads();
   const bool beta_now = MODE == F_RNB && cnt[1] == 0;  // no reliable update pending: direction now
-  if (beta_now && lane == 0) {
+  if (beta_now && lane == 0)
     for (int g = wv; g < ns; g += 4) {
-      if (f.si[I_ACTIVE * ns + g]) {
-        const double s2       = f.sd[D_RZN * ns + g];
-        f.sd[D_BETA * ns + g] = s2 / f.sd[D_RZ * ns + g];
-        f.sd[D_RZ * ns + g]   = s2;
-      }
+      const SegRef sg{f.sd, f.si, ns, g};
+      seg_beta(sg, sg.d(D_RZN));
     }
-  }
   if (threadIdx.x == 0 && !skip) {
     f.status[ST_ACTIVE] = cnt[0];
     f.status[ST_REL]    = cnt[1];
     f.status[ST_BAD]    = cnt[2];
     f.status[ST_PEND]   = cnt[3];
+    f.status[ST_STAG]   = cnt[4];
     if (MODE == F_INIT) {
-      f.status[ST_IT]      = 0;
+      f.status[ST_IT]      = restart ? it : 0;
       f.status[ST_STEPPED] = 0;
     }
     if (MODE == F_RNB && !beta_now) f.status[ST_STEPPED] = 1;
@@ -532,7 +460,7 @@ static void cg_layout(System& S, CgState& C, const std::vector<int>& seg_off) {
   C.sd.alloc(D_COUNT * (size_t)ns);
   C.si.alloc(I_COUNT * (size_t)ns);
   C.status.alloc(ST_COUNT);
-  C.params.alloc(2);
+  C.params.alloc(P_COUNT);
   C.hist_cap = 0;
   C.hist.release();
   if (!C.h_status) {
@@ -764,7 +692,12 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
     else
       C.rec64 = r64;
   }
-  const double params[2] = {thresh, S.reliable_drop};
+  double params[P_COUNT];
+  params[P_THRESH]   = thresh;
+  params[P_DROP]     = S.reliable_drop;
+  params[P_PROGRESS] = S.refine_min_progress;
+  params[P_STALL]    = S.refine_stall;
+  params[P_RESTART]  = 0.0;
   CPPE_HIP_CHECK(hipMemcpyAsync(C.params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
   // r0 = rhs - B x0, z0 = alpha r0, p = z0. The mixed form takes r0 from an FP32 apply: it is
   // replaced by an FP64 true residual at the first reliable update, before any convergence test.
@@ -774,7 +707,8 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
   ++C.expect;  // every launch sequence below runs exactly one final kernel
   read_status(S, C);
   int k = 0, nrel = 0;
-  const int m = mixed ? 1 : 0;
+  int m = mixed ? 1 : 0;
+  res.fp64_fallback = -1;
   // Loop invariant at the top: every active segment has completed step, replacement and
   // direction of its last iteration. Pending segments (mixed form) are confirmed together: inside a
   // replacement that runs anyway, or once no segment is active.
@@ -784,11 +718,41 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
     read_status(S, C);
     ++nrel;
   };
-  if (C.h_status[ST_REL] > 0) reliable();
+  // A stagnating refinement (a true residual that did not fall below min_progress times the
+  // previous one) continues in the plain form: every unfinished segment restarts from its FP64
+  // true residual, which the replacement apply has just left in t (beta = 0), and every later
+  // apply runs in FP64. Returns whether it happened.
+  auto fallback = [&]() -> bool {
+    if (!mixed || C.h_status[ST_STAG] == 0) return false;
+    mixed = false;
+    m     = 0;
+    if (C.nrhs == 1) {
+      float4* r32;
+      double4* r64;
+      fmm_dipole_records(S, &r32, &r64);
+      C.rec32 = nullptr;
+      C.rec64 = r64;
+    }
+    params[P_RESTART] = 1.0;
+    CPPE_HIP_CHECK(hipMemcpyAsync(C.params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_sg_init, dim3(C.nchunk), dim3(256), 0, st, vecs(S, C));
+    launch_final<F_INIT>(S, C, false);
+    params[P_RESTART] = 0.0;
+    CPPE_HIP_CHECK(hipMemcpyAsync(C.params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
+    ++C.expect;
+    read_status(S, C);
+    res.fp64_fallback = k;
+    return true;
+  };
+  if (C.h_status[ST_REL] > 0) {
+    reliable();
+    fallback();
+  }
   while (k < maxiter) {
     if (C.h_status[ST_ACTIVE] == 0) {
       if (C.h_status[ST_PEND] == 0) break;
       reliable();
+      if (fallback()) continue;
       if (C.h_status[ST_ACTIVE] == 0) break;
       run_graph(S, C.g_dir[m], [&] { seq_direction(S, C, mixed); });
       ++C.expect;
@@ -803,13 +767,14 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
     // no-op on the vectors; segments that resume then take theirs after the confirmation.
     if (C.h_status[ST_REL] > 0 || (C.h_status[ST_ACTIVE] == 0 && C.h_status[ST_PEND] > 0)) {
       reliable();
+      if (fallback()) continue;
       if (C.h_status[ST_ACTIVE] > 0) {
         run_graph(S, C.g_dir[m], [&] { seq_direction(S, C, mixed); });
         ++C.expect;
       }
     }
   }
-  if (C.h_status[ST_PEND] > 0) reliable();  // maxiter reached with unconfirmed segments
+  if (mixed && C.h_status[ST_PEND] > 0) reliable();  // maxiter reached with unconfirmed segments
   // per-segment results
   std::vector<int> si((size_t)I_COUNT * ns);
   std::vector<double> sd((size_t)D_COUNT * ns);

@@ -73,6 +73,7 @@ struct CgResult {
   std::vector<std::vector<double>> norms;  // residual norms per iteration, every segment
   int loop_iters = 0;          // iterations of the shared loop (max over segments)
   int reliable_updates = 0;    // FP64 true-residual applies of the mixed form
+  int fp64_fallback = -1;      // mixed form: iteration at which it continued in FP64 (-1: never)
   bool all_converged = false;
 };
 
@@ -142,6 +143,9 @@ struct System {
   CgState cg, cgm;
   bool use_graphs = true;
   double reliable_drop = 1e-5;  // mixed form: residual drop that triggers an FP64 true residual
+  int refine_stall = 10;        // mixed form: steps without halving that trigger one as well
+  double refine_min_progress = 0.5;  // mixed form: a true residual above this fraction of the
+                                     // previous one continues the solve in FP64
   CgResult last;            // outcome of the last solve
   dbuf<double> io_a, io_b;  // AoS3 staging in caller order
   dbuf<float> f32_src;      // FP32 staging for mixed-precision applies

@@ -46,8 +46,9 @@ void CppeState::set_potentials(std::vector<Potential> potentials) {  // cppe_sta
       ++k;
     }
   }
+  // the response operator belongs to the old sites; m_make_guess is left as it was (the
+  // reference's set_potentials zeroes the moments only, so a later solve starts from zero)
   m_solver.reset();
-  m_make_guess = true;
 }
 
 void CppeState::calculate_static_energies_and_fields() {  // cppe_state.cc:57-71

@@ -173,7 +173,18 @@ int cppe_hip_set_graphs(cppe_hip_system* sys, int enable);
 /* Mixed form: factor by which a residual must fall before the FP64 true residual replaces the
  * recursive one (default 1e-5; convergence is always declared on an FP64 true residual). */
 int cppe_hip_set_reliable_drop(cppe_hip_system* sys, double drop);
-/* Last solve: info[0] iterations of the shared loop, info[1] FP64 true-residual applies. */
+/*
+ * Mixed form, all knobs of the refinement: `drop` as above; `stall`: a recursive residual that
+ * has not halved for this many steps also asks for an FP64 true residual (default 10);
+ * `min_progress`: an FP64 true residual above this fraction of the previous one (after at least
+ * one step) means the FP32 refinement stagnates, and the solve continues from the current moments
+ * in the plain FP64 form (default 0.5). "Failed to converge" is returned only when that, too,
+ * does not converge within maxiter. No counterpart in the reference (whose solve is FP64 only,
+ * electric_fields.cc:191-239).
+ */
+int cppe_hip_set_refinement(cppe_hip_system* sys, double drop, int stall, double min_progress);
+/* Last solve: info[0] iterations of the shared loop, info[1] FP64 true-residual applies,
+ * info[2] iteration at which a stagnating mixed solve continued in FP64 (-1: it did not). */
 int cppe_hip_last_solve_info(cppe_hip_system* sys, int* info);
 
 /*

@@ -215,6 +215,30 @@ def test_printer_lines():
 
 
 @pytest.mark.gpu
+def test_set_potentials_keeps_guess_mode():
+    """The reference's set_potentials (cppe_state.cc:34-55) zeroes the moments and leaves the
+    first-guess flag alone: after one SCF update, a new set of potentials is solved from zero."""
+    lines = []
+    opts = {"potfile": POT, "damp_induced": True}
+    s = cppe.CppeState(opts, pna_molecule(), lines.append)
+    s.calculate_static_energies_and_fields()
+    zeros = np.zeros_like(s.static_fields)
+    s.update_induced_moments(zeros, False)
+    pots = s.potentials
+    s.set_potentials(pots)
+    np.testing.assert_array_equal(s.get_induced_moments(), 0.0)
+    del lines[:]
+    s.update_induced_moments(zeros, False)
+    from_zero, from_alpha_f = [], []
+    for make_guess, out in ((False, from_zero), (True, from_alpha_f)):
+        im = cppe.InducedMoments(pots, opts)
+        im.set_print_callback(out.append)
+        im.compute(s.static_fields, make_guess)
+    assert lines == from_zero
+    assert lines != from_alpha_f
+
+
+@pytest.mark.gpu
 def test_polarization_energies():
     s = solve_induced_moments({"potfile": POT, "damp_induced": True}, pna_molecule())
     mu = s.get_induced_moments()

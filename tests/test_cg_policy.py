@@ -1,0 +1,184 @@
+"""CPU tests of the device CG's per-segment decisions (cppe_amd/csrc/device/cg_policy.hh), built for
+the host by tests/cg_policy_host_driver.cc.
+
+Plain form: the reference recurrence's stopping rule (electric_fields.cc:213-229: one step before
+the first test, stop when ||r|| < thresh). Mixed form: reliable updates on a residual drop, on a
+stalled recursive residual, confirmation of a residual below the threshold by an FP64 true
+residual, and the stagnation test that sends the solve on in the plain FP64 form.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DEV = os.path.join(ROOT, "cppe_amd", "csrc", "device")
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int)
+
+# indices of the per-segment tables (cg_policy.hh SegD / SegI)
+D_RZ, D_BETA, D_RNORM, D_RLAST, D_RZN, D_RBEST = range(6)
+I_ACTIVE, I_NEEDREL, I_ITERS, I_BAD, I_PENDING, I_ELIGIBLE, I_SINCE, I_STEPS, I_STAG = range(9)
+THRESH = 1e-8
+
+
+@pytest.fixture(scope="module")
+def pol(tmp_path_factory):
+    out = tmp_path_factory.mktemp("cgpolicy") / "libcgpolicy.so"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", DEV,
+                    os.path.join(ROOT, "tests", "cg_policy_host_driver.cc"), "-o", str(out)], check=True)
+    L = ctypes.CDLL(str(out))
+    dd, ii = ctypes.c_double, ctypes.c_int
+    L.pol_sizes.argtypes = [_i, _i]
+    L.pol_init.argtypes = [_d, _i, dd, dd, ii, _d]
+    L.pol_restart.argtypes = [_d, _i, dd, dd, ii, _d]
+    L.pol_step.argtypes = [_d, _i, dd, ii, ii, _d, _d, ii]
+    L.pol_true.argtypes = [_d, _i, dd, ii, ii, _d, _d, ii]
+    L.pol_beta.argtypes = [_d, _i, dd]
+    nd, ni = ctypes.c_int(), ctypes.c_int()
+    L.pol_sizes(ctypes.byref(nd), ctypes.byref(ni))
+    assert (nd.value, ni.value) == (6, 9)
+    return L
+
+
+class Seg:
+    def __init__(self, L, drop=1e-5, stall=10, min_progress=0.5, cap=64):
+        self.L = L
+        self.sd = np.zeros(6)
+        self.si = np.zeros(9, dtype=np.int32)
+        self.prm = np.array([THRESH, drop, min_progress, stall], dtype=np.float64)
+        self.hist = np.full(cap, np.nan)
+
+    def _a(self):
+        return self.sd.ctypes.data_as(_d), self.si.ctypes.data_as(_i)
+
+    def init(self, rn, mixed):
+        self.L.pol_init(*self._a(), 1.0, rn * rn, int(mixed), self.prm.ctypes.data_as(_d))
+
+    def restart(self, rn, it):
+        self.L.pol_restart(*self._a(), 1.0, rn * rn, it, self.prm.ctypes.data_as(_d))
+
+    def step(self, rn, it, mixed):
+        self.L.pol_step(*self._a(), rn * rn, it, int(mixed), self.prm.ctypes.data_as(_d),
+                        self.hist.ctypes.data_as(_d), self.hist.size)
+
+    def true(self, rn, it, stepped):
+        self.L.pol_true(*self._a(), rn * rn, it, stepped, self.prm.ctypes.data_as(_d),
+                        self.hist.ctypes.data_as(_d), self.hist.size)
+
+
+def test_plain_form_reference_stopping_rule(pol):
+    s = Seg(pol)
+    s.init(1e-9, mixed=False)  # below the threshold: still one step (electric_fields.cc:213-229)
+    assert s.si[I_ACTIVE] == 1 and s.si[I_NEEDREL] == 0
+    s.step(5e-10, 0, mixed=False)
+    assert s.si[I_ACTIVE] == 0 and s.si[I_ITERS] == 1 and s.hist[0] == 5e-10
+    z = Seg(pol)
+    z.init(0.0, mixed=False)  # an exactly zero residual stops at 0 iterations
+    assert z.si[I_ACTIVE] == 0
+    # a plain solve never asks for replacements, however flat its residual
+    p = Seg(pol, stall=2)
+    p.init(1.0, mixed=False)
+    for it in range(8):
+        p.step(0.9, it, mixed=False)
+        assert p.si[I_NEEDREL] == 0 and p.si[I_ACTIVE] == 1 and p.si[I_STAG] == 0
+    assert np.all(p.hist[:8] == 0.9)
+
+
+def test_mixed_form_drop_pending_confirmation(pol):
+    s = Seg(pol)
+    s.init(1.0, mixed=True)
+    assert s.si[I_ACTIVE] == 1 and s.si[I_NEEDREL] == 0
+    rn, it = 1.0, 0
+    while True:  # a healthy recursive residual, 2.5x per step
+        rn /= 2.5
+        s.step(rn, it, mixed=True)
+        if s.si[I_NEEDREL]:
+            break
+        it += 1
+    assert rn < 1e-5 <= rn * 2.5  # asked exactly when it fell by the drop
+    s.true(1.3e-5, it, 1)  # FP64 true residual: a little drift, real progress
+    assert s.si[I_STAG] == 0 and s.si[I_ACTIVE] == 1 and s.sd[D_RLAST] == 1.3e-5 and s.hist[it] == 1.3e-5
+    it += 1
+    rn = 1.3e-5
+    while s.si[I_ACTIVE]:
+        rn /= 2.5
+        s.step(rn, it, mixed=True)
+        it += 1
+    assert s.si[I_PENDING] == 1 and s.si[I_ITERS] == it and rn < THRESH
+    s.true(2e-8, it, 0)  # not confirmed: resumes from the true residual
+    assert s.si[I_ACTIVE] == 1 and s.si[I_PENDING] == 0 and s.si[I_STAG] == 0
+    assert s.hist[it - 1] == 2e-8
+    s.step(4e-9, it, mixed=True)
+    assert s.si[I_PENDING] == 1
+    s.true(5e-9, it + 1, 0)  # confirmed
+    assert s.si[I_ACTIVE] == 0 and s.si[I_PENDING] == 0 and s.si[I_ITERS] == it + 1
+
+
+def test_mixed_form_stalled_residual_asks_for_true_residual(pol):
+    s = Seg(pol, drop=1e-12, stall=10)
+    s.init(1.0, mixed=True)
+    # with so small a drop, the first FP32 residual lies within one drop of the threshold and is
+    # replaced before any step (no stagnation: no step taken)
+    assert s.si[I_NEEDREL] == 1
+    s.true(1.0, 0, 0)
+    assert s.si[I_NEEDREL] == 0 and s.si[I_STAG] == 0 and s.si[I_ACTIVE] == 1
+    for it, rn in enumerate([0.4, 0.15, 0.06, 0.02]):
+        s.step(rn, it, mixed=True)
+    assert s.si[I_NEEDREL] == 0 and s.si[I_SINCE] == 0
+    # levels off: no halving for `stall` steps
+    for k in range(10):
+        s.step(0.015 * (1 - 0.01 * k), 4 + k, mixed=True)
+        assert s.si[I_NEEDREL] == (1 if k == 9 else 0), k
+    s.true(0.016, 13, 1)  # the true residual made progress since the start: no stagnation
+    assert s.si[I_STAG] == 0 and s.si[I_SINCE] == 0 and s.si[I_STEPS] == 0
+
+
+def test_mixed_form_stagnation_then_plain_restart(pol):
+    s = Seg(pol, stall=3)
+    s.init(1.0, mixed=True)
+    s.step(0.4, 0, mixed=True)
+    s.step(0.35, 1, mixed=True)
+    s.step(0.3, 2, mixed=True)
+    s.step(0.3, 3, mixed=True)
+    assert s.si[I_NEEDREL] == 1
+    s.true(0.9, 3, 1)  # true residual barely below the last one: the refinement stagnates
+    assert s.si[I_STAG] == 1 and s.si[I_ACTIVE] == 1
+    s.restart(0.9, 4)
+    assert s.si[I_STAG] == 0 and s.si[I_ACTIVE] == 1 and s.sd[D_BETA] == 0.0 and s.sd[D_RLAST] == 0.9
+    # a finished segment is left alone by the restart
+    f = Seg(pol)
+    f.init(1.0, mixed=True)
+    f.si[I_ACTIVE] = 0
+    f.si[I_ITERS] = 7
+    f.restart(0.5, 12)
+    assert f.si[I_ACTIVE] == 0 and f.si[I_ITERS] == 7 and f.sd[D_RLAST] == 1.0
+    # a restart whose true residual already meets the threshold finishes at the current count
+    c = Seg(pol)
+    c.init(1.0, mixed=True)
+    c.restart(5e-9, 12)
+    assert c.si[I_ACTIVE] == 0 and c.si[I_ITERS] == 12
+
+
+def test_mixed_form_no_false_stagnation_before_a_step(pol):
+    # a guess whose FP32 residual is near the threshold is replaced before any step; the true
+    # residual equals the FP32 one, which is not stagnation
+    s = Seg(pol)
+    s.init(5e-5, mixed=True)
+    assert s.si[I_NEEDREL] == 1
+    s.true(5.0001e-5, 0, 0)
+    assert s.si[I_STAG] == 0 and s.si[I_ACTIVE] == 1
+    c = Seg(pol)
+    c.init(5e-9, mixed=True)
+    c.true(6e-9, 0, 0)  # confirmed before any step: 0 iterations
+    assert c.si[I_ACTIVE] == 0 and c.si[I_ITERS] == 0
+
+
+def test_beta(pol):
+    s = Seg(pol)
+    s.init(1.0, mixed=False)
+    s.sd[D_RZ] = 4.0
+    pol.pol_beta(s.sd.ctypes.data_as(_d), s.si.ctypes.data_as(_i), 1.0)
+    assert s.sd[D_BETA] == 0.25 and s.sd[D_RZ] == 1.0

@@ -131,6 +131,50 @@ def test_pair_field_vs_sympy(m, damped):
         np.testing.assert_allclose(F, Fs, rtol=1e-12, atol=1e-14 * np.abs(Fs).max())
 
 
+@pytest.mark.parametrize("m", [0, 1, 2])
+def test_pair_field_vs_reference_p2p(m):
+    """The oracle's pair fields equal the reference's generated P2P<m,3> kernels, undamped and
+    Thole-damped (fmm/field_m{0,1,2}.cc; a = damping (alpha_i alpha_j)^(-1/6) as
+    calculate.cc:60-65 passes it), on random displacements and damping factors."""
+    R = O.ref_kernels()
+    if R is None:
+        pytest.skip("oracle/_ref not built (reference checkout absent)")
+    rng = np.random.default_rng(40 + m)
+    dp = O._d
+    for k in range(64):
+        d = rng.normal(size=3) * rng.uniform(0.3, 8.0)
+        S = rng.normal(size=O.ncomp(m))
+        a = rng.uniform(0.1, 3.0)
+        for damped in (False, True):
+            F = np.zeros(3)
+            Sc = S.copy()
+            if damped:
+                R.ref_p2p_damped(m, *d, a, Sc.ctypes.data_as(dp), F.ctypes.data_as(dp))
+            else:
+                R.ref_p2p(m, *d, Sc.ctypes.data_as(dp), F.ctypes.data_as(dp))
+            mine = O.pair_field(m, d, S, a, damped)
+            np.testing.assert_allclose(mine, F, rtol=1e-12, atol=1e-13 * np.abs(F).max())
+        # the damped form differs from the undamped one at these separations (a live check)
+        if np.linalg.norm(d) * a < 2.0:
+            assert not np.allclose(O.pair_field(m, d, S, a, True), O.pair_field(m, d, S, a, False), rtol=1e-6)
+
+
+@pytest.mark.parametrize("m", [0, 1, 2])
+@pytest.mark.parametrize("damping", [0.0, 2.1304])
+def test_direct_field_targets_equals_full_sum_rows(m, damping):
+    """The sampled direct field (the checker of the device's sampled field) is the full direct
+    sum restricted to the targets, exclusions and damping included; the full sum is pinned by
+    the golden fields and the sympy pair fields above."""
+    pot = O.read_potfile(os.path.join(GOLDEN, "pna_6w.pot"))
+    rng = np.random.default_rng(60 + m)
+    S = rng.normal(size=(pot.n, O.ncomp(m)))
+    al = pot.alpha_iso() if damping else None
+    full = O.direct_field(m, pot.pos, S, al, pot.excl_csr(), damping).reshape(-1, 3)
+    tg = np.sort(rng.choice(pot.n, pot.n // 2, replace=False))
+    part = O.direct_field_targets(m, pot.pos, S, tg, al, pot.excl_csr(), damping).reshape(-1, 3)
+    np.testing.assert_allclose(part, full[tg], rtol=1e-14, atol=1e-15 * np.abs(full).max())
+
+
 def test_coulomb_derivatives_vs_sympy():
     rng = np.random.default_rng(3)
     p = 6
