@@ -87,6 +87,77 @@ def ref_kernels():
     return _ref
 
 
+class CpuBaseline:
+    """The reference's solve on the host cores (oracle/cpu_baseline.cc, -O3 -march=native, OpenMP):
+    per-apply tree rebuild, straight-line FMM operators or the direct sum, Jacobi-preconditioned
+    CG from alpha F. ``path`` selects a copy built for the running machine."""
+
+    _libs = {}
+
+    def __init__(self, pos, alpha6, excl, damping=0.0, scheme="direct", ncrit=64, order=5, theta=0.5,
+                 path=None):
+        self.L = self.load(path or os.path.join(HERE, "libcpu_baseline.so"))
+        self.pos = _f64(pos).reshape(-1)
+        self.n = self.pos.size // 3
+        self.alpha6 = _f64(alpha6).reshape(-1)
+        self.off = _i32(excl[0])
+        self.idx = _i32(excl[1]) if len(excl[1]) else np.zeros(1, dtype=np.int32)
+        self.h = self.L.cpb_create(self.n, _p(self.pos), _p(self.alpha6), _p(self.off, _i), _p(self.idx, _i),
+                                   float(damping), int(scheme == "fmm"), int(ncrit), int(order), float(theta))
+        if not self.h:
+            raise RuntimeError(self.L.orc_last_error().decode())
+
+    @classmethod
+    def load(cls, path):
+        if path not in cls._libs:
+            if not os.path.exists(path):
+                subprocess.run(["make", "-s", "-C", HERE, "baseline", "CPB_OUT=" + path], check=True)
+            L = ctypes.CDLL(path)
+            L.orc_last_error.restype = ctypes.c_char_p
+            L.cpb_flags.restype = ctypes.c_char_p
+            L.orc_num_threads.restype = ctypes.c_int
+            L.cpb_create.restype = ctypes.c_void_p
+            L.cpb_create.argtypes = [ctypes.c_int, _d, _d, _i, _i, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_double]
+            L.cpb_free.argtypes = [ctypes.c_void_p]
+            L.cpb_apply.argtypes = [ctypes.c_void_p, _d, _d, _d]
+            L.cpb_solve.argtypes = [ctypes.c_void_p, _d, ctypes.c_double, ctypes.c_int, _d, _i, _d]
+            cls._libs[path] = L
+        return cls._libs[path]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.cpb_free(self.h)
+            self.h = None
+
+    @property
+    def flags(self):
+        return self.L.cpb_flags().decode()
+
+    @property
+    def threads(self):
+        return int(self.L.orc_num_threads())
+
+    def apply(self, x):
+        """B x (tree rebuilt for FMM); returns (Bx, tree ms, field ms)."""
+        x = _f64(x)
+        out = np.zeros(3 * self.n)
+        t = np.zeros(2)
+        self.L.cpb_apply(self.h, _p(x), _p(out), _p(t))
+        return out, float(t[0]), float(t[1])
+
+    def solve(self, rhs, thresh=1e-8, maxiter=50):
+        """CG from alpha rhs; returns (mu, iterations, total ms, tree ms, field ms)."""
+        rhs = _f64(rhs)
+        x = np.zeros_like(rhs)
+        it = ctypes.c_int(0)
+        t = np.zeros(3)
+        rc = self.L.cpb_solve(self.h, _p(rhs), float(thresh), int(maxiter), _p(x), ctypes.byref(it), _p(t))
+        if rc:
+            raise RuntimeError("Failed to converge induced dipole moments.")
+        return x, it.value, float(t[0]), float(t[1]), float(t[2])
+
+
 def _p(a, t=_d):
     return None if a is None else a.ctypes.data_as(t)
 

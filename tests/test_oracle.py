@@ -368,3 +368,24 @@ def test_potfile_reader_fields():
     # quadrupole trace removed on read (potential.hh:23-34)
     th = pot.mul[:, 4:10]
     np.testing.assert_allclose(th[:, 0] + th[:, 3] + th[:, 5], 0.0, atol=1e-14)
+
+
+@pytest.mark.parametrize("scheme,ncrit", [("fmm", 16), ("fmm", 64), ("direct", 64)])
+def test_cpu_baseline_equals_oracle(scheme, ncrit):
+    """The CPU baseline that bench.py times (oracle/cpu_baseline.cc: -O3 -march=native, straight-line
+    FMM operators, per-apply tree rebuild) computes the oracle's B apply and CG solve."""
+    from cppe_amd import synthetic
+    w = synthetic.water_box(400, 11, anisotropic=True)
+    x = np.random.default_rng(1).normal(size=3 * w.n)
+    Bo = O.BSystem(w.pos, w.pol, w.excl_csr(), damping=2.1304, scheme=scheme, ncrit=ncrit)
+    cb = O.CpuBaseline(w.pos, w.pol, w.excl_csr(), damping=2.1304, scheme=scheme, ncrit=ncrit)
+    assert "-O3" in cb.flags and "-march=native" in cb.flags
+    y, t_tree, t_field = cb.apply(x)
+    ref = Bo.apply(x)
+    assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max()
+    assert (t_tree > 0) == (scheme == "fmm") and t_field > 0
+    rhs = O.nuclear_fields(w.pos, w.atoms_z.astype(float), w.atoms_pos)
+    mu_o, it_o, _ = Bo.solve(rhs, thresh=1e-8)
+    mu, it, ms, _, _ = cb.solve(rhs, thresh=1e-8)
+    assert it == it_o and ms > 0
+    assert np.abs(mu - mu_o).max() <= 1e-12 * np.abs(mu_o).max()
