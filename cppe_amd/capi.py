@@ -19,7 +19,7 @@ EXPORTS = [
     "cppe_hip_field", "cppe_hip_multipole_fields", "cppe_hip_apply_b", "cppe_hip_apply_alpha",
     "cppe_hip_apply_alpha_inv", "cppe_hip_dense_b", "cppe_hip_dense_b_inverse", "cppe_hip_solve", "cppe_hip_upload_rhs",
     "cppe_hip_solve_resident", "cppe_hip_download_mu", "cppe_hip_nuclear_fields",
-    "cppe_hip_nuclear_multipole_energy", "cppe_hip_dot", "cppe_hip_tree_stats", "cppe_hip_tree_export",
+    "cppe_hip_nuclear_multipole_energy", "cppe_hip_nuclear_multipole_energy_noalloc", "cppe_hip_dot", "cppe_hip_tree_stats", "cppe_hip_tree_export",
     "cppe_hip_profile_apply", "cppe_hip_batch_create", "cppe_hip_segment_results", "cppe_hip_set_graphs",
     "cppe_hip_autotune", "cppe_hip_set_tune", "cppe_hip_set_reliable_drop", "cppe_hip_set_refinement",
     "cppe_hip_last_solve_info",
@@ -74,6 +74,9 @@ def lib():
         L.cppe_hip_download_mu.argtypes = [vp, _d]
         L.cppe_hip_nuclear_fields.argtypes = [ctypes.c_int, _d, ctypes.c_int, _d, _d, _d]
         L.cppe_hip_nuclear_multipole_energy.argtypes = [ctypes.c_int, _d, _i, _d, ctypes.c_int, _d, _d, _d]
+        # device pointers (integers) and a stream handle
+        L.cppe_hip_nuclear_multipole_energy_noalloc.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int, vp, vp, vp,
+                                                                vp]
         L.cppe_hip_nuclear_field_gradient.argtypes = [ctypes.c_int, _d, ctypes.c_int, _d, _d, _d]
         L.cppe_hip_apply_b_multi.argtypes = [vp, ctypes.c_int, _d, _d]
         L.cppe_hip_solve_multi.argtypes = [vp, ctypes.c_int, _d, _d, ctypes.c_int, ctypes.c_double, ctypes.c_int,
@@ -384,6 +387,13 @@ def nuclear_multipole_energy(pos, kmax, mul, charges, atom_pos):
     _check(lib().cppe_hip_nuclear_multipole_energy(pos.size // 3, _p(pos), _p(km, _i), _p(mul), ch.size, _p(ch),
                                                    _p(ap), ctypes.byref(e)))
     return e.value
+
+
+def nuclear_multipole_energy_noalloc(n, pos, kmax, mul, natoms, charges, atom_pos, energy, stream=0):
+    """Allocation-free energy: every argument but n / natoms is a device address (int), energy a
+    device double written on `stream` without synchronisation (cppe_hip_nuclear_multipole_energy_noalloc)."""
+    _check(lib().cppe_hip_nuclear_multipole_energy_noalloc(int(n), pos, kmax, mul, int(natoms), charges, atom_pos,
+                                                           energy, stream))
 
 
 def nuclear_field_gradient(pos, charges, atom_pos):

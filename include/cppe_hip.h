@@ -239,6 +239,20 @@ int cppe_hip_nuclear_multipole_energy(int n, const double* pos, const int* kmax,
                                       double* energy);
 
 /*
+ * Allocation-free variant of cppe_hip_nuclear_multipole_energy. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph capture, loops that
+ * re-evaluate the energy as the nuclei move). Every array argument and `energy` are DEVICE
+ * pointers; the kernels are enqueued on `stream` (a hipStream_t, NULL = the null stream) and
+ * *energy is written there, without synchronisation. The partial sums live in a fixed device
+ * array of 16384 doubles (128 KiB per GPU, kept until the process ends): at most 4,194,304 sites,
+ * larger n returns CPPE_HIP_ERR_ARG. Calls that overlap on one GPU race on that array.
+ * Same kernels and summation order as cppe_hip_nuclear_multipole_energy (same result bits).
+ */
+int cppe_hip_nuclear_multipole_energy_noalloc(int n, const double* pos, const int* kmax, const double* mul,
+                                              int natoms, const double* charges, const double* atom_pos,
+                                              double* energy, void* stream);
+
+/*
  * NuclearFields::nuclear_gradient (electric_fields.cc:77-104): derivative of the nuclear fields at
  * the n polarizable sites with respect to the atom positions, row-major (3 natoms) x (3 n):
  * grad[(3a + p) * 3n + 3i + q] = Z_a T2(r_i - R_a)_pq.
