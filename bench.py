@@ -208,6 +208,10 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuned ones")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the 2,000-site direct-summation check")
     ap.add_argument("--no-eef", action="store_true", help="skip the three-unit-field (induced_moments_eef) timing")
+    ap.add_argument("--precond", default="local", choices=["jacobi", "local"],
+                    help="CG preconditioner of the headline form (the plain FP64 form always uses the "
+                         "reference's alpha r): local = alpha r + w alpha T_loc alpha r over pairs within 3 A")
+    ap.add_argument("--precond-weight", type=float, default=0.8, help="weight w of the local preconditioner")
     ap.add_argument("--reliable-drop", type=float, default=None,
                     help="mixed form: residual drop that triggers an FP64 true residual (library default 1e-5)")
     ap.add_argument("--apply-damping", type=float, default=2.1304, help="Thole factor of --apply-only runs")
@@ -257,6 +261,7 @@ def main():
     S32.upload_rhs(rhs)
     if args.reliable_drop:
         S32.set_reliable_drop(args.reliable_drop)
+    S32.set_preconditioner(args.precond, weight=args.precond_weight)
     log("setup (octree + lists) ms", round(t_setup, 1))
     tune = None
     if not args.no_tune:
@@ -303,6 +308,27 @@ def main():
     S32.set_graphs(False)
     walln, _, _ = time_solves(S32, max(1, args.steps // 2), 1, 0)
     S32.set_graphs(True)
+    # the preconditioner's share: the headline form with the reference's alpha r, and the plain FP64
+    # form with the local preconditioner
+    other = "jacobi" if args.precond == "local" else "local"
+    S32.set_preconditioner(other, weight=args.precond_weight)
+    if ml_used:
+        for _ in range(max(1, args.warmup // 2)):
+            head_solve()
+        legs = []
+        for _ in range(max(1, args.steps // 2)):
+            t0 = time.perf_counter()
+            it_o = head_solve()[0]
+            legs.append(((time.perf_counter() - t0) * 1e3, it_o))
+        wallo, itso = float(np.mean([l[0] for l in legs])), [l[1] for l in legs]
+    else:
+        wallo, _, itso = time_solves(S32, max(1, args.steps // 2), 1, 0)
+    S32.set_preconditioner(args.precond, weight=args.precond_weight)
+    S64.set_preconditioner("local", weight=args.precond_weight)
+    wall64l, _, its64l = time_solves(S64, max(1, args.steps // 2), 1, 0)
+    S64.set_preconditioner("jacobi")
+    S64.solve_resident(0, 1e-8, 200)  # the plain form's moments again
+    log("headline with %s preconditioner" % other, wallo, itso, "fp64 local", wall64l, its64l)
     mu64 = S64.download_mu()
     S32.solve_resident(0, 1e-8, 200)  # the headline form's moments, from alpha F
     mu32 = S32.download_mu()
@@ -446,6 +472,10 @@ def main():
         "mixed_alphaF_ms": round(wallm, 3),
         "mixed_alphaF_iterations": itsm,
         "mixed_alphaF_no_graphs_ms": round(walln, 3),
+        "preconditioner": {"headline": args.precond, "weight": args.precond_weight,
+                           "headline_with_%s_ms" % other: round(wallo, 3),
+                           "headline_with_%s_iterations" % other: itso,
+                           "fp64_local_ms": round(wall64l, 3), "fp64_local_iterations": its64l},
         "mixed_zero_guess_ms": round(wallz, 3),
         "mixed_zero_guess_iterations": itsz,
         "guess_dipole_rel_err": guess_err,

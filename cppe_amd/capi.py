@@ -21,7 +21,7 @@ EXPORTS = [
     "cppe_hip_solve_resident", "cppe_hip_download_mu", "cppe_hip_nuclear_fields",
     "cppe_hip_nuclear_multipole_energy", "cppe_hip_nuclear_multipole_energy_noalloc", "cppe_hip_dot", "cppe_hip_tree_stats", "cppe_hip_tree_export",
     "cppe_hip_profile_apply", "cppe_hip_batch_create", "cppe_hip_segment_results", "cppe_hip_set_graphs",
-    "cppe_hip_autotune", "cppe_hip_set_tune", "cppe_hip_set_reliable_drop", "cppe_hip_set_refinement",
+    "cppe_hip_autotune", "cppe_hip_set_tune", "cppe_hip_set_reliable_drop", "cppe_hip_set_refinement", "cppe_hip_set_preconditioner",
     "cppe_hip_last_solve_info",
     "cppe_hip_field_targets", "cppe_hip_nuclear_field_gradient", "cppe_hip_nuclear_multipole_gradient",
     "cppe_hip_solve_multi", "cppe_hip_apply_b_multi",
@@ -94,6 +94,7 @@ def lib():
         L.cppe_hip_set_tune.argtypes = [vp, _i]
         L.cppe_hip_set_reliable_drop.argtypes = [vp, ctypes.c_double]
         L.cppe_hip_set_refinement.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+        L.cppe_hip_set_preconditioner.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.cppe_hip_last_solve_info.argtypes = [vp, _i]
         L.cppe_hip_field_targets.argtypes = [vp, _d, ctypes.c_int, _i, _d]
         _f = ctypes.POINTER(ctypes.c_float)
@@ -259,7 +260,8 @@ class System:
         for k, prec in enumerate(["fp64", "fp32"]):
             m = ms[w * k:w * (k + 1)]
             out[prec] = dict(near_variant=int(choice[4 * k]), near_block=int(choice[4 * k + 1]),
-                             m2l_group=int(choice[4 * k + 2]), overlap=int(choice[4 * k + 3]),
+                             m2l_group=int(choice[4 * k + 2]), overlap=int(choice[4 * k + 3]) & 3,
+                             near_launches=1 << (int(choice[4 * k + 3]) >> 2),
                              apply_ms={"series": round(float(m[nn + nm]), 4), "overlap": round(float(m[nn + nm + 1]), 4)},
                              near_ms={"%s/%d" % (self.NEAR_NAMES[v], b): round(float(t), 4)
                                       for (v, b), t in zip(self.NEAR_CANDIDATES, m[:nn])},
@@ -285,6 +287,14 @@ class System:
     def set_refinement(self, drop=1e-5, stall=10, min_progress=0.5):
         """Mixed form: reliable-update drop, stall steps and the stagnation test (cppe_hip_set_refinement)."""
         _check(lib().cppe_hip_set_refinement(self.h, float(drop), int(stall), float(min_progress)))
+
+    PRECOND_RADIUS = 5.6691783738  # 3 A in bohr
+
+    def set_preconditioner(self, kind="jacobi", radius=PRECOND_RADIUS, weight=0.8):
+        """CG preconditioner: "jacobi" (alpha r, the reference's) or "local" (alpha r + weight alpha T_loc
+        alpha r over pairs closer than radius bohr) (cppe_hip_set_preconditioner)."""
+        k = {"jacobi": 0, "local": 1}[kind]
+        _check(lib().cppe_hip_set_preconditioner(self.h, k, float(radius), float(weight)))
 
     def last_solve_info(self):
         info = np.zeros(3, dtype=np.int32)

@@ -17,6 +17,11 @@ void fmm_autotune(System& S, int reps, int* choice, double* ms);
 void field1_targets(System& S, const double* mu_soa, int nt, const int* tdev, double* out);
 void fmm_set_tune(System& S, const int* choice);
 bool near_variant_allowed(System& S, int variant);
+bool decode_overlap(int code, int& form, int& splits);
+static bool overlap_code_valid(int code) {
+  int form = 0, splits = 1;
+  return decode_overlap(code, form, splits);
+}
 
 __global__ void k_scatter_xyz(int n, const double* __restrict__ aos, double* x, double* y, double* z) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -80,6 +85,7 @@ System::~System() {
   cg_graphs_release(*this);
   if (fmm) fmm_destroy(*this);
   if (ml_grid) ml_grid_destroy(*this);
+  if (pc) precond_destroy(*this);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -674,7 +680,7 @@ int cppe_hip_set_tune(cppe_hip_system* sys, const int* choice) {
   for (int prec = 0; prec < 2; ++prec) {
     const int v = choice[4 * prec], b = choice[4 * prec + 1], g = choice[4 * prec + 2], o = choice[4 * prec + 3];
     if (v < 0 || v > 8 || (b != 64 && b != 128 && b != 256) || (v == 2 && b != 64) ||
-        (g != 16 && g != 32 && g != 64) || o < 0 || o > 2)
+        (g != 16 && g != 32 && g != 64) || !overlap_code_valid(o))
       return fail(CPPE_HIP_ERR_ARG, "invalid launch shape");
     if (!near_variant_allowed(sys->S, v))
       return fail(CPPE_HIP_ERR_UNSUPPORTED, "near-field variant 2 needs leaves of at most 1024 sites");
@@ -699,6 +705,21 @@ int cppe_hip_set_refinement(cppe_hip_system* sys, double drop, int stall, double
   sys->S.reliable_drop       = drop;
   sys->S.refine_stall        = stall;
   sys->S.refine_min_progress = min_progress;
+  return CPPE_HIP_OK;
+  CPPE_API_END
+}
+
+int cppe_hip_set_preconditioner(cppe_hip_system* sys, int kind, double radius, double weight) {
+  CPPE_API_BEGIN
+  if (!sys || (kind != CPPE_HIP_PRECOND_JACOBI && kind != CPPE_HIP_PRECOND_LOCAL) || !(radius > 0.0) ||
+      !(weight >= 0.0))
+    return fail(CPPE_HIP_ERR_ARG, "preconditioner: kind 0 or 1, radius > 0, weight >= 0");
+  System& S = sys->S;
+  if (kind == CPPE_HIP_PRECOND_LOCAL && !S.has_alpha) return fail(CPPE_HIP_ERR_ARG, "system has no polarizabilities");
+  S.precond        = kind;
+  S.precond_rc     = radius;
+  S.precond_weight = weight;
+  cg_graphs_release(S);  // the captured iterations hold the preconditioner's buffers and weight
   return CPPE_HIP_OK;
   CPPE_API_END
 }

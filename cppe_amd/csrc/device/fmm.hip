@@ -249,7 +249,8 @@ void fmm_reserve(System& S) {
 // through wave-private LDS.
 // Then, for a staged near variant, the whole apply is timed with the far field in series, beside
 // the near field on a second stream, and with only M2L / L2L beside it after the upward pass; the
-// fastest is kept (the overlapped timing reported is the faster of the two overlapped forms).
+// fastest is kept (the overlapped timing reported is the fastest overlapped form); each
+// overlapped form is timed with the near field in 1, 2 and 4 launches.
 // ms receives the timings: per precision (FP64 then FP32) kNearCand near values, kM2lCand M2L
 // values, then the apply in series and overlapped; choice receives (near variant, near block, m2l
 // group, overlap) per precision.
@@ -280,6 +281,16 @@ int max_leaf_size(System& S) {
 
 bool near_variant_allowed(System& S, int variant) {
   return variant != 2 || max_leaf_size(S) <= kMaxLeafVariant2;
+}
+
+// The overlap entry of a launch-shape choice: the far/near form (0, 1, 2) plus 4 log2 of the
+// near-field launches (1, 2 or 4; forms 1 and 2 only).
+int encode_overlap(int form, int splits) { return form + 4 * (splits >= 4 ? 2 : splits >= 2 ? 1 : 0); }
+bool decode_overlap(int code, int& form, int& splits) {
+  form         = code & 3;
+  const int sl = code >> 2;
+  splits       = 1 << sl;
+  return code >= 0 && form <= 2 && sl <= 2 && (form > 0 || sl == 0);
 }
 
 void fmm_autotune(System& S, int reps, int* choice, double* ms) {
@@ -342,18 +353,30 @@ void fmm_autotune(System& S, int reps, int* choice, double* ms) {
       F.tune.m2l_group[prec] = kM2lCand[bm];
       m[kNNear + kNM2l]      = timed_total(fp32);
       m[kNNear + kNM2l + 1]  = 0.0;
-      if (F.tune.near_variant[prec] >= 5) {  // overlapped: the faster of the two forms
-        F.tune.overlap[prec]  = 1;
-        const double t1       = timed_total(fp32);
-        F.tune.overlap[prec]  = 2;
-        const double t2       = timed_total(fp32);
-        m[kNNear + kNM2l + 1] = std::min(t1, t2);
-        F.tune.overlap[prec]  = m[kNNear + kNM2l + 1] < m[kNNear + kNM2l] ? (t2 < t1 ? 2 : 1) : 0;
+      F.tune.near_splits[prec] = 1;
+      if (F.tune.near_variant[prec] >= 5) {  // overlapped: the fastest of the two forms x near splits
+        int bo = 0, bs = 1;
+        double bt = 0.0;
+        for (int form = 1; form <= 2; ++form)
+          for (int sp = 1; sp <= 4; sp *= 2) {
+            F.tune.overlap[prec]     = form;
+            F.tune.near_splits[prec] = sp;
+            const double t           = timed_total(fp32);
+            if (bo == 0 || t < bt) {
+              bo = form;
+              bs = sp;
+              bt = t;
+            }
+          }
+        m[kNNear + kNM2l + 1]    = bt;
+        const bool ovl           = bt < m[kNNear + kNM2l];
+        F.tune.overlap[prec]     = ovl ? bo : 0;
+        F.tune.near_splits[prec] = ovl ? bs : 1;
       }
       choice[4 * prec]     = F.tune.near_variant[prec];
       choice[4 * prec + 1] = F.tune.near_block[prec];
       choice[4 * prec + 2] = F.tune.m2l_group[prec];
-      choice[4 * prec + 3] = F.tune.overlap[prec];
+      choice[4 * prec + 3] = encode_overlap(F.tune.overlap[prec], F.tune.near_splits[prec]);
     }
   } catch (...) {
     S.prof = nullptr;
@@ -370,7 +393,7 @@ void fmm_set_tune(System& S, const int* choice) {
     F.tune.near_variant[prec] = choice[4 * prec];
     F.tune.near_block[prec]   = choice[4 * prec + 1];
     F.tune.m2l_group[prec]    = choice[4 * prec + 2];
-    F.tune.overlap[prec]      = choice[4 * prec + 3];
+    decode_overlap(choice[4 * prec + 3], F.tune.overlap[prec], F.tune.near_splits[prec]);
   }
   cg_graphs_release(S);
 }

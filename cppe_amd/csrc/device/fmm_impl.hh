@@ -47,6 +47,8 @@ struct FmmTune {
   int m2l_group[2]    = {64, 64};  // lanes per target cell in M2L (16, 32 or 64)
   int overlap[2]      = {0, 0};    // far field on a second stream beside the near field (staged variants):
                                    // 0 none, 1 the whole far field, 2 M2L / L2L after the upward pass
+  int near_splits[2]  = {1, 1};    // overlapped forms: the near field in 1, 2 or 4 launches, so the
+                                   // far field's kernels get the device between them
 };
 
 struct FmmState {
@@ -1407,17 +1409,26 @@ void launch_near(System& S, const NearArgs& a, const MU4* mu4, const R* L, bool 
     const int ni = ensure_near_items(S, ci);
     build_near_expanded(S);
     const Octree& T = S.tree;
-    const dim3 g(ceil_div(ni, block / 64)), b(block);
+    const dim3 b(block);
     const int chunk = 64 >> ci;
     // the branch-free FP32 pair needs every site damped
     constexpr bool kB2 = sizeof(R) == 4 && DAMP;
     const bool b2      = kB2 && S.all_damped;
     // far-source stages: valid for damping factors at least the classification's
     const long long* xs = (DAMP && a.damping >= T.xdamp) ? T.xsplit[prec].p : nullptr;
+    // Beside a far field on the second stream (no fused L2P), the work items may be issued in
+    // several launches: the device dispatches the far field's small dependent kernels between
+    // them instead of behind the whole near field.
+    const int splits = with_l2p ? 1 : std::max(1, F.tune.near_splits[prec]);
     auto go = [&](auto l2p, auto cht, auto bb) {
-      hipLaunchKernelGGL((k_fmm_near_st<P, R, DAMP, MODE, MU4, decltype(l2p)::value, decltype(cht)::value,
-                                        decltype(bb)::value, K>),
-                         g, b, 0, st, a, F.near_items[ci].p, ni, chunk, T.xoff.p, T.xsrc.p, mu4, L, lstride, xs);
+      for (int k = 0; k < splits; ++k) {
+        const int i0 = (int)((long long)ni * k / splits), i1 = (int)((long long)ni * (k + 1) / splits);
+        if (i1 <= i0) continue;
+        hipLaunchKernelGGL((k_fmm_near_st<P, R, DAMP, MODE, MU4, decltype(l2p)::value, decltype(cht)::value,
+                                          decltype(bb)::value, K>),
+                           dim3(ceil_div(i1 - i0, block / 64)), b, 0, st, a, F.near_items[ci].p + i0, i1 - i0, chunk,
+                           T.xoff.p, T.xsrc.p, mu4, L, lstride, xs);
+      }
     };
     auto go2 = [&](auto l2p, auto bb) {
       // the unrolled full-chunk path pays for branch-free pairs only (a damped FP64 pair branches

@@ -132,6 +132,13 @@ struct CgVecs {
   const int* status;  // ST_COUNT ints
   float4* rec32;      // when set: near-field dipole records of p for the next FP32 apply
   double4* rec64;     // when set: the same for the next FP64 apply
+  // local preconditioner (pc_off set): the kernels that form z write w = alpha r instead, and
+  // k_sg_prec forms z = w + weight alpha T_loc w and the partial r.z
+  double* w;
+  const int* pc_off;
+  const int* pc_nj;
+  const float* pc_blk;
+  double pc_weight;
 };
 
 // p of site i (components p0, p1, p2) into the near-field dipole record the next apply reads
@@ -166,6 +173,13 @@ __global__ __launch_bounds__(256) void k_sg_init(CgVecs v) {
     v.r[k0] = r0;
     v.r[k1] = r1;
     v.r[k2] = r2;
+    rr += r0 * r0 + r1 * r1 + r2 * r2;
+    if (v.pc_off) {  // z, p, the records and r.z follow in k_sg_prec
+      v.w[k0] = z0;
+      v.w[k1] = z1;
+      v.w[k2] = z2;
+      continue;
+    }
     v.z[k0] = z0;
     v.z[k1] = z1;
     v.z[k2] = z2;
@@ -174,12 +188,11 @@ __global__ __launch_bounds__(256) void k_sg_init(CgVecs v) {
     v.p[k2] = z2;
     put_record(v, i + ci.ia, z0, z1, z2);
     rz += r0 * z0 + r1 * z1 + r2 * z2;
-    rr += r0 * r0 + r1 * r1 + r2 * r2;
   }
   const double s1 = block_sum(rz, red);
   const double s2 = block_sum(rr, red);
   if (threadIdx.x == 0) {
-    v.part[c]            = s1;
+    if (!v.pc_off) v.part[c] = s1;
     v.part[v.nchunk + c] = s2;
   }
 }
@@ -236,17 +249,18 @@ __global__ __launch_bounds__(256) void k_sg_xrz(CgVecs v) {
       double z0, z1, z2;
       z_of(v.a6, n, i + ci.ia, rq[0], rq[1], rq[2], z0, z1, z2);
       const size_t k0 = ci.b + i;
-      v.z[k0]         = z0;
-      v.z[k0 + n]     = z1;
-      v.z[k0 + 2 * (size_t)n] = z2;
+      double* zo      = v.pc_off ? v.w : v.z;  // local preconditioner: w, then k_sg_prec
+      zo[k0]          = z0;
+      zo[k0 + n]      = z1;
+      zo[k0 + 2 * (size_t)n] = z2;
       srz += rq[0] * z0 + rq[1] * z1 + rq[2] * z2;
     }
   }
   const double s1 = block_sum(srr, red);
   const double s2 = block_sum(srz, red);
   if (threadIdx.x == 0) {
-    v.part[c]            = s1;
-    v.part[v.nchunk + c] = s2;
+    v.part[c] = s1;
+    if (!v.pc_off) v.part[v.nchunk + c] = s2;
   }
 }
 
@@ -286,13 +300,57 @@ __global__ __launch_bounds__(256) void k_sg_z(CgVecs v) {
       const double r0 = v.r[k0], r1 = v.r[k1], r2 = v.r[k2];
       double z0, z1, z2;
       z_of(v.a6, n, i + ci.ia, r0, r1, r2, z0, z1, z2);
-      v.z[k0] = z0;
-      v.z[k1] = z1;
-      v.z[k2] = z2;
+      double* zo = v.pc_off ? v.w : v.z;  // local preconditioner: w, then k_sg_prec
+      zo[k0]     = z0;
+      zo[k1]     = z1;
+      zo[k2]     = z2;
       s += r0 * z0 + r1 * z1 + r2 * z2;
     }
   s = block_sum(s, red);
-  if (threadIdx.x == 0) v.part[c] = s;
+  if (threadIdx.x == 0 && !v.pc_off) v.part[c] = s;
+}
+
+// Local preconditioner, after a kernel that wrote w = alpha r: z = w + weight alpha (T_loc w) and
+// the chunk partial r.z into part[slot * nchunk + c]. MODE 0 (after k_sg_init, every segment):
+// also p = z and p's near-field records. MODE 1 (after k_sg_xrz) and MODE 2 (after k_sg_z):
+// active segments, MODE 2 not while a reliable update is pending.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_sg_prec(CgVecs v, int slot) {
+  __shared__ double red[4];
+  if (MODE == 2 && v.status[ST_REL] > 0) return;
+  const int c = blockIdx.x, n = v.na;
+  const ChunkIdx ci(v, c);
+  double s = 0.0;
+  if (MODE == 0 || v.si[I_ACTIVE * v.nseg + v.cseg[c]])
+    for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x) {
+      const int ia = i + ci.ia;  // site within its right-hand side
+      double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+      for (int q = v.pc_off[ia]; q < v.pc_off[ia + 1]; ++q) {
+        const size_t kj = ci.b + (size_t)(v.pc_nj[q] - ci.ia);
+        const double w0 = v.w[kj], w1 = v.w[kj + n], w2 = v.w[kj + 2 * (size_t)n];
+        const float* t  = v.pc_blk + 6 * (size_t)q;
+        y0 += (double)t[0] * w0 + (double)t[1] * w1 + (double)t[2] * w2;
+        y1 += (double)t[1] * w0 + (double)t[3] * w1 + (double)t[4] * w2;
+        y2 += (double)t[2] * w0 + (double)t[4] * w1 + (double)t[5] * w2;
+      }
+      double a0, a1, a2;
+      z_of(v.a6, n, ia, y0, y1, y2, a0, a1, a2);
+      const size_t k0 = ci.b + i, k1 = k0 + n, k2 = k1 + n;
+      const double z0 = v.w[k0] + v.pc_weight * a0, z1 = v.w[k1] + v.pc_weight * a1,
+                   z2 = v.w[k2] + v.pc_weight * a2;
+      v.z[k0] = z0;
+      v.z[k1] = z1;
+      v.z[k2] = z2;
+      if (MODE == 0) {
+        v.p[k0] = z0;
+        v.p[k1] = z1;
+        v.p[k2] = z2;
+        put_record(v, ia, z0, z1, z2);
+      }
+      s += v.r[k0] * z0 + v.r[k1] * z1 + v.r[k2] * z2;
+    }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) v.part[(size_t)slot * v.nchunk + c] = s;
 }
 
 // p = z + beta p (active segments)
@@ -551,7 +609,20 @@ static CgVecs vecs(System& S, CgState& C) {
   v.status = C.status.p;
   v.rec32  = C.rec32;
   v.rec64  = C.rec64;
+  const bool pc = S.precond == 1 && S.pc;
+  v.w         = pc ? C.wbuf.p : nullptr;
+  v.pc_off    = pc ? S.pc->off.p : nullptr;
+  v.pc_nj     = pc ? S.pc->nj.p : nullptr;
+  v.pc_blk    = pc ? S.pc->blk.p : nullptr;
+  v.pc_weight = S.precond_weight;
   return v;
+}
+
+// z from w for the local preconditioner (MODE as in k_sg_prec); nothing for the Jacobi form
+template <int MODE>
+static void launch_prec(System& S, CgState& C, int slot) {
+  if (S.precond != 1) return;
+  hipLaunchKernelGGL(k_sg_prec<MODE>, dim3(C.nchunk), dim3(256), 0, S.stream, vecs(S, C), slot);
 }
 
 static CgFinal finals(CgState& C) {
@@ -594,6 +665,7 @@ static void seq_reliable(System& S, CgState& C) {
 static void seq_direction(System& S, CgState& C, bool mixed) {
   const dim3 g(C.nchunk), b(256);
   hipLaunchKernelGGL(k_sg_z, g, b, 0, S.stream, vecs(S, C));
+  launch_prec<2>(S, C, 0);
   launch_final<F_BETA>(S, C, mixed);
   hipLaunchKernelGGL(k_sg_p, g, b, 0, S.stream, vecs(S, C));
 }
@@ -608,6 +680,7 @@ static void seq_step_direction(System& S, CgState& C, bool mixed) {
   S.prepacked = nullptr;
   hipLaunchKernelGGL(k_sg_pap, g, b, 0, S.stream, vecs(S, C));
   hipLaunchKernelGGL(k_sg_xrz, g, b, 0, S.stream, vecs(S, C));
+  launch_prec<1>(S, C, 1);
   launch_final<F_RNB>(S, C, mixed);
   hipLaunchKernelGGL(k_sg_p, g, b, 0, S.stream, vecs(S, C));
 }
@@ -679,6 +752,10 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
   }
   fmm_reserve(S);  // no allocation inside a graph
   if (C.nrhs > 1) fmm_reserve_multi(S);
+  if (S.precond == 1) {
+    precond_build(S);
+    C.wbuf.alloc(3 * (size_t)C.na * C.nrhs);
+  }
   // single right-hand side: the direction kernels also write p's near-field dipole records in
   // the precision of the step's apply, which then skips its own packing
   C.rec32 = nullptr;
@@ -703,6 +780,7 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
   // replaced by an FP64 true residual at the first reliable update, before any convergence test.
   apply(S, C, C.x, C.t, mixed);
   hipLaunchKernelGGL(k_sg_init, dim3(C.nchunk), dim3(256), 0, st, vecs(S, C));
+  launch_prec<0>(S, C, 0);
   launch_final<F_INIT>(S, C, mixed);
   ++C.expect;  // every launch sequence below runs exactly one final kernel
   read_status(S, C);
@@ -736,6 +814,7 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
     params[P_RESTART] = 1.0;
     CPPE_HIP_CHECK(hipMemcpyAsync(C.params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_sg_init, dim3(C.nchunk), dim3(256), 0, st, vecs(S, C));
+    launch_prec<0>(S, C, 0);
     launch_final<F_INIT>(S, C, false);
     params[P_RESTART] = 0.0;
     CPPE_HIP_CHECK(hipMemcpyAsync(C.params.p, params, sizeof(params), hipMemcpyHostToDevice, st));

@@ -63,6 +63,15 @@ struct Octree {
 };
 
 struct FmmState;  // FMM expansions and operator scratch (fmm.hip)
+
+// Pair list of the local Neumann preconditioner (precond.hip): CSR over sites (device order),
+// neighbour index and the FP32 damped dipole tensor block (6 packed values) of each pair.
+struct Precond {
+  double rc = 0.0, damping = 0.0;
+  long long npairs = 0;
+  dbuf<int> off, nj;
+  dbuf<float> blk;
+};
 struct MlGrid;    // neighbour grid of the learned warm start (ml_features.hip)
 
 // Outcome of a device CG solve, per segment.
@@ -97,6 +106,7 @@ struct CgState {
   float4* rec32   = nullptr;  // near-field dipole records of p written by the direction kernels
   double4* rec64 = nullptr;   // (one of them, in the precision of the step's apply; none: nrhs > 1)
   dbuf<double> store;  // backing of the vectors when this state owns them
+  dbuf<double> wbuf;   // alpha r of the local preconditioner (same layout as r)
   void release_graphs();
   ~CgState();
 };
@@ -142,6 +152,12 @@ struct System {
   // cgm for solves of several right-hand sides at once (built on first use)
   CgState cg, cgm;
   bool use_graphs = true;
+  // CG preconditioner: 0 = alpha r (the reference's Jacobi form), 1 = local first-order Neumann
+  // alpha r + weight alpha T_loc alpha r over pairs closer than precond_rc (precond.hip)
+  int precond = 0;
+  double precond_rc = 5.6691783738;  // 3 A in bohr
+  double precond_weight = 0.8;
+  Precond* pc = nullptr;
   double reliable_drop = 1e-5;  // mixed form: residual drop that triggers an FP64 true residual
   int refine_stall = 10;        // mixed form: steps without halving that trigger one as well
   double refine_min_progress = 0.5;  // mixed form: a true residual above this fraction of the
@@ -191,6 +207,8 @@ void fmm_static(System& S, const double* q, const double* mu, const double* th, 
 void fmm_destroy(System& S);
 int fmm_near_variant(const System& S, bool fp32);  // tuned near-field variant (0 before the first apply)
 void ml_grid_destroy(System& S);
+void precond_build(System& S);
+void precond_destroy(System& S);
 
 // layout conversions between caller AoS (caller order) and device SoA (device order)
 void aos_to_soa(System& S, const double* aos, double* soa, int ncomp);

@@ -49,8 +49,12 @@ DeviceSystem::DeviceSystem(const std::vector<Potential>& sites, const PeOptions&
   }
   if (idx.empty()) idx.push_back(-1);
   auto o = make_options(opt, damping);
+  if (opt.solver_preconditioner != "jacobi" && opt.solver_preconditioner != "local")
+    throw std::invalid_argument("solver_preconditioner must be 'jacobi' or 'local'.");
   hip_check(cppe_hip_system_create(&m_sys, m_n, pos.data(), with_alpha ? a6.data() : nullptr, iso.data(),
                                    off.data(), idx.data(), &o));
+  if (with_alpha && opt.solver_preconditioner == "local")
+    hip_check(cppe_hip_set_preconditioner(m_sys, CPPE_HIP_PRECOND_LOCAL, 5.6691783738, 0.8));
 }
 
 DeviceSystem::~DeviceSystem() {

@@ -29,6 +29,10 @@ struct PeOptions {
   // (FP32 field applies inside FP64 iterative refinement; same FP64 stopping test).
   // No counterpart in the reference (see extension_option_keys).
   std::string solver_precision{"fp64"};
+  // CG preconditioner of the induced-dipole solve: "jacobi" (alpha r, the reference's) or "local"
+  // (alpha r plus the first Neumann correction over pairs closer than 3 A; cppe_hip.h).
+  // No counterpart in the reference (see extension_option_keys).
+  std::string solver_preconditioner{"jacobi"};
 };
 
 static const std::vector<std::string> valid_option_keys{"potfile",
@@ -51,6 +55,6 @@ static const std::vector<std::string> valid_option_keys{"potfile",
                                                         "border_redist_pol"};
 
 // Keys this build accepts beyond the reference's list (valid_option_keys stays identical to it).
-static const std::vector<std::string> extension_option_keys{"solver_precision"};
+static const std::vector<std::string> extension_option_keys{"solver_precision", "solver_preconditioner"};
 
 }  // namespace libcppe

@@ -75,6 +75,7 @@ PeOptions dict_to_options(const py::dict& d) {
     else if (key == "border_redist_order") o.border_redist_order = v.cast<int>();
     else if (key == "border_redist_pol") o.border_redist_pol = v.cast<bool>();
     else if (key == "solver_precision") o.solver_precision = v.cast<std::string>();
+    else if (key == "solver_preconditioner") o.solver_preconditioner = v.cast<std::string>();
   }
   return o;
 }
@@ -100,6 +101,7 @@ py::dict options_to_dict(const PeOptions& o) {  // export_state.cc:78-123
   d["border_redist_order"]      = o.border_redist_order;
   d["border_redist_pol"]        = o.border_redist_pol;
   if (o.solver_precision != "fp64") d["solver_precision"] = o.solver_precision;
+  if (o.solver_preconditioner != "jacobi") d["solver_preconditioner"] = o.solver_preconditioner;
   return d;
 }
 

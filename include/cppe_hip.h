@@ -34,6 +34,9 @@ extern "C" {
 #define CPPE_HIP_PREC_FP64 0      /* plain form: FP64 throughout */
 #define CPPE_HIP_PREC_MIXED 1     /* FP32 field applies inside FP64 iterative refinement */
 
+#define CPPE_HIP_PRECOND_JACOBI 0 /* z = alpha r, the reference's preconditioner (default) */
+#define CPPE_HIP_PRECOND_LOCAL 1  /* z = alpha r + weight alpha T_loc alpha r (close pairs) */
+
 typedef struct cppe_hip_options {
   int summation;    /* PeOptions::summation_induced_fields: 0 "direct", 1 "fmm" (pe_options.hh:28-29) */
   int ncrit;        /* PeOptions::tree_ncrit (pe_options.hh:30) */
@@ -183,6 +186,18 @@ int cppe_hip_set_reliable_drop(cppe_hip_system* sys, double drop);
  * electric_fields.cc:191-239).
  */
 int cppe_hip_set_refinement(cppe_hip_system* sys, double drop, int stall, double min_progress);
+/*
+ * CG preconditioner of this system's solves (no counterpart in the reference, whose
+ * InducedMoments::compute preconditions with alpha r, electric_fields.cc:191-239).
+ * kind CPPE_HIP_PRECOND_JACOBI: z = alpha r (default). kind CPPE_HIP_PRECOND_LOCAL: the first term
+ * of the Neumann series of B^-1 = (alpha^-1 - T)^-1 beyond alpha, kept over close pairs:
+ * z = alpha r + weight alpha T_loc alpha r, T_loc_ij the system's (Thole-damped) dipole tensor
+ * -B_ij for pairs closer than `radius` (bohr) in partner leaves of the octree's near lists,
+ * exclusions and the site itself left out; T_loc is symmetric, so CG keeps its recurrence and its
+ * unpreconditioned stopping test. radius > 0 (5.669 = 3 A: about six neighbours of a water
+ * site), weight >= 0 (0.8 measured best on water boxes).
+ */
+int cppe_hip_set_preconditioner(cppe_hip_system* sys, int kind, double radius, double weight);
 /* Last solve: info[0] iterations of the shared loop, info[1] FP64 true-residual applies,
  * info[2] iteration at which a stagnating mixed solve continued in FP64 (-1: it did not). */
 int cppe_hip_last_solve_info(cppe_hip_system* sys, int* info);
@@ -205,10 +220,12 @@ int cppe_hip_last_solve_info(cppe_hip_system* sys, int* info);
  *      (target, source slot), block 64/128/256.
  * Then, for a staged near variant (5-8), the whole apply is timed with the far field (P2M, M2M,
  * M2L, L2L) in series, on a second stream beside the near field (joined before an L2P pass), and
- * with only M2L / L2L beside the near field after the upward pass (P2M, M2M) has run; the fastest
- * is kept.
- * choice[8]: (near variant, near block, m2l group, overlap 0/1/2) for FP64 then FP32; overlap 0 =
- *            series, 1 = whole far field beside the near field, 2 = upward pass first.
+ * with only M2L / L2L beside the near field after the upward pass (P2M, M2M) has run, each
+ * overlapped form with the near field in 1, 2 and 4 launches; the fastest is kept.
+ * choice[8]: (near variant, near block, m2l group, overlap) for FP64 then FP32; overlap = form +
+ *            4 log2(near launches): form 0 = series, 1 = whole far field beside the near field,
+ *            2 = upward pass first; with forms 1 and 2 the near field may be issued as 1, 2 or 4
+ *            launches of consecutive work items (so 0, 1, 2, 5, 6, 9, 10 are valid).
  * ms[60]: per precision, the 25 near timings (order above), the 3 M2L timings (ms/launch), then
  *         the apply in series and overlapped (the faster overlapped form; 0 when not timed).
  * Launch shapes never change results beyond FP summation order.
