@@ -132,9 +132,10 @@ struct CgVecs {
   const int* status;  // ST_COUNT ints
   float4* rec32;      // when set: near-field dipole records of p for the next FP32 apply
   double4* rec64;     // when set: the same for the next FP64 apply
-  // local preconditioner (pc_off set): the kernels that form z write w = alpha r instead, and
-  // k_sg_prec forms z = w + weight alpha T_loc w and the partial r.z
-  double* w;
+  // local preconditioner (pc_off set): the kernels that form z write w = alpha r instead (one
+  // double4 record per site, stacked numbering), and k_sg_prec forms z = w + weight alpha T_loc w
+  // and the partial r.z
+  double4* w;
   const int* pc_off;
   const int* pc_nj;
   const float* pc_blk;
@@ -175,9 +176,7 @@ __global__ __launch_bounds__(256) void k_sg_init(CgVecs v) {
     v.r[k2] = r2;
     rr += r0 * r0 + r1 * r1 + r2 * r2;
     if (v.pc_off) {  // z, p, the records and r.z follow in k_sg_prec
-      v.w[k0] = z0;
-      v.w[k1] = z1;
-      v.w[k2] = z2;
+      v.w[i] = make_double4(z0, z1, z2, 0.0);
       continue;
     }
     v.z[k0] = z0;
@@ -249,11 +248,14 @@ __global__ __launch_bounds__(256) void k_sg_xrz(CgVecs v) {
       double z0, z1, z2;
       z_of(v.a6, n, i + ci.ia, rq[0], rq[1], rq[2], z0, z1, z2);
       const size_t k0 = ci.b + i;
-      double* zo      = v.pc_off ? v.w : v.z;  // local preconditioner: w, then k_sg_prec
-      zo[k0]          = z0;
-      zo[k0 + n]      = z1;
-      zo[k0 + 2 * (size_t)n] = z2;
-      srz += rq[0] * z0 + rq[1] * z1 + rq[2] * z2;
+      if (v.pc_off) {  // local preconditioner: w, then k_sg_prec
+        v.w[i] = make_double4(z0, z1, z2, 0.0);
+      } else {
+        v.z[k0]                 = z0;
+        v.z[k0 + n]             = z1;
+        v.z[k0 + 2 * (size_t)n] = z2;
+        srz += rq[0] * z0 + rq[1] * z1 + rq[2] * z2;
+      }
     }
   }
   const double s1 = block_sum(srr, red);
@@ -300,10 +302,13 @@ __global__ __launch_bounds__(256) void k_sg_z(CgVecs v) {
       const double r0 = v.r[k0], r1 = v.r[k1], r2 = v.r[k2];
       double z0, z1, z2;
       z_of(v.a6, n, i + ci.ia, r0, r1, r2, z0, z1, z2);
-      double* zo = v.pc_off ? v.w : v.z;  // local preconditioner: w, then k_sg_prec
-      zo[k0]     = z0;
-      zo[k1]     = z1;
-      zo[k2]     = z2;
+      if (v.pc_off) {  // local preconditioner: w, then k_sg_prec
+        v.w[i] = make_double4(z0, z1, z2, 0.0);
+        continue;
+      }
+      v.z[k0] = z0;
+      v.z[k1] = z1;
+      v.z[k2] = z2;
       s += r0 * z0 + r1 * z1 + r2 * z2;
     }
   s = block_sum(s, red);
@@ -314,40 +319,55 @@ __global__ __launch_bounds__(256) void k_sg_z(CgVecs v) {
 // the chunk partial r.z into part[slot * nchunk + c]. MODE 0 (after k_sg_init, every segment):
 // also p = z and p's near-field records. MODE 1 (after k_sg_xrz) and MODE 2 (after k_sg_z):
 // active segments, MODE 2 not while a reliable update is pending.
+// kPcLanes lanes per site take consecutive entries of the site's pair row (coalesced index and
+// block loads, one 32-byte record gather per pair); the lane sums are combined by a fixed xor tree.
+constexpr int kPcLanes = 8;
 template <int MODE>
 __global__ __launch_bounds__(256) void k_sg_prec(CgVecs v, int slot) {
   __shared__ double red[4];
   if (MODE == 2 && v.status[ST_REL] > 0) return;
   const int c = blockIdx.x, n = v.na;
   const ChunkIdx ci(v, c);
+  const int g = threadIdx.x % kPcLanes;
   double s = 0.0;
   if (MODE == 0 || v.si[I_ACTIVE * v.nseg + v.cseg[c]])
-    for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x) {
-      const int ia = i + ci.ia;  // site within its right-hand side
+    for (int base = v.clo[c]; base < v.clo[c + 1]; base += 256 / kPcLanes) {
+      const int i     = base + threadIdx.x / kPcLanes;
+      const bool site = i < v.clo[c + 1];
+      const int ia    = site ? i + ci.ia : 0;  // site within its right-hand side
       double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-      for (int q = v.pc_off[ia]; q < v.pc_off[ia + 1]; ++q) {
-        const size_t kj = ci.b + (size_t)(v.pc_nj[q] - ci.ia);
-        const double w0 = v.w[kj], w1 = v.w[kj + n], w2 = v.w[kj + 2 * (size_t)n];
-        const float* t  = v.pc_blk + 6 * (size_t)q;
-        y0 += (double)t[0] * w0 + (double)t[1] * w1 + (double)t[2] * w2;
-        y1 += (double)t[1] * w0 + (double)t[3] * w1 + (double)t[4] * w2;
-        y2 += (double)t[2] * w0 + (double)t[4] * w1 + (double)t[5] * w2;
+      if (site)
+        for (int q = v.pc_off[ia] + g; q < v.pc_off[ia + 1]; q += kPcLanes) {
+          const double4 wj = v.w[v.pc_nj[q] - ci.ia];
+          const float2* t2 = reinterpret_cast<const float2*>(v.pc_blk + 6 * (size_t)q);
+          const float2 ta = t2[0], tb = t2[1], tc = t2[2];  // xx xy | xz yy | yz zz
+          y0 += (double)ta.x * wj.x + (double)ta.y * wj.y + (double)tb.x * wj.z;
+          y1 += (double)ta.y * wj.x + (double)tb.y * wj.y + (double)tc.x * wj.z;
+          y2 += (double)tb.x * wj.x + (double)tc.x * wj.y + (double)tc.y * wj.z;
+        }
+#pragma unroll
+      for (int off = kPcLanes / 2; off > 0; off >>= 1) {
+        y0 += __shfl_xor(y0, off);
+        y1 += __shfl_xor(y1, off);
+        y2 += __shfl_xor(y2, off);
       }
-      double a0, a1, a2;
-      z_of(v.a6, n, ia, y0, y1, y2, a0, a1, a2);
-      const size_t k0 = ci.b + i, k1 = k0 + n, k2 = k1 + n;
-      const double z0 = v.w[k0] + v.pc_weight * a0, z1 = v.w[k1] + v.pc_weight * a1,
-                   z2 = v.w[k2] + v.pc_weight * a2;
-      v.z[k0] = z0;
-      v.z[k1] = z1;
-      v.z[k2] = z2;
-      if (MODE == 0) {
-        v.p[k0] = z0;
-        v.p[k1] = z1;
-        v.p[k2] = z2;
-        put_record(v, ia, z0, z1, z2);
+      if (site && g == 0) {
+        double a0, a1, a2;
+        z_of(v.a6, n, ia, y0, y1, y2, a0, a1, a2);
+        const double4 wi = v.w[i];
+        const size_t k0 = ci.b + i, k1 = k0 + n, k2 = k1 + n;
+        const double z0 = wi.x + v.pc_weight * a0, z1 = wi.y + v.pc_weight * a1, z2 = wi.z + v.pc_weight * a2;
+        v.z[k0] = z0;
+        v.z[k1] = z1;
+        v.z[k2] = z2;
+        if (MODE == 0) {
+          v.p[k0] = z0;
+          v.p[k1] = z1;
+          v.p[k2] = z2;
+          put_record(v, ia, z0, z1, z2);
+        }
+        s += v.r[k0] * z0 + v.r[k1] * z1 + v.r[k2] * z2;
       }
-      s += v.r[k0] * z0 + v.r[k1] * z1 + v.r[k2] * z2;
     }
   s = block_sum(s, red);
   if (threadIdx.x == 0) v.part[(size_t)slot * v.nchunk + c] = s;
@@ -371,6 +391,7 @@ __global__ __launch_bounds__(256) void k_sg_p(CgVecs v) {
     put_record(v, i + ci.ia, pq[0], pq[1], pq[2]);
   }
 }
+
 
 struct CgFinal {
   int nseg, nchunk;
@@ -610,7 +631,7 @@ static CgVecs vecs(System& S, CgState& C) {
   v.rec32  = C.rec32;
   v.rec64  = C.rec64;
   const bool pc = S.precond == 1 && S.pc;
-  v.w         = pc ? C.wbuf.p : nullptr;
+  v.w         = pc ? reinterpret_cast<double4*>(C.wbuf.p) : nullptr;
   v.pc_off    = pc ? S.pc->off.p : nullptr;
   v.pc_nj     = pc ? S.pc->nj.p : nullptr;
   v.pc_blk    = pc ? S.pc->blk.p : nullptr;
@@ -754,7 +775,7 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
   if (C.nrhs > 1) fmm_reserve_multi(S);
   if (S.precond == 1) {
     precond_build(S);
-    C.wbuf.alloc(3 * (size_t)C.na * C.nrhs);
+    C.wbuf.alloc(4 * (size_t)C.na * C.nrhs);  // one double4 record per site and right-hand side
   }
   // single right-hand side: the direction kernels also write p's near-field dipole records in
   // the precision of the step's apply, which then skips its own packing
