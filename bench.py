@@ -105,8 +105,10 @@ def run_batch(args, capi, synthetic):
         B.upload_rhs(rhs)
         tune = None if args.no_tune else B.autotune(reps=2)
         res = {}
-        for graphs in (True, False):
+        # graphs of up to 8 steps per launch (the default), graphs of one step, kernel by kernel
+        for name, graphs, gsteps in (("one_step_graphs", True, 1), ("no_graphs", False, 8), ("graphs", True, 8)):
             B.set_graphs(graphs)
+            B.set_graph_steps(gsteps)
             for _ in range(args.warmup):
                 B.solve_resident(0, 1e-8, 200)
             wall, dev = [], []
@@ -116,8 +118,8 @@ def run_batch(args, capi, synthetic):
                 wall.append((time.perf_counter() - t1) * 1e3)
                 dev.append(ms)
             iters, rn, conv = B.segment_results()
-            res["graphs" if graphs else "no_graphs"] = dict(ms=round(float(np.mean(wall)), 3),
-                                                           device_ms=round(float(np.mean(dev)), 3))
+            res[name] = dict(ms=round(float(np.mean(wall)), 3), device_ms=round(float(np.mean(dev)), 3),
+                             **B.last_solve_launches())
         assert conv.all()
         mu = B.download_mu()
         epol = [-0.5 * float(m @ f) for m, f in zip(B.split(mu), B.split(rhs))]
@@ -277,6 +279,7 @@ def main():
     model = None if (args.no_ml or args.summation != "fmm") else warmstart.load()
     ml_used = model is not None
     wall64, dev64, its64 = time_solves(S64, args.steps, args.warmup, 0)
+    launch64 = S64.last_solve_launches()
     log("fp64 solve ms", wall64, "iters", its64)
     guess_ms = []
     if ml_used:
@@ -299,12 +302,17 @@ def main():
     else:
         wall32, dev32, its32 = time_solves(S32, args.steps, args.warmup, 0)
     info32 = S32.last_solve_info()
+    launch32 = S32.last_solve_launches()
     log("headline solve ms", wall32, "iters", its32, "ml", ml_used, "fp64 true-residual applies",
-        info32["reliable_updates"])
+        info32["reliable_updates"], launch32)
     # the same mixed solve from alpha F and from zero, for the ML gain; then once more launched
     # kernel by kernel
     wallm, devm, itsm = time_solves(S32, max(1, args.steps // 2), 0, 0)
     wallz, devz, itsz = time_solves(S32, max(1, args.steps // 2), 0, 2)
+    S32.set_graph_steps(1)  # one step per launch, a host status read after each
+    wall1, _, _ = time_solves(S32, max(1, args.steps // 2), 1, 0)
+    launch1 = S32.last_solve_launches()
+    S32.set_graph_steps(8)
     S32.set_graphs(False)
     walln, _, _ = time_solves(S32, max(1, args.steps // 2), 1, 0)
     S32.set_graphs(True)
@@ -472,6 +480,8 @@ def main():
         "mixed_alphaF_ms": round(wallm, 3),
         "mixed_alphaF_iterations": itsm,
         "mixed_alphaF_no_graphs_ms": round(walln, 3),
+        "launches": {"headline": launch32, "fp64": launch64, "alphaF_one_step_per_launch_ms": round(wall1, 3),
+                     "alphaF_one_step_per_launch": launch1},
         "preconditioner": {"headline": args.precond, "weight": args.precond_weight,
                            "headline_with_%s_ms" % other: round(wallo, 3),
                            "headline_with_%s_iterations" % other: itso,

@@ -22,7 +22,7 @@ EXPORTS = [
     "cppe_hip_nuclear_multipole_energy", "cppe_hip_nuclear_multipole_energy_noalloc", "cppe_hip_dot", "cppe_hip_tree_stats", "cppe_hip_tree_export",
     "cppe_hip_profile_apply", "cppe_hip_batch_create", "cppe_hip_segment_results", "cppe_hip_set_graphs",
     "cppe_hip_autotune", "cppe_hip_set_tune", "cppe_hip_set_reliable_drop", "cppe_hip_set_refinement", "cppe_hip_set_preconditioner",
-    "cppe_hip_last_solve_info",
+    "cppe_hip_last_solve_info", "cppe_hip_set_graph_steps", "cppe_hip_last_solve_launches",
     "cppe_hip_field_targets", "cppe_hip_nuclear_field_gradient", "cppe_hip_nuclear_multipole_gradient",
     "cppe_hip_solve_multi", "cppe_hip_apply_b_multi",
     "cppe_hip_mlp_create", "cppe_hip_mlp_destroy", "cppe_hip_mlp_set", "cppe_hip_mlp_get", "cppe_hip_ml_features",
@@ -96,6 +96,8 @@ def lib():
         L.cppe_hip_set_refinement.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double]
         L.cppe_hip_set_preconditioner.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.cppe_hip_last_solve_info.argtypes = [vp, _i]
+        L.cppe_hip_set_graph_steps.argtypes = [vp, ctypes.c_int]
+        L.cppe_hip_last_solve_launches.argtypes = [vp, _i]
         L.cppe_hip_field_targets.argtypes = [vp, _d, ctypes.c_int, _i, _d]
         _f = ctypes.POINTER(ctypes.c_float)
         L.cppe_hip_mlp_create.argtypes = [ctypes.POINTER(vp)]
@@ -303,6 +305,17 @@ class System:
 
     def set_graphs(self, enable=True):
         _check(lib().cppe_hip_set_graphs(self.h, int(bool(enable))))
+
+    def set_graph_steps(self, steps=8):
+        """Most CG steps launched as one sequence between host status reads: 1, 2, 4 or 8
+        (cppe_hip_set_graph_steps)."""
+        _check(lib().cppe_hip_set_graph_steps(self.h, int(steps)))
+
+    def last_solve_launches(self):
+        """Step-sequence launches, steps launched in them and host status reads of the last solve."""
+        a = np.zeros(3, dtype=np.int32)
+        _check(lib().cppe_hip_last_solve_launches(self.h, _p(a, _i)))
+        return dict(step_launches=int(a[0]), steps_launched=int(a[1]), host_reads=int(a[2]))
 
     def download_mu(self):
         mu = np.zeros(3 * self.n)

@@ -742,6 +742,25 @@ int cppe_hip_set_graphs(cppe_hip_system* sys, int enable) {
   CPPE_API_END
 }
 
+int cppe_hip_set_graph_steps(cppe_hip_system* sys, int steps) {
+  CPPE_API_BEGIN
+  if (!sys || !(steps == 1 || steps == 2 || steps == 4 || steps == 8))
+    return fail(CPPE_HIP_ERR_ARG, "steps per launch must be 1, 2, 4 or 8");
+  sys->S.graph_steps = steps;
+  return CPPE_HIP_OK;
+  CPPE_API_END
+}
+
+int cppe_hip_last_solve_launches(cppe_hip_system* sys, int* launches) {
+  CPPE_API_BEGIN
+  if (!sys || !launches) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
+  launches[0] = sys->S.last.step_launches;
+  launches[1] = sys->S.last.steps_launched;
+  launches[2] = sys->S.last.host_reads;
+  return CPPE_HIP_OK;
+  CPPE_API_END
+}
+
 int cppe_hip_download_mu(cppe_hip_system* sys, double* mu) {
   CPPE_API_BEGIN
   if (!sys || !mu) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");

@@ -15,6 +15,9 @@
 //  * a true residual that is not below `min_progress` times the previous true residual after at
 //    least one step marks the segment as stagnating (STAG): the refinement is not converging and
 //    the solve continues from the current moments in the plain FP64 form.
+// Launch horizon: after every decision a segment also predicts how many more steps it will take
+// before it needs the host (seg_horizon), from its mean log reduction per step; the host launches
+// that many steps as one graph, and steps launched beyond an event do nothing on the device.
 #pragma once
 #include <cmath>
 
@@ -27,11 +30,13 @@
 namespace cppe_dev {
 
 using std::isfinite;
+using std::log;
 using std::sqrt;
 
 // per-segment doubles; D_RZN: the new r.z of a step, kept between the final kernel's two passes;
-// D_RBEST: smallest recursive residual since the last true residual (halving steps only)
-enum SegD { D_RZ = 0, D_BETA, D_RNORM, D_RLAST, D_RZN, D_RBEST, D_COUNT };
+// D_RBEST: smallest recursive residual since the last true residual (halving steps only);
+// D_LOGQ: mean log reduction of the residual norm per step (0: not known yet)
+enum SegD { D_RZ = 0, D_BETA, D_RNORM, D_RLAST, D_RZN, D_RBEST, D_LOGQ, D_COUNT };
 // per-segment ints. I_PENDING: frozen below the threshold until an FP64 true residual confirms it;
 // I_ELIGIBLE: within 100x of the replacement trigger, replaced whenever a replacement runs for
 // another segment (segments share FP64 applies); I_SINCE: steps since D_RBEST last halved;
@@ -69,6 +74,7 @@ CPPE_HD inline void seg_init(const SegRef& s, double rz, double rr, bool mixed, 
   s.d(D_RLAST) = rn;
   s.d(D_RBEST) = rn;
   s.d(D_BETA)  = 0.0;
+  s.d(D_LOGQ)  = 0.0;
   const bool bad = !isfinite(rn);
   s.i(I_BAD)      = bad;
   s.i(I_ITERS)    = 0;
@@ -121,7 +127,11 @@ CPPE_HD inline void seg_restart(const SegRef& s, double rz, double rr, int it, c
 // Residual norm after the step of iteration `it` (rr = r.r of the recursive residual).
 CPPE_HD inline void seg_step(const SegRef& s, double rr, int it, bool mixed, const Refine& p, const Hist& h) {
   if (!s.i(I_ACTIVE)) return;
-  const double rn = sqrt(rr);
+  const double rn = sqrt(rr), prev = s.d(D_RNORM);
+  if (isfinite(rn) && rn > 0.0 && isfinite(prev) && prev > 0.0) {  // running mean, weight 1/2
+    const double l = log(rn / prev);
+    s.d(D_LOGQ)    = s.d(D_LOGQ) == 0.0 ? l : 0.5 * (s.d(D_LOGQ) + l);
+  }
   s.d(D_RNORM)    = rn;
   s.i(I_ELIGIBLE) = mixed && isfinite(rn) && rn < 100.0 * p.drop * s.d(D_RLAST);
   if (!isfinite(rn)) {
@@ -197,6 +207,41 @@ CPPE_HD inline void seg_true_residual(const SegRef& s, double rr, int it, int st
     } else if (!(rn < p.thresh)) {
       s.i(I_ACTIVE) = 1;  // resumes from the true residual
     }
+  }
+}
+
+// Steps an active segment is expected to take before it needs the host, from its mean log
+// reduction per step, rounded down (a launch of that many steps ends at or before the event):
+// to_rel, steps before it asks for a true residual (mixed form: the replacement trigger or the
+// stall limit; kFarSteps when neither applies, always in the plain form), to_done, steps before it
+// leaves the active set at the threshold. At least 1 each; 1 while the rate is unknown or the
+// residual is not falling.
+// Inactive segments: to_rel = kFarSteps, to_done = 0; a pending replacement request: both 0.
+constexpr int kFarSteps = 1 << 20;
+CPPE_HD inline void seg_horizon(const SegRef& s, bool mixed, const Refine& p, int& to_rel, int& to_done) {
+  to_rel  = kFarSteps;
+  to_done = 0;
+  if (!s.i(I_ACTIVE)) return;
+  if (s.i(I_NEEDREL)) {
+    to_rel = 0;
+    return;
+  }
+  const double rn = s.d(D_RNORM), q = s.d(D_LOGQ);
+  if (!(q < -1e-3) || !isfinite(rn)) {
+    if (mixed) to_rel = 1;
+    to_done = 1;
+    return;
+  }
+  auto steps = [&](double target) -> int {  // floor(log(target / rn) / q), at least 1
+    const double k = log(target / rn) / q;
+    return k >= (double)kFarSteps ? kFarSteps : (k < 1.0 ? 1 : (int)k);
+  };
+  to_done = steps(p.thresh);
+  if (mixed) {
+    const double trig = p.drop * s.d(D_RLAST);
+    if (trig > p.thresh) to_rel = steps(trig);
+    const int st = p.stall - s.i(I_SINCE);
+    if (st < to_rel) to_rel = st < 1 ? 1 : st;
   }
 }
 

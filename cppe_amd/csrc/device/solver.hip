@@ -18,10 +18,16 @@
 // forms stop on the same FP64 criterion ||rhs - B x||_2 < thresh, per segment.
 //
 // One iteration is three fixed launch sequences -- step (apply p, step length, x/r update,
-// residual norm), reliable (FP64 apply of x, residual replacement; mixed form, when a segment
-// asks for it) and direction (preconditioner, beta, p update). Each is captured once into a HIP
-// graph and replayed; all scalars live on the device, and the host reads a 4-int status block
-// after the step and reliable graphs to decide what to launch next.
+// residual norm, and the direction when no segment asks for a replacement), reliable (FP64 apply
+// of x, residual replacement; mixed form, when a segment asks for it) and direction
+// (preconditioner, beta, p update, after a replacement). Each is captured once into a HIP graph
+// and replayed; all scalars and decisions live on the device, and the host reads a small status
+// block (published to host-mapped memory by every final kernel) after a graph to decide what to
+// launch next. Several steps run as one graph: each final kernel also publishes a horizon, the
+// number of steps every segment can take before one of them needs the host (a replacement, or
+// no segment left active), predicted from the segments' residual reduction rates (cg_policy.hh);
+// the host launches a graph of up to that many steps (1, 2, 4 or 8), and a step that starts after
+// such an event leaves every vector and scalar as it is (its apply result is never read).
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -41,9 +47,16 @@ constexpr int kChunk = 256;  // one site per thread: enough blocks in flight to 
 
 // status block: [iteration, active segments, segments asking for a reliable update, bad,
 // step taken in the current iteration, segments pending an FP64 confirmation, stagnating segments
-// (mixed form), number of final kernels run (each final kernel publishes the block to a
-// host-mapped mirror, this count last)]
-enum Status { ST_IT = 0, ST_ACTIVE, ST_REL, ST_BAD, ST_STEPPED, ST_PEND, ST_STAG, ST_SEQ, ST_COUNT };
+// (mixed form), steps that can be launched before the host is needed (0: none active), number of
+// final kernels run (each final kernel publishes the block to a host-mapped mirror, this count last)]
+enum Status { ST_IT = 0, ST_ACTIVE, ST_REL, ST_BAD, ST_STEPPED, ST_PEND, ST_STAG, ST_SAFE, ST_SEQ, ST_COUNT };
+
+// A step that starts after the host is needed (a segment asked for a replacement, or no segment
+// is active) does nothing: launches of several steps end with such steps when a prediction fell
+// short. Read from the status block the previous final kernel wrote.
+__device__ __forceinline__ bool step_halted(const int* status) {
+  return status[ST_REL] > 0 || status[ST_ACTIVE] == 0;
+}
 // solve parameters on the device: [thresh, reliable drop, min progress, stall steps, restart]
 enum Param { P_THRESH = 0, P_DROP, P_PROGRESS, P_STALL, P_RESTART, P_COUNT };
 // F_RNB: the residual norm and its tests, then -- unless a segment asked for a reliable update --
@@ -199,6 +212,7 @@ __global__ __launch_bounds__(256) void k_sg_init(CgVecs v) {
 // partial p.Ap (active segments), into the third block of the partials (read by k_sg_xrz)
 __global__ __launch_bounds__(256) void k_sg_pap(CgVecs v) {
   __shared__ double red[4];
+  if (step_halted(v.status)) return;
   const int c = blockIdx.x, n = v.na;
   const ChunkIdx ci(v, c);
   double s = 0.0;
@@ -218,6 +232,7 @@ __global__ __launch_bounds__(256) void k_sg_pap(CgVecs v) {
 __global__ __launch_bounds__(256) void k_sg_xrz(CgVecs v) {
   __shared__ double red[4];
   __shared__ double alpha;
+  if (step_halted(v.status)) return;
   const int c = blockIdx.x, n = v.na, g = v.cseg[c];
   const ChunkIdx ci(v, c);
   const bool active = v.si[I_ACTIVE * v.nseg + g];
@@ -318,14 +333,14 @@ __global__ __launch_bounds__(256) void k_sg_z(CgVecs v) {
 // Local preconditioner, after a kernel that wrote w = alpha r: z = w + weight alpha (T_loc w) and
 // the chunk partial r.z into part[slot * nchunk + c]. MODE 0 (after k_sg_init, every segment):
 // also p = z and p's near-field records. MODE 1 (after k_sg_xrz) and MODE 2 (after k_sg_z):
-// active segments, MODE 2 not while a reliable update is pending.
+// active segments, MODE 1 not in a halted step, MODE 2 not while a reliable update is pending.
 // kPcLanes lanes per site take consecutive entries of the site's pair row (coalesced index and
 // block loads, one 32-byte record gather per pair); the lane sums are combined by a fixed xor tree.
 constexpr int kPcLanes = 8;
 template <int MODE>
 __global__ __launch_bounds__(256) void k_sg_prec(CgVecs v, int slot) {
   __shared__ double red[4];
-  if (MODE == 2 && v.status[ST_REL] > 0) return;
+  if ((MODE == 1 && step_halted(v.status)) || (MODE == 2 && v.status[ST_REL] > 0)) return;
   const int c = blockIdx.x, n = v.na;
   const ChunkIdx ci(v, c);
   const int g = threadIdx.x % kPcLanes;
@@ -411,13 +426,16 @@ struct CgFinal {
 // shuffle tree combines the lanes -- deterministic for a given chunk split. Lane 0 then applies
 // the segment's decision (cg_policy.hh); the status block is refreshed at the end.
 // F_INIT with params[P_RESTART] set: the plain-form restart of unfinished segments after the
-// mixed form stagnated (iteration count and history kept).
+// mixed form stagnated (iteration count and history kept). Every mode also publishes the launch
+// horizon ST_SAFE: the smallest number of steps before a segment asks for a replacement, capped by
+// the largest number before every segment has left the active set.
 template <int MODE, bool MIXED>
 __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
-  __shared__ int cnt[5];
-  // F_BETA while a reliable update is pending: no update, only the publication below
-  const bool skip = MODE == F_BETA && f.status[ST_REL] > 0;
-  if (threadIdx.x < 5) cnt[threadIdx.x] = 0;
+  __shared__ int cnt[7];
+  // F_BETA while a reliable update is pending and F_RNB of a halted step: no update, only the
+  // publication below
+  const bool skip = (MODE == F_BETA && f.status[ST_REL] > 0) || (MODE == F_RNB && step_halted(f.status));
+  if (threadIdx.x < 7) cnt[threadIdx.x] = threadIdx.x == 5 ? kFarSteps : 0;
   __syncthreads();
   const int ns = skip ? 0 : f.nseg;
   const bool restart = MODE == F_INIT && f.params[P_RESTART] != 0.0;
@@ -426,7 +444,7 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
   const Refine pr{f.params[P_THRESH], f.params[P_DROP], f.params[P_PROGRESS], (int)f.params[P_STALL]};
   const Hist h{f.hist, f.hist_cap};
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  int na = 0, nr = 0, nb = 0, np = 0, nst = 0;
+  int na = 0, nr = 0, nb = 0, np = 0, nst = 0, h_rel = kFarSteps, h_done = 0;
   for (int g = wv; g < ns; g += 4) {
     double s1 = lane_chunk_sum(f.part, f.seg_chunk[g], f.seg_chunk[g + 1], lane), s2 = 0.0;
     if (MODE == F_INIT || MODE == F_RNB) s2 = lane_chunk_sum(f.part + f.nchunk, f.seg_chunk[g], f.seg_chunk[g + 1], lane);
@@ -455,12 +473,18 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
     nb += sg.i(I_BAD);
     np += sg.i(I_PENDING);
     nst += sg.i(I_STAG);
+    int to_rel, to_done;
+    seg_horizon(sg, MIXED, pr, to_rel, to_done);
+    h_rel  = min(h_rel, to_rel);
+    h_done = max(h_done, to_done);
   }
   if (na) atomicAdd(&cnt[0], na);
   if (nr) atomicAdd(&cnt[1], nr);
   if (nb) atomicAdd(&cnt[2], nb);
   if (np) atomicAdd(&cnt[3], np);
   if (nst) atomicAdd(&cnt[4], nst);
+  if (h_rel < kFarSteps) atomicMin(&cnt[5], h_rel);
+  if (h_done) atomicMax(&cnt[6], h_done);
   __syncthreads();
   const bool beta_now = MODE == F_RNB && cnt[1] == 0;  // no reliable update pending: direction now
   if (beta_now && lane == 0)
@@ -474,6 +498,7 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
     f.status[ST_BAD]    = cnt[2];
     f.status[ST_PEND]   = cnt[3];
     f.status[ST_STAG]   = cnt[4];
+    f.status[ST_SAFE]   = cnt[0] == 0 ? 0 : min(cnt[5], cnt[6]);
     if (MODE == F_INIT) {
       f.status[ST_IT]      = restart ? it : 0;
       f.status[ST_STEPPED] = 0;
@@ -499,11 +524,15 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
 
 // ---- host side -------------------------------------------------------------------------------
 void CgState::release_graphs() {
-  for (auto* g : {&g_step[0], &g_step[1], &g_rel, &g_dir[0], &g_dir[1]})
-    if (*g) {
-      (void)hipGraphExecDestroy(*g);
-      *g = nullptr;
-    }
+  auto drop = [](hipGraphExec_t& g) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  };
+  for (auto& form : g_step)
+    for (auto& g : form) drop(g);
+  drop(g_rel);
+  drop(g_dir[0]);
+  drop(g_dir[1]);
 }
 
 CgState::~CgState() {
@@ -743,14 +772,15 @@ static void run_graph(System& S, hipGraphExec_t& exec, const std::function<void(
 // Waits for the status block of the last final kernel launched (C.expect of them so far). The
 // final kernels publish it to the host-mapped mirror, so the host can decide and enqueue the next
 // launch sequence while the rest of the current one (the new direction) still runs. A mirror that
-// has not caught up within 20 ms (a long apply, or mapped writes that are late) is replaced by a
-// synchronous copy of the device block, which is authoritative.
-static void read_status(System& S, CgState& C) {
+// has not caught up within 20 ms per step launched (a long apply, or mapped writes that are late)
+// is replaced by a synchronous copy of the device block, which is authoritative.
+static void read_status(System& S, CgState& C, int steps = 1) {
   volatile int* hs = C.h_status;
   const auto t0    = std::chrono::steady_clock::now();
+  const auto limit = std::chrono::milliseconds(20 * steps);
   int spins        = 0;
   while (hs[ST_SEQ] != C.expect) {
-    if (++spins % 1024 == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
+    if (++spins % 1024 == 0 && std::chrono::steady_clock::now() - t0 > limit) {
       CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
       CPPE_HIP_CHECK(hipMemcpy(C.h_status, C.status.p, sizeof(int) * ST_COUNT, hipMemcpyDeviceToHost));
       C.expect = C.h_status[ST_SEQ];  // every launched final has run: the device count is exact
@@ -803,9 +833,9 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
   hipLaunchKernelGGL(k_sg_init, dim3(C.nchunk), dim3(256), 0, st, vecs(S, C));
   launch_prec<0>(S, C, 0);
   launch_final<F_INIT>(S, C, mixed);
-  ++C.expect;  // every launch sequence below runs exactly one final kernel
+  ++C.expect;  // every launch sequence below runs one final kernel per step
   read_status(S, C);
-  int k = 0, nrel = 0;
+  int k = 0, nrel = 0, nlaunch = 0, nsteps = 0, nread = 1;
   int m = mixed ? 1 : 0;
   res.fp64_fallback = -1;
   // Loop invariant at the top: every active segment has completed step, replacement and
@@ -816,6 +846,24 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
     ++C.expect;
     read_status(S, C);
     ++nrel;
+    ++nread;
+  };
+  // 2^l steps as one launch, 2^l at most the device's horizon, the iterations left and the
+  // system's limit; k = steps taken (steps that start after an event take none)
+  auto steps = [&] {
+    const int lim = std::min({C.h_status[ST_SAFE], maxiter - k, S.graph_steps});
+    int l = 0;
+    while (l < 3 && (2 << l) <= lim) ++l;
+    const int cnt = 1 << l;
+    run_graph(S, C.g_step[m][l], [&] {
+      for (int q = 0; q < cnt; ++q) seq_step_direction(S, C, mixed);
+    });
+    C.expect += cnt;
+    read_status(S, C, cnt);
+    k = C.h_status[ST_IT] + C.h_status[ST_STEPPED];
+    ++nlaunch;
+    nsteps += cnt;
+    ++nread;
   };
   // A stagnating refinement (a true residual that did not fall below min_progress times the
   // previous one) continues in the plain form: every unfinished segment restarts from its FP64
@@ -841,6 +889,7 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
     CPPE_HIP_CHECK(hipMemcpyAsync(C.params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
     ++C.expect;
     read_status(S, C);
+    ++nread;
     res.fp64_fallback = k;
     return true;
   };
@@ -858,10 +907,7 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
       ++C.expect;
       continue;
     }
-    run_graph(S, C.g_step[m], [&] { seq_step_direction(S, C, mixed); });
-    ++C.expect;
-    read_status(S, C);
-    ++k;
+    steps();
     // The direction already ran unless a reliable update was asked for (it is then taken after
     // the update). Confirming pending segments when none is active leaves the direction pass a
     // no-op on the vectors; segments that resume then take theirs after the confirmation.
@@ -902,6 +948,9 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
   res.norms0           = res.norms[0];
   res.loop_iters       = k;
   res.reliable_updates = nrel;
+  res.step_launches    = nlaunch;
+  res.steps_launched   = nsteps;
+  res.host_reads       = nread;
   return k;
 }
 

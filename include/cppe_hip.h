@@ -173,6 +173,18 @@ int cppe_hip_field_targets(cppe_hip_system* sys, const double* mu, int nt, const
 int cppe_hip_segment_results(cppe_hip_system* sys, int* iters, double* rnorm, int* converged);
 /* CG iterations replayed as captured HIP graphs (default 1) or launched kernel by kernel (0). */
 int cppe_hip_set_graphs(cppe_hip_system* sys, int enable);
+/*
+ * Most CG steps launched as one sequence without the host in between (1, 2, 4 or 8; default 8).
+ * After every decision the device publishes how many steps each snapshot can take before one of
+ * them needs the host (an FP64 true residual, or none left active), predicted from their residual
+ * reduction rates; the host launches that many steps (rounded down to 1, 2, 4 or 8) and reads the
+ * status once. A step that starts after such an event changes nothing, so results do not depend
+ * on this setting. No counterpart in the reference (a host loop, electric_fields.cc:213-234).
+ */
+int cppe_hip_set_graph_steps(cppe_hip_system* sys, int steps);
+/* Last solve: launches[0] step sequences launched, [1] steps in them (those that started after an
+ * event took none), [2] status reads of the host loop. */
+int cppe_hip_last_solve_launches(cppe_hip_system* sys, int* launches);
 /* Mixed form: factor by which a residual must fall before the FP64 true residual replaces the
  * recursive one (default 1e-5; convergence is always declared on an FP64 true residual). */
 int cppe_hip_set_reliable_drop(cppe_hip_system* sys, double drop);

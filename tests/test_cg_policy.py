@@ -19,7 +19,7 @@ _d = ctypes.POINTER(ctypes.c_double)
 _i = ctypes.POINTER(ctypes.c_int)
 
 # indices of the per-segment tables (cg_policy.hh SegD / SegI)
-D_RZ, D_BETA, D_RNORM, D_RLAST, D_RZN, D_RBEST = range(6)
+D_RZ, D_BETA, D_RNORM, D_RLAST, D_RZN, D_RBEST, D_LOGQ = range(7)
 I_ACTIVE, I_NEEDREL, I_ITERS, I_BAD, I_PENDING, I_ELIGIBLE, I_SINCE, I_STEPS, I_STAG = range(9)
 THRESH = 1e-8
 
@@ -37,16 +37,17 @@ def pol(tmp_path_factory):
     L.pol_step.argtypes = [_d, _i, dd, ii, ii, _d, _d, ii]
     L.pol_true.argtypes = [_d, _i, dd, ii, ii, _d, _d, ii]
     L.pol_beta.argtypes = [_d, _i, dd]
+    L.pol_horizon.argtypes = [_d, _i, ii, _d, _i, _i]
     nd, ni = ctypes.c_int(), ctypes.c_int()
     L.pol_sizes(ctypes.byref(nd), ctypes.byref(ni))
-    assert (nd.value, ni.value) == (6, 9)
+    assert (nd.value, ni.value) == (7, 9)
     return L
 
 
 class Seg:
     def __init__(self, L, drop=1e-5, stall=10, min_progress=0.5, cap=64):
         self.L = L
-        self.sd = np.zeros(6)
+        self.sd = np.zeros(7)
         self.si = np.zeros(9, dtype=np.int32)
         self.prm = np.array([THRESH, drop, min_progress, stall], dtype=np.float64)
         self.hist = np.full(cap, np.nan)
@@ -63,6 +64,11 @@ class Seg:
     def step(self, rn, it, mixed):
         self.L.pol_step(*self._a(), rn * rn, it, int(mixed), self.prm.ctypes.data_as(_d),
                         self.hist.ctypes.data_as(_d), self.hist.size)
+
+    def horizon(self, mixed):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self.L.pol_horizon(*self._a(), int(mixed), self.prm.ctypes.data_as(_d), ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
 
     def true(self, rn, it, stepped):
         self.L.pol_true(*self._a(), rn * rn, it, stepped, self.prm.ctypes.data_as(_d),
@@ -182,3 +188,61 @@ def test_beta(pol):
     s.sd[D_RZ] = 4.0
     pol.pol_beta(s.sd.ctypes.data_as(_d), s.si.ctypes.data_as(_i), 1.0)
     assert s.sd[D_BETA] == 0.25 and s.sd[D_RZ] == 1.0
+
+
+def test_launch_horizon_predicts_events(pol):
+    """The horizon (steps before a segment needs the host) follows the mean log reduction per step,
+    rounded down, so a launch of that many steps ends at or before the event; 1 while the rate is
+    unknown; 0 while a replacement is asked for; inactive segments never limit it."""
+    far = pol.pol_far()
+    s = Seg(pol)
+    s.init(1.0, mixed=False)
+    assert s.horizon(False) == (far, 1)  # no rate yet: one step at a time
+    s.step(0.25, 0, mixed=False)
+    assert s.sd[D_LOGQ] == pytest.approx(np.log(0.25))
+    # log(1e-8 / 0.25) / log(0.25) = 12.29 -> 12 steps to the threshold
+    assert s.horizon(False) == (far, 12)
+    s.step(0.0625 * 0.5, 1, mixed=False)  # log q = mean(log 0.25, log 0.125)
+    q = 0.5 * (np.log(0.25) + np.log(0.125))
+    assert s.horizon(False) == (far, int(np.log(1e-8 / 0.03125) / q))
+    # replay a geometric sequence: the predicted step count never overshoots the event
+    for rate in (0.2, 0.3, 0.45):
+        g = Seg(pol)
+        g.init(1.0, mixed=False)
+        rn, it, stops = 1.0, 0, []
+        while g.si[I_ACTIVE]:
+            rn *= rate
+            g.step(rn, it, mixed=False)
+            it += 1
+            if g.si[I_ACTIVE]:
+                stops.append((it, g.horizon(False)[1]))
+        for at, h in stops:
+            assert at + h <= it  # the segment leaves the active set at step `it`
+    # mixed form: the replacement trigger (drop x last true residual) comes first
+    m = Seg(pol, drop=1e-5)
+    m.init(1.0, mixed=True)
+    m.step(0.25, 0, mixed=True)
+    to_rel, to_done = m.horizon(True)
+    assert to_rel == int(np.log(1e-5 / 0.25) / np.log(0.25)) == 7
+    assert to_done == 12
+    # stall limit caps it
+    st = Seg(pol, drop=1e-5, stall=3)
+    st.init(1.0, mixed=True)
+    st.step(0.9, 0, mixed=True)  # not halved: one step towards the stall limit
+    assert st.si[I_SINCE] == 1
+    assert st.horizon(True)[0] <= 2
+    # a residual that does not fall: one step at a time
+    u = Seg(pol)
+    u.init(1.0, mixed=True)
+    u.step(1.2, 0, mixed=True)
+    assert u.horizon(True) == (1, 1)
+    # a pending replacement request: the host is needed now
+    r = Seg(pol, drop=1e-2)
+    r.init(1.0, mixed=True)
+    r.step(1e-3, 0, mixed=True)
+    assert r.si[I_NEEDREL] == 1 and r.horizon(True) == (0, 0)
+    # an inactive (converged) segment never limits the launch
+    d = Seg(pol)
+    d.init(1.0, mixed=False)
+    d.step(1e-9, 0, mixed=False)
+    assert d.si[I_ACTIVE] == 0 and d.horizon(False) == (far, 0)
