@@ -399,7 +399,8 @@ def main():
     log("phases fp64", prof64)
 
     near_names = {0: "k_fmm_near", 1: "k_fmm_near_lw", 2: "k_fmm_p2p", 3: "k_fmm_near_sl", 4: "k_fmm_near_cl",
-                  5: "k_fmm_near_st", 6: "k_fmm_near_st", 7: "k_fmm_near_st", 8: "k_fmm_near_st"}
+                  5: "k_fmm_near_st", 6: "k_fmm_near_st", 7: "k_fmm_near_st", 8: "k_fmm_near_st",
+                  9: "k_fmm_near_sym", 10: "k_fmm_near_sym", 11: "k_fmm_near_sym"}
 
     def roofline(prof, peak, prec):
         near_flop = FLOP_PAIR_DAMPED * pairs
@@ -411,6 +412,13 @@ def main():
             name = "%s (%s P2P + alpha^-1 mu)" % (near_names[v], "near field" if args.summation == "fmm"
                                                    else "all pairs, one-leaf tree")
             flop, ms = near_flop, (prof["near_kernel"] if prof["near_kernel"] > 0 else prof["near"])
+            if v in (9, 10, 11):
+                # symmetric form: each unordered pair evaluated once; the work is counted as the two
+                # ordered pairs it replaces, over the pair kernel plus the gather of the reverse
+                # blocks (with the L2P term) and the exclusion pass (the whole near pass)
+                name = ("k_fmm_near_sym + k_fmm_sym_gather (symmetric near field P2P + alpha^-1 mu; "
+                        "work counted as ordered pairs)")
+                ms = prof["near"]
         ach = flop / (ms * 1e-3)
         traffic = None
         if args.workload == "300k" and "k_fmm_near_st" in name and tune:

@@ -246,9 +246,10 @@ class System:
 
     NEAR_CANDIDATES = [(0, 64), (0, 128), (0, 256), (1, 64), (1, 128), (1, 256), (2, 64), (3, 64), (3, 128),
                        (3, 256), (4, 64), (4, 128), (4, 256), (5, 64), (5, 128), (5, 256), (6, 64), (6, 128),
-                       (6, 256), (7, 64), (7, 128), (7, 256), (8, 64), (8, 128), (8, 256)]
+                       (6, 256), (7, 64), (7, 128), (7, 256), (8, 64), (8, 128), (8, 256), (9, 64), (9, 128),
+                       (9, 256), (10, 64), (10, 128), (10, 256), (11, 64), (11, 128), (11, 256)]
     NEAR_NAMES = {0: "thread", 1: "leafwave", 2: "ldsleaf", 3: "slotted", 4: "cluster", 5: "staged64", 6: "staged32",
-                  7: "staged16", 8: "staged8"}
+                  7: "staged16", 8: "staged8", 9: "sym-large", 10: "sym-mid", 11: "sym-small"}
     M2L_CANDIDATES = [16, 32, 64]
 
     def autotune(self, reps=3):

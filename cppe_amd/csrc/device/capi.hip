@@ -679,7 +679,7 @@ int cppe_hip_set_tune(cppe_hip_system* sys, const int* choice) {
   if (!sys || !choice) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
   for (int prec = 0; prec < 2; ++prec) {
     const int v = choice[4 * prec], b = choice[4 * prec + 1], g = choice[4 * prec + 2], o = choice[4 * prec + 3];
-    if (v < 0 || v > 8 || (b != 64 && b != 128 && b != 256) || (v == 2 && b != 64) ||
+    if (v < 0 || v > 11 || (b != 64 && b != 128 && b != 256) || (v == 2 && b != 64) ||
         (g != 16 && g != 32 && g != 64) || !overlap_code_valid(o))
       return fail(CPPE_HIP_ERR_ARG, "invalid launch shape");
     if (!near_variant_allowed(sys->S, v))

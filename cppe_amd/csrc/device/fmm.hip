@@ -232,7 +232,8 @@ void fmm_reserve(System& S) {
   for (int prec = 0; prec < 2; ++prec) {
     const int v = F.tune.near_variant[prec];
     if (v == 3 || v >= 5) build_near_expanded(S);
-    if (v >= 5) ensure_near_items(S, v - 5);
+    if (v >= 5) ensure_near_items(S, near_sym_variant(v) ? 3 : v - 5);  // symmetric: other launches use chunks of 8
+    if (near_sym_variant(v)) ensure_sym(S, prec, kSymTs[prec][v - 9], S.opts.damping);
   }
   if (F.tune.near_variant[0] == 4 || F.tune.near_variant[1] == 4) ensure_clusters(S);
   CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
@@ -246,7 +247,9 @@ void fmm_reserve(System& S) {
 // source slot) lanes over the expanded near list, 4 wavefront per target cluster (<= 64 sites of
 // consecutive leaves) over the union of their near lists with lane masks, 5 wavefront per leaf
 // chunk (5: 64 targets, 6: 32, 7: 16, 8: 8), several per block, sources of the expanded near list staged
-// through wave-private LDS.
+// through wave-private LDS, 9, 10, 11 symmetric (near_sym.hh: each unordered pair once, lanes =
+// sources, chunks of 16 / 12 / 8 targets in FP32 and 8 / 6 / 4 in FP64; the system's other
+// launches then use variant 8).
 // Then, for a staged near variant, the whole apply is timed with the far field in series, beside
 // the near field on a second stream, and with only M2L / L2L beside it after the upward pass; the
 // fastest is kept (the overlapped timing reported is the fastest overlapped form); each
@@ -258,9 +261,10 @@ static const int kNearCand[][2] = {{0, 64}, {0, 128}, {0, 256}, {1, 64}, {1, 128
                                    {2, 64}, {3, 64},  {3, 128}, {3, 256}, {4, 64}, {4, 128},
                                    {4, 256}, {5, 64}, {5, 128}, {5, 256}, {6, 64}, {6, 128},
                                    {6, 256}, {7, 64}, {7, 128}, {7, 256}, {8, 64}, {8, 128},
-                                   {8, 256}};
+                                   {8, 256}, {9, 64}, {9, 128}, {9, 256}, {10, 64}, {10, 128},
+                                   {10, 256}, {11, 64}, {11, 128}, {11, 256}};
 static const int kM2lCand[]     = {16, 32, 64};
-constexpr int kNNear = 25, kNM2l = 3;
+constexpr int kNNear = 34, kNM2l = 3;
 
 // The LDS-staged wavefront-per-leaf kernel (variant 2) gives one wavefront all targets of a leaf:
 // its time grows with the square of the leaf size, so it is only a candidate for small leaves.

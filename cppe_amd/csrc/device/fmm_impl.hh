@@ -85,6 +85,15 @@ struct FmmState {
   dbuf<float> M32k, L32k;
   dbuf<double4> mu4k;
   dbuf<float4> mu4fk;
+  // symmetric near field (near_sym.hh): per-site target records -- FP32: offset from the leaf
+  // centre and damping rate (times log2 e for the branch-free pair), FP64: position and damping
+  // rate -- for the damping factor sym_tdamp, and the reverse slots of each precision
+  dbuf<float4> sym_t32;
+  dbuf<double4> sym_t64;
+  double sym_tdamp[2] = {-1.0, -1.0};
+  bool sym_tb2        = false;
+  dbuf<float> rev32;
+  dbuf<double> rev64;
 };
 
 // right-hand sides whose near fields one staged launch evaluates together
@@ -1219,6 +1228,28 @@ __global__ __launch_bounds__(256) void k_fmm_l2p_add(int n, const int* __restric
   out[2 * (size_t)n + i] += (double)lz;
 }
 
+#include "near_sym.hh"
+
+// Target records of the symmetric near field (see FmmState::sym_t32 / sym_t64).
+static __global__ void k_sym_trec32(int n, const double4* __restrict__ pos4, const int* __restrict__ leaf_of,
+                                    const double* __restrict__ cx, const double* __restrict__ cy,
+                                    const double* __restrict__ cz, double damping, int b2, float4* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c      = leaf_of[i];
+  const double4 p  = pos4[i];
+  const float di   = b2 ? staged_di<float, true>(damping, p.w) : staged_di<float, false>(damping, p.w);
+  out[i]           = make_float4((float)(p.x - cx[c]), (float)(p.y - cy[c]), (float)(p.z - cz[c]), di);
+}
+
+static __global__ void k_sym_trec64(int n, const double4* __restrict__ pos4, double damping,
+                                    double4* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 p = pos4[i];
+  out[i]          = make_double4(p.x, p.y, p.z, staged_di<double, false>(damping, p.w));
+}
+
 // ---- host drivers ----------------------------------------------------------------------------
 // FP32 M2L pair records (target-grouped list order): the displacement is formed in FP64 and
 // rounded, as the FP64-centre form of the kernel does, so both give the same FP32 operands.
@@ -1397,13 +1428,129 @@ inline void ensure_posf4(System& S) {
 
 void ensure_clusters(System& S);  // fmm.hip
 
+// targets per work item of the symmetric near-field variants 9, 10, 11: FP64 8, 6, 4 and FP32 16,
+// 12, 8 (larger chunks receive fewer reverse blocks; smaller ones leave registers for more
+// wavefronts per SIMD)
+constexpr int kSymTs[2][3] = {{8, 6, 4}, {16, 12, 8}};
+inline bool near_sym_variant(int v) { return v >= 9 && v <= 11; }
+
+// Lists (chunks of ts targets), target records (for this damping factor) and reverse slots of the
+// symmetric near field in precision prec (0 FP64, 1 FP32).
+inline SymLists& ensure_sym(System& S, int prec, int ts, double damping) {
+  FmmState& F     = *S.fmm;
+  const Octree& T = S.tree;
+  SymLists& Y     = build_sym_lists(S, ts);
+  ensure_pos4(S);
+  const dim3 g(ceil_div(S.n, 256)), b(256);
+  if (prec == 1) {
+    const bool b2 = S.all_damped && damping > 0.0;
+    if (!F.sym_t32.p || F.sym_tdamp[1] != damping || F.sym_tb2 != b2) {
+      F.sym_t32.alloc(S.n);
+      hipLaunchKernelGGL(k_sym_trec32, g, b, 0, S.stream, S.n, F.pos4.p, T.leaf_of.p, T.cx.p, T.cy.p, T.cz.p,
+                         damping, b2 ? 1 : 0, F.sym_t32.p);
+      F.sym_tdamp[1] = damping;
+      F.sym_tb2      = b2;
+    }
+  } else if (!F.sym_t64.p || F.sym_tdamp[0] != damping) {
+    F.sym_t64.alloc(S.n);
+    hipLaunchKernelGGL(k_sym_trec64, g, b, 0, S.stream, S.n, F.pos4.p, damping, F.sym_t64.p);
+    F.sym_tdamp[0] = damping;
+  }
+  if (prec)
+    ensure_buf(F.rev32, (size_t)std::max<long long>(Y.nrev, 1));
+  else
+    ensure_buf(F.rev64, (size_t)std::max<long long>(Y.nrev, 1));
+  return Y;
+}
+
+// out += the reverse blocks of each site's leaf (and its L2P term when L is given).
+template <int P, typename R>
+void launch_sym_gather(System& S, const SymLists& Y, const R* L, double* out) {
+  FmmState& F     = *S.fmm;
+  const Octree& T = S.tree;
+  const R* rev;
+  if constexpr (sizeof(R) == 4)
+    rev = F.rev32.p;
+  else
+    rev = F.rev64.p;
+  hipLaunchKernelGGL((k_fmm_sym_gather<P, R>), dim3(ceil_div(S.n, 256)), dim3(256), 0, S.stream, S.n, T.leaf_of.p,
+                     T.start.p, T.count.p, Y.rev_off.p, Y.nin.p, rev, F.pos4.p, T.cx.p, T.cy.p, T.cz.p, L, out);
+}
+
+// Symmetric near field of a B apply (variants 9-11, TS targets per chunk): the pair kernel, the
+// exclusion pass and the gather of the reverse blocks (beside the far field when it runs on the
+// second stream), then -- when the far field has run before it (with_l2p) -- the L2P pass;
+// otherwise the caller adds L2P after joining the far field.
+template <int P, typename R, bool DAMP, typename MU4, int TS>
+void launch_near_sym(System& S, const NearArgs& a, const MU4* mu4, const R* L, bool with_l2p) {
+  FmmState& F        = *S.fmm;
+  const Octree& T    = S.tree;
+  constexpr int prec = sizeof(R) == 4;
+  SymLists& Y        = ensure_sym(S, prec, TS, a.damping);
+  constexpr bool kB2 = prec && DAMP;
+  const bool b2      = kB2 && S.all_damped;
+  const long long* usp = (DAMP && prec && a.damping >= T.xdamp) ? Y.usplit.p : nullptr;
+  const int block    = F.tune.near_block[prec];
+  const int splits   = with_l2p ? 1 : std::max(1, F.tune.near_splits[prec]);
+  hipStream_t st     = S.stream;
+  if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[6], st));
+  for (int k = 0; k < splits; ++k) {
+    const int i0 = (int)((long long)Y.nitems * k / splits), i1 = (int)((long long)Y.nitems * (k + 1) / splits);
+    if (i1 <= i0) continue;
+    const SymArgs sa{Y.items.p + i0, i1 - i0, Y.uoff.p, usp, Y.usrc.p, Y.udst.p};
+    const dim3 g(ceil_div(i1 - i0, block / 64)), b(block);
+    if constexpr (prec) {
+      if (b2)
+        hipLaunchKernelGGL((k_fmm_near_sym<R, DAMP, kB2, TS, MU4, float4>), g, b, 0, st, a, sa, F.sym_t32.p, mu4,
+                           F.rev32.p);
+      else
+        hipLaunchKernelGGL((k_fmm_near_sym<R, DAMP, false, TS, MU4, float4>), g, b, 0, st, a, sa, F.sym_t32.p, mu4,
+                           F.rev32.p);
+    } else {
+      hipLaunchKernelGGL((k_fmm_near_sym<R, DAMP, false, TS, MU4, double4>), g, b, 0, st, a, sa, F.sym_t64.p, mu4,
+                         F.rev64.p);
+    }
+  }
+  if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[7], st));
+  if (a.eoff) {
+    if (b2)
+      hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, kB2, 1>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a, mu4);
+    else
+      hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, false, 1>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a,
+                         mu4);
+  }
+  launch_sym_gather<P, R>(S, Y, nullptr, a.out);
+  if (with_l2p)
+    hipLaunchKernelGGL((k_fmm_l2p_add<P, R>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, S.n, T.leaf_of.p,
+                       F.pos4.p, T.cx.p, T.cy.p, T.cz.p, L, a.out);
+}
+
+template <int P, typename R, bool DAMP, typename MU4>
+void launch_near_sym_v(System& S, int var, const NearArgs& a, const MU4* mu4, const R* L, bool with_l2p) {
+  constexpr int prec = sizeof(R) == 4;
+  if (var == 9)
+    launch_near_sym<P, R, DAMP, MU4, kSymTs[prec][0]>(S, a, mu4, L, with_l2p);
+  else if (var == 10)
+    launch_near_sym<P, R, DAMP, MU4, kSymTs[prec][1]>(S, a, mu4, L, with_l2p);
+  else
+    launch_near_sym<P, R, DAMP, MU4, kSymTs[prec][2]>(S, a, mu4, L, with_l2p);
+}
+
+
 template <int P, typename R, int MODE, bool DAMP, typename MU4, int K = 1>
 void launch_near(System& S, const NearArgs& a, const MU4* mu4, const R* L, bool with_l2p, size_t lstride = 0) {
   FmmState& F     = *S.fmm;
   const int prec  = sizeof(R) == 4;
   const int block = F.tune.near_block[prec];
-  const int var   = F.tune.near_variant[prec];
+  int var         = F.tune.near_variant[prec];
   hipStream_t st  = S.stream;
+  if (near_sym_variant(var)) {
+    if constexpr (K == 1 && MODE == MODE_BAPPLY) {
+      launch_near_sym_v<P, R, DAMP, MU4>(S, var, a, mu4, L, with_l2p);
+      return;
+    }
+    var = 8;
+  }
   if (var >= 5) {  // 5, 6, 7, 8: staged, chunks of 64, 32, 16, 8 targets
     const int ci = var - 5;
     const int ni = ensure_near_items(S, ci);

@@ -636,6 +636,163 @@ void build_near_expanded(System& S) {
   T.xdamp = S.opts.damping;
 }
 
+// ---- lists of the symmetric near field (near_sym.hh) -------------------------------------------
+// Per leaf B (thread per cell): over its near list (sorted by source leaf), the number of chunks
+// of the leaves A < B before each entry (block index of A's first chunk among B's incoming
+// reverse blocks) and in total.
+__global__ void k_sym_pre(int ncells, const int* __restrict__ off, const int* __restrict__ src,
+                          const int* __restrict__ nq, int* __restrict__ pre, int* __restrict__ nin) {
+  const int B = blockIdx.x * blockDim.x + threadIdx.x;
+  if (B >= ncells) return;
+  int run = 0;
+  for (int k = off[B]; k < off[B + 1]; ++k) {
+    const int A = src[k];
+    pre[k]      = run;
+    if (A < B) run += nq[A];
+  }
+  nin[B] = run;
+}
+
+// Per leaf A: upper-list length (own sites + sites of near leaves B > A) and reverse slots it
+// receives (incoming blocks x 3 |A|).
+__global__ void k_sym_count(int ncells, const int* __restrict__ off, const int* __restrict__ src,
+                            const int* __restrict__ count, const int* __restrict__ nin,
+                            long long* __restrict__ ucnt, long long* __restrict__ rsz) {
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A > ncells) return;
+  if (A == ncells || off[A + 1] == off[A]) {  // the scans' last entry; cells without a near list
+    ucnt[A] = 0;
+    rsz[A]  = 0;
+    return;
+  }
+  long long c = count[A];
+  for (int k = off[A]; k < off[A + 1]; ++k)
+    if (src[k] > A) c += count[src[k]];
+  ucnt[A] = c;
+  rsz[A]  = (long long)nin[A] * 3 * count[A];
+}
+
+// One wavefront per leaf A: its own sites, then the entries of its expanded near list whose leaf
+// is above A (order kept, compaction by ballot), each with its reverse slot; usplit: position of
+// the first kept entry at or after the FP32 far split of the expanded list.
+__global__ __launch_bounds__(64) void k_sym_fill(const long long* __restrict__ xoff, const int* __restrict__ xsrc,
+                                                 const long long* __restrict__ xs32, const int* __restrict__ leaf_of,
+                                                 const int* __restrict__ start, const int* __restrict__ count,
+                                                 const int* __restrict__ p2p_off, const int* __restrict__ p2p_src,
+                                                 const int* __restrict__ pre, const long long* __restrict__ rev_off,
+                                                 const long long* __restrict__ uoff, int* __restrict__ usrc,
+                                                 int2* __restrict__ udst, long long* __restrict__ usplit,
+                                                 int* __restrict__ bad) {
+  const int A = blockIdx.x, lane = threadIdx.x;
+  const long long u0 = uoff[A], u1 = uoff[A + 1];
+  if (u1 == u0) {
+    if (lane == 0) usplit[A] = u1;
+    return;
+  }
+  const int na = count[A];
+  for (int u = lane; u < na; u += 64) {
+    usrc[u0 + u] = start[A] + u;
+    udst[u0 + u] = make_int2(0, 0);
+  }
+  long long pos   = u0 + na;
+  const long long x0 = xoff[A], x1 = xoff[A + 1];
+  const long long split = xs32 ? xs32[A] : x1;
+  long long usp   = -1;
+  for (long long e0 = x0; e0 < x1; e0 += 64) {
+    const long long e = e0 + lane;
+    const bool valid  = e < x1;
+    const int j       = valid ? xsrc[e] : 0;
+    const int B       = valid ? leaf_of[j] : -1;
+    const bool keep   = valid && B > A;
+    const unsigned long long m = __ballot(keep);
+    if (usp < 0 && split < e0 + 64) usp = pos + __popcll(__ballot(keep && e < split));
+    if (keep) {
+      const long long idx = pos + __popcll(m & ((1ull << lane) - 1ull));
+      int lo = p2p_off[B], hi = p2p_off[B + 1];  // A in B's near list (sorted by source leaf)
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (p2p_src[mid] < A)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      usrc[idx] = j;
+      if (lo < p2p_off[B + 1] && p2p_src[lo] == A) {
+        const long long slot = rev_off[B] + (long long)pre[lo] * 3 * count[B] + (j - start[B]);
+        udst[idx]            = make_int2((int)slot, count[B]);
+      } else {  // B near A but not A near B: the lists are not symmetric
+        udst[idx] = make_int2(0, 0);
+        *bad      = 1;
+      }
+    }
+    pos += __popcll(m);
+  }
+  if (usp < 0) usp = pos;
+  if (lane == 0) usplit[A] = usp;
+}
+
+SymLists& build_sym_lists(System& S, int ts) {
+  Octree& T    = S.tree;
+  SymLists& Y  = T.sym[sym_slot(ts)];
+  if (Y.ts == ts) return Y;
+  build_near_expanded(S);
+  hipStream_t st = S.stream;
+  const int nc   = T.ncells;
+  std::vector<int> ids(T.nleaves), cnt(nc), nq(nc, 0);
+  CPPE_HIP_CHECK(hipMemcpyAsync(ids.data(), T.leaf_ids.p, sizeof(int) * T.nleaves, hipMemcpyDeviceToHost, st));
+  CPPE_HIP_CHECK(hipMemcpyAsync(cnt.data(), T.count.p, sizeof(int) * nc, hipMemcpyDeviceToHost, st));
+  CPPE_HIP_CHECK(hipStreamSynchronize(st));
+  std::vector<int4> items;
+  items.reserve(T.nleaves + T.nleaves / 4);
+  for (int c : ids) {
+    nq[c] = (cnt[c] + ts - 1) / ts;
+    for (int q = 0; q < nq[c]; ++q) items.push_back(make_int4(c, q * ts, std::min(ts, cnt[c] - q * ts), q));
+  }
+  dbuf<int> d_nq, pre;
+  d_nq.alloc(nc);
+  pre.alloc(std::max<long long>(T.n_p2p, 1));
+  Y.nin.alloc(nc);
+  CPPE_HIP_CHECK(hipMemcpyAsync(d_nq.p, nq.data(), sizeof(int) * nc, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_sym_pre, dim3(ceil_div(nc, 256)), dim3(256), 0, st, nc, T.p2p_off.p, T.p2p_src.p, d_nq.p,
+                     pre.p, Y.nin.p);
+  dbuf<long long> ucnt, rsz;
+  ucnt.alloc(nc + 1);
+  rsz.alloc(nc + 1);
+  hipLaunchKernelGGL(k_sym_count, dim3(ceil_div(nc + 1, 256)), dim3(256), 0, st, nc, T.p2p_off.p, T.p2p_src.p,
+                     T.count.p, Y.nin.p, ucnt.p, rsz.p);
+  Y.uoff.alloc(nc + 1);
+  Y.rev_off.alloc(nc + 1);
+  dbuf<char> tmp;
+  exclusive_scan(ucnt.p, Y.uoff.p, nc + 1, st, tmp);
+  exclusive_scan(rsz.p, Y.rev_off.p, nc + 1, st, tmp);
+  long long tot[2] = {0, 0};
+  CPPE_HIP_CHECK(hipMemcpyAsync(&tot[0], Y.uoff.p + nc, sizeof(long long), hipMemcpyDeviceToHost, st));
+  CPPE_HIP_CHECK(hipMemcpyAsync(&tot[1], Y.rev_off.p + nc, sizeof(long long), hipMemcpyDeviceToHost, st));
+  CPPE_HIP_CHECK(hipStreamSynchronize(st));
+  if (tot[1] >= (1ll << 31) - 1)
+    throw hip_error("symmetric near field: more reverse slots than a 32-bit index addresses");
+  Y.usrc.alloc(std::max<long long>(tot[0], 1));
+  Y.udst.alloc(std::max<long long>(tot[0], 1));
+  Y.usplit.alloc(nc);
+  const long long* xs = S.opts.damping >= T.xdamp ? T.xsplit[1].p : nullptr;
+  dbuf<int> bad;
+  bad.alloc(1);
+  int h_bad = 0;
+  CPPE_HIP_CHECK(hipMemcpyAsync(bad.p, &h_bad, sizeof(int), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_sym_fill, dim3(nc), dim3(64), 0, st, T.xoff.p, T.xsrc.p, xs, T.leaf_of.p, T.start.p,
+                     T.count.p, T.p2p_off.p, T.p2p_src.p, pre.p, Y.rev_off.p, Y.uoff.p, Y.usrc.p, Y.udst.p,
+                     Y.usplit.p, bad.p);
+  Y.items.alloc(std::max<size_t>(items.size(), 1));
+  CPPE_HIP_CHECK(hipMemcpyAsync(Y.items.p, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, st));
+  CPPE_HIP_CHECK(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  CPPE_HIP_CHECK(hipStreamSynchronize(st));
+  if (h_bad) throw hip_error("symmetric near field: the near lists of this tree are not symmetric");
+  Y.nitems = (int)items.size();
+  Y.nrev   = tot[1];
+  Y.ts     = ts;
+  return Y;
+}
+
 void build_lists(System& S) {
   Octree& T      = S.tree;
   hipStream_t st = S.stream;

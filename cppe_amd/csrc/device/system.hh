@@ -34,6 +34,22 @@ struct dbuf {
   operator T*() const { return p; }
 };
 
+// Lists of the symmetric near-field kernel (near_sym.hh) for chunks of ts targets: the work items;
+// per leaf A its upper list (A's own sites, then the sites of its near leaves B > A in the order of
+// the expanded near list), with the reverse slot of each entry (for chunk 0 and component x, and
+// |B|); per leaf B the layout of its incoming reverse blocks (first slot, block count).
+struct SymLists {
+  int ts = 0;            // targets per chunk (0: not built)
+  int nitems = 0;
+  long long nrev = 0;    // reverse slots
+  dbuf<int4> items;      // (leaf, first target, targets, chunk index)
+  dbuf<long long> uoff;  // ncells + 1
+  dbuf<long long> usplit;  // per cell: first entry of the far class (FP32 cutoff)
+  dbuf<long long> rev_off; // ncells + 1
+  dbuf<int> usrc, nin;
+  dbuf<int2> udst;
+};
+
 // Octree in level order (level 0 = root). Sites are sorted by Morton key ("device order");
 // every cell covers the contiguous device-order range [start, start + count).
 struct Octree {
@@ -59,6 +75,7 @@ struct Octree {
   dbuf<long long> xsplit[2];
   dbuf<double> xbox;
   double xdamp = 0.0;  // damping factor of the classification (valid for kernels with damping >= xdamp)
+  SymLists sym[5];     // symmetric near field with chunks of 4, 6, 8, 12, 16 targets (sym_slot)
   bool built = false;
 };
 
@@ -184,6 +201,10 @@ struct System {
 void build_octree(System& S);
 void build_lists(System& S);
 void build_near_expanded(System& S);
+// lists of the symmetric near-field kernel with chunks of ts targets (4, 6, 8, 12 or 16) into
+// S.tree.sym[sym_slot(ts)]
+inline int sym_slot(int ts) { return ts <= 4 ? 0 : ts <= 6 ? 1 : ts <= 8 ? 2 : ts <= 12 ? 3 : 4; }
+SymLists& build_sym_lists(System& S, int ts);
 
 // field of order-m sources given in device-order SoA (ncomp(m) arrays of n) into F (3n SoA)
 void direct_field(System& S, int m, const double* src, double* F, double damping);

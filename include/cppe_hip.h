@@ -229,8 +229,15 @@ int cppe_hip_last_solve_info(cppe_hip_system* sys, int* info);
  *      of their near lists with per-source-leaf lane masks, block 64/128/256;
  *   5, 6, 7, 8: one wavefront per leaf chunk of at most 64, 32, 16, 8 targets, several per block, the
  *      expanded near list staged 64 sources at a time through wavefront-private LDS, lanes =
- *      (target, source slot), block 64/128/256.
- * Then, for a staged near variant (5-8), the whole apply is timed with the far field (P2M, M2M,
+ *      (target, source slot), block 64/128/256;
+ *   9, 10, 11: symmetric: each unordered near pair evaluated once for the field at both sites
+ *      (the tensor is even in the displacement); one wavefront per chunk of at most 16 / 12 / 8
+ *      (FP32) or 8 / 6 / 4 (FP64) targets of a leaf walking its own sites and those of its near
+ *      leaves with a higher index, lanes = sources; the field at those sources goes to per-leaf
+ *      reverse blocks that a gather pass adds in a fixed order (with the L2P term); block
+ *      64/128/256. Single-right-hand-side B applies only: the system's other near-field launches
+ *      then use variant 8.
+ * Then, for a staged or symmetric near variant (5-11), the whole apply is timed with the far field (P2M, M2M,
  * M2L, L2L) in series, on a second stream beside the near field (joined before an L2P pass), and
  * with only M2L / L2L beside the near field after the upward pass (P2M, M2M) has run, each
  * overlapped form with the near field in 1, 2 and 4 launches; the fastest is kept.
@@ -238,7 +245,7 @@ int cppe_hip_last_solve_info(cppe_hip_system* sys, int* info);
  *            4 log2(near launches): form 0 = series, 1 = whole far field beside the near field,
  *            2 = upward pass first; with forms 1 and 2 the near field may be issued as 1, 2 or 4
  *            launches of consecutive work items (so 0, 1, 2, 5, 6, 9, 10 are valid).
- * ms[60]: per precision, the 25 near timings (order above), the 3 M2L timings (ms/launch), then
+ * ms[78]: per precision, the 34 near timings (order above), the 3 M2L timings (ms/launch), then
  *         the apply in series and overlapped (the faster overlapped form; 0 when not timed).
  * Launch shapes never change results beyond FP summation order.
  */

@@ -91,7 +91,7 @@ def test_solve_with_each_overlap_form(precision):
 def test_set_tune_rejects_unknown_shapes():
     _, S = system("fp64")
     with pytest.raises(capi.CppeHipError):
-        S.set_tune([9, 256, 64, 0, 0, 256, 64, 0])
+        S.set_tune([12, 256, 64, 0, 0, 256, 64, 0])
     with pytest.raises(capi.CppeHipError):
         S.set_tune([0, 96, 64, 0, 0, 256, 64, 0])
     for bad_overlap in (3, 4, 7, 12, 13, -1):  # form 3, split series form, 8 launches, ...
