@@ -27,7 +27,8 @@ EXPORTS = [
     "cppe_hip_solve_multi", "cppe_hip_apply_b_multi",
     "cppe_hip_mlp_create", "cppe_hip_mlp_destroy", "cppe_hip_mlp_set", "cppe_hip_mlp_get", "cppe_hip_ml_features",
     "cppe_hip_mlp_forward", "cppe_hip_mlp_loss_grad", "cppe_hip_mlp_reserve", "cppe_hip_mlp_collect",
-    "cppe_hip_mlp_prepare", "cppe_hip_mlp_train", "cppe_hip_ml_guess",
+    "cppe_hip_mlp_prepare", "cppe_hip_mlp_train", "cppe_hip_ml_guess", "cppe_hip_mlp_set_objective",
+    "cppe_hip_mlp_samples",
 ]
 MLP_IN = 64
 MLP_PARAMS = 25219
@@ -112,6 +113,8 @@ def lib():
         L.cppe_hip_mlp_prepare.argtypes = [vp]
         L.cppe_hip_mlp_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint, _d]
         L.cppe_hip_ml_guess.argtypes = [vp, vp, _d]
+        L.cppe_hip_mlp_set_objective.argtypes = [vp, ctypes.c_int]
+        L.cppe_hip_mlp_samples.argtypes = [vp, ctypes.c_longlong, ctypes.c_longlong, _f, _f, _f]
         _lib = L
     return _lib
 
@@ -517,6 +520,20 @@ class Mlp:
 
     def prepare(self):
         _check(lib().cppe_hip_mlp_prepare(self.h))
+
+    OBJECTIVES = {"dipole": 0, "alpha_inv": 1}
+
+    def set_objective(self, objective):
+        """Training objective (before collecting): 'dipole' error, or its 'alpha_inv' norm (cppe_hip_mlp_set_objective)."""
+        _check(lib().cppe_hip_mlp_set_objective(self.h, self.OBJECTIVES[objective]))
+
+    def samples(self, s0, count):
+        """Collected samples as the loss sees them: V (count, 9), T (count, 3), weights (count,)."""
+        V = np.zeros((count, 9), dtype=np.float32)
+        T = np.zeros((count, 3), dtype=np.float32)
+        w = np.zeros(count, dtype=np.float32)
+        _check(lib().cppe_hip_mlp_samples(self.h, int(s0), int(count), _pf(V), _pf(T), _pf(w)))
+        return V, T, w
 
     def train(self, epochs, batch=4096, lr=1e-3, seed=0):
         losses = np.zeros(max(epochs, 1))

@@ -13,16 +13,41 @@ namespace cppe_dev {
 void precondition(System& S, const double* in, double* out);
 int capi_fail(int code, const std::string& msg);
 
-// samples from device-order features/bases/moments: V (9 floats), T (3 floats), weight 1/|v1|^2
+// Samples from device-order features/bases/moments: V (9 floats), T (3 floats), weight 1/|v1|^2,
+// so the loss sum_s w_s |sum_a y_sa V_sa - T_s|^2 is the squared dipole error relative to alpha F.
+// With the alpha^-1 objective (ainv6 given) the bases and the target are mapped by L^T, where
+// L L^T = alpha^-1 is the site's Cholesky factor: |L^T e|^2 = e^T alpha^-1 e, the site's diagonal
+// block of the B-norm e^T B e (B = alpha^-1 - T) that CG minimises, relative to |v1| in that norm.
 __global__ void k_ml_samples(int n, const double* __restrict__ Vsoa, const double* __restrict__ mu,
-                             float* __restrict__ V, float* __restrict__ T, float* __restrict__ W) {
+                             const double* __restrict__ ainv6, float* __restrict__ V, float* __restrict__ T,
+                             float* __restrict__ W) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  double v[3][3], t[3];
   for (int a = 0; a < 3; ++a)
-    for (int c = 0; c < 3; ++c) V[(size_t)i * 9 + 3 * a + c] = (float)Vsoa[(size_t)(3 * a + c) * n + i];
-  for (int c = 0; c < 3; ++c) T[(size_t)i * 3 + c] = mu ? (float)mu[(size_t)c * n + i] : 0.f;
-  const double v0 = Vsoa[i], v1 = Vsoa[n + i], v2 = Vsoa[2 * (size_t)n + i];
-  W[i] = (float)(1.0 / (v0 * v0 + v1 * v1 + v2 * v2 + 1e-30));
+    for (int c = 0; c < 3; ++c) v[a][c] = Vsoa[(size_t)(3 * a + c) * n + i];
+  for (int c = 0; c < 3; ++c) t[c] = mu ? mu[(size_t)c * n + i] : 0.0;
+  if (ainv6) {
+    const double a11 = ainv6[i], a12 = ainv6[n + i], a13 = ainv6[2 * (size_t)n + i];
+    const double a22 = ainv6[3 * (size_t)n + i], a23 = ainv6[4 * (size_t)n + i], a33 = ainv6[5 * (size_t)n + i];
+    const double l11 = sqrt(a11), l21 = a12 / l11, l31 = a13 / l11;
+    const double l22 = sqrt(a22 - l21 * l21), l32 = (a23 - l31 * l21) / l22;
+    const double l33 = sqrt(a33 - l31 * l31 - l32 * l32);
+    auto lt = [&](double* u) {  // u <- L^T u
+      const double u0 = l11 * u[0] + l21 * u[1] + l31 * u[2];
+      const double u1 = l22 * u[1] + l32 * u[2];
+      const double u2 = l33 * u[2];
+      u[0] = u0;
+      u[1] = u1;
+      u[2] = u2;
+    };
+    for (int a = 0; a < 3; ++a) lt(v[a]);
+    lt(t);
+  }
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) V[(size_t)i * 9 + 3 * a + c] = (float)v[a][c];
+  for (int c = 0; c < 3; ++c) T[(size_t)i * 3 + c] = (float)t[c];
+  W[i] = (float)(1.0 / (v[0][0] * v[0][0] + v[0][1] * v[0][1] + v[0][2] * v[0][2] + 1e-30));
 }
 
 // guess mu = sum_a y_a v_a into 3n SoA (device order)
@@ -79,6 +104,7 @@ struct cppe_hip_mlp {
   Mlp M;
   dbuf<float> rawX;  // collected raw features (before prepare)
   bool prepared = false;
+  int objective = 0;  // 0: dipole error, 1: alpha^-1-norm error (set before collecting)
 };
 struct cppe_hip_system {
   System S;
@@ -271,10 +297,32 @@ int cppe_hip_mlp_collect(cppe_hip_mlp* m, cppe_hip_system* sys) {
   dV.alloc(9 * (size_t)S.n);
   ml_features(S, m->rawX.p + (size_t)M.ns * kMlpIn, dV.p);
   hipLaunchKernelGGL(k_ml_samples, dim3(ceil_div(S.n, 256)), dim3(256), 0, S.stream, S.n, dV.p, S.v_mu.p,
-                     M.dV.p + (size_t)M.ns * 9, M.dT.p + (size_t)M.ns * 3, M.dS.p + M.ns);
+                     m->objective == 1 ? S.ainv6.p : nullptr, M.dV.p + (size_t)M.ns * 9, M.dT.p + (size_t)M.ns * 3,
+                     M.dS.p + M.ns);
   CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
   M.ns += S.n;
   m->prepared = false;
+  return CPPE_HIP_OK;
+  ML_END
+}
+
+int cppe_hip_mlp_set_objective(cppe_hip_mlp* m, int objective) {
+  ML_BEGIN
+  if (!m || objective < 0 || objective > 1) return capi_fail(CPPE_HIP_ERR_ARG, "objective must be 0 or 1");
+  if (m->M.ns > 0) return capi_fail(CPPE_HIP_ERR_ARG, "set the objective before collecting");
+  m->objective = objective;
+  return CPPE_HIP_OK;
+  ML_END
+}
+
+int cppe_hip_mlp_samples(cppe_hip_mlp* m, long long s0, long long count, float* V, float* T, float* w) {
+  ML_BEGIN
+  if (!m || s0 < 0 || count < 0 || s0 + count > m->M.ns || !V || !T || !w)
+    return capi_fail(CPPE_HIP_ERR_ARG, "invalid arguments");
+  Mlp& M = m->M;
+  CPPE_HIP_CHECK(hipMemcpy(V, M.dV.p + s0 * 9, sizeof(float) * 9 * count, hipMemcpyDeviceToHost));
+  CPPE_HIP_CHECK(hipMemcpy(T, M.dT.p + s0 * 3, sizeof(float) * 3 * count, hipMemcpyDeviceToHost));
+  CPPE_HIP_CHECK(hipMemcpy(w, M.dS.p + s0, sizeof(float) * count, hipMemcpyDeviceToHost));
   return CPPE_HIP_OK;
   ML_END
 }

@@ -5,8 +5,10 @@ FP64 solver itself produces: seeded synthetic water boxes of 3k-30k sites (the b
 recipe with per-snapshot seeds), static fields as right-hand side, FMM summation (theta 0.5,
 order 5, ncrit 64), Thole damping 2.1304, CG to 1e-8. Every site of every snapshot is a sample.
 
-    python -m cppe_amd.warmstart --snapshots 2000 --epochs 8 --out cppe_amd/data/warmstart_mlp.npz
+    python -m cppe_amd.warmstart --snapshots 2000 --epochs 8 --objective alpha_inv --out cppe_amd/data/warmstart_mlp.npz
 
+Objectives: 'dipole' (squared dipole error relative to alpha F) or 'alpha_inv' (the error in the
+alpha^-1 norm, e^T alpha^-1 e per site: the diagonal blocks of the B-norm e^T B e that CG minimises).
 The model file holds the FP32 parameters, the feature scaling and the training metadata.
 """
 import argparse
@@ -47,9 +49,10 @@ def solved_system(w, thresh=1e-8, summation="fmm"):
     return S, it
 
 
-def train(n_snapshots=2000, epochs=8, batch=4096, lr=1e-3, log=print):
+def train(n_snapshots=2000, epochs=8, batch=4096, lr=1e-3, objective="dipole", log=print):
     sizes = snapshot_sizes(n_snapshots)
     M = capi.Mlp()
+    M.set_objective(objective)
     M.reserve(int(3 * sizes.sum()))
     t0 = time.time()
     iters = []
@@ -69,7 +72,7 @@ def train(n_snapshots=2000, epochs=8, batch=4096, lr=1e-3, log=print):
         losses += list(M.train(1, batch, float(lr_e), seed=1 + e))
         log("epoch %d lr %.2e loss %.5e (%.0f s)" % (e, lr_e, losses[-1], time.time() - t0))
     meta = dict(n_snapshots=int(n_snapshots), sites=int(3 * sizes.sum()), epochs=int(epochs), batch=int(batch),
-                lr=float(lr), lr_schedule="cosine to 5%", losses=[float(x) for x in losses],
+                lr=float(lr), lr_schedule="cosine to 5%", objective=objective, losses=[float(x) for x in losses],
                 mean_cg_iterations=float(np.mean(iters)), seconds=time.time() - t0)
     meta["validation"] = validate(M, log=log)
     return M, meta
@@ -79,27 +82,38 @@ VALID_SEED0 = 40000
 
 
 def validate(M, n=16, log=print):
-    """Held-out snapshots (seeds outside the training range): dipole error of the learned guess and
-    of alpha F against the FP64 solution, and CG iterations from each guess to 1e-8."""
+    """Held-out snapshots (seeds outside the training range): error of the learned guess and of alpha F
+    against the FP64 solution (dipole norm, and B-norm sqrt(e^T B e) with the system's FP64 operator),
+    and CG iterations from each guess to 1e-8 with the alpha r preconditioner and with the local one."""
     rows = []
     for k, nw in enumerate(snapshot_sizes(n, seed=VALID_SEED0)):
         w = synthetic.water_box(int(nw), VALID_SEED0 + 1 + k, anisotropic=True)
         S, it_aF = solved_system(w)
         mu = S.download_mu()
+        F = static_rhs(w)
+        aF = S.apply_alpha(F)
         capi.ml_guess(S, M)
         g = S.download_mu()
         it_ml, _, _ = S.solve_resident(1, 1e-8, 500)
-        aF = S.apply_alpha(static_rhs(w))
+        S.set_preconditioner("local")
+        it_aF_loc, _, _ = S.solve_resident(0, 1e-8, 500)
+        capi.ml_guess(S, M)
+        it_ml_loc, _, _ = S.solve_resident(1, 1e-8, 500)
+        bnorm = lambda e: float(np.sqrt(max(e @ S.apply_b(e), 0.0)))
         rows.append(dict(sites=int(w.n), err_ml=float(np.linalg.norm(g - mu) / np.linalg.norm(mu)),
                          err_alphaF=float(np.linalg.norm(aF - mu) / np.linalg.norm(mu)),
-                         iters_alphaF=int(it_aF), iters_ml=int(it_ml)))
+                         berr_ml=bnorm(g - mu) / bnorm(mu), berr_alphaF=bnorm(aF - mu) / bnorm(mu),
+                         iters_alphaF=int(it_aF), iters_ml=int(it_ml),
+                         iters_alphaF_local=int(it_aF_loc), iters_ml_local=int(it_ml_loc)))
         S.close()
-    out = dict(snapshots=rows, mean_err_ml=float(np.mean([r["err_ml"] for r in rows])),
-               mean_err_alphaF=float(np.mean([r["err_alphaF"] for r in rows])),
-               mean_iters_alphaF=float(np.mean([r["iters_alphaF"] for r in rows])),
-               mean_iters_ml=float(np.mean([r["iters_ml"] for r in rows])))
-    log("validation: relative dipole error %.3e (alpha F %.3e), CG iterations %.2f (alpha F %.2f)"
-        % (out["mean_err_ml"], out["mean_err_alphaF"], out["mean_iters_ml"], out["mean_iters_alphaF"]))
+    out = dict(snapshots=rows)
+    for key in ("err_ml", "err_alphaF", "berr_ml", "berr_alphaF", "iters_alphaF", "iters_ml", "iters_alphaF_local",
+                "iters_ml_local"):
+        out["mean_" + key] = float(np.mean([r[key] for r in rows]))
+    log("validation: relative dipole error %.3e (alpha F %.3e), B-norm %.3e (alpha F %.3e), CG iterations "
+        "%.2f (alpha F %.2f), with the local preconditioner %.2f (alpha F %.2f)"
+        % (out["mean_err_ml"], out["mean_err_alphaF"], out["mean_berr_ml"], out["mean_berr_alphaF"],
+           out["mean_iters_ml"], out["mean_iters_alphaF"], out["mean_iters_ml_local"], out["mean_iters_alphaF_local"]))
     return out
 
 
@@ -123,10 +137,16 @@ def main():
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--objective", default="alpha_inv", choices=sorted(capi.Mlp.OBJECTIVES))
     ap.add_argument("--out", default=DEFAULT_MODEL)
+    ap.add_argument("--validate-only", metavar="MODEL", default=None,
+                    help="validate an existing model file and print its metrics")
     args = ap.parse_args()
-    M, meta = train(args.snapshots, args.epochs, args.batch, args.lr,
-                    log=lambda *a: print(*a, file=sys.stderr, flush=True))
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    if args.validate_only:
+        print(json.dumps(validate(load(args.validate_only), log=log)))
+        return 0
+    M, meta = train(args.snapshots, args.epochs, args.batch, args.lr, args.objective, log=log)
     save(M, meta, args.out)
     print(json.dumps(meta))
     return 0

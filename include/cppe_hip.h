@@ -350,6 +350,13 @@ int cppe_hip_mlp_loss_grad(cppe_hip_mlp* m, const float* X, const float* V, cons
 int cppe_hip_mlp_reserve(cppe_hip_mlp* m, long long ns);
 int cppe_hip_mlp_collect(cppe_hip_mlp* m, cppe_hip_system* sys);
 int cppe_hip_mlp_prepare(cppe_hip_mlp* m);
+/* Training objective, set before collecting: 0 (default) the squared dipole error relative to alpha F,
+ * 1 the error in the alpha^-1 norm (e^T alpha^-1 e per site: the diagonal blocks of the B-norm that
+ * CG minimises) relative to alpha F in that norm -- the collected bases and targets are mapped by
+ * L^T, L L^T = alpha^-1 (Cholesky), so the same loss kernel trains either objective. */
+int cppe_hip_mlp_set_objective(cppe_hip_mlp* m, int objective);
+/* Collected samples s0 .. s0 + count - 1 as the loss sees them: V (9 floats), T (3), weight (1). */
+int cppe_hip_mlp_samples(cppe_hip_mlp* m, long long s0, long long count, float* V, float* T, float* w);
 /* Adam epochs over the collected set (minibatches of `batch` samples, tiles shuffled with `seed`);
  * losses receives the mean training loss of each epoch. */
 int cppe_hip_mlp_train(cppe_hip_mlp* m, int epochs, int batch, float lr, unsigned seed, double* losses);

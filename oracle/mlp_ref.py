@@ -106,6 +106,19 @@ def loss_grad(p, mean, inv_std, X, V, T, w, emulate_bf16=True):
 
 
 # ---------------------------------------------------------------------------------------------
+def alpha_inv_factor(alpha6):
+    """L^T with L L^T = alpha^-1 (Cholesky) per site, alpha6 packed (xx, xy, xz, yy, yz, zz): the map
+    under which the squared dipole error becomes the alpha^-1-norm error, |L^T e|^2 = e^T alpha^-1 e
+    (the training samples of the 'alpha_inv' objective are V and T mapped by it)."""
+    a = np.asarray(alpha6, dtype=np.float64).reshape(-1, 6)
+    A = np.empty((a.shape[0], 3, 3))
+    for k, (r, c) in enumerate([(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]):
+        A[:, r, c] = a[:, k]
+        A[:, c, r] = a[:, k]
+    L = np.linalg.cholesky(np.linalg.inv(A))
+    return np.transpose(L, (0, 2, 1))
+
+
 def _thole(u):
     e = np.exp(-u)
     fE = 1.0 - (1.0 + u + 0.5 * u * u) * e

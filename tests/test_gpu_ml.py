@@ -5,7 +5,10 @@
 * forward: BF16-MFMA network vs the FP64 reference with the same bfloat16 rounding points
 * gradient: GPU backward vs the FP64 reference gradient (bf16-emulated tight, pure FP64 loose)
 * training + guess: a small training run lowers the loss and the learned guess needs fewer CG
-  iterations than alpha F on a held-out snapshot, with the same 1e-8 FP64 stopping test.
+  iterations than alpha F on a held-out snapshot, with the same 1e-8 FP64 stopping test (both
+  training objectives: dipole error and its alpha^-1 norm)
+* alpha^-1 objective: the collected samples are the dipole-objective samples mapped by L^T,
+  L L^T = alpha^-1 (oracle: numpy Cholesky; |L^T e|^2 = e^T alpha^-1 e checked on its own)
 """
 import numpy as np
 import pytest
@@ -79,10 +82,41 @@ def test_gradient_vs_reference():
     assert np.linalg.norm(g - g64) <= 5e-2 * np.linalg.norm(g64)
 
 
-def test_training_and_guess_gain():
+def test_alpha_inv_objective_samples():
+    """Every site gets the same anisotropic polarizability, so the sample map is one known L^T."""
+    b = synthetic.water_box(300, 3100, anisotropic=True)
+    a6 = np.array([5.0, 0.7, -0.3, 4.0, 0.4, 6.0])
+    pol = np.tile(a6, (b.n, 1))
+    iso = np.full(b.n, (a6[0] + a6[3] + a6[5]) / 3.0)
+    S = capi.System(b.pos, pol, iso, b.excl_csr(), damping=2.1304)
+    S.upload_rhs(rhs_of(b))
+    S.solve_resident(0, 1e-10, 200)
+    M0, M1 = capi.Mlp(), capi.Mlp()
+    M1.set_objective("alpha_inv")
+    for M in (M0, M1):
+        M.reserve(b.n)
+        M.collect(S)
+    V0, T0, w0 = M0.samples(0, b.n)
+    V1, T1, w1 = M1.samples(0, b.n)
+    Lt = R.alpha_inv_factor(a6)[0]
+    Ainv = np.linalg.inv(np.array([[a6[0], a6[1], a6[2]], [a6[1], a6[3], a6[4]], [a6[2], a6[4], a6[5]]]))
+    e = np.random.default_rng(5).normal(size=(50, 3))
+    assert np.allclose(np.sum((e @ Lt.T) ** 2, axis=1), np.einsum("si,ij,sj->s", e, Ainv, e), rtol=1e-12)
+    exp_V = (V0.reshape(-1, 3, 3).astype(np.float64) @ Lt.T).reshape(-1, 9)
+    exp_T = T0.astype(np.float64) @ Lt.T
+    assert np.abs(V1 - exp_V).max() < 1e-6 * np.abs(exp_V).max()
+    assert np.abs(T1 - exp_T).max() < 1e-6 * np.abs(exp_T).max()
+    assert np.allclose(w1, 1.0 / np.sum(exp_V[:, :3] ** 2, axis=1), rtol=1e-5)
+    with pytest.raises(capi.CppeHipError):  # the objective is fixed once samples are collected
+        M1.set_objective("dipole")
+
+
+@pytest.mark.parametrize("objective", ["dipole", "alpha_inv"])
+def test_training_and_guess_gain(objective):
     """Train on a few small boxes; on a held-out box the learned guess must be closer to the solution
     than alpha F (error and initial residual), and CG from it must not need more iterations."""
     M = capi.Mlp()
+    M.set_objective(objective)
     boxes = [synthetic.water_box(int(nw), 3000 + k, anisotropic=True) for k, nw in enumerate([500, 700, 600, 800])]
     M.reserve(sum(b.n for b in boxes))
     for b in boxes:
