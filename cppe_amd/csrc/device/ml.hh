@@ -42,12 +42,12 @@ struct MlGrid {
   GridInfo g;
   dbuf<int> cstart, sorted;
   dbuf<unsigned> cellof;
-  // neighbour list within rc (CSR over sites, device order), geometry only
+  // neighbour list within rc (CSR over sites, device order): indices only, the pair geometry is
+  // recomputed from the site records in each pass
   long long npairs = 0;
   dbuf<long long> noff;  // n + 1
-  dbuf<int> nj;          // neighbour index
-  dbuf<float4> ngeo;     // (r, unit vector i -> j)
-  dbuf<float> nk;        // 6 per pair: fc(r) T2d_ij packed xx,xy,xz,yy,yz,zz (0 for excluded pairs)
+  dbuf<int> nj;          // neighbour index, bit 31 set for excluded pairs (no tensor block)
+  dbuf<double4> pos;     // per site: position and damping rate damping * alpha_iso^(-1/6)
   dbuf<float> iso;       // alpha_iso of each site
   // per-site records gathered by the list kernels: (F / |F|, alpha_iso), and v1 / v2 in FP32
   dbuf<float4> frec, w1, w2;

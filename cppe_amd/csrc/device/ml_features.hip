@@ -20,14 +20,15 @@
 //     56    (v2 . v1)/|v1|^2,  57 |v2|/|v1|,  58 (v3 . v1)/|v1|^2,  59 |v3|/|v1|
 //     60    (v3 . v2)/(|v2||v3|),  61 log|v1|,  62 (v1 . F)/(|v1||F|),  63 1
 // Neighbours come from a uniform grid of cell size rc/2 built on the device once per system
-// (sort by cell; a neighbourhood is 25 contiguous rows of 5 cells), stored as a neighbour list
-// with each pair's distance, unit vector and damped tensor block fc T2d (geometry only, built
-// once per system like the octree). The descriptors are
-// evaluated in FP32 (the eight radial Gaussians by the recurrence g_{k+1} = g_k A B_k,
-// A = exp(2 eta dr (r - r_0)), B_k = exp(-eta dr^2 (2k + 1)), two exponentials per neighbour).
-// The bases: tensor blocks formed in FP64 (Thole-damped dipole tensor as in the response
-// operator) and stored in FP32, neighbour vectors gathered as FP32 records, sums in FP64 -- the
-// list passes are bound by the bytes they move (a guess needs no more than FP32 accuracy).
+// (sort by cell; a neighbourhood is 25 contiguous rows of 5 cells), stored as a neighbour list of
+// indices (the exclusion flag in bit 31). Each pass recomputes the pair geometry from the
+// neighbour's site record: the displacement formed in FP64 and rounded, the rest in FP32 --
+// reading a cached FP32 tensor block (24 B per pair, ~650 MB at 300k sites) cost a pass more than
+// the ~30 instructions that form it. The descriptors: the eight radial Gaussians by the recurrence
+// g_{k+1} = g_k A B_k, A = exp(2 eta dr (r - r_0)), B_k = exp(-eta dr^2 (2k + 1)), two exponentials
+// per neighbour. The bases: Thole-damped dipole tensor as in the response operator (FP32),
+// neighbour vectors gathered as FP32 records, sums in FP64 (a guess needs no more than FP32
+// accuracy).
 #include <hipcub/hipcub.hpp>
 
 #include "ml.hh"
@@ -132,54 +133,56 @@ __global__ __launch_bounds__(128) void k_ml_ncount(FeatArgs a, long long* __rest
   cnt[i] = c;
 }
 
-// fc(r) T2d(d), packed xx,xy,xz,yy,yz,zz: (3 fT d_a d_b / r^5 - fE delta_ab / r^3) fc
-__device__ __forceinline__ void t2_block(double dx, double dy, double dz, double r, double a, double fc, double* k) {
-  const double ri  = 1.0 / r;
-  const double ri3 = ri * ri * ri;
-  double fE = 1.0, fT = 1.0;
-  if (a > 0.0) {
-    const double u = a * r;
-    if (u < kTholeCutoff) thole_ET(u, exp(-u), fE, fT);
-  }
-  const double c5 = 3.0 * fT * ri3 * ri * ri * fc, c3 = fE * ri3 * fc;
-  k[0] = c5 * dx * dx - c3;
-  k[1] = c5 * dx * dy;
-  k[2] = c5 * dx * dz;
-  k[3] = c5 * dy * dy - c3;
-  k[4] = c5 * dy * dz;
-  k[5] = c5 * dz * dz - c3;
+// Per-site records of the list passes: position and alpha_iso^(-1/6) (0: undamped site).
+__global__ void k_ml_pos(int n, const double* __restrict__ x, const double* __restrict__ y,
+                         const double* __restrict__ z, const double* __restrict__ s, double4* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_double4(x[i], y[i], z[i], s ? s[i] : 0.0);
 }
 
+// Neighbour indices (pairs within rc, j != i; excluded pairs flagged in bit 31: the densities
+// still see them, the bases take no tensor block for them).
 __global__ __launch_bounds__(128) void k_ml_nfill(FeatArgs a, const long long* __restrict__ noff, int* __restrict__ nj,
-                                                  float4* __restrict__ ngeo, float* __restrict__ nk,
                                                   float* __restrict__ iso) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const int n = a.n;
   iso[i] = (float)((a.a6[i] + a.a6[3 * (size_t)n + i] + a.a6[5 * (size_t)n + i]) / 3.0);
-  const double di = a.damping * a.s[i];
   const int e0 = a.eoff ? a.eoff[i] : 0, e1 = a.eoff ? a.eoff[i + 1] : 0;
   long long p = noff[i];
-  for_neighbours(a, i, a.x[i], a.y[i], a.z[i], [&](int j, double dx, double dy, double dz, double r2) {
-    const double r  = sqrt(r2);
-    const double fc = 0.5 * (cos(M_PI * r / kRc) + 1.0);
+  for_neighbours(a, i, a.x[i], a.y[i], a.z[i], [&](int j, double, double, double, double) {
     bool ex = false;
     for (int k = e0; k < e1; ++k) ex |= (a.eidx[k] == j);
-    nj[p]   = j;
-    ngeo[p] = make_float4((float)r, (float)(dx / r), (float)(dy / r), (float)(dz / r));
-    double k[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (!ex) t2_block(dx, dy, dz, r, di * a.s[j], fc, k);
-    for (int c = 0; c < 6; ++c) nk[6 * p + c] = (float)k[c];
+    nj[p] = ex ? (int)((unsigned)j | 0x80000000u) : j;
     ++p;
   });
 }
 
+// Pair geometry of the list passes: d = r_j - r_i formed in FP64 and rounded, r, 1/r in FP32.
+struct MlPair {
+  float dx, dy, dz, r2, ri;
+};
+__device__ __forceinline__ MlPair ml_pair(const double4& pi, const double4& pj) {
+  MlPair q;
+  q.dx = (float)(pj.x - pi.x);
+  q.dy = (float)(pj.y - pi.y);
+  q.dz = (float)(pj.z - pi.z);
+  q.r2 = q.dx * q.dx + q.dy * q.dy + q.dz * q.dz;
+  q.ri = __builtin_amdgcn_rsqf(q.r2);
+  return q;
+}
+
+// cutoff function fc(r) = (cos(pi r / rc) + 1) / 2 through the hardware cosine (argument in turns)
+__device__ __forceinline__ float ml_fc(float r) {
+  return 0.5f * (__builtin_amdgcn_cosf(r * (float)(0.5 / kRc)) + 1.0f);
+}
+
 struct ListArgs {
   int n;
+  double damping;  // Thole factor of the bases
   const long long* noff;
   const int* nj;
-  const float4* ngeo;
-  const float* nk;
+  const double4* pos;
   const float* iso;
   float4* frec;  // per site: F / |F| and alpha_iso (written by k_ml_v1)
   float4* w1;    // per site: v1 in FP32 (k_ml_v1)
@@ -217,6 +220,7 @@ __global__ __launch_bounds__(256) void k_ml_desc(ListArgs a) {
   const int i    = ok ? i0 : a.n - 1;  // trailing lanes shadow the last site and store nothing
   const int n = a.n;
   const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
+  const double4 pi = a.pos[i];
   const double Fx = a.F[i], Fy = a.F[n + i], Fz = a.F[2 * (size_t)n + i];
   const double Fn = sqrt(Fx * Fx + Fy * Fy + Fz * Fz) + 1e-30;
   const double fx = Fx / Fn, fy = Fy / Fn, fz = Fz / Fn;
@@ -225,16 +229,24 @@ __global__ __launch_bounds__(256) void k_ml_desc(ListArgs a) {
 #pragma unroll
   for (int k = 0; k < 48; ++k) feat[k] = 0.f;
   float f54 = 0.f, f55 = 0.f;
-  constexpr float kPiRc = (float)(M_PI / kRc);
   constexpr float kA    = (float)(2.0 * kEta * kRkStep);
   const long long p1 = ok ? a.noff[i + 1] : 0;
-  for (long long p = a.noff[i] + sub; p < p1; p += kDescLanes) {
-    const int j     = a.nj[p];
-    const float4 gq = a.ngeo[p];
-    const float r = gq.x, ux = gq.y, uy = gq.z, uz = gq.w;
-    const float ri  = 1.0f / r;
-    const float fc  = 0.5f * (cosf(kPiRc * r) + 1.0f);
-    const float4 fj = a.frec[j];  // F_j / |F_j|, alpha_iso_j
+  // the lane's pairs in order, two at a time (both indices, then both records, requested first)
+  for (long long pb = a.noff[i] + sub; pb < p1; pb += 2 * kDescLanes) {
+    const bool has2 = pb + kDescLanes < p1;
+    const int j0    = a.nj[pb] & 0x7fffffff;
+    const int j1    = has2 ? a.nj[pb + kDescLanes] & 0x7fffffff : j0;
+    const double4 pq[2] = {a.pos[j0], a.pos[j1]};
+    const float4 fq[2]  = {a.frec[j0], a.frec[j1]};
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+    if (b == 1 && !has2) break;
+    const MlPair q  = ml_pair(pi, pq[b]);
+    const float ri  = q.ri;
+    const float r   = q.r2 * ri;
+    const float ux = q.dx * ri, uy = q.dy * ri, uz = q.dz * ri;
+    const float fc  = ml_fc(r);
+    const float4 fj = fq[b];  // F_j / |F_j|, alpha_iso_j
     const float iso = fj.w;
     const float cth = ux * ffx + uy * ffy + uz * ffz;
     const float p2  = 1.5f * cth * cth - 0.5f;
@@ -255,6 +267,7 @@ __global__ __launch_bounds__(256) void k_ml_desc(ListArgs a) {
     }
     f54 += fc;
     f55 += fc * ri * ri * ri * 100.0f;
+    }
   }
 #pragma unroll
   for (int k = 0; k < 48; ++k) feat[k] = lane_sum<kDescLanes>(feat[k]);
@@ -320,7 +333,9 @@ __global__ void k_ml_v1(ListArgs a) {
 }
 
 // PASS 1: v2 = alpha_i sum_j K_ij v1_j.  PASS 2: v3 = alpha_i sum_j K_ij v2_j, descriptors 56-63.
-// K_ij = fc(r) T2d_ij from the neighbour list and v_j as FP32 records; products summed in FP64.
+// K_ij = fc(r) T2d_ij, the Thole-damped dipole tensor (3 fT d d^T / r^5 - fE / r^3) fc of the
+// response operator, formed per pair in FP32 (excluded pairs: none); v_j as FP32 records; each
+// pair's product summed in FP64.
 template <int PASS>
 __global__ __launch_bounds__(256) void k_ml_basis(ListArgs a) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -330,15 +345,50 @@ __global__ __launch_bounds__(256) void k_ml_basis(ListArgs a) {
   const int i   = ok ? i0 : a.n - 1;  // trailing lanes shadow the last site and store nothing
   const int n = a.n;
   const float4* W = PASS == 1 ? a.w1 : a.w2;
+  const double4 pi = a.pos[i];
+  const float di   = (float)(a.damping * pi.w);
+  constexpr float kLog2e = 1.44269504088896340736f;
   double sx = 0.0, sy = 0.0, sz = 0.0;
   const long long p1 = ok ? a.noff[i + 1] : 0;
-  for (long long p = a.noff[i] + sub; p < p1; p += kBasisLanes) {
-    const int j    = a.nj[p];
-    const float* k = a.nk + 6 * p;
-    const float4 w = W[j];
-    sx += (double)(k[0] * w.x + k[1] * w.y + k[2] * w.z);
-    sy += (double)(k[1] * w.x + k[3] * w.y + k[4] * w.z);
-    sz += (double)(k[2] * w.x + k[4] * w.y + k[5] * w.z);
+  // the lane's pairs p, p + G, p + 2G, ... in order, four at a time: every index, then every
+  // record, is requested before the first pair is evaluated
+  constexpr int kU = 4;
+  for (long long pb = a.noff[i] + sub; pb < p1; pb += kU * kBasisLanes) {
+    int jb[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const long long p = pb + (long long)u * kBasisLanes;
+      jb[u]             = p < p1 ? a.nj[p] : -1;  // negative: excluded pair or none
+    }
+    double4 pb4[kU];
+    float4 wb[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int jj = jb[u] < 0 ? i : jb[u];
+      pb4[u]       = a.pos[jj];
+      wb[u]        = W[jj];
+    }
+#pragma unroll
+    for (int b = 0; b < kU; ++b) {
+    if (jb[b] < 0) continue;
+    const double4 pj = pb4[b];
+    const float4 w   = wb[b];
+    const MlPair q   = ml_pair(pi, pj);
+    const float ri2 = q.ri * q.ri, ri3 = ri2 * q.ri;
+    const float r   = q.r2 * q.ri;
+    const float u   = di * (float)pj.w * r;  // Thole argument; 0: a pair with an undamped site
+    const float e   = u > 0.0f ? __builtin_amdgcn_exp2f(-u * kLog2e) : 0.0f;  // e = 0: fE = fT = 1
+    const float u2  = u * u;
+    const float fE  = 1.0f - (1.0f + u + 0.5f * u2) * e;
+    const float fT  = fE - (u2 * u * (1.0f / 6.0f)) * e;
+    const float fc  = ml_fc(r);
+    const float c3  = fE * ri3 * fc, c5 = 3.0f * fT * ri3 * ri2 * fc;
+    const float dw  = q.dx * w.x + q.dy * w.y + q.dz * w.z;
+    // K w = c5 (d . w) d - c3 w
+    sx += (double)(c5 * dw * q.dx - c3 * w.x);
+    sy += (double)(c5 * dw * q.dy - c3 * w.y);
+    sz += (double)(c5 * dw * q.dz - c3 * w.z);
+    }
   }
   sx = lane_sum<kBasisLanes>(sx);
   sy = lane_sum<kBasisLanes>(sy);
@@ -470,12 +520,13 @@ MlGrid& ml_grid(System& S) {
     CPPE_HIP_CHECK(hipMemcpyAsync(&G->npairs, G->noff.p + n, sizeof(long long), hipMemcpyDeviceToHost, st));
     CPPE_HIP_CHECK(hipStreamSynchronize(st));
     const size_t np = (size_t)std::max<long long>(G->npairs, 1);
+    if (G->npairs >= (1ll << 31)) throw std::runtime_error("ml features: more neighbour pairs than 32-bit indices");
     G->nj.alloc(np);
-    G->ngeo.alloc(np);
-    G->nk.alloc(6 * np);
     G->iso.alloc(n);
-    hipLaunchKernelGGL(k_ml_nfill, dim3(ceil_div(n, 128)), dim3(128), 0, st, fa, G->noff.p, G->nj.p, G->ngeo.p,
-                       G->nk.p, G->iso.p);
+    G->pos.alloc(n);
+    hipLaunchKernelGGL(k_ml_nfill, dim3(ceil_div(n, 128)), dim3(128), 0, st, fa, G->noff.p, G->nj.p, G->iso.p);
+    hipLaunchKernelGGL(k_ml_pos, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, S.x.p, S.y.p, S.z.p, S.s.p,
+                       G->pos.p);
     CPPE_HIP_CHECK(hipStreamSynchronize(st));
   } catch (...) {
     delete G;
@@ -493,7 +544,7 @@ void ml_features(System& S, float* X, double* V) {
   G.frec.alloc(n);
   G.w1.alloc(n);
   G.w2.alloc(n);
-  ListArgs a{n, G.noff.p, G.nj.p, G.ngeo.p, G.nk.p, G.iso.p, G.frec.p, G.w1.p, G.w2.p, S.x.p, S.y.p, S.z.p,
+  ListArgs a{n, S.opts.damping, G.noff.p, G.nj.p, G.pos.p, G.iso.p, G.frec.p, G.w1.p, G.w2.p, S.x.p, S.y.p, S.z.p,
              S.alpha6.p, S.v_rhs.p, S.excl_off.p, S.excl_idx.p, X, V};
   hipLaunchKernelGGL(k_ml_v1, dim3(ceil_div(n, 256)), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_ml_desc, dim3(ceil_div((long long)n * kDescLanes, 256)), dim3(256), 0, st, a);
