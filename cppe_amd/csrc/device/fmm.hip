@@ -233,7 +233,12 @@ void fmm_reserve(System& S) {
     const int v = F.tune.near_variant[prec];
     if (v == 3 || v >= 5) build_near_expanded(S);
     if (v >= 5) ensure_near_items(S, near_sym_variant(v) ? 3 : v - 5);  // symmetric: other launches use chunks of 8
-    if (near_sym_variant(v)) ensure_sym(S, prec, kSymTs[prec][v - 9], S.opts.damping);
+    if (near_sym_variant(v)) {
+      if (one_leaf_system(S))
+        ensure_dsym(S, prec, kSymTs[prec][v - 9], S.opts.damping);
+      else
+        ensure_sym(S, prec, kSymTs[prec][v - 9], S.opts.damping);
+    }
   }
   if (F.tune.near_variant[0] == 4 || F.tune.near_variant[1] == 4) ensure_clusters(S);
   CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));

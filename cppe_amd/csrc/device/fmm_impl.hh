@@ -94,6 +94,17 @@ struct FmmState {
   bool sym_tb2        = false;
   dbuf<float> rev32;
   dbuf<double> rev64;
+  // symmetric direct summation (one leaf): tiles of W TS targets, items (tile, source segment,
+  // index within the tile), forward-block offsets per tile, and the blocks of each precision
+  struct DSym {
+    int ts = 0, tb = 0, nitems = 0, ntile = 0;
+    long long nfwd = 0, nrev = 0;
+    double c[3] = {0.0, 0.0, 0.0};  // the leaf's centre
+    dbuf<int4> items;
+    dbuf<long long> fwd_off;
+  } dsym[2];
+  dbuf<float> dfwd32, drev32;
+  dbuf<double> dfwd64, drev64;
 };
 
 // right-hand sides whose near fields one staged launch evaluates together
@@ -1434,12 +1445,10 @@ void ensure_clusters(System& S);  // fmm.hip
 constexpr int kSymTs[2][3] = {{8, 6, 4}, {16, 12, 8}};
 inline bool near_sym_variant(int v) { return v >= 9 && v <= 11; }
 
-// Lists (chunks of ts targets), target records (for this damping factor) and reverse slots of the
-// symmetric near field in precision prec (0 FP64, 1 FP32).
-inline SymLists& ensure_sym(System& S, int prec, int ts, double damping) {
+// Target records of the symmetric forms for this damping factor (precision prec: 0 FP64, 1 FP32).
+inline void ensure_sym_trec(System& S, int prec, double damping) {
   FmmState& F     = *S.fmm;
   const Octree& T = S.tree;
-  SymLists& Y     = build_sym_lists(S, ts);
   ensure_pos4(S);
   const dim3 g(ceil_div(S.n, 256)), b(256);
   if (prec == 1) {
@@ -1456,6 +1465,14 @@ inline SymLists& ensure_sym(System& S, int prec, int ts, double damping) {
     hipLaunchKernelGGL(k_sym_trec64, g, b, 0, S.stream, S.n, F.pos4.p, damping, F.sym_t64.p);
     F.sym_tdamp[0] = damping;
   }
+}
+
+// Lists (chunks of ts targets), target records (for this damping factor) and reverse slots of the
+// symmetric near field in precision prec (0 FP64, 1 FP32).
+inline SymLists& ensure_sym(System& S, int prec, int ts, double damping) {
+  FmmState& F = *S.fmm;
+  SymLists& Y = build_sym_lists(S, ts);
+  ensure_sym_trec(S, prec, damping);
   if (prec)
     ensure_buf(F.rev32, (size_t)std::max<long long>(Y.nrev, 1));
   else
@@ -1475,6 +1492,112 @@ void launch_sym_gather(System& S, const SymLists& Y, const R* L, double* out) {
     rev = F.rev64.p;
   hipLaunchKernelGGL((k_fmm_sym_gather<P, R>), dim3(ceil_div(S.n, 256)), dim3(256), 0, S.stream, S.n, T.leaf_of.p,
                      T.start.p, T.count.p, Y.rev_off.p, Y.nin.p, rev, F.pos4.p, T.cx.p, T.cy.p, T.cz.p, L, out);
+}
+
+// wavefronts per block of the symmetric direct-summation kernel
+constexpr int kDSymW = 8;
+// sources per item of the symmetric direct-summation kernel (rounded to whole tiles)
+constexpr int kDSymSeg = 8192;
+
+// Items and blocks of the symmetric direct summation with TS targets per wavefront (systems whose
+// tree is one leaf); built on the host once per chunk size.
+inline FmmState::DSym& ensure_dsym(System& S, int prec, int ts, double damping) {
+  FmmState& F       = *S.fmm;
+  FmmState::DSym& D = F.dsym[prec];
+  ensure_sym_trec(S, prec, damping);
+  if (D.ts != ts) {
+    const int n = S.n, TB = kDSymW * ts;
+    const int ntile  = (n + TB - 1) / TB;
+    const long long seg = (long long)TB * std::max(1, kDSymSeg / TB);
+    std::vector<int4> items;
+    std::vector<long long> off(ntile + 1, 0);
+    for (int I = 0; I < ntile; ++I) {
+      const long long e = std::min<long long>(n, (long long)(I + 1) * TB);
+      items.push_back(make_int4(I, I * TB, (int)e, 0));
+      int k = 1;
+      for (long long s0 = (long long)(I + 1) * TB; s0 < n; s0 += seg, ++k)
+        items.push_back(make_int4(I, (int)s0, (int)std::min<long long>(n, s0 + seg), k));
+      off[I + 1] = off[I] + 3ll * TB * k;
+    }
+    // longest items first (the tile's own sites are short)
+    std::stable_sort(items.begin(), items.end(), [](const int4& x, const int4& y) { return x.z - x.y > y.z - y.y; });
+    D.items.alloc(items.size());
+    D.fwd_off.alloc(off.size());
+    CPPE_HIP_CHECK(hipMemcpyAsync(D.items.p, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, S.stream));
+    CPPE_HIP_CHECK(hipMemcpyAsync(D.fwd_off.p, off.data(), sizeof(long long) * off.size(), hipMemcpyHostToDevice,
+                                  S.stream));
+    const Octree& T = S.tree;
+    CPPE_HIP_CHECK(hipMemcpyAsync(&D.c[0], T.cx.p, sizeof(double), hipMemcpyDeviceToHost, S.stream));
+    CPPE_HIP_CHECK(hipMemcpyAsync(&D.c[1], T.cy.p, sizeof(double), hipMemcpyDeviceToHost, S.stream));
+    CPPE_HIP_CHECK(hipMemcpyAsync(&D.c[2], T.cz.p, sizeof(double), hipMemcpyDeviceToHost, S.stream));
+    CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
+    D.ts     = ts;
+    D.tb     = TB;
+    D.ntile  = ntile;
+    D.nitems = (int)items.size();
+    D.nfwd   = off[ntile];
+    D.nrev   = 3ll * TB * ((long long)ntile * (ntile - 1) / 2);
+  }
+  const size_t nf = (size_t)std::max<long long>(D.nfwd, 1), nr = (size_t)std::max<long long>(D.nrev, 1);
+  if (prec) {
+    ensure_buf(F.dfwd32, nf);
+    ensure_buf(F.drev32, nr);
+  } else {
+    ensure_buf(F.dfwd64, nf);
+    ensure_buf(F.drev64, nr);
+  }
+  return D;
+}
+
+// a system whose tree is one leaf holding every site (direct summation)
+inline bool one_leaf_system(const System& S) { return S.nseg == 1 && S.tree.nleaves == 1; }
+
+// Symmetric direct summation of a B apply: the tile kernel, the gather (which writes out), the
+// exclusion pass, then -- when the far field has run before it (with_l2p) -- the (empty) L2P pass.
+template <int P, typename R, bool DAMP, typename MU4, int TS>
+void launch_direct_sym(System& S, const NearArgs& a, const MU4* mu4, const R* L, bool with_l2p) {
+  FmmState& F        = *S.fmm;
+  const Octree& T    = S.tree;
+  constexpr int prec = sizeof(R) == 4;
+  FmmState::DSym& D  = ensure_dsym(S, prec, TS, a.damping);
+  constexpr bool kB2 = prec && DAMP;
+  const bool b2      = kB2 && S.all_damped;
+  hipStream_t st     = S.stream;
+  const DSymArgs da{D.items.p, D.nitems, S.n, D.fwd_off.p, D.c[0], D.c[1], D.c[2]};
+  R *fwd, *rev;
+  if constexpr (prec) {
+    fwd = F.dfwd32.p;
+    rev = F.drev32.p;
+  } else {
+    fwd = F.dfwd64.p;
+    rev = F.drev64.p;
+  }
+  if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[6], st));
+  const dim3 g(D.nitems), b(64 * kDSymW);
+  if constexpr (prec) {
+    if (b2)
+      hipLaunchKernelGGL((k_direct_sym<R, DAMP, kB2, TS, kDSymW, MU4, float4>), g, b, 0, st, a, da, F.sym_t32.p, mu4,
+                         fwd, rev);
+    else
+      hipLaunchKernelGGL((k_direct_sym<R, DAMP, false, TS, kDSymW, MU4, float4>), g, b, 0, st, a, da, F.sym_t32.p,
+                         mu4, fwd, rev);
+  } else {
+    hipLaunchKernelGGL((k_direct_sym<R, DAMP, false, TS, kDSymW, MU4, double4>), g, b, 0, st, a, da, F.sym_t64.p,
+                       mu4, fwd, rev);
+  }
+  if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[7], st));
+  hipLaunchKernelGGL(k_direct_sym_gather<R>, dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a, D.tb, D.fwd_off.p, fwd,
+                     rev);
+  if (a.eoff) {
+    if (b2)
+      hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, kB2, 1>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a, mu4);
+    else
+      hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, false, 1>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a,
+                         mu4);
+  }
+  if (with_l2p)
+    hipLaunchKernelGGL((k_fmm_l2p_add<P, R>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, S.n, T.leaf_of.p,
+                       F.pos4.p, T.cx.p, T.cy.p, T.cz.p, L, a.out);
 }
 
 // Symmetric near field of a B apply (variants 9-11, TS targets per chunk): the pair kernel, the
@@ -1528,12 +1651,20 @@ void launch_near_sym(System& S, const NearArgs& a, const MU4* mu4, const R* L, b
 template <int P, typename R, bool DAMP, typename MU4>
 void launch_near_sym_v(System& S, int var, const NearArgs& a, const MU4* mu4, const R* L, bool with_l2p) {
   constexpr int prec = sizeof(R) == 4;
+  const bool direct  = one_leaf_system(S);
+  auto go = [&](auto ts) {
+    constexpr int TS = decltype(ts)::value;
+    if (direct)
+      launch_direct_sym<P, R, DAMP, MU4, TS>(S, a, mu4, L, with_l2p);
+    else
+      launch_near_sym<P, R, DAMP, MU4, TS>(S, a, mu4, L, with_l2p);
+  };
   if (var == 9)
-    launch_near_sym<P, R, DAMP, MU4, kSymTs[prec][0]>(S, a, mu4, L, with_l2p);
+    go(std::integral_constant<int, kSymTs[prec][0]>{});
   else if (var == 10)
-    launch_near_sym<P, R, DAMP, MU4, kSymTs[prec][1]>(S, a, mu4, L, with_l2p);
+    go(std::integral_constant<int, kSymTs[prec][1]>{});
   else
-    launch_near_sym<P, R, DAMP, MU4, kSymTs[prec][2]>(S, a, mu4, L, with_l2p);
+    go(std::integral_constant<int, kSymTs[prec][2]>{});
 }
 
 

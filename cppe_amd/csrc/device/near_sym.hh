@@ -258,3 +258,148 @@ __global__ __launch_bounds__(256) void k_fmm_sym_gather(int n, const int* __rest
   out[2 * (size_t)n + i] += g[2];
 }
 
+
+// ---- direct summation (one leaf holding every site) ---------------------------------------------
+// All pairs are near, so the symmetric form tiles them: W wavefronts of a block hold TS targets
+// each (a tile of TB = W TS consecutive sites); an item is a tile I against a segment of the
+// sources above it (both directions), or against its own sites (ordered pairs, field at the tile
+// only). Lanes are sources, 64 per stage, shared by the block's wavefronts; after each stage the
+// wavefronts' fields at the stage's sources are summed through LDS in wavefront order and stored
+// once per (tile, source): reverse block (J, I) of the receiving tile J, 3 TB values, at
+// 3 TB (J (J - 1) / 2 + I). The forward sums of an item go to its own block of the tile
+// (fwd_off[I] + k 3 TB, k = the item's index among the tile's items, its own sites first).
+// k_direct_sym_gather adds, per site, its tile's forward blocks and then its reverse blocks in
+// block order, with alpha^-1 mu: every sum in a fixed order, no atomics.
+struct DSymArgs {
+  const int4* items;  // (tile, first source, end source, index of the item among the tile's items)
+  int nitems, n;
+  const long long* fwd_off;  // per tile: first forward slot
+  double cx, cy, cz;         // the leaf's centre (FP32 offsets)
+};
+
+__device__ __forceinline__ long long dsym_rev_block(int J, int I, int TB) {
+  return 3ll * TB * ((long long)J * (J - 1) / 2 + I);
+}
+
+template <typename R, bool DAMP, bool B2, int TS, int W, typename MU4, typename TREC>
+__global__ __launch_bounds__(64 * W) void k_direct_sym(NearArgs a, DSymArgs b, const TREC* __restrict__ trec,
+                                                       const MU4* __restrict__ mu4, R* __restrict__ fwd,
+                                                       R* __restrict__ rev) {
+  constexpr int TB = W * TS;
+  __shared__ SymTarget<R> tl[W][TS];
+  __shared__ R red[2][W][3][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int4 it  = b.items[blockIdx.x];
+  const int I = it.x, kb = it.w;
+  const long long s0 = it.y, s1 = it.z;
+  const bool diag = s0 == (long long)I * TB;  // the tile's own sites (uniform over the block)
+  const int t0    = I * TB + wv * TS;
+  const int T     = max(0, min(TS, b.n - t0));
+  constexpr R kFar = sizeof(R) == 4 ? R(1.0e4) : R(1.0e8);
+  SymTarget<R>* tg = tl[wv];
+  if (lane < T) {
+    const TREC tr = trec[t0 + lane];
+    const MU4 mt  = mu4[t0 + lane];
+    tg[lane].p    = SymVec4<R>{(R)tr.x, (R)tr.y, (R)tr.z, (R)tr.w};
+    tg[lane].m    = SymVec4<R>{(R)mt.x, (R)mt.y, (R)mt.z, R(0)};
+  }
+  __syncthreads();
+  R f[TS][3];
+#pragma unroll
+  for (int t = 0; t < TS; ++t) f[t][0] = f[t][1] = f[t][2] = R(0);
+  int buf = 0;
+  for (long long q0 = s0; q0 < s1; q0 += 64) {
+    const long long j = q0 + lane;
+    const bool act    = j < s1;
+    R xs = kFar, ys = kFar, zs = kFar, ss = R(0), mx = R(0), my = R(0), mz = R(0);
+    if (act) {
+      const double4 pj = a.pos4[j];
+      const MU4 mj     = mu4[j];
+      xs = sizeof(R) == 8 ? (R)pj.x : (R)(pj.x - b.cx);
+      ys = sizeof(R) == 8 ? (R)pj.y : (R)(pj.y - b.cy);
+      zs = sizeof(R) == 8 ? (R)pj.z : (R)(pj.z - b.cz);
+      ss = (R)pj.w;
+      mx = (R)mj.x;
+      my = (R)mj.y;
+      mz = (R)mj.z;
+    }
+    if (diag) {  // ordered pairs inside the tile (the self pair gives zero)
+#pragma unroll
+      for (int t = 0; t < TS; ++t) {
+        if (t < T) {
+          const SymVec4<R> tp = tg[t].p;
+          const R dx = tp.x - xs, dy = tp.y - ys, dz = tp.z - zs;
+          R c3, cf;
+          staged_geo<R, DAMP, B2, false>(dx, dy, dz, tp.w * ss, c3, cf);
+          acc_m1(c3, cf, dx, dy, dz, mx, my, mz, f[t][0], f[t][1], f[t][2]);
+        }
+      }
+      continue;
+    }
+    R rx = R(0), ry = R(0), rz = R(0);
+#pragma unroll
+    for (int t = 0; t < TS; ++t)
+      if (t < T) sym_pair<R, DAMP, B2, false>(tg[t], xs, ys, zs, ss, mx, my, mz, f[t][0], f[t][1], f[t][2], rx, ry, rz);
+    red[buf][wv][0][lane] = rx;
+    red[buf][wv][1][lane] = ry;
+    red[buf][wv][2][lane] = rz;
+    __syncthreads();  // (also orders this buffer's reads two stages back before it is rewritten)
+    if (threadIdx.x < 3 * 64) {
+      const int c = threadIdx.x >> 6, l = threadIdx.x & 63;
+      const long long jj = q0 + l;
+      if (jj < s1) {
+        R s = red[buf][0][c][l];
+#pragma unroll
+        for (int w = 1; w < W; ++w) s += red[buf][w][c][l];
+        const int J = (int)(jj / TB), u = (int)(jj - (long long)J * TB);
+        rev[dsym_rev_block(J, I, TB) + c * TB + u] = s;
+      }
+    }
+    buf ^= 1;
+  }
+  // the item's forward sums of each target: wavefront sums, lane t stores target t
+  R* blk = fwd + b.fwd_off[I] + 3ll * TB * kb + wv * TS;
+#pragma unroll
+  for (int t = 0; t < TS; ++t) {
+    if (t < T) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const R v = sym_wave_sum(f[t][c]);
+        if (lane == t) blk[c * TB + t] = v;
+      }
+    }
+  }
+}
+
+// out = (the tile's forward blocks) + (reverse blocks from the tiles below) + alpha^-1 mu
+template <typename R>
+__global__ __launch_bounds__(256) void k_direct_sym_gather(NearArgs a, int TB, const long long* __restrict__ fwd_off,
+                                                           const R* __restrict__ fwd, const R* __restrict__ rev) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = a.n;
+  if (i >= n) return;
+  const int J = i / TB, u = i - J * TB;
+  const int nk = (int)((fwd_off[J + 1] - fwd_off[J]) / (3ll * TB));
+  double g[3] = {0.0, 0.0, 0.0};
+  const R* p = fwd + fwd_off[J] + u;
+  for (int k = 0; k < nk; ++k, p += 3 * TB) {
+    g[0] += (double)p[0];
+    g[1] += (double)p[TB];
+    g[2] += (double)p[2 * TB];
+  }
+  const R* q = rev + dsym_rev_block(J, 0, TB) + u;
+#pragma unroll 8
+  for (int I = 0; I < J; ++I, q += 3 * TB) {
+    g[0] += (double)q[0];
+    g[1] += (double)q[TB];
+    g[2] += (double)q[2 * TB];
+  }
+  const double* ai = a.ainv;
+  double a6[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) a6[c] = ai[(size_t)c * n + i];
+  const double m0 = a.src[i], m1 = a.src[n + i], m2 = a.src[2 * (size_t)n + i];
+  a.out[i]                 = g[0] + (a6[0] * m0 + a6[1] * m1 + a6[2] * m2);
+  a.out[n + i]             = g[1] + (a6[1] * m0 + a6[3] * m1 + a6[4] * m2);
+  a.out[2 * (size_t)n + i] = g[2] + (a6[2] * m0 + a6[4] * m1 + a6[5] * m2);
+}

@@ -11,6 +11,8 @@ the higher leaf goes through per-leaf reverse blocks that a gather pass adds in 
   small leaves (ncrit 8); undamped sites (the branching FP32 pair) and no damping at all
 * solves in both forms stop on a residual that the oracle's FP64 operator confirms (< 1e-8)
 * snapshot batches: every snapshot's apply equals the staged form's
+* direct summation (one leaf holding every site): the tiled form -- tiles of 8 wavefronts, reverse
+  sums combined across them in LDS -- against the staged one-leaf apply and the oracle's direct B
 """
 import sys
 
@@ -141,3 +143,46 @@ def test_batch_apply(precision):
         y = B.apply_b(x)
         for a, b in zip(B.split(y), B.split(ref)):
             assert rel(a, b) < (1e-13 if precision == "fp64" else 1e-6), v
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_direct_summation_tiles(precision):
+    w = synthetic.water_box(2300, 7320, anisotropic=True)  # several tiles, the last one partial
+    S = capi.System(w.pos, w.pol, w.alpha_iso(), w.excl_csr(), summation="direct", damping=DAMP,
+                    precision=precision)
+    x = np.random.default_rng(25).normal(size=3 * w.n)
+    S.set_tune(tune(STAGED))
+    ref = S.apply_b(x)
+    B = O.BSystem(w.pos, w.pol, w.excl_csr(), damping=DAMP, scheme="direct") if precision == "fp64" else None
+    for v in SYMS:
+        say("direct sym", precision, v, w.n)
+        S.set_tune(tune(v))
+        y = S.apply_b(x)
+        assert rel(y, ref) < (1e-13 if precision == "fp64" else 1e-6), v
+        if B is not None:
+            assert rel(y, B.apply(x)) < 1e-12, v
+        for o in (1, 5):
+            S.set_tune(tune(v, o))
+            assert np.array_equal(S.apply_b(x), y), (v, o)
+
+
+def test_direct_summation_tiles_undamped_and_solve():
+    w = synthetic.water_box(1400, 7321, anisotropic=True)
+    iso = w.alpha_iso().copy()
+    iso[::5] = 0.0
+    x = np.random.default_rng(26).normal(size=3 * w.n)
+    for damping in (DAMP, 0.0):
+        S = capi.System(w.pos, w.pol, iso, w.excl_csr(), summation="direct", damping=damping)
+        S.set_tune(tune(STAGED))
+        ref = S.apply_b(x)
+        S.set_tune(tune(SYM))
+        assert rel(S.apply_b(x), ref) < 1e-13, damping
+    F = capi.nuclear_fields(w.pos, w.atoms_z.astype(float), w.atoms_pos)
+    S = capi.System(w.pos, w.pol, w.alpha_iso(), w.excl_csr(), summation="direct", damping=DAMP, precision="mixed")
+    S.set_tune(tune(SYM))
+    S.upload_rhs(F)
+    it, norms, _ = S.solve_resident(0, 1e-8, 100)
+    mu = S.download_mu()
+    B = O.BSystem(w.pos, w.pol, w.excl_csr(), damping=DAMP, scheme="direct")
+    say("direct sym solve", it, norms[-1])
+    assert np.linalg.norm(F - B.apply(mu)) < 1e-8
