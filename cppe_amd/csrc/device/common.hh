@@ -58,17 +58,4 @@ __device__ __forceinline__ void thole_ET(R v, R e, R& fE, R& fT) {
 constexpr double kTholeCutoff  = 40.0;
 constexpr float kTholeCutoff32 = 25.0f;
 
-// Class of a near-field source for the targets of one leaf: a lower bound of the Thole argument
-// v = damping s_i s_j R over the leaf's targets (s_min of the leaf, R from the source to the leaf's
-// bounding box), 0: v_min < 25, 1: 25 <= v_min < 40, 2: v_min >= 40 (small margin above each
-// cutoff). box = (lo x, y, z, hi x, y, z, s_min, unused).
-__host__ __device__ inline int near_class(const double* box, double damping, double xj, double yj, double zj,
-                                          double sj) {
-  const double dx = fmax(fmax(box[0] - xj, xj - box[3]), 0.0);
-  const double dy = fmax(fmax(box[1] - yj, yj - box[4]), 0.0);
-  const double dz = fmax(fmax(box[2] - zj, zj - box[5]), 0.0);
-  const double v  = damping * box[6] * sj * sqrt(dx * dx + dy * dy + dz * dz);
-  return v < (double)kTholeCutoff32 * (1.0 + 1e-9) ? 0 : (v < kTholeCutoff * (1.0 + 1e-9) ? 1 : 2);
-}
-
 }  // namespace cppe_dev

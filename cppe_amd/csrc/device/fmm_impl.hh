@@ -958,11 +958,20 @@ __device__ __forceinline__ void staged_geo(R dx, R dy, R dz, R a, R& c3, R& cf) 
 template <typename R, bool DAMP, bool B2, int K, bool FAR = false>
 __device__ __forceinline__ void staged_pair(R xi, R yi, R zi, R di, const StagedRec<R, K>& r, R* f) {
   const R dx = xi - r.x, dy = yi - r.y, dz = zi - r.z;
-  R c3, cf;
-  staged_geo<R, DAMP, B2, FAR>(dx, dy, dz, di * r.s, c3, cf);
+  if constexpr (FAR && sizeof(R) == 4) {
+    float ri3, m3;
+    geo_far_f32(dx, dy, dz, ri3, m3);
 #pragma unroll
-  for (int k = 0; k < K; ++k)
-    acc_m1(c3, cf, dx, dy, dz, r.m[3 * k], r.m[3 * k + 1], r.m[3 * k + 2], f[3 * k], f[3 * k + 1], f[3 * k + 2]);
+    for (int k = 0; k < K; ++k)
+      acc_far_f32(ri3, m3, dx, dy, dz, r.m[3 * k], r.m[3 * k + 1], r.m[3 * k + 2], f[3 * k], f[3 * k + 1],
+                  f[3 * k + 2]);
+  } else {
+    R c3, cf;
+    staged_geo<R, DAMP, B2, FAR>(dx, dy, dz, di * r.s, c3, cf);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      acc_m1(c3, cf, dx, dy, dz, r.m[3 * k], r.m[3 * k + 1], r.m[3 * k + 2], f[3 * k], f[3 * k + 1], f[3 * k + 2]);
+  }
 }
 
 template <typename R, bool B2>

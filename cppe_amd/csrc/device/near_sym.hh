@@ -53,10 +53,17 @@ __device__ __forceinline__ void sym_pair(const SymTarget<R>& t, R xs, R ys, R zs
                                          R& fy, R& fz, R& rx, R& ry, R& rz) {
   const SymVec4<R> tp = t.p, tm = t.m;
   const R dx = tp.x - xs, dy = tp.y - ys, dz = tp.z - zs;
-  R c3, cf;
-  staged_geo<R, DAMP, B2, FAR>(dx, dy, dz, tp.w * ss, c3, cf);
-  acc_m1(c3, cf, dx, dy, dz, mxs, mys, mzs, fx, fy, fz);
-  acc_m1(c3, cf, dx, dy, dz, tm.x, tm.y, tm.z, rx, ry, rz);
+  if constexpr (FAR && sizeof(R) == 4) {
+    float ri3, m3;
+    geo_far_f32(dx, dy, dz, ri3, m3);
+    acc_far_f32(ri3, m3, dx, dy, dz, mxs, mys, mzs, fx, fy, fz);
+    acc_far_f32(ri3, m3, dx, dy, dz, tm.x, tm.y, tm.z, rx, ry, rz);
+  } else {
+    R c3, cf;
+    staged_geo<R, DAMP, B2, FAR>(dx, dy, dz, tp.w * ss, c3, cf);
+    acc_m1(c3, cf, dx, dy, dz, mxs, mys, mzs, fx, fy, fz);
+    acc_m1(c3, cf, dx, dy, dz, tm.x, tm.y, tm.z, rx, ry, rz);
+  }
 }
 
 // FP32 DPP sum over a wavefront; the total is uniform (lane 63's value, read into a scalar).

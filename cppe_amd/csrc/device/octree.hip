@@ -561,46 +561,64 @@ __global__ void k_xcount(int ncells, const int* __restrict__ off, const int* __r
   xcnt[c] = t;
 }
 
-// One wavefront per cell: the bounding box and smallest s of the cell's own sites, then its near
-// sources written class by class (near_class), each class in the order of the near list (stable
-// compaction with a ballot per 64 sources).
+// Class of every entry of a leaf's expanded near list (near-list order), one wavefront per cell,
+// lanes = sources: the smallest Thole argument v = damping s_i s_j R_ij over the leaf's own sites i
+// (exact, FP64), then 0: v < 25, 1: 25 <= v < 40, 2: v >= 40 (a small margin above each cutoff).
+// Undamped sites (s = 0) and the leaf's own sites (the site itself is among the targets, R = 0)
+// give class 0 without the search, so one-leaf (direct-summation) systems cost O(n) here.
+__global__ __launch_bounds__(64) void k_xclass(const int* __restrict__ off, const int* __restrict__ src,
+                                               const int* __restrict__ start, const int* __restrict__ count,
+                                               const double* __restrict__ x, const double* __restrict__ y,
+                                               const double* __restrict__ z, const double* __restrict__ s,
+                                               double damping, const long long* __restrict__ xoff,
+                                               unsigned char* __restrict__ xcls) {
+  const int c = blockIdx.x, lane = threadIdx.x;
+  const int i0 = start[c], ni = count[c];
+  long long q = xoff[c];
+  for (int k = off[c]; k < off[c + 1]; ++k) {
+    const int B = src[k], j0 = start[B], nb = count[B];
+    for (int u0 = 0; u0 < nb; u0 += 64) {
+      const int u = u0 + lane;
+      if (u < nb && (B == c || s[j0 + u] == 0.0)) {
+        xcls[q + u] = 0;
+      } else if (u < nb) {
+        const int j = j0 + u;
+        const double xj = x[j], yj = y[j], zj = z[j];
+        double m = 1e300;
+        for (int i = i0; i < i0 + ni; ++i) {  // uniform index: the leaf's sites are read by every lane
+          const double dx = xj - x[i], dy = yj - y[i], dz = zj - z[i];
+          m = fmin(m, (s[i] * s[i]) * (dx * dx + dy * dy + dz * dz));
+        }
+        const double v = damping * s[j] * sqrt(m);
+        xcls[q + u] = v < (double)kTholeCutoff32 * (1.0 + 1e-9) ? 0 : (v < kTholeCutoff * (1.0 + 1e-9) ? 1 : 2);
+      }
+    }
+    q += nb;
+  }
+}
+
+// One wavefront per cell: the cell's near sources written class by class (k_xclass), each class
+// in the order of the near list (stable compaction with a ballot per 64 sources); xcls = null puts
+// every source in class 0.
 __global__ __launch_bounds__(64) void k_xfill(const int* __restrict__ off, const int* __restrict__ src,
                                               const int* __restrict__ start, const int* __restrict__ count,
-                                              const double* __restrict__ x, const double* __restrict__ y,
-                                              const double* __restrict__ z, const double* __restrict__ s,
-                                              double damping, const long long* __restrict__ xoff,
-                                              int* __restrict__ xsrc, long long* __restrict__ xs64,
-                                              long long* __restrict__ xs32, double* __restrict__ xbox) {
+                                              const unsigned char* __restrict__ xcls,
+                                              const long long* __restrict__ xoff, int* __restrict__ xsrc,
+                                              long long* __restrict__ xs64, long long* __restrict__ xs32) {
   const int c = blockIdx.x, lane = threadIdx.x;
   long long pos = xoff[c];
-  double b[8] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300, 1e300, 0.0};
-  for (int i = start[c] + lane; i < start[c] + count[c]; i += 64) {
-    b[0] = fmin(b[0], x[i]);
-    b[1] = fmin(b[1], y[i]);
-    b[2] = fmin(b[2], z[i]);
-    b[3] = fmax(b[3], x[i]);
-    b[4] = fmax(b[4], y[i]);
-    b[5] = fmax(b[5], z[i]);
-    b[6] = fmin(b[6], s ? s[i] : 0.0);
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    for (int q = 0; q < 3; ++q) b[q] = fmin(b[q], __shfl_xor(b[q], o));
-    for (int q = 3; q < 6; ++q) b[q] = fmax(b[q], __shfl_xor(b[q], o));
-    b[6] = fmin(b[6], __shfl_xor(b[6], o));
-  }
-  if (count[c] == 0 || !s || !(damping > 0.0)) b[6] = 0.0;  // no bound: every source in class 0
-  if (lane < 8) xbox[8 * (size_t)c + lane] = b[lane];
   for (int pass = 0; pass < 3; ++pass) {
+    long long q = xoff[c];
     for (int k = off[c]; k < off[c + 1]; ++k) {
       const int B = src[k], j0 = start[B], nb = count[B];
       for (int u0 = 0; u0 < nb; u0 += 64) {
         const int u  = u0 + lane;
-        const int j  = j0 + u;
-        const int cl = u < nb ? near_class(b, damping, x[j], y[j], z[j], s ? s[j] : 0.0) : -1;
+        const int cl = u < nb ? (xcls ? (int)xcls[q + u] : 0) : -1;
         const unsigned long long m = __ballot(cl == pass);
-        if (cl == pass) xsrc[pos + __popcll(m & ((1ull << lane) - 1ull))] = j;
+        if (cl == pass) xsrc[pos + __popcll(m & ((1ull << lane) - 1ull))] = j0 + u;
         pos += __popcll(m);
       }
+      q += nb;
     }
     if (lane == 0 && pass == 0) xs32[c] = pos;
     if (lane == 0 && pass == 1) xs64[c] = pos;
@@ -627,10 +645,15 @@ void build_near_expanded(System& S) {
   T.xsrc.alloc(std::max<long long>(total, 1));
   T.xsplit[0].alloc(nc);
   T.xsplit[1].alloc(nc);
-  T.xbox.alloc(8 * (size_t)nc);
-  hipLaunchKernelGGL(k_xfill, dim3(nc), dim3(64), 0, st, T.p2p_off.p, T.p2p_src.p, T.start.p, T.count.p, S.x.p,
-                     S.y.p, S.z.p, S.s.p, S.opts.damping, T.xoff.p, T.xsrc.p, T.xsplit[0].p, T.xsplit[1].p,
-                     T.xbox.p);
+  dbuf<unsigned char> cls;
+  const bool damped = S.s.p && S.opts.damping > 0.0;
+  if (damped) {
+    cls.alloc(std::max<long long>(total, 1));
+    hipLaunchKernelGGL(k_xclass, dim3(nc), dim3(64), 0, st, T.p2p_off.p, T.p2p_src.p, T.start.p, T.count.p, S.x.p,
+                       S.y.p, S.z.p, S.s.p, S.opts.damping, T.xoff.p, cls.p);
+  }
+  hipLaunchKernelGGL(k_xfill, dim3(nc), dim3(64), 0, st, T.p2p_off.p, T.p2p_src.p, T.start.p, T.count.p,
+                     damped ? cls.p : nullptr, T.xoff.p, T.xsrc.p, T.xsplit[0].p, T.xsplit[1].p);
   CPPE_HIP_CHECK(hipStreamSynchronize(st));
   T.nx    = total;
   T.xdamp = S.opts.damping;
