@@ -306,12 +306,16 @@ def main():
         wall32, dev32, its32 = time_solves(S32, args.steps, args.warmup, 0)
     info32 = S32.last_solve_info()
     launch32 = S32.last_solve_launches()
+    # the headline moments' residual under the FP64 operator (outside the timed region): the solve
+    # confirms convergence from an FP64 true residual plus an FP32 apply of the correction since
+    head_res = float(np.linalg.norm(rhs - S64.apply_b(S32.download_mu())))
+    log("headline moments: FP64 residual", head_res)
     log("headline solve ms", wall32, "iters", its32, "ml", ml_used, "fp64 true-residual applies",
         info32["reliable_updates"], launch32)
     # the same mixed solve from alpha F and from zero, for the ML gain; then once more launched
     # kernel by kernel
-    wallm, devm, itsm = time_solves(S32, max(1, args.steps // 2), 0, 0)
-    wallz, devz, itsz = time_solves(S32, max(1, args.steps // 2), 0, 2)
+    wallm, devm, itsm = time_solves(S32, max(1, args.steps // 2), 1, 0)
+    wallz, devz, itsz = time_solves(S32, max(1, args.steps // 2), 1, 2)
     S32.set_graph_steps(1)  # one step per launch, a host status read after each
     wall1, _, _ = time_solves(S32, max(1, args.steps // 2), 1, 0)
     launch1 = S32.last_solve_launches()
@@ -487,6 +491,8 @@ def main():
         "iterations": {"headline": its32, "fp64": its64},
         "ml_warm_start": ml_used,
         "fp64_true_residual_applies": info32["reliable_updates"],
+        "fp32_corrected_confirmations": info32["confirmations"],
+        "headline_fp64_residual": head_res,
         "ml_guess_ms": round(float(np.mean(guess_ms)), 3) if guess_ms else None,
         "mixed_alphaF_ms": round(wallm, 3),
         "mixed_alphaF_iterations": itsm,

@@ -28,7 +28,7 @@ EXPORTS = [
     "cppe_hip_mlp_create", "cppe_hip_mlp_destroy", "cppe_hip_mlp_set", "cppe_hip_mlp_get", "cppe_hip_ml_features",
     "cppe_hip_mlp_forward", "cppe_hip_mlp_loss_grad", "cppe_hip_mlp_reserve", "cppe_hip_mlp_collect",
     "cppe_hip_mlp_prepare", "cppe_hip_mlp_train", "cppe_hip_ml_guess", "cppe_hip_mlp_set_objective",
-    "cppe_hip_mlp_samples",
+    "cppe_hip_mlp_samples", "cppe_hip_set_confirmation", "cppe_hip_last_solve_confirmations",
 ]
 MLP_IN = 64
 MLP_PARAMS = 25219
@@ -97,6 +97,8 @@ def lib():
         L.cppe_hip_set_refinement.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double]
         L.cppe_hip_set_preconditioner.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.cppe_hip_last_solve_info.argtypes = [vp, _i]
+        L.cppe_hip_set_confirmation.argtypes = [vp, ctypes.c_double]
+        L.cppe_hip_last_solve_confirmations.argtypes = [vp, _i]
         L.cppe_hip_set_graph_steps.argtypes = [vp, ctypes.c_int]
         L.cppe_hip_last_solve_launches.argtypes = [vp, _i]
         L.cppe_hip_field_targets.argtypes = [vp, _d, ctypes.c_int, _i, _d]
@@ -302,10 +304,19 @@ class System:
         k = {"jacobi": 0, "local": 1}[kind]
         _check(lib().cppe_hip_set_preconditioner(self.h, k, float(radius), float(weight)))
 
+    def set_confirmation(self, eta=1e-5):
+        """Mixed form: confirm a converged recursive residual from the last FP64 true residual and an
+        FP32 apply of the correction since, accepted with the error bound eta (0: FP64 true
+        residual only) (cppe_hip_set_confirmation)."""
+        _check(lib().cppe_hip_set_confirmation(self.h, float(eta)))
+
     def last_solve_info(self):
         info = np.zeros(3, dtype=np.int32)
         _check(lib().cppe_hip_last_solve_info(self.h, _p(info, _i)))
-        return dict(iterations=int(info[0]), reliable_updates=int(info[1]), fp64_fallback=int(info[2]))
+        conf = np.zeros(1, dtype=np.int32)
+        _check(lib().cppe_hip_last_solve_confirmations(self.h, _p(conf, _i)))
+        return dict(iterations=int(info[0]), reliable_updates=int(info[1]), fp64_fallback=int(info[2]),
+                    confirmations=int(conf[0]))
 
     def set_graphs(self, enable=True):
         _check(lib().cppe_hip_set_graphs(self.h, int(bool(enable))))

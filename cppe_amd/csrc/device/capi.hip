@@ -709,6 +709,22 @@ int cppe_hip_set_refinement(cppe_hip_system* sys, double drop, int stall, double
   CPPE_API_END
 }
 
+int cppe_hip_set_confirmation(cppe_hip_system* sys, double eta) {
+  CPPE_API_BEGIN
+  if (!sys || !(eta >= 0.0) || !std::isfinite(eta)) return fail(CPPE_HIP_ERR_ARG, "confirmation: eta >= 0");
+  sys->S.confirm_eta = eta;
+  return CPPE_HIP_OK;
+  CPPE_API_END
+}
+
+int cppe_hip_last_solve_confirmations(cppe_hip_system* sys, int* count) {
+  CPPE_API_BEGIN
+  if (!sys || !count) return fail(CPPE_HIP_ERR_ARG, "invalid arguments");
+  count[0] = sys->S.last.confirmations;
+  return CPPE_HIP_OK;
+  CPPE_API_END
+}
+
 int cppe_hip_set_preconditioner(cppe_hip_system* sys, int kind, double radius, double weight) {
   CPPE_API_BEGIN
   if (!sys || (kind != CPPE_HIP_PRECOND_JACOBI && kind != CPPE_HIP_PRECOND_LOCAL) || !(radius > 0.0) ||

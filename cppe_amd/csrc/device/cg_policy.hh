@@ -14,7 +14,12 @@
 //    operator the recursive residual can level off before it reaches the trigger;
 //  * a true residual that is not below `min_progress` times the previous true residual after at
 //    least one step marks the segment as stagnating (STAG): the refinement is not converging and
-//    the solve continues from the current moments in the plain FP64 form.
+//    the solve continues from the current moments in the plain FP64 form;
+//  * a pending segment with an FP64 true residual r_t at moments x_t can instead be confirmed from
+//    it: r_c = r_t - B32 (x - x_t), one FP32 apply of the correction since then. It is accepted
+//    when ||r_c|| + eta ||B32 (x - x_t)|| < thresh, eta a bound on the FP32 apply's relative error,
+//    so that ||rhs - B x|| < thresh holds for the FP64 operator; otherwise an FP64 true residual
+//    decides as before.
 // Launch horizon: after every decision a segment also predicts how many more steps it will take
 // before it needs the host (seg_horizon), from its mean log reduction per step; the host launches
 // that many steps as one graph, and steps launched beyond an event do nothing on the device.
@@ -40,8 +45,9 @@ enum SegD { D_RZ = 0, D_BETA, D_RNORM, D_RLAST, D_RZN, D_RBEST, D_LOGQ, D_COUNT 
 // per-segment ints. I_PENDING: frozen below the threshold until an FP64 true residual confirms it;
 // I_ELIGIBLE: within 100x of the replacement trigger, replaced whenever a replacement runs for
 // another segment (segments share FP64 applies); I_SINCE: steps since D_RBEST last halved;
-// I_STEPS: steps since the last true residual; I_STAG: stagnating refinement
-enum SegI { I_ACTIVE = 0, I_NEEDREL, I_ITERS, I_BAD, I_PENDING, I_ELIGIBLE, I_SINCE, I_STEPS, I_STAG, I_COUNT };
+// I_STEPS: steps since the last true residual; I_STAG: stagnating refinement; I_HASREL: an FP64
+// true residual of this solve is stored (moments and residual at that point)
+enum SegI { I_ACTIVE = 0, I_NEEDREL, I_ITERS, I_BAD, I_PENDING, I_ELIGIBLE, I_SINCE, I_STEPS, I_STAG, I_HASREL, I_COUNT };
 
 struct Refine {
   double thresh;        // ||r||_2 below which a segment has converged
@@ -83,6 +89,7 @@ CPPE_HD inline void seg_init(const SegRef& s, double rz, double rr, bool mixed, 
   s.i(I_SINCE)    = 0;
   s.i(I_STEPS)    = 0;
   s.i(I_STAG)     = 0;
+  s.i(I_HASREL)   = 0;
   if (mixed) {
     // an FP32 residual within one drop of the threshold is replaced by the FP64 true residual
     // before any step (a guess that already converges stops at 0 iterations)
@@ -184,6 +191,7 @@ CPPE_HD inline void seg_true_residual(const SegRef& s, double rr, int it, int st
     const double rn = sqrt(rr);
     s.i(I_ELIGIBLE) = 0;
     s.i(I_NEEDREL)  = 0;
+    s.i(I_HASREL)   = 1;
     s.d(D_RNORM)    = rn;
     seg_progress(s, rn, p);
     h.put(s, it, rn);
@@ -200,6 +208,7 @@ CPPE_HD inline void seg_true_residual(const SegRef& s, double rr, int it, int st
     const int k0    = s.i(I_ITERS) - 1;  // iteration at which it froze
     s.d(D_RNORM)    = rn;
     s.i(I_PENDING)  = 0;
+    s.i(I_HASREL)   = 1;
     seg_progress(s, rn, p);
     h.put(s, k0, rn);
     if (!isfinite(rn)) {
@@ -208,6 +217,21 @@ CPPE_HD inline void seg_true_residual(const SegRef& s, double rr, int it, int st
       s.i(I_ACTIVE) = 1;  // resumes from the true residual
     }
   }
+}
+
+// Confirmation of a pending segment from its stored FP64 true residual (mixed form): rr =
+// ||r_t - B32 (x - x_t)||^2, yy = ||B32 (x - x_t)||^2. Accepted (the segment has converged, its
+// norm recorded at the iteration where it froze) when ||r_c|| + eta ||B32 (x - x_t)|| < thresh;
+// otherwise it stays pending for an FP64 true residual. Returns whether it was tried.
+CPPE_HD inline bool seg_confirm32(const SegRef& s, double rr, double yy, double eta, const Refine& p, const Hist& h) {
+  if (!s.i(I_PENDING) || !s.i(I_HASREL)) return false;
+  const double rn = sqrt(rr), bound = rn + eta * sqrt(yy);
+  if (isfinite(bound) && bound < p.thresh) {
+    s.i(I_PENDING) = 0;
+    s.d(D_RNORM)   = rn;
+    h.put(s, s.i(I_ITERS) - 1, rn);
+  }
+  return true;
 }
 
 // Steps an active segment is expected to take before it needs the host, from its mean log

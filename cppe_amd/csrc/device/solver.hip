@@ -57,11 +57,12 @@ enum Status { ST_IT = 0, ST_ACTIVE, ST_REL, ST_BAD, ST_STEPPED, ST_PEND, ST_STAG
 __device__ __forceinline__ bool step_halted(const int* status) {
   return status[ST_REL] > 0 || status[ST_ACTIVE] == 0;
 }
-// solve parameters on the device: [thresh, reliable drop, min progress, stall steps, restart]
-enum Param { P_THRESH = 0, P_DROP, P_PROGRESS, P_STALL, P_RESTART, P_COUNT };
+// solve parameters on the device: [thresh, reliable drop, min progress, stall steps, restart,
+// eta of the FP32-corrected confirmation]
+enum Param { P_THRESH = 0, P_DROP, P_PROGRESS, P_STALL, P_RESTART, P_ETA, P_COUNT };
 // F_RNB: the residual norm and its tests, then -- unless a segment asked for a reliable update --
-// F_BETA's update, in one launch
-enum FinalMode { F_INIT = 0, F_REL, F_BETA, F_RNB };
+// F_BETA's update, in one launch; F_CONF: FP32-corrected confirmation of pending segments
+enum FinalMode { F_INIT = 0, F_REL, F_BETA, F_RNB, F_CONF };
 
 // Both summations apply B through the tree path: a direct-summation system is a tree of one leaf
 // whose near list is itself (every pair, exclusions and Thole damping as in the direct sum).
@@ -139,6 +140,7 @@ struct CgVecs {
   const double* a6;
   const double* rhs;
   double *x, *r, *z, *p, *Ap, *t;
+  double *xrel, *rrel, *dv;  // mixed form: moments and residual at the last FP64 true residual, correction
   double* part;     // 2 * nchunk partials
   double* sd;       // D_COUNT * nseg
   int* si;          // I_COUNT * nseg
@@ -281,7 +283,9 @@ __global__ __launch_bounds__(256) void k_sg_xrz(CgVecs v) {
   }
 }
 
-// r = rhs - t for the segments that asked for a reliable update or await confirmation; partial r.r
+// r = rhs - t for the segments that asked for a reliable update or await confirmation; partial r.r.
+// With xrel set, the moments and the true residual are also kept for later FP32-corrected
+// confirmations.
 __global__ __launch_bounds__(256) void k_sg_resid(CgVecs v) {
   __shared__ double red[4];
   const int c = blockIdx.x, n = v.na, g = v.cseg[c];
@@ -294,10 +298,54 @@ __global__ __launch_bounds__(256) void k_sg_resid(CgVecs v) {
         const size_t k  = ci.b + (size_t)q * n + i;
         const double rk = v.rhs[k] - v.t[k];
         v.r[k] = rk;
+        if (v.xrel) {
+          v.xrel[k] = v.x[k];
+          v.rrel[k] = rk;
+        }
         s += rk * rk;
       }
   s = block_sum(s, red);
   if (threadIdx.x == 0) v.part[c] = s;
+}
+
+__device__ __forceinline__ bool conf_seg(const CgVecs& v, int g) {
+  return v.si[I_PENDING * v.nseg + g] && v.si[I_HASREL * v.nseg + g];
+}
+
+// dv = x - xrel for pending segments with a stored true residual, 0 elsewhere
+__global__ __launch_bounds__(256) void k_sg_defect(CgVecs v) {
+  const int c = blockIdx.x, n = v.na;
+  const ChunkIdx ci(v, c);
+  const bool on = conf_seg(v, v.cseg[c]);
+  for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const size_t k = ci.b + (size_t)q * n + i;
+      v.dv[k]        = on ? v.x[k] - v.xrel[k] : 0.0;
+    }
+}
+
+// partials of ||rrel - t||^2 (part[c]) and ||t||^2 (part[nchunk + c]), t = B32 dv, for the same segments
+__global__ __launch_bounds__(256) void k_sg_confirm(CgVecs v) {
+  __shared__ double red[4];
+  const int c = blockIdx.x, n = v.na;
+  const ChunkIdx ci(v, c);
+  double s1 = 0.0, s2 = 0.0;
+  if (conf_seg(v, v.cseg[c]))
+    for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const size_t k  = ci.b + (size_t)q * n + i;
+        const double tk = v.t[k], rk = v.rrel[k] - tk;
+        s1 += rk * rk;
+        s2 += tk * tk;
+      }
+  s1 = block_sum(s1, red);
+  s2 = block_sum(s2, red);
+  if (threadIdx.x == 0) {
+    v.part[c]            = s1;
+    v.part[v.nchunk + c] = s2;
+  }
 }
 
 // The direction (beta in F_RNB, k_sg_p) follows the step in the same launch sequence and does
@@ -447,11 +495,12 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
   int na = 0, nr = 0, nb = 0, np = 0, nst = 0, h_rel = kFarSteps, h_done = 0;
   for (int g = wv; g < ns; g += 4) {
     double s1 = lane_chunk_sum(f.part, f.seg_chunk[g], f.seg_chunk[g + 1], lane), s2 = 0.0;
-    if (MODE == F_INIT || MODE == F_RNB) s2 = lane_chunk_sum(f.part + f.nchunk, f.seg_chunk[g], f.seg_chunk[g + 1], lane);
+    constexpr bool TWO = MODE == F_INIT || MODE == F_RNB || MODE == F_CONF;
+    if (TWO) s2 = lane_chunk_sum(f.part + f.nchunk, f.seg_chunk[g], f.seg_chunk[g + 1], lane);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       s1 += __shfl_xor(s1, off);
-      if (MODE == F_INIT || MODE == F_RNB) s2 += __shfl_xor(s2, off);
+      if (TWO) s2 += __shfl_xor(s2, off);
     }
     if (lane != 0) continue;
     const SegRef sg{f.sd, f.si, ns, g};
@@ -465,6 +514,8 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
       seg_step(sg, s1, it, MIXED, pr, h);
     } else if (MODE == F_REL) {
       seg_true_residual(sg, s1, it, f.status[ST_STEPPED], pr, h);
+    } else if (MODE == F_CONF) {
+      seg_confirm32(sg, s1, s2, f.params[P_ETA], pr, h);
     } else {  // F_BETA
       seg_beta(sg, s1);
     }
@@ -531,6 +582,7 @@ void CgState::release_graphs() {
   for (auto& form : g_step)
     for (auto& g : form) drop(g);
   drop(g_rel);
+  drop(g_conf);
   drop(g_dir[0]);
   drop(g_dir[1]);
 }
@@ -653,6 +705,9 @@ static CgVecs vecs(System& S, CgState& C) {
   v.p      = C.p;
   v.Ap     = C.Ap;
   v.t      = C.t;
+  v.xrel   = C.xrel;
+  v.rrel   = C.rrel;
+  v.dv     = C.dv;
   v.part   = C.part.p;
   v.sd     = C.sd.p;
   v.si     = C.si.p;
@@ -710,6 +765,15 @@ static void seq_reliable(System& S, CgState& C) {
   apply(S, C, C.x, C.t, false);
   hipLaunchKernelGGL(k_sg_resid, dim3(C.nchunk), dim3(256), 0, S.stream, vecs(S, C));
   launch_final<F_REL>(S, C, true);
+}
+
+// FP32-corrected confirmation of the pending segments (cg_policy.hh, seg_confirm32)
+static void seq_confirm(System& S, CgState& C) {
+  const dim3 g(C.nchunk), b(256);
+  hipLaunchKernelGGL(k_sg_defect, g, b, 0, S.stream, vecs(S, C));
+  apply(S, C, C.dv, C.t, true);
+  hipLaunchKernelGGL(k_sg_confirm, g, b, 0, S.stream, vecs(S, C));
+  launch_final<F_CONF>(S, C, true);
 }
 
 static void seq_direction(System& S, CgState& C, bool mixed) {
@@ -820,12 +884,28 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
     else
       C.rec64 = r64;
   }
+  // mixed form: moments and residual of the last FP64 true residual, for FP32-corrected confirmations
+  const bool conf = mixed && S.confirm_eta > 0.0;
+  if (conf) {
+    const size_t nv = 3 * (size_t)C.na * C.nrhs;
+    if (C.rstore.n < 3 * nv) {
+      C.rstore.alloc(3 * nv);
+      C.release_graphs();  // the graphs hold the vectors
+    }
+    C.xrel = C.rstore.p;
+    C.rrel = C.rstore.p + nv;
+    C.dv   = C.rstore.p + 2 * nv;
+  } else if (C.xrel) {
+    C.xrel = C.rrel = C.dv = nullptr;
+    C.release_graphs();
+  }
   double params[P_COUNT];
   params[P_THRESH]   = thresh;
   params[P_DROP]     = S.reliable_drop;
   params[P_PROGRESS] = S.refine_min_progress;
   params[P_STALL]    = S.refine_stall;
   params[P_RESTART]  = 0.0;
+  params[P_ETA]      = S.confirm_eta;
   CPPE_HIP_CHECK(hipMemcpyAsync(C.params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
   // r0 = rhs - B x0, z0 = alpha r0, p = z0. The mixed form takes r0 from an FP32 apply: it is
   // replaced by an FP64 true residual at the first reliable update, before any convergence test.
@@ -835,7 +915,7 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
   launch_final<F_INIT>(S, C, mixed);
   ++C.expect;  // every launch sequence below runs one final kernel per step
   read_status(S, C);
-  int k = 0, nrel = 0, nlaunch = 0, nsteps = 0, nread = 1;
+  int k = 0, nrel = 0, nlaunch = 0, nsteps = 0, nread = 1, nconf = 0;
   int m = mixed ? 1 : 0;
   res.fp64_fallback = -1;
   // Loop invariant at the top: every active segment has completed step, replacement and
@@ -847,6 +927,19 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
     read_status(S, C);
     ++nrel;
     ++nread;
+  };
+  // Pending segments with nothing else to do: first the FP32-corrected confirmation, then an FP64
+  // true residual for those it did not confirm.
+  auto confirm = [&] {
+    if (mixed && conf && C.h_status[ST_REL] == 0) {
+      run_graph(S, C.g_conf, [&] { seq_confirm(S, C); });
+      ++C.expect;
+      read_status(S, C);
+      ++nconf;
+      ++nread;
+      if (C.h_status[ST_PEND] == 0) return;
+    }
+    reliable();
   };
   // 2^l steps as one launch, 2^l at most the device's horizon, the iterations left and the
   // system's limit; k = steps taken (steps that start after an event take none)
@@ -900,7 +993,7 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
   while (k < maxiter) {
     if (C.h_status[ST_ACTIVE] == 0) {
       if (C.h_status[ST_PEND] == 0) break;
-      reliable();
+      confirm();
       if (fallback()) continue;
       if (C.h_status[ST_ACTIVE] == 0) break;
       run_graph(S, C.g_dir[m], [&] { seq_direction(S, C, mixed); });
@@ -912,7 +1005,10 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
     // the update). Confirming pending segments when none is active leaves the direction pass a
     // no-op on the vectors; segments that resume then take theirs after the confirmation.
     if (C.h_status[ST_REL] > 0 || (C.h_status[ST_ACTIVE] == 0 && C.h_status[ST_PEND] > 0)) {
-      reliable();
+      if (C.h_status[ST_REL] > 0)
+        reliable();
+      else
+        confirm();
       if (fallback()) continue;
       if (C.h_status[ST_ACTIVE] > 0) {
         run_graph(S, C.g_dir[m], [&] { seq_direction(S, C, mixed); });
@@ -948,6 +1044,7 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
   res.norms0           = res.norms[0];
   res.loop_iters       = k;
   res.reliable_updates = nrel;
+  res.confirmations    = nconf;
   res.step_launches    = nlaunch;
   res.steps_launched   = nsteps;
   res.host_reads       = nread;

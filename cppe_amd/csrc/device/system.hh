@@ -98,6 +98,7 @@ struct CgResult {
   std::vector<std::vector<double>> norms;  // residual norms per iteration, every segment
   int loop_iters = 0;          // iterations of the shared loop (max over segments)
   int reliable_updates = 0;    // FP64 true-residual applies of the mixed form
+  int confirmations    = 0;    // FP32-corrected confirmations of the mixed form (one FP32 apply each)
   int fp64_fallback = -1;      // mixed form: iteration at which it continued in FP64 (-1: never)
   int step_launches = 0;       // launches of step sequences (each of 1, 2, 4 or 8 steps)
   int steps_launched = 0;      // steps in them (those that start after an event take none)
@@ -121,12 +122,16 @@ struct CgState {
   int* d_hstat  = nullptr;  // its device address
   int expect    = 0;        // final kernels launched so far: the mirror's count once they have run
   // g_step[form][l]: 2^l consecutive steps (form 0 FP64, 1 mixed)
-  hipGraphExec_t g_step[2][4] = {}, g_rel = nullptr, g_dir[2] = {nullptr, nullptr};
+  hipGraphExec_t g_step[2][4] = {}, g_rel = nullptr, g_conf = nullptr, g_dir[2] = {nullptr, nullptr};
   double *x = nullptr, *rhs = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr, *t = nullptr;
   float4* rec32   = nullptr;  // near-field dipole records of p written by the direction kernels
   double4* rec64 = nullptr;   // (one of them, in the precision of the step's apply; none: nrhs > 1)
   dbuf<double> store;  // backing of the vectors when this state owns them
   dbuf<double> wbuf;   // alpha r of the local preconditioner (same layout as r)
+  // mixed form: moments and residual at the last FP64 true residual and the correction since
+  // (FP32-corrected confirmation), in rstore
+  double *xrel = nullptr, *rrel = nullptr, *dv = nullptr;
+  dbuf<double> rstore;
   void release_graphs();
   ~CgState();
 };
@@ -183,6 +188,7 @@ struct System {
   int refine_stall = 10;        // mixed form: steps without halving that trigger one as well
   double refine_min_progress = 0.5;  // mixed form: a true residual above this fraction of the
                                      // previous one continues the solve in FP64
+  double confirm_eta = 1e-5;  // mixed form: relative-error bound of the FP32-corrected confirmation (0: off)
   CgResult last;            // outcome of the last solve
   dbuf<double> io_a, io_b;  // AoS3 staging in caller order
   dbuf<float> f32_src;      // FP32 staging for mixed-precision applies

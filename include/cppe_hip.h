@@ -186,8 +186,23 @@ int cppe_hip_set_graph_steps(cppe_hip_system* sys, int steps);
  * event took none), [2] status reads of the host loop. */
 int cppe_hip_last_solve_launches(cppe_hip_system* sys, int* launches);
 /* Mixed form: factor by which a residual must fall before the FP64 true residual replaces the
- * recursive one (default 1e-5; convergence is always declared on an FP64 true residual). */
+ * recursive one (default 1e-5; convergence is declared on an FP64 true residual or on its
+ * FP32-corrected form, cppe_hip_set_confirmation). */
 int cppe_hip_set_reliable_drop(cppe_hip_system* sys, double drop);
+/*
+ * Mixed form: how a recursive residual below the threshold is confirmed (no counterpart in the
+ * reference, whose FP64 CG stops on its recursive residual, electric_fields.cc:213-229).
+ * eta > 0 (default 1e-5): first from the last FP64 true residual r_t, taken at moments x_t, and
+ * one FP32 apply of the correction since: r_c = r_t - B32 (x - x_t), accepted when
+ * ||r_c|| + eta ||B32 (x - x_t)|| < thresh. eta bounds the FP32 apply's relative error (measured
+ * at or below 1e-6), so acceptance keeps the FP64 residual ||rhs - B x|| below the threshold.
+ * Where that test fails, before any FP64 true residual, and with eta = 0: an FP64 true residual
+ * decides. Needs 3 more vectors of 3 n doubles per right-hand side while eta > 0.
+ */
+int cppe_hip_set_confirmation(cppe_hip_system* sys, double eta);
+/* Last solve: FP32-corrected confirmations run (one FP32 apply each; FP64 true residuals are
+ * counted by cppe_hip_last_solve_info). */
+int cppe_hip_last_solve_confirmations(cppe_hip_system* sys, int* count);
 /*
  * Mixed form, all knobs of the refinement: `drop` as above; `stall`: a recursive residual that
  * has not halved for this many steps also asks for an FP64 true residual (default 10);

@@ -24,6 +24,9 @@ void pol_step(double* sd, int* si, double rr, int it, int mixed, const double* p
 void pol_true(double* sd, int* si, double rr, int it, int stepped, const double* prm, double* hist, int cap) {
   seg_true_residual(SegRef{sd, si, 1, 0}, rr, it, stepped, refine(prm), Hist{hist, cap});
 }
+int pol_confirm(double* sd, int* si, double rr, double yy, double eta, const double* prm, double* hist, int cap) {
+  return seg_confirm32(SegRef{sd, si, 1, 0}, rr, yy, eta, refine(prm), Hist{hist, cap});
+}
 void pol_beta(double* sd, int* si, double rz_new) { seg_beta(SegRef{sd, si, 1, 0}, rz_new); }
 void pol_horizon(double* sd, int* si, int mixed, const double* prm, int* to_rel, int* to_done) {
   seg_horizon(SegRef{sd, si, 1, 0}, mixed != 0, refine(prm), *to_rel, *to_done);

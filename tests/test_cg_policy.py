@@ -4,7 +4,8 @@ the host by tests/cg_policy_host_driver.cc.
 Plain form: the reference recurrence's stopping rule (electric_fields.cc:213-229: one step before
 the first test, stop when ||r|| < thresh). Mixed form: reliable updates on a residual drop, on a
 stalled recursive residual, confirmation of a residual below the threshold by an FP64 true
-residual, and the stagnation test that sends the solve on in the plain FP64 form.
+residual or by its FP32-corrected form (accepted only with the error bound below the threshold),
+and the stagnation test that sends the solve on in the plain FP64 form.
 """
 import ctypes
 import os
@@ -20,7 +21,7 @@ _i = ctypes.POINTER(ctypes.c_int)
 
 # indices of the per-segment tables (cg_policy.hh SegD / SegI)
 D_RZ, D_BETA, D_RNORM, D_RLAST, D_RZN, D_RBEST, D_LOGQ = range(7)
-I_ACTIVE, I_NEEDREL, I_ITERS, I_BAD, I_PENDING, I_ELIGIBLE, I_SINCE, I_STEPS, I_STAG = range(9)
+I_ACTIVE, I_NEEDREL, I_ITERS, I_BAD, I_PENDING, I_ELIGIBLE, I_SINCE, I_STEPS, I_STAG, I_HASREL = range(10)
 THRESH = 1e-8
 
 
@@ -37,10 +38,11 @@ def pol(tmp_path_factory):
     L.pol_step.argtypes = [_d, _i, dd, ii, ii, _d, _d, ii]
     L.pol_true.argtypes = [_d, _i, dd, ii, ii, _d, _d, ii]
     L.pol_beta.argtypes = [_d, _i, dd]
+    L.pol_confirm.argtypes = [_d, _i, dd, dd, dd, _d, _d, ii]
     L.pol_horizon.argtypes = [_d, _i, ii, _d, _i, _i]
     nd, ni = ctypes.c_int(), ctypes.c_int()
     L.pol_sizes(ctypes.byref(nd), ctypes.byref(ni))
-    assert (nd.value, ni.value) == (7, 9)
+    assert (nd.value, ni.value) == (7, 10)
     return L
 
 
@@ -48,7 +50,7 @@ class Seg:
     def __init__(self, L, drop=1e-5, stall=10, min_progress=0.5, cap=64):
         self.L = L
         self.sd = np.zeros(7)
-        self.si = np.zeros(9, dtype=np.int32)
+        self.si = np.zeros(10, dtype=np.int32)
         self.prm = np.array([THRESH, drop, min_progress, stall], dtype=np.float64)
         self.hist = np.full(cap, np.nan)
 
@@ -73,6 +75,10 @@ class Seg:
     def true(self, rn, it, stepped):
         self.L.pol_true(*self._a(), rn * rn, it, stepped, self.prm.ctypes.data_as(_d),
                         self.hist.ctypes.data_as(_d), self.hist.size)
+
+    def confirm(self, rc, y, eta):
+        return self.L.pol_confirm(*self._a(), rc * rc, y * y, eta, self.prm.ctypes.data_as(_d),
+                                  self.hist.ctypes.data_as(_d), self.hist.size)
 
 
 def test_plain_form_reference_stopping_rule(pol):
@@ -121,6 +127,41 @@ def test_mixed_form_drop_pending_confirmation(pol):
     assert s.si[I_PENDING] == 1
     s.true(5e-9, it + 1, 0)  # confirmed
     assert s.si[I_ACTIVE] == 0 and s.si[I_PENDING] == 0 and s.si[I_ITERS] == it + 1
+
+
+def test_mixed_form_fp32_corrected_confirmation(pol):
+    s = Seg(pol)
+    s.init(1.0, mixed=True)
+    assert s.si[I_HASREL] == 0
+    s.step(0.5e-8, 0, mixed=True)  # below the threshold before any FP64 true residual
+    assert s.si[I_PENDING] == 1
+    assert s.confirm(1e-9, 1.0, 1e-5) == 0 and s.si[I_PENDING] == 1  # nothing stored: not tried
+    s.true(3e-6, 1, 0)  # FP64: not confirmed, resumes; its moments and residual are now stored
+    assert s.si[I_ACTIVE] == 1 and s.si[I_HASREL] == 1
+    rn, it = 3e-6, 1
+    while s.si[I_ACTIVE]:
+        rn /= 3.0
+        s.step(rn, it, mixed=True)
+        it += 1
+    assert s.si[I_PENDING] == 1 and s.si[I_ITERS] == it
+    # the corrected residual is below the threshold, but not with the error bound: stays pending
+    assert s.confirm(9.5e-9, 3e-6, 1e-3) == 1
+    assert s.si[I_PENDING] == 1 and s.si[I_ACTIVE] == 0 and np.isnan(s.hist[it - 1])
+    # corrected residual and bound below the threshold: converged at the iteration where it froze
+    assert s.confirm(9.5e-9, 3e-6, 1e-5) == 1
+    assert s.si[I_PENDING] == 0 and s.si[I_ACTIVE] == 0 and s.si[I_ITERS] == it
+    assert s.hist[it - 1] == 9.5e-9 and s.sd[D_RNORM] == 9.5e-9
+    # a non-finite correction is never accepted
+    t = Seg(pol)
+    t.init(1.0, mixed=True)
+    t.step(5e-6, 0, mixed=True)  # fell by the drop: FP64 true residual
+    assert t.si[I_NEEDREL] == 1
+    t.true(4e-6, 0, 1)
+    assert t.si[I_HASREL] == 1 and t.si[I_ACTIVE] == 1
+    t.step(1e-9, 1, mixed=True)
+    assert t.si[I_PENDING] == 1
+    assert t.confirm(float("nan"), 1.0, 1e-5) == 1
+    assert t.si[I_PENDING] == 1
 
 
 def test_mixed_form_stalled_residual_asks_for_true_residual(pol):

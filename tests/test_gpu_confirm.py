@@ -1,0 +1,148 @@
+"""GPU checks of the FP32-corrected confirmation of the mixed form (cppe_hip_set_confirmation): a
+recursive residual below the threshold is confirmed from the last FP64 true residual r_t (at
+moments x_t) and one FP32 apply of the correction since, r_c = r_t - B32 (x - x_t), accepted when
+||r_c|| + eta ||B32 (x - x_t)|| < thresh; otherwise an FP64 true residual decides.
+
+* default eta: one FP64 true residual fewer, and the oracle's FP64 operator confirms
+  ||rhs - B mu|| < thresh (fmm/calculate.cc:205-225 and the direct sum for the operator)
+* an eta that the bound never meets: every confirmation falls back to the FP64 true residual and
+  the solve is bit for bit the one with eta = 0 (the attempt changes no state the solve reads)
+* batches (snapshots confirmed together, tree and direct), several right-hand sides
+"""
+import functools
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cppe_amd import capi, synthetic
+
+pytestmark = pytest.mark.gpu
+DAMP = 2.1304
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@functools.lru_cache(None)
+def case():
+    w = synthetic.water_box(1000, 1051, anisotropic=True)
+    rhs = O.nuclear_fields(w.pos, w.atoms_z.astype(float), w.atoms_pos)
+    Bo = O.BSystem(w.pos, w.pol, w.excl_csr(), damping=DAMP, scheme="fmm", ncrit=32)
+    return w, rhs, Bo
+
+
+def mixed_system(w, summation="fmm"):
+    return capi.System(w.pos, w.pol, w.alpha_iso(), w.excl_csr(), summation=summation, ncrit=32, damping=DAMP,
+                       precision="mixed")
+
+
+@pytest.mark.parametrize("precond", ["jacobi", "local"])
+def test_confirmation_saves_an_fp64_residual(precond):
+    w, rhs, Bo = case()
+    S = mixed_system(w)
+    S.set_preconditioner(precond)
+    out = {}
+    for eta in (0.0, 1e-5):
+        say("confirm", precond, "eta", eta)
+        S.set_confirmation(eta)
+        mu, it, norms = S.solve(rhs, thresh=1e-8, maxiter=60)
+        out[eta] = (mu, it, norms, S.last_solve_info())
+        say("iterations", it, out[eta][3], "last norm", norms[-1])
+        assert np.linalg.norm(rhs - Bo.apply(mu)) < 1e-8
+        assert len(norms) == it and norms[-1] < 1e-8
+    i0, i1 = out[0.0][3], out[1e-5][3]
+    assert i0["confirmations"] == 0
+    assert i1["confirmations"] >= 1 and i1["reliable_updates"] == i0["reliable_updates"] - 1
+    assert out[1e-5][1] == out[0.0][1]
+    assert np.array_equal(out[1e-5][0], out[0.0][0])  # the confirmation does not touch the moments
+    # the corrected residual agrees with the FP64 true residual to far below the threshold
+    assert abs(out[1e-5][2][-1] - out[0.0][2][-1]) < 1e-11
+    S.close()
+
+
+def test_unmet_bound_falls_back_bitwise():
+    w, rhs, Bo = case()
+    S = mixed_system(w)
+    S.set_confirmation(0.0)
+    mu0, it0, n0 = S.solve(rhs, thresh=1e-8, maxiter=60)
+    info0 = S.last_solve_info()
+    S.set_confirmation(1e6)  # eta ||B32 dx|| is far above the threshold: never accepted
+    mu1, it1, n1 = S.solve(rhs, thresh=1e-8, maxiter=60)
+    info1 = S.last_solve_info()
+    say("unmet bound", info0, info1)
+    assert info1["confirmations"] >= 1 and info1["reliable_updates"] == info0["reliable_updates"]
+    assert it1 == it0 and np.array_equal(mu1, mu0) and np.array_equal(n1, n0)
+    S.close()
+
+
+def test_confirmation_in_a_batch():
+    sizes = [150, 420, 300]
+    snaps = [synthetic.water_box(nw, 7150 + k, anisotropic=True) for k, nw in enumerate(sizes)]
+    direct = [False, True, False]
+    rhs = np.concatenate([O.nuclear_fields(s.pos, s.atoms_z.astype(float), s.atoms_pos) for s in snaps])
+    res = {}
+    for eta in (0.0, 1e-5):
+        B = capi.Batch(snaps, direct, ncrit=32, damping=DAMP, precision="mixed")
+        B.set_confirmation(eta)
+        B.upload_rhs(rhs)
+        B.solve_resident(0, 1e-8, 60)
+        iters, rn, conv = B.segment_results()
+        info = B.last_solve_info()
+        say("batch eta", eta, iters, rn, info)
+        assert conv.all()
+        mu = B.split(B.download_mu())
+        for g, s in enumerate(snaps):
+            Bo = O.BSystem(s.pos, s.pol, s.excl_csr(), damping=DAMP, scheme="direct" if direct[g] else "fmm",
+                           ncrit=32)
+            assert np.linalg.norm(B.split(rhs)[g] - Bo.apply(mu[g])) < 1e-8, g
+        res[eta] = (iters, info)
+        B.close()
+    # a snapshot whose corrected residual leaves less than the error bound below the threshold
+    # is confirmed in FP64 after all, so the saving is not guaranteed here
+    assert res[1e-5][1]["confirmations"] >= 1
+    assert res[1e-5][1]["reliable_updates"] <= res[0.0][1]["reliable_updates"]
+    assert np.array_equal(res[1e-5][0], res[0.0][0])
+
+
+def test_confirmation_several_right_hand_sides():
+    w, rhs, Bo = case()
+    S = mixed_system(w)
+    R = np.stack([rhs, np.roll(rhs, 3), 0.5 * rhs + np.roll(rhs, 9)])
+    out = {}
+    for eta in (0.0, 1e-5):
+        S.set_confirmation(eta)
+        mu, it, norms = S.solve_multi(R, thresh=1e-8, maxiter=60)
+        say("multi eta", eta, it, [n[-1] for n in norms])
+        for k in range(R.shape[0]):
+            assert np.linalg.norm(R[k] - Bo.apply(mu[k])) < 1e-8, k
+        out[eta] = (mu, it, norms)
+    assert np.array_equal(out[1e-5][0], out[0.0][0]) and np.array_equal(out[1e-5][1], out[0.0][1])
+    # a right-hand side confirmed from its corrected residual records that norm, not the FP64 one
+    last = [(a[-1], b[-1]) for a, b in zip(out[1e-5][2], out[0.0][2])]
+    assert any(a != b for a, b in last) and all(abs(a - b) < 1e-11 for a, b in last)
+    S.close()
+
+
+def test_direct_summation_confirmation():
+    w = synthetic.water_box(700, 1052, anisotropic=True)
+    rhs = O.nuclear_fields(w.pos, w.atoms_z.astype(float), w.atoms_pos)
+    S = mixed_system(w, "direct")
+    mu, it, norms = S.solve(rhs, thresh=1e-8, maxiter=60)
+    info = S.last_solve_info()
+    say("direct", it, info)
+    Bo = O.BSystem(w.pos, w.pol, w.excl_csr(), damping=DAMP, scheme="direct")
+    assert np.linalg.norm(rhs - Bo.apply(mu)) < 1e-8
+    S.close()
+
+
+def test_confirmation_argument_rejected():
+    w, rhs, Bo = case()
+    S = mixed_system(w)
+    for bad in (-1e-5, float("nan"), float("inf")):
+        with pytest.raises(capi.CppeHipError):
+            S.set_confirmation(bad)
+    S.close()
