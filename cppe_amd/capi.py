@@ -97,7 +97,7 @@ def lib():
         L.cppe_hip_set_refinement.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double]
         L.cppe_hip_set_preconditioner.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.cppe_hip_last_solve_info.argtypes = [vp, _i]
-        L.cppe_hip_set_confirmation.argtypes = [vp, ctypes.c_double]
+        L.cppe_hip_set_confirmation.argtypes = [vp, ctypes.c_double, ctypes.c_int]
         L.cppe_hip_last_solve_confirmations.argtypes = [vp, _i]
         L.cppe_hip_set_graph_steps.argtypes = [vp, ctypes.c_int]
         L.cppe_hip_last_solve_launches.argtypes = [vp, _i]
@@ -304,11 +304,12 @@ class System:
         k = {"jacobi": 0, "local": 1}[kind]
         _check(lib().cppe_hip_set_preconditioner(self.h, k, float(radius), float(weight)))
 
-    def set_confirmation(self, eta=1e-5):
-        """Mixed form: confirm a converged recursive residual from the last FP64 true residual and an
-        FP32 apply of the correction since, accepted with the error bound eta (0: FP64 true
-        residual only) (cppe_hip_set_confirmation)."""
-        _check(lib().cppe_hip_set_confirmation(self.h, float(eta)))
+    def set_confirmation(self, eta=1e-5, gap=True, corrected=True):
+        """Mixed form: how a converged recursive residual is confirmed, eta bounding an FP32 apply's
+        relative error: by the gap bound since the last FP64 true residual (gap), by that residual
+        and an FP32 apply of the correction since (corrected), else by an FP64 true residual
+        (cppe_hip_set_confirmation)."""
+        _check(lib().cppe_hip_set_confirmation(self.h, float(eta), int(bool(gap)) | 2 * int(bool(corrected))))
 
     def last_solve_info(self):
         info = np.zeros(3, dtype=np.int32)

@@ -709,10 +709,12 @@ int cppe_hip_set_refinement(cppe_hip_system* sys, double drop, int stall, double
   CPPE_API_END
 }
 
-int cppe_hip_set_confirmation(cppe_hip_system* sys, double eta) {
+int cppe_hip_set_confirmation(cppe_hip_system* sys, double eta, int forms) {
   CPPE_API_BEGIN
-  if (!sys || !(eta >= 0.0) || !std::isfinite(eta)) return fail(CPPE_HIP_ERR_ARG, "confirmation: eta >= 0");
-  sys->S.confirm_eta = eta;
+  if (!sys || !(eta >= 0.0) || !std::isfinite(eta) || forms < 0 || forms > 3)
+    return fail(CPPE_HIP_ERR_ARG, "confirmation: eta >= 0, forms in 0..3");
+  sys->S.confirm_eta   = eta;
+  sys->S.confirm_forms = forms;
   return CPPE_HIP_OK;
   CPPE_API_END
 }

@@ -15,11 +15,15 @@
 //  * a true residual that is not below `min_progress` times the previous true residual after at
 //    least one step marks the segment as stagnating (STAG): the refinement is not converging and
 //    the solve continues from the current moments in the plain FP64 form;
-//  * a pending segment with an FP64 true residual r_t at moments x_t can instead be confirmed from
-//    it: r_c = r_t - B32 (x - x_t), one FP32 apply of the correction since then. It is accepted
-//    when ||r_c|| + eta ||B32 (x - x_t)|| < thresh, eta a bound on the FP32 apply's relative error,
-//    so that ||rhs - B x|| < thresh holds for the FP64 operator; otherwise an FP64 true residual
-//    decides as before.
+//  * after an FP64 true residual the gap between true and recursive residual is bounded by
+//    eta sum_k |alpha_k| ||B32 p_k|| over the steps since (each FP32 apply's error at most eta times
+//    its result; residual replacement after van der Vorst & Ye, SIAM J. Sci. Comput. 22, 835
+//    (2000)): a recursive residual with ||r|| + that bound below the threshold has converged for
+//    the FP64 operator without a confirmation;
+//  * otherwise a pending segment with an FP64 true residual r_t at moments x_t can be confirmed
+//    from it: r_c = r_t - B32 (x - x_t), one FP32 apply of the correction since then, accepted
+//    when ||r_c|| + eta ||B32 (x - x_t)|| < thresh; else an FP64 true residual decides as before.
+//    Either form can be switched off (its eta 0); with both off an FP64 true residual confirms.
 // Launch horizon: after every decision a segment also predicts how many more steps it will take
 // before it needs the host (seg_horizon), from its mean log reduction per step; the host launches
 // that many steps as one graph, and steps launched beyond an event do nothing on the device.
@@ -34,14 +38,16 @@
 
 namespace cppe_dev {
 
+using std::fabs;
 using std::isfinite;
 using std::log;
 using std::sqrt;
 
 // per-segment doubles; D_RZN: the new r.z of a step, kept between the final kernel's two passes;
 // D_RBEST: smallest recursive residual since the last true residual (halving steps only);
-// D_LOGQ: mean log reduction of the residual norm per step (0: not known yet)
-enum SegD { D_RZ = 0, D_BETA, D_RNORM, D_RLAST, D_RZN, D_RBEST, D_LOGQ, D_COUNT };
+// D_LOGQ: mean log reduction of the residual norm per step (0: not known yet); D_GAP: sum of
+// |alpha_k| ||B32 p_k|| since the last true residual
+enum SegD { D_RZ = 0, D_BETA, D_RNORM, D_RLAST, D_RZN, D_RBEST, D_LOGQ, D_GAP, D_COUNT };
 // per-segment ints. I_PENDING: frozen below the threshold until an FP64 true residual confirms it;
 // I_ELIGIBLE: within 100x of the replacement trigger, replaced whenever a replacement runs for
 // another segment (segments share FP64 applies); I_SINCE: steps since D_RBEST last halved;
@@ -54,6 +60,8 @@ struct Refine {
   double drop;          // mixed: recursive-residual drop that asks for a true residual
   double min_progress;  // mixed: a true residual above this fraction of the previous one stagnates
   int stall;            // mixed: steps without halving that ask for a true residual
+  double eta;           // mixed: bound of an FP32 apply's relative error in the corrected confirmation (0: off)
+  double eta_gap;       // mixed: the same bound in the gap bound of the recursive residual (0: off)
 };
 
 struct SegRef {
@@ -81,6 +89,7 @@ CPPE_HD inline void seg_init(const SegRef& s, double rz, double rr, bool mixed, 
   s.d(D_RBEST) = rn;
   s.d(D_BETA)  = 0.0;
   s.d(D_LOGQ)  = 0.0;
+  s.d(D_GAP)   = 0.0;
   const bool bad = !isfinite(rn);
   s.i(I_BAD)      = bad;
   s.i(I_ITERS)    = 0;
@@ -131,8 +140,10 @@ CPPE_HD inline void seg_restart(const SegRef& s, double rz, double rr, int it, c
   }
 }
 
-// Residual norm after the step of iteration `it` (rr = r.r of the recursive residual).
-CPPE_HD inline void seg_step(const SegRef& s, double rr, int it, bool mixed, const Refine& p, const Hist& h) {
+// Residual norm after the step of iteration `it` (rr = r.r of the recursive residual; pap = p.Ap and
+// apap = Ap.Ap of the step, mixed form).
+CPPE_HD inline void seg_step(const SegRef& s, double rr, int it, bool mixed, const Refine& p, const Hist& h,
+                             double pap = 0.0, double apap = 0.0) {
   if (!s.i(I_ACTIVE)) return;
   const double rn = sqrt(rr), prev = s.d(D_RNORM);
   if (isfinite(rn) && rn > 0.0 && isfinite(prev) && prev > 0.0) {  // running mean, weight 1/2
@@ -150,6 +161,16 @@ CPPE_HD inline void seg_step(const SegRef& s, double rr, int it, bool mixed, con
   }
   if (mixed) {
     s.i(I_STEPS) += 1;
+    if (p.eta_gap > 0.0) {  // alpha_k = (r.z) / (p.Ap); a step without a finite term voids the bound
+      const double g = pap != 0.0 ? fabs(s.d(D_RZ) / pap) * sqrt(apap) : INFINITY;
+      s.d(D_GAP)     = isfinite(g) ? s.d(D_GAP) + g : INFINITY;
+    }
+    if (rn < p.thresh && p.eta_gap > 0.0 && s.i(I_HASREL) && rn + p.eta_gap * s.d(D_GAP) < p.thresh) {
+      h.put(s, it, rn);  // converged within the gap bound
+      s.i(I_ACTIVE) = 0;
+      s.i(I_ITERS)  = it + 1;
+      return;
+    }
     if (rn < 0.5 * s.d(D_RBEST)) {
       s.d(D_RBEST) = rn;
       s.i(I_SINCE) = 0;
@@ -179,6 +200,7 @@ CPPE_HD inline void seg_progress(const SegRef& s, double rn, const Refine& p) {
   if (s.i(I_STEPS) > 0 && isfinite(rn) && !(rn < p.thresh) && rn > p.min_progress * s.d(D_RLAST)) s.i(I_STAG) = 1;
   s.d(D_RLAST) = rn;
   s.d(D_RBEST) = rn;
+  s.d(D_GAP)   = 0.0;
   s.i(I_SINCE) = 0;
   s.i(I_STEPS) = 0;
 }
@@ -223,9 +245,9 @@ CPPE_HD inline void seg_true_residual(const SegRef& s, double rr, int it, int st
 // ||r_t - B32 (x - x_t)||^2, yy = ||B32 (x - x_t)||^2. Accepted (the segment has converged, its
 // norm recorded at the iteration where it froze) when ||r_c|| + eta ||B32 (x - x_t)|| < thresh;
 // otherwise it stays pending for an FP64 true residual. Returns whether it was tried.
-CPPE_HD inline bool seg_confirm32(const SegRef& s, double rr, double yy, double eta, const Refine& p, const Hist& h) {
+CPPE_HD inline bool seg_confirm32(const SegRef& s, double rr, double yy, const Refine& p, const Hist& h) {
   if (!s.i(I_PENDING) || !s.i(I_HASREL)) return false;
-  const double rn = sqrt(rr), bound = rn + eta * sqrt(yy);
+  const double rn = sqrt(rr), bound = rn + p.eta * sqrt(yy);
   if (isfinite(bound) && bound < p.thresh) {
     s.i(I_PENDING) = 0;
     s.d(D_RNORM)   = rn;

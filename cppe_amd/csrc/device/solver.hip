@@ -58,8 +58,8 @@ __device__ __forceinline__ bool step_halted(const int* status) {
   return status[ST_REL] > 0 || status[ST_ACTIVE] == 0;
 }
 // solve parameters on the device: [thresh, reliable drop, min progress, stall steps, restart,
-// eta of the FP32-corrected confirmation]
-enum Param { P_THRESH = 0, P_DROP, P_PROGRESS, P_STALL, P_RESTART, P_ETA, P_COUNT };
+// eta of the FP32-corrected confirmation, eta of the gap bound]
+enum Param { P_THRESH = 0, P_DROP, P_PROGRESS, P_STALL, P_RESTART, P_ETA, P_GAP, P_COUNT };
 // F_RNB: the residual norm and its tests, then -- unless a segment asked for a reliable update --
 // F_BETA's update, in one launch; F_CONF: FP32-corrected confirmation of pending segments
 enum FinalMode { F_INIT = 0, F_REL, F_BETA, F_RNB, F_CONF };
@@ -211,20 +211,27 @@ __global__ __launch_bounds__(256) void k_sg_init(CgVecs v) {
   }
 }
 
-// partial p.Ap (active segments), into the third block of the partials (read by k_sg_xrz)
+// partial p.Ap (active segments), into the third block of the partials (read by k_sg_xrz), and
+// partial Ap.Ap into the fourth (the gap bound of the mixed form)
 __global__ __launch_bounds__(256) void k_sg_pap(CgVecs v) {
   __shared__ double red[4];
   if (step_halted(v.status)) return;
   const int c = blockIdx.x, n = v.na;
   const ChunkIdx ci(v, c);
-  double s = 0.0;
+  double s = 0.0, q = 0.0;
   if (v.si[I_ACTIVE * v.nseg + v.cseg[c]])
     for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x) {
       const size_t k0 = ci.b + i, k1 = k0 + n, k2 = k1 + n;
-      s += v.p[k0] * v.Ap[k0] + v.p[k1] * v.Ap[k1] + v.p[k2] * v.Ap[k2];
+      const double a0 = v.Ap[k0], a1 = v.Ap[k1], a2 = v.Ap[k2];
+      s += v.p[k0] * a0 + v.p[k1] * a1 + v.p[k2] * a2;
+      q += a0 * a0 + a1 * a1 + a2 * a2;
     }
   s = block_sum(s, red);
-  if (threadIdx.x == 0) v.part[2 * (size_t)v.nchunk + c] = s;
+  q = block_sum(q, red);
+  if (threadIdx.x == 0) {
+    v.part[2 * (size_t)v.nchunk + c] = s;
+    v.part[3 * (size_t)v.nchunk + c] = q;
+  }
 }
 
 // Step update and preconditioner in one pass (active segments): x += alpha p ; r -= alpha Ap ; z = alpha r ;
@@ -489,18 +496,29 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
   const bool restart = MODE == F_INIT && f.params[P_RESTART] != 0.0;
   // iterations so far; a restart also counts the step taken before the stagnating replacement
   const int it = f.status[ST_IT] + (restart ? f.status[ST_STEPPED] : 0);
-  const Refine pr{f.params[P_THRESH], f.params[P_DROP], f.params[P_PROGRESS], (int)f.params[P_STALL]};
+  const Refine pr{f.params[P_THRESH], f.params[P_DROP], f.params[P_PROGRESS], (int)f.params[P_STALL],
+                  f.params[P_ETA], f.params[P_GAP]};
   const Hist h{f.hist, f.hist_cap};
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int na = 0, nr = 0, nb = 0, np = 0, nst = 0, h_rel = kFarSteps, h_done = 0;
   for (int g = wv; g < ns; g += 4) {
-    double s1 = lane_chunk_sum(f.part, f.seg_chunk[g], f.seg_chunk[g + 1], lane), s2 = 0.0;
+    double s1 = lane_chunk_sum(f.part, f.seg_chunk[g], f.seg_chunk[g + 1], lane), s2 = 0.0, s3 = 0.0, s4 = 0.0;
     constexpr bool TWO = MODE == F_INIT || MODE == F_RNB || MODE == F_CONF;
+    // mixed steps with the gap bound: p.Ap (summed as k_sg_xrz sums it for alpha) and Ap.Ap
+    const bool gap = MIXED && MODE == F_RNB && pr.eta_gap > 0.0;
     if (TWO) s2 = lane_chunk_sum(f.part + f.nchunk, f.seg_chunk[g], f.seg_chunk[g + 1], lane);
+    if (gap) {
+      s3 = lane_chunk_sum(f.part + 2 * (size_t)f.nchunk, f.seg_chunk[g], f.seg_chunk[g + 1], lane);
+      s4 = lane_chunk_sum(f.part + 3 * (size_t)f.nchunk, f.seg_chunk[g], f.seg_chunk[g + 1], lane);
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       s1 += __shfl_xor(s1, off);
       if (TWO) s2 += __shfl_xor(s2, off);
+      if (gap) {
+        s3 += __shfl_xor(s3, off);
+        s4 += __shfl_xor(s4, off);
+      }
     }
     if (lane != 0) continue;
     const SegRef sg{f.sd, f.si, ns, g};
@@ -511,11 +529,11 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
         seg_init(sg, s1, s2, MIXED, pr);
     } else if (MODE == F_RNB) {
       sg.d(D_RZN) = s2;  // read back by this thread after the barrier
-      seg_step(sg, s1, it, MIXED, pr, h);
+      seg_step(sg, s1, it, MIXED, pr, h, s3, s4);
     } else if (MODE == F_REL) {
       seg_true_residual(sg, s1, it, f.status[ST_STEPPED], pr, h);
     } else if (MODE == F_CONF) {
-      seg_confirm32(sg, s1, s2, f.params[P_ETA], pr, h);
+      seg_confirm32(sg, s1, s2, pr, h);
     } else {  // F_BETA
       seg_beta(sg, s1);
     }
@@ -616,7 +634,7 @@ static void cg_layout(System& S, CgState& C, const std::vector<int>& seg_off) {
   up(C.chunk_lo, clo);
   up(C.chunk_seg, cseg);
   up(C.seg_chunk, schunk);
-  C.part.alloc(3 * (size_t)C.nchunk);  // [r.r or p.Ap | r.z | p.Ap of the step]
+  C.part.alloc(4 * (size_t)C.nchunk);  // [r.r or p.Ap | r.z | p.Ap of the step | Ap.Ap of the step]
   C.sd.alloc(D_COUNT * (size_t)ns);
   C.si.alloc(I_COUNT * (size_t)ns);
   C.status.alloc(ST_COUNT);
@@ -885,7 +903,7 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
       C.rec64 = r64;
   }
   // mixed form: moments and residual of the last FP64 true residual, for FP32-corrected confirmations
-  const bool conf = mixed && S.confirm_eta > 0.0;
+  const bool conf = mixed && S.confirm_eta > 0.0 && (S.confirm_forms & 2);
   if (conf) {
     const size_t nv = 3 * (size_t)C.na * C.nrhs;
     if (C.rstore.n < 3 * nv) {
@@ -905,7 +923,8 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
   params[P_PROGRESS] = S.refine_min_progress;
   params[P_STALL]    = S.refine_stall;
   params[P_RESTART]  = 0.0;
-  params[P_ETA]      = S.confirm_eta;
+  params[P_ETA]      = conf ? S.confirm_eta : 0.0;
+  params[P_GAP]      = (S.confirm_forms & 1) ? S.confirm_eta : 0.0;
   CPPE_HIP_CHECK(hipMemcpyAsync(C.params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
   // r0 = rhs - B x0, z0 = alpha r0, p = z0. The mixed form takes r0 from an FP32 apply: it is
   // replaced by an FP64 true residual at the first reliable update, before any convergence test.

@@ -188,7 +188,8 @@ struct System {
   int refine_stall = 10;        // mixed form: steps without halving that trigger one as well
   double refine_min_progress = 0.5;  // mixed form: a true residual above this fraction of the
                                      // previous one continues the solve in FP64
-  double confirm_eta = 1e-5;  // mixed form: relative-error bound of the FP32-corrected confirmation (0: off)
+  double confirm_eta = 1e-5;  // mixed form: relative-error bound of an FP32 apply (0: FP64 confirmations only)
+  int confirm_forms  = 3;     // bit 0: the gap bound of the recursive residual, bit 1: corrected confirmation
   CgResult last;            // outcome of the last solve
   dbuf<double> io_a, io_b;  // AoS3 staging in caller order
   dbuf<float> f32_src;      // FP32 staging for mixed-precision applies

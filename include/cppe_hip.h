@@ -191,15 +191,21 @@ int cppe_hip_last_solve_launches(cppe_hip_system* sys, int* launches);
 int cppe_hip_set_reliable_drop(cppe_hip_system* sys, double drop);
 /*
  * Mixed form: how a recursive residual below the threshold is confirmed (no counterpart in the
- * reference, whose FP64 CG stops on its recursive residual, electric_fields.cc:213-229).
- * eta > 0 (default 1e-5): first from the last FP64 true residual r_t, taken at moments x_t, and
- * one FP32 apply of the correction since: r_c = r_t - B32 (x - x_t), accepted when
- * ||r_c|| + eta ||B32 (x - x_t)|| < thresh. eta bounds the FP32 apply's relative error (measured
- * at or below 1e-6), so acceptance keeps the FP64 residual ||rhs - B x|| below the threshold.
- * Where that test fails, before any FP64 true residual, and with eta = 0: an FP64 true residual
- * decides. Needs 3 more vectors of 3 n doubles per right-hand side while eta > 0.
+ * reference, whose FP64 CG stops on its recursive residual, electric_fields.cc:213-229). eta
+ * (default 1e-5) bounds an FP32 apply's relative error (measured at or below 1e-6); forms, a bit
+ * set (default 3):
+ *  CPPE_HIP_CONFIRM_GAP (1): after an FP64 true residual, the gap between true and recursive
+ *    residual is at most eta sum_k |alpha_k| ||B32 p_k|| over the steps since; a recursive
+ *    residual r with ||r|| + that bound below the threshold has converged without further work;
+ *  CPPE_HIP_CONFIRM_CORRECTED (2): otherwise, from the last FP64 true residual r_t (at moments
+ *    x_t) and one FP32 apply of the correction since, r_c = r_t - B32 (x - x_t), accepted when
+ *    ||r_c|| + eta ||B32 (x - x_t)|| < thresh (3 more vectors of 3 n doubles per right-hand side).
+ * Both keep the FP64 residual ||rhs - B x|| below the threshold. What neither accepts, and
+ * everything with eta = 0 or forms = 0, is decided by an FP64 true residual.
  */
-int cppe_hip_set_confirmation(cppe_hip_system* sys, double eta);
+#define CPPE_HIP_CONFIRM_GAP 1
+#define CPPE_HIP_CONFIRM_CORRECTED 2
+int cppe_hip_set_confirmation(cppe_hip_system* sys, double eta, int forms);
 /* Last solve: FP32-corrected confirmations run (one FP32 apply each; FP64 true residuals are
  * counted by cppe_hip_last_solve_info). */
 int cppe_hip_last_solve_confirmations(cppe_hip_system* sys, int* count);

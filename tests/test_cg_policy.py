@@ -20,7 +20,7 @@ _d = ctypes.POINTER(ctypes.c_double)
 _i = ctypes.POINTER(ctypes.c_int)
 
 # indices of the per-segment tables (cg_policy.hh SegD / SegI)
-D_RZ, D_BETA, D_RNORM, D_RLAST, D_RZN, D_RBEST, D_LOGQ = range(7)
+D_RZ, D_BETA, D_RNORM, D_RLAST, D_RZN, D_RBEST, D_LOGQ, D_GAP = range(8)
 I_ACTIVE, I_NEEDREL, I_ITERS, I_BAD, I_PENDING, I_ELIGIBLE, I_SINCE, I_STEPS, I_STAG, I_HASREL = range(10)
 THRESH = 1e-8
 
@@ -35,23 +35,23 @@ def pol(tmp_path_factory):
     L.pol_sizes.argtypes = [_i, _i]
     L.pol_init.argtypes = [_d, _i, dd, dd, ii, _d]
     L.pol_restart.argtypes = [_d, _i, dd, dd, ii, _d]
-    L.pol_step.argtypes = [_d, _i, dd, ii, ii, _d, _d, ii]
+    L.pol_step.argtypes = [_d, _i, dd, ii, ii, _d, _d, ii, dd, dd]
     L.pol_true.argtypes = [_d, _i, dd, ii, ii, _d, _d, ii]
     L.pol_beta.argtypes = [_d, _i, dd]
-    L.pol_confirm.argtypes = [_d, _i, dd, dd, dd, _d, _d, ii]
+    L.pol_confirm.argtypes = [_d, _i, dd, dd, _d, _d, ii]
     L.pol_horizon.argtypes = [_d, _i, ii, _d, _i, _i]
     nd, ni = ctypes.c_int(), ctypes.c_int()
     L.pol_sizes(ctypes.byref(nd), ctypes.byref(ni))
-    assert (nd.value, ni.value) == (7, 10)
+    assert (nd.value, ni.value) == (8, 10)
     return L
 
 
 class Seg:
-    def __init__(self, L, drop=1e-5, stall=10, min_progress=0.5, cap=64):
+    def __init__(self, L, drop=1e-5, stall=10, min_progress=0.5, eta=0.0, cap=64):
         self.L = L
-        self.sd = np.zeros(7)
+        self.sd = np.zeros(8)
         self.si = np.zeros(10, dtype=np.int32)
-        self.prm = np.array([THRESH, drop, min_progress, stall], dtype=np.float64)
+        self.prm = np.array([THRESH, drop, min_progress, stall, eta, eta], dtype=np.float64)
         self.hist = np.full(cap, np.nan)
 
     def _a(self):
@@ -63,9 +63,9 @@ class Seg:
     def restart(self, rn, it):
         self.L.pol_restart(*self._a(), 1.0, rn * rn, it, self.prm.ctypes.data_as(_d))
 
-    def step(self, rn, it, mixed):
+    def step(self, rn, it, mixed, pap=0.0, apap=0.0):
         self.L.pol_step(*self._a(), rn * rn, it, int(mixed), self.prm.ctypes.data_as(_d),
-                        self.hist.ctypes.data_as(_d), self.hist.size)
+                        self.hist.ctypes.data_as(_d), self.hist.size, pap, apap)
 
     def horizon(self, mixed):
         a, b = ctypes.c_int(), ctypes.c_int()
@@ -77,7 +77,8 @@ class Seg:
                         self.hist.ctypes.data_as(_d), self.hist.size)
 
     def confirm(self, rc, y, eta):
-        return self.L.pol_confirm(*self._a(), rc * rc, y * y, eta, self.prm.ctypes.data_as(_d),
+        self.prm[4] = eta  # the corrected confirmation's eta (the gap bound keeps its own)
+        return self.L.pol_confirm(*self._a(), rc * rc, y * y, self.prm.ctypes.data_as(_d),
                                   self.hist.ctypes.data_as(_d), self.hist.size)
 
 
@@ -162,6 +163,37 @@ def test_mixed_form_fp32_corrected_confirmation(pol):
     assert t.si[I_PENDING] == 1
     assert t.confirm(float("nan"), 1.0, 1e-5) == 1
     assert t.si[I_PENDING] == 1
+
+
+def test_mixed_form_gap_bound(pol):
+    """After an FP64 true residual, ||r|| + eta sum |alpha| ||B32 p|| below the threshold ends the
+    solve without a confirmation; anything short of that waits for one."""
+    def replaced(eta):
+        s = Seg(pol, eta=eta)
+        s.init(1.0, mixed=True)  # D_RZ = 1: alpha = 1 / p.Ap below
+        s.step(5e-6, 0, mixed=True, pap=1.0, apap=1e-2)
+        assert s.si[I_NEEDREL] == 1 and s.sd[D_GAP] == (pytest.approx(0.1) if eta > 0 else 0.0)
+        s.true(4e-6, 0, 1)
+        assert s.si[I_HASREL] == 1 and s.sd[D_GAP] == 0.0  # the gap restarts at a true residual
+        for it, rn in ((1, 4e-7), (2, 4e-8), (3, 2e-8)):  # ||alpha B32 p|| = 1e-4 per step
+            s.step(rn, it, mixed=True, pap=2.0, apap=4e-8)
+        assert s.si[I_ACTIVE] == 1
+        assert s.sd[D_GAP] == (pytest.approx(3e-4) if eta > 0 else 0.0)  # tracked only with eta > 0
+        return s
+
+    s = replaced(1e-5)  # bound 5e-9 + 3e-9 < 1e-8: converged, never pending
+    s.step(5e-9, 4, mixed=True, pap=1e9, apap=0.0)
+    assert s.si[I_ACTIVE] == 0 and s.si[I_PENDING] == 0 and s.si[I_ITERS] == 5 and s.hist[4] == 5e-9
+    s = replaced(1e-5)  # 8e-9 + 3e-9 > 1e-8: waits for a confirmation
+    s.step(8e-9, 4, mixed=True, pap=1e9, apap=0.0)
+    assert s.si[I_PENDING] == 1 and s.si[I_ITERS] == 5
+    s = replaced(0.0)  # eta 0: every confirmation in FP64
+    s.step(1e-12, 4, mixed=True, pap=1e9, apap=0.0)
+    assert s.si[I_PENDING] == 1
+    s = Seg(pol, eta=1e-5)  # no FP64 true residual yet: the FP32 initial residual bounds nothing
+    s.init(1e-2, mixed=True)
+    s.step(1e-12, 0, mixed=True, pap=1.0, apap=0.0)
+    assert s.si[I_PENDING] == 1
 
 
 def test_mixed_form_stalled_residual_asks_for_true_residual(pol):

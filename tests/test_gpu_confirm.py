@@ -1,13 +1,17 @@
-"""GPU checks of the FP32-corrected confirmation of the mixed form (cppe_hip_set_confirmation): a
-recursive residual below the threshold is confirmed from the last FP64 true residual r_t (at
-moments x_t) and one FP32 apply of the correction since, r_c = r_t - B32 (x - x_t), accepted when
-||r_c|| + eta ||B32 (x - x_t)|| < thresh; otherwise an FP64 true residual decides.
+"""GPU checks of how the mixed form confirms a recursive residual below the threshold
+(cppe_hip_set_confirmation), eta bounding an FP32 apply's relative error:
 
-* default eta: one FP64 true residual fewer, and the oracle's FP64 operator confirms
-  ||rhs - B mu|| < thresh (fmm/calculate.cc:205-225 and the direct sum for the operator)
-* an eta that the bound never meets: every confirmation falls back to the FP64 true residual and
-  the solve is bit for bit the one with eta = 0 (the attempt changes no state the solve reads)
-* batches (snapshots confirmed together, tree and direct), several right-hand sides
+* gap bound: after an FP64 true residual, ||r|| + eta sum_k |alpha_k| ||B32 p_k|| < thresh ends the
+  solve without further work;
+* corrected confirmation: r_c = r_t - B32 (x - x_t) from the last FP64 true residual r_t (at
+  moments x_t) and one FP32 apply, accepted when ||r_c|| + eta ||B32 (x - x_t)|| < thresh;
+* otherwise an FP64 true residual, as with eta = 0.
+
+Checked: each form saves the final FP64 true residual and leaves the moments bit for bit as with
+eta = 0, the oracle's FP64 operator confirms ||rhs - B mu|| < thresh (fmm/calculate.cc:205-225,
+or the direct sum); an eta that no bound meets falls back to the FP64 path bit for bit (the
+attempts change no state the solve reads); batches (snapshots confirmed together, tree and
+direct) and several right-hand sides.
 """
 import functools
 import sys
@@ -40,27 +44,32 @@ def mixed_system(w, summation="fmm"):
                        precision="mixed")
 
 
+FORMS = {"fp64": dict(eta=0.0), "gap": dict(corrected=False), "corrected": dict(gap=False), "both": {}}
+
+
 @pytest.mark.parametrize("precond", ["jacobi", "local"])
-def test_confirmation_saves_an_fp64_residual(precond):
+def test_each_form_saves_an_fp64_residual(precond):
     w, rhs, Bo = case()
     S = mixed_system(w)
     S.set_preconditioner(precond)
     out = {}
-    for eta in (0.0, 1e-5):
-        say("confirm", precond, "eta", eta)
-        S.set_confirmation(eta)
+    for name, kw in FORMS.items():
+        S.set_confirmation(**kw)
         mu, it, norms = S.solve(rhs, thresh=1e-8, maxiter=60)
-        out[eta] = (mu, it, norms, S.last_solve_info())
-        say("iterations", it, out[eta][3], "last norm", norms[-1])
+        out[name] = (mu, it, norms, S.last_solve_info())
+        say("confirm", precond, name, "iterations", it, out[name][3], "last norm", norms[-1])
         assert np.linalg.norm(rhs - Bo.apply(mu)) < 1e-8
         assert len(norms) == it and norms[-1] < 1e-8
-    i0, i1 = out[0.0][3], out[1e-5][3]
-    assert i0["confirmations"] == 0
-    assert i1["confirmations"] >= 1 and i1["reliable_updates"] == i0["reliable_updates"] - 1
-    assert out[1e-5][1] == out[0.0][1]
-    assert np.array_equal(out[1e-5][0], out[0.0][0])  # the confirmation does not touch the moments
-    # the corrected residual agrees with the FP64 true residual to far below the threshold
-    assert abs(out[1e-5][2][-1] - out[0.0][2][-1]) < 1e-11
+    ref = out["fp64"]
+    assert ref[3]["confirmations"] == 0
+    assert out["gap"][3]["confirmations"] == 0
+    assert out["corrected"][3]["confirmations"] >= 1
+    for name in ("gap", "corrected", "both"):
+        mu, it, norms, info = out[name]
+        assert info["reliable_updates"] == ref[3]["reliable_updates"] - 1, name
+        assert it == ref[1] and np.array_equal(mu, ref[0]), name  # confirmations do not touch the moments
+        # the accepted norm (recursive or corrected) agrees with the FP64 true residual
+        assert abs(norms[-1] - ref[2][-1]) < 1e-10, name
     S.close()
 
 
@@ -70,7 +79,7 @@ def test_unmet_bound_falls_back_bitwise():
     S.set_confirmation(0.0)
     mu0, it0, n0 = S.solve(rhs, thresh=1e-8, maxiter=60)
     info0 = S.last_solve_info()
-    S.set_confirmation(1e6)  # eta ||B32 dx|| is far above the threshold: never accepted
+    S.set_confirmation(1e6)  # both bounds are far above the threshold: never accepted
     mu1, it1, n1 = S.solve(rhs, thresh=1e-8, maxiter=60)
     info1 = S.last_solve_info()
     say("unmet bound", info0, info1)
@@ -101,9 +110,8 @@ def test_confirmation_in_a_batch():
             assert np.linalg.norm(B.split(rhs)[g] - Bo.apply(mu[g])) < 1e-8, g
         res[eta] = (iters, info)
         B.close()
-    # a snapshot whose corrected residual leaves less than the error bound below the threshold
-    # is confirmed in FP64 after all, so the saving is not guaranteed here
-    assert res[1e-5][1]["confirmations"] >= 1
+    # a snapshot that neither bound accepts is confirmed in FP64 after all, so the saving is not
+    # guaranteed here
     assert res[1e-5][1]["reliable_updates"] <= res[0.0][1]["reliable_updates"]
     assert np.array_equal(res[1e-5][0], res[0.0][0])
 
@@ -121,7 +129,7 @@ def test_confirmation_several_right_hand_sides():
             assert np.linalg.norm(R[k] - Bo.apply(mu[k])) < 1e-8, k
         out[eta] = (mu, it, norms)
     assert np.array_equal(out[1e-5][0], out[0.0][0]) and np.array_equal(out[1e-5][1], out[0.0][1])
-    # a right-hand side confirmed from its corrected residual records that norm, not the FP64 one
+    # a right-hand side accepted by a bound records its recursive or corrected norm, not the FP64 one
     last = [(a[-1], b[-1]) for a, b in zip(out[1e-5][2], out[0.0][2])]
     assert any(a != b for a, b in last) and all(abs(a - b) < 1e-11 for a, b in last)
     S.close()
@@ -145,4 +153,6 @@ def test_confirmation_argument_rejected():
     for bad in (-1e-5, float("nan"), float("inf")):
         with pytest.raises(capi.CppeHipError):
             S.set_confirmation(bad)
+    with pytest.raises(capi.CppeHipError):
+        capi._check(capi.lib().cppe_hip_set_confirmation(S.h, 1e-5, 4))
     S.close()
