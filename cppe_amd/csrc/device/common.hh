@@ -57,5 +57,10 @@ __device__ __forceinline__ void thole_ET(R v, R e, R& fE, R& fT) {
 // far below the rounding of the forms that use them.
 constexpr double kTholeCutoff  = 40.0;
 constexpr float kTholeCutoff32 = 25.0f;
+// Near-list class boundary of the FP32 near field: sources with v >= 18 for every target of a leaf
+// take the undamped pair. A single pair's screening terms there reach 1e-5 relative, but they
+// are a small part of each site's field: on water boxes the field of the FP32 apply moves by
+// 8e-8 (2-norm, relative), below its own rounding error (2e-7).
+constexpr double kFarClass32 = 18.0;
 
 }  // namespace cppe_dev

@@ -940,8 +940,9 @@ struct alignas(K == 1 ? 4 * sizeof(R) : 16) StagedRec {  // K = 1: two 16-byte (
 // Geometry of one near-field pair of the staged kernel and of the exclusion pass (identical
 // arithmetic, so an excluded pair is taken out exactly as it was added). B2: the branch-free FP32
 // form for systems whose sites are all damped; di then carries the factor log2(e).
-// FAR: a source whose damping is below the precision's cutoff for every target of the leaf
-// (never the target itself): the undamped geometry, exactly what the damped forms give there.
+// FAR: a source past the precision's far class for every target of the leaf (never the target
+// itself): the undamped geometry (FP32: from v >= 18 on, where the screening it leaves out moves
+// the field by less than the FP32 rounding; FP64 takes no far stages).
 template <typename R, bool DAMP, bool B2, bool FAR = false>
 __device__ __forceinline__ void staged_geo(R dx, R dy, R dz, R a, R& c3, R& cf) {
   if constexpr (sizeof(R) == 8)

@@ -563,7 +563,7 @@ __global__ void k_xcount(int ncells, const int* __restrict__ off, const int* __r
 
 // Class of every entry of a leaf's expanded near list (near-list order), one wavefront per cell,
 // lanes = sources: the smallest Thole argument v = damping s_i s_j R_ij over the leaf's own sites i
-// (exact, FP64), then 0: v < 25, 1: 25 <= v < 40, 2: v >= 40 (a small margin above each cutoff).
+// (exact, FP64), then 0: v < 18, 1: 18 <= v < 40, 2: v >= 40 (a small margin above each bound).
 // Undamped sites (s = 0) and the leaf's own sites (the site itself is among the targets, R = 0)
 // give class 0 without the search, so one-leaf (direct-summation) systems cost O(n) here.
 __global__ __launch_bounds__(64) void k_xclass(const int* __restrict__ off, const int* __restrict__ src,
@@ -590,7 +590,7 @@ __global__ __launch_bounds__(64) void k_xclass(const int* __restrict__ off, cons
           m = fmin(m, (s[i] * s[i]) * (dx * dx + dy * dy + dz * dz));
         }
         const double v = damping * s[j] * sqrt(m);
-        xcls[q + u] = v < (double)kTholeCutoff32 * (1.0 + 1e-9) ? 0 : (v < kTholeCutoff * (1.0 + 1e-9) ? 1 : 2);
+        xcls[q + u] = v < kFarClass32 * (1.0 + 1e-9) ? 0 : (v < kTholeCutoff * (1.0 + 1e-9) ? 1 : 2);
       }
     }
     q += nb;

@@ -69,9 +69,9 @@ struct Octree {
   dbuf<long long> xoff;        // ncells+1 offsets
   dbuf<int> xsrc;              // source sites (device order)
   // Each leaf's expanded list is ordered by the smallest Thole argument v of each source over the
-  // leaf's targets: v_min < 25, then 25 <= v_min < 40, then v_min >= 40 (stable within each class).
-  // xsplit[1] (FP32) / xsplit[0] (FP64): offset of the first source whose damping is below that
-  // precision's cutoff for every target of the leaf.
+  // leaf's targets: v_min < 18, then 18 <= v_min < 40, then v_min >= 40 (stable within each class).
+  // xsplit[1] (FP32) / xsplit[0] (FP64): offset of the first source that takes the undamped pair
+  // in that precision for every target of the leaf (kFarClass32, kTholeCutoff).
   dbuf<long long> xsplit[2];
   double xdamp = 0.0;  // damping factor of the classification (valid for kernels with damping >= xdamp)
   SymLists sym[5];     // symmetric near field with chunks of 4, 6, 8, 12, 16 targets (sym_slot)
