@@ -123,6 +123,29 @@ __device__ __forceinline__ double lane_chunk_sum(const double* __restrict__ part
   return (a0 + a1) + (a2 + a3);
 }
 
+// lane_chunk_sum of two partial arrays at once (the same order for each; twice the loads in flight)
+__device__ __forceinline__ void lane_chunk_sum2(const double* __restrict__ pa, const double* __restrict__ pb, int lo,
+                                                int hi, int lane, double& ra, double& rb) {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
+  int c = lo + lane;
+  for (; c + 192 < hi; c += 256) {
+    a0 += pa[c];
+    a1 += pa[c + 64];
+    a2 += pa[c + 128];
+    a3 += pa[c + 192];
+    b0 += pb[c];
+    b1 += pb[c + 64];
+    b2 += pb[c + 128];
+    b3 += pb[c + 192];
+  }
+  for (; c < hi; c += 64) {
+    a0 += pa[c];
+    b0 += pb[c];
+  }
+  ra = (a0 + a1) + (a2 + a3);
+  rb = (b0 + b1) + (b2 + b3);
+}
+
 __device__ __forceinline__ void z_of(const double* __restrict__ a6, int n, int i, double r0, double r1, double r2,
                                      double& z0, double& z1, double& z2) {
   const double a0 = a6[i], a1 = a6[n + i], a2 = a6[2 * n + i], a3 = a6[3 * n + i], a4 = a6[4 * n + i],
@@ -141,7 +164,7 @@ struct CgVecs {
   const double* rhs;
   double *x, *r, *z, *p, *Ap, *t;
   double *xrel, *rrel, *dv;  // mixed form: moments and residual at the last FP64 true residual, correction
-  double* part;     // 2 * nchunk partials
+  double* part;     // 4 * nchunk partials
   double* sd;       // D_COUNT * nseg
   int* si;          // I_COUNT * nseg
   const int* status;  // ST_COUNT ints
@@ -238,13 +261,26 @@ __global__ __launch_bounds__(256) void k_sg_pap(CgVecs v) {
 // partials r.r (part[c]) and r.z (part[nchunk + c]), each summed as in the separate kernels. The
 // z of a segment that converges or asks for a reliable update here is not used (the direction
 // after a reliable update recomputes it).
-__global__ __launch_bounds__(256) void k_sg_xrz(CgVecs v) {
+__global__ __launch_bounds__(kChunk) void k_sg_xrz(CgVecs v) {
   __shared__ double red[4];
   __shared__ double alpha;
   if (step_halted(v.status)) return;
   const int c = blockIdx.x, n = v.na, g = v.cseg[c];
   const ChunkIdx ci(v, c);
   const bool active = v.si[I_ACTIVE * v.nseg + g];
+  // one site per thread (chunks hold at most kChunk sites): its vectors are loaded before alpha
+  // is known, so the loads overlap the sum below
+  const int i     = v.clo[c] + threadIdx.x;
+  const bool site = active && i < v.clo[c + 1];
+  double xq[3], pq[3], rq[3], aq[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const size_t k = ci.b + (size_t)q * n + (site ? i : v.clo[c]);
+    xq[q] = site ? v.x[k] : 0.0;
+    pq[q] = site ? v.p[k] : 0.0;
+    rq[q] = site ? v.r[k] : 0.0;
+    aq[q] = site ? v.Ap[k] : 0.0;
+  }
   // alpha = (r.z) / (p.Ap) of the segment, from k_sg_pap's partials: every block sums them in one
   // fixed order (lane_chunk_sum, then an xor-shuffle tree), so all blocks of a segment take the
   // same alpha, without a launch of its own
@@ -256,30 +292,26 @@ __global__ __launch_bounds__(256) void k_sg_xrz(CgVecs v) {
   }
   __syncthreads();
   double srr = 0.0, srz = 0.0;
-  if (active) {
+  if (site) {
     const double al = alpha;
-    for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x) {
-      double rq[3];
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const size_t k = ci.b + (size_t)q * n + i;
-        v.x[k] += al * v.p[k];
-        const double rk = v.r[k] - al * v.Ap[k];
-        v.r[k] = rk;
-        srr += rk * rk;
-        rq[q] = rk;
-      }
-      double z0, z1, z2;
-      z_of(v.a6, n, i + ci.ia, rq[0], rq[1], rq[2], z0, z1, z2);
-      const size_t k0 = ci.b + i;
-      if (v.pc_off) {  // local preconditioner: w, then k_sg_prec
-        v.w[i] = make_double4(z0, z1, z2, 0.0);
-      } else {
-        v.z[k0]                 = z0;
-        v.z[k0 + n]             = z1;
-        v.z[k0 + 2 * (size_t)n] = z2;
-        srz += rq[0] * z0 + rq[1] * z1 + rq[2] * z2;
-      }
+    for (int q = 0; q < 3; ++q) {
+      const size_t k = ci.b + (size_t)q * n + i;
+      v.x[k] = xq[q] + al * pq[q];
+      rq[q]  = rq[q] - al * aq[q];
+      v.r[k] = rq[q];
+      srr += rq[q] * rq[q];
+    }
+    double z0, z1, z2;
+    z_of(v.a6, n, i + ci.ia, rq[0], rq[1], rq[2], z0, z1, z2);
+    const size_t k0 = ci.b + i;
+    if (v.pc_off) {  // local preconditioner: w, then k_sg_prec
+      v.w[i] = make_double4(z0, z1, z2, 0.0);
+    } else {
+      v.z[k0]                 = z0;
+      v.z[k0 + n]             = z1;
+      v.z[k0 + 2 * (size_t)n] = z2;
+      srz += rq[0] * z0 + rq[1] * z1 + rq[2] * z2;
     }
   }
   const double s1 = block_sum(srr, red);
@@ -389,55 +421,44 @@ __global__ __launch_bounds__(256) void k_sg_z(CgVecs v) {
 // the chunk partial r.z into part[slot * nchunk + c]. MODE 0 (after k_sg_init, every segment):
 // also p = z and p's near-field records. MODE 1 (after k_sg_xrz) and MODE 2 (after k_sg_z):
 // active segments, MODE 1 not in a halted step, MODE 2 not while a reliable update is pending.
-// kPcLanes lanes per site take consecutive entries of the site's pair row (coalesced index and
-// block loads, one 32-byte record gather per pair); the lane sums are combined by a fixed xor tree.
-constexpr int kPcLanes = 8;
+// One thread per site walks its pair row in order (a handful of pairs: four in flight at a
+// time), so every site of the chunk gathers at once; the sum over a row has a fixed order.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_sg_prec(CgVecs v, int slot) {
   __shared__ double red[4];
   if ((MODE == 1 && step_halted(v.status)) || (MODE == 2 && v.status[ST_REL] > 0)) return;
   const int c = blockIdx.x, n = v.na;
   const ChunkIdx ci(v, c);
-  const int g = threadIdx.x % kPcLanes;
   double s = 0.0;
   if (MODE == 0 || v.si[I_ACTIVE * v.nseg + v.cseg[c]])
-    for (int base = v.clo[c]; base < v.clo[c + 1]; base += 256 / kPcLanes) {
-      const int i     = base + threadIdx.x / kPcLanes;
-      const bool site = i < v.clo[c + 1];
-      const int ia    = site ? i + ci.ia : 0;  // site within its right-hand side
+    for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x) {
+      const int ia = i + ci.ia;  // site within its right-hand side
       double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-      if (site)
-        for (int q = v.pc_off[ia] + g; q < v.pc_off[ia + 1]; q += kPcLanes) {
-          const double4 wj = v.w[v.pc_nj[q] - ci.ia];
-          const float2* t2 = reinterpret_cast<const float2*>(v.pc_blk + 6 * (size_t)q);
-          const float2 ta = t2[0], tb = t2[1], tc = t2[2];  // xx xy | xz yy | yz zz
-          y0 += (double)ta.x * wj.x + (double)ta.y * wj.y + (double)tb.x * wj.z;
-          y1 += (double)ta.y * wj.x + (double)tb.y * wj.y + (double)tc.x * wj.z;
-          y2 += (double)tb.x * wj.x + (double)tc.x * wj.y + (double)tc.y * wj.z;
-        }
-#pragma unroll
-      for (int off = kPcLanes / 2; off > 0; off >>= 1) {
-        y0 += __shfl_xor(y0, off);
-        y1 += __shfl_xor(y1, off);
-        y2 += __shfl_xor(y2, off);
+      const int q0 = v.pc_off[ia], q1 = v.pc_off[ia + 1];
+#pragma unroll 4
+      for (int q = q0; q < q1; ++q) {
+        const double4 wj = v.w[v.pc_nj[q] - ci.ia];
+        const float2* t2 = reinterpret_cast<const float2*>(v.pc_blk + 6 * (size_t)q);
+        const float2 ta = t2[0], tb = t2[1], tc = t2[2];  // xx xy | xz yy | yz zz
+        y0 += (double)ta.x * wj.x + (double)ta.y * wj.y + (double)tb.x * wj.z;
+        y1 += (double)ta.y * wj.x + (double)tb.y * wj.y + (double)tc.x * wj.z;
+        y2 += (double)tb.x * wj.x + (double)tc.x * wj.y + (double)tc.y * wj.z;
       }
-      if (site && g == 0) {
-        double a0, a1, a2;
-        z_of(v.a6, n, ia, y0, y1, y2, a0, a1, a2);
-        const double4 wi = v.w[i];
-        const size_t k0 = ci.b + i, k1 = k0 + n, k2 = k1 + n;
-        const double z0 = wi.x + v.pc_weight * a0, z1 = wi.y + v.pc_weight * a1, z2 = wi.z + v.pc_weight * a2;
-        v.z[k0] = z0;
-        v.z[k1] = z1;
-        v.z[k2] = z2;
-        if (MODE == 0) {
-          v.p[k0] = z0;
-          v.p[k1] = z1;
-          v.p[k2] = z2;
-          put_record(v, ia, z0, z1, z2);
-        }
-        s += v.r[k0] * z0 + v.r[k1] * z1 + v.r[k2] * z2;
+      double a0, a1, a2;
+      z_of(v.a6, n, ia, y0, y1, y2, a0, a1, a2);
+      const double4 wi = v.w[i];
+      const size_t k0 = ci.b + i, k1 = k0 + n, k2 = k1 + n;
+      const double z0 = wi.x + v.pc_weight * a0, z1 = wi.y + v.pc_weight * a1, z2 = wi.z + v.pc_weight * a2;
+      v.z[k0] = z0;
+      v.z[k1] = z1;
+      v.z[k2] = z2;
+      if (MODE == 0) {
+        v.p[k0] = z0;
+        v.p[k1] = z1;
+        v.p[k2] = z2;
+        put_record(v, ia, z0, z1, z2);
       }
+      s += v.r[k0] * z0 + v.r[k1] * z1 + v.r[k2] * z2;
     }
   s = block_sum(s, red);
   if (threadIdx.x == 0) v.part[(size_t)slot * v.nchunk + c] = s;
@@ -474,6 +495,7 @@ struct CgFinal {
   const double* params;  // P_COUNT values
   double* hist;          // residual norms, iteration-major [it * nseg + g]
   int hist_cap;          // iterations held by hist
+  int publish;           // copy the status block to the host mirror (0: steps inside a launch but its last)
 };
 
 // Per-segment finalisation of the chunk partials: one wavefront per segment (the block's four
@@ -502,15 +524,17 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int na = 0, nr = 0, nb = 0, np = 0, nst = 0, h_rel = kFarSteps, h_done = 0;
   for (int g = wv; g < ns; g += 4) {
-    double s1 = lane_chunk_sum(f.part, f.seg_chunk[g], f.seg_chunk[g + 1], lane), s2 = 0.0, s3 = 0.0, s4 = 0.0;
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
     constexpr bool TWO = MODE == F_INIT || MODE == F_RNB || MODE == F_CONF;
-    // mixed steps with the gap bound: p.Ap (summed as k_sg_xrz sums it for alpha) and Ap.Ap
+    // mixed steps with the gap bound also need p.Ap and Ap.Ap of the step (k_sg_pap's partials)
     const bool gap = MIXED && MODE == F_RNB && pr.eta_gap > 0.0;
-    if (TWO) s2 = lane_chunk_sum(f.part + f.nchunk, f.seg_chunk[g], f.seg_chunk[g + 1], lane);
-    if (gap) {
-      s3 = lane_chunk_sum(f.part + 2 * (size_t)f.nchunk, f.seg_chunk[g], f.seg_chunk[g + 1], lane);
-      s4 = lane_chunk_sum(f.part + 3 * (size_t)f.nchunk, f.seg_chunk[g], f.seg_chunk[g + 1], lane);
-    }
+    if (TWO)
+      lane_chunk_sum2(f.part, f.part + f.nchunk, f.seg_chunk[g], f.seg_chunk[g + 1], lane, s1, s2);
+    else
+      s1 = lane_chunk_sum(f.part, f.seg_chunk[g], f.seg_chunk[g + 1], lane);
+    if (gap)
+      lane_chunk_sum2(f.part + 2 * (size_t)f.nchunk, f.part + 3 * (size_t)f.nchunk, f.seg_chunk[g],
+                      f.seg_chunk[g + 1], lane, s3, s4);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       s1 += __shfl_xor(s1, off);
@@ -521,6 +545,7 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
       }
     }
     if (lane != 0) continue;
+
     const SegRef sg{f.sd, f.si, ns, g};
     if (MODE == F_INIT) {
       if (restart)
@@ -585,9 +610,11 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
   if (threadIdx.x == 0) {  // publish: the fields, a system-scope fence, then the count
     const int seq    = f.status[ST_SEQ] + 1;
     f.status[ST_SEQ] = seq;
-    for (int q = 0; q < ST_SEQ; ++q) f.hstat[q] = f.status[q];
-    __threadfence_system();
-    f.hstat[ST_SEQ] = seq;
+    if (f.publish) {
+      for (int q = 0; q < ST_SEQ; ++q) f.hstat[q] = f.status[q];
+      __threadfence_system();
+      f.hstat[ST_SEQ] = seq;
+    }
   }
 }
 
@@ -635,6 +662,7 @@ static void cg_layout(System& S, CgState& C, const std::vector<int>& seg_off) {
   up(C.chunk_seg, cseg);
   up(C.seg_chunk, schunk);
   C.part.alloc(4 * (size_t)C.nchunk);  // [r.r or p.Ap | r.z | p.Ap of the step | Ap.Ap of the step]
+
   C.sd.alloc(D_COUNT * (size_t)ns);
   C.si.alloc(I_COUNT * (size_t)ns);
   C.status.alloc(ST_COUNT);
@@ -761,15 +789,18 @@ static CgFinal finals(CgState& C) {
   f.params    = C.params.p;
   f.hist      = C.hist.p;
   f.hist_cap  = C.hist_cap;
+  f.publish   = 1;
   return f;
 }
 
 template <int MODE>
-static void launch_final(System& S, CgState& C, bool mixed) {
+static void launch_final(System& S, CgState& C, bool mixed, bool publish = true) {
+  CgFinal f  = finals(C);
+  f.publish  = publish ? 1 : 0;
   if (mixed)
-    hipLaunchKernelGGL((k_sg_final<MODE, true>), dim3(1), dim3(256), 0, S.stream, finals(C));
+    hipLaunchKernelGGL((k_sg_final<MODE, true>), dim3(1), dim3(256), 0, S.stream, f);
   else
-    hipLaunchKernelGGL((k_sg_final<MODE, false>), dim3(1), dim3(256), 0, S.stream, finals(C));
+    hipLaunchKernelGGL((k_sg_final<MODE, false>), dim3(1), dim3(256), 0, S.stream, f);
 }
 
 static void apply(System& S, CgState& C, const double* x, double* out, bool fp32) {
@@ -805,7 +836,7 @@ static void seq_direction(System& S, CgState& C, bool mixed) {
 // One iteration: the step (apply, p.Ap, alpha, x / r / z update, residual norm) and -- unless a
 // segment asked for a reliable update, which the final kernel and k_sg_p test on the device --
 // beta and the new direction, so that one host round trip per iteration suffices.
-static void seq_step_direction(System& S, CgState& C, bool mixed) {
+static void seq_step_direction(System& S, CgState& C, bool mixed, bool publish) {
   const dim3 g(C.nchunk), b(256);
   if (C.rec32 || C.rec64) S.prepacked = C.p;  // records written by k_sg_init / k_sg_p
   apply(S, C, C.p, C.Ap, mixed);
@@ -813,7 +844,7 @@ static void seq_step_direction(System& S, CgState& C, bool mixed) {
   hipLaunchKernelGGL(k_sg_pap, g, b, 0, S.stream, vecs(S, C));
   hipLaunchKernelGGL(k_sg_xrz, g, b, 0, S.stream, vecs(S, C));
   launch_prec<1>(S, C, 1);
-  launch_final<F_RNB>(S, C, mixed);
+  launch_final<F_RNB>(S, C, mixed, publish);
   hipLaunchKernelGGL(k_sg_p, g, b, 0, S.stream, vecs(S, C));
 }
 
@@ -968,7 +999,8 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
     while (l < 3 && (2 << l) <= lim) ++l;
     const int cnt = 1 << l;
     run_graph(S, C.g_step[m][l], [&] {
-      for (int q = 0; q < cnt; ++q) seq_step_direction(S, C, mixed);
+      // only the last step's final kernel publishes: the host waits for the launch as a whole
+      for (int q = 0; q < cnt; ++q) seq_step_direction(S, C, mixed, q == cnt - 1);
     });
     C.expect += cnt;
     read_status(S, C, cnt);
