@@ -41,9 +41,10 @@ FLOP_M2L_M1_P5 = 950
 # precision and launch shape (near variant, block) the autotuner picks there, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE runs calibrated on a copy kernel (profiles/near_field_300k_hbm_traffic.md);
 # other shapes: not measured (null)
-MEASURED_TRAFFIC_300K = {("fp32", 8, 256): 941.3e6, ("fp64", 8, 256): 1542.7e6,
-                         ("fp32", 8, 128): 1184.0e6, ("fp64", 8, 128): 1381.2e6,
-                         ("fp32", 7, 128): 1292.6e6, ("fp64", 7, 128): 1465.6e6}
+MEASURED_TRAFFIC_300K = {("fp32", 8, 256): 871.1e6, ("fp64", 8, 256): 1472.4e6,
+                         ("fp32", 8, 128): 1095.0e6, ("fp64", 8, 128): 1345.0e6,
+                         ("fp32", 7, 128): 1215.5e6, ("fp64", 7, 128): 1417.4e6,
+                         ("fp32", 7, 256): 1027.2e6, ("fp64", 7, 256): 1648.3e6}
 
 WORKLOADS = {
     "300k": dict(n_waters=100000, seed=1003, anisotropic=True),
