@@ -323,7 +323,7 @@ class System:
         _check(lib().cppe_hip_set_graphs(self.h, int(bool(enable))))
 
     def set_graph_steps(self, steps=8):
-        """Most CG steps launched as one sequence between host status reads: 1, 2, 4 or 8
+        """Most CG steps launched as one sequence between host status reads: 1 to 8
         (cppe_hip_set_graph_steps)."""
         _check(lib().cppe_hip_set_graph_steps(self.h, int(steps)))
 
@@ -560,7 +560,12 @@ def ml_features(system):
     return X, V
 
 
-def ml_guess(system, mlp):
+def ml_guess(system, mlp, timed=True):
+    """Learned guess into the system's resident moments; timed: wait and return its device ms,
+    otherwise only enqueue it ahead of the next solve (returns None)."""
+    if not timed:
+        _check(lib().cppe_hip_ml_guess(system.h, mlp.h, None))
+        return None
     ms = ctypes.c_double(0.0)
     _check(lib().cppe_hip_ml_guess(system.h, mlp.h, ctypes.byref(ms)))
     return ms.value

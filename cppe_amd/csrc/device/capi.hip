@@ -762,8 +762,8 @@ int cppe_hip_set_graphs(cppe_hip_system* sys, int enable) {
 
 int cppe_hip_set_graph_steps(cppe_hip_system* sys, int steps) {
   CPPE_API_BEGIN
-  if (!sys || !(steps == 1 || steps == 2 || steps == 4 || steps == 8))
-    return fail(CPPE_HIP_ERR_ARG, "steps per launch must be 1, 2, 4 or 8");
+  if (!sys || steps < 1 || steps > CgState::kMaxGraphSteps)
+    return fail(CPPE_HIP_ERR_ARG, "steps per launch must be 1 to 8");
   sys->S.graph_steps = steps;
   return CPPE_HIP_OK;
   CPPE_API_END

@@ -402,25 +402,29 @@ int cppe_hip_ml_guess(cppe_hip_system* sys, cppe_hip_mlp* m, double* device_ms) 
   if (!S.has_alpha) return capi_fail(CPPE_HIP_ERR_ARG, "system has no polarizabilities");
   hipStream_t st = S.stream;
   const int n = S.n;
-  hipEvent_t e0, e1;
-  CPPE_HIP_CHECK(hipEventCreate(&e0));
-  CPPE_HIP_CHECK(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (device_ms) {
+    CPPE_HIP_CHECK(hipEventCreate(&e0));
+    CPPE_HIP_CHECK(hipEventCreate(&e1));
+  }
   MlGrid& G = ml_grid(S);  // neighbour grid and work buffers, kept by the system
   G.X.alloc((size_t)n * kMlpIn);
   G.y.alloc((size_t)n * 3);
   G.V.alloc(9 * (size_t)n);
-  CPPE_HIP_CHECK(hipEventRecord(e0, st));
+  if (device_ms) CPPE_HIP_CHECK(hipEventRecord(e0, st));
   ml_features(S, G.X.p, G.V.p);
   CPPE_HIP_CHECK(hipStreamSynchronize(m->M.stream));
   mlp_forward(m->M, G.X.p, n, G.y.p, st);
   hipLaunchKernelGGL(k_ml_combine, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, G.y.p, G.V.p, S.v_mu.p);
-  CPPE_HIP_CHECK(hipEventRecord(e1, st));
-  CPPE_HIP_CHECK(hipEventSynchronize(e1));
-  float ms = 0.f;
-  CPPE_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (device_ms) *device_ms = ms;
+  if (device_ms) {  // timed: wait for the guess; otherwise it stays queued before what follows on the stream
+    CPPE_HIP_CHECK(hipEventRecord(e1, st));
+    CPPE_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CPPE_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *device_ms = ms;
+  }
   S.mu_valid = true;
   return CPPE_HIP_OK;
   ML_END

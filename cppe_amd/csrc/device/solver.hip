@@ -26,7 +26,7 @@
 // launch next. Several steps run as one graph: each final kernel also publishes a horizon, the
 // number of steps every segment can take before one of them needs the host (a replacement, or
 // no segment left active), predicted from the segments' residual reduction rates (cg_policy.hh);
-// the host launches a graph of up to that many steps (1, 2, 4 or 8), and a step that starts after
+// the host launches a graph of up to that many steps (1 to 8), and a step that starts after
 // such an event leaves every vector and scalar as it is (its apply result is never read).
 #include <algorithm>
 #include <atomic>
@@ -991,14 +991,11 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
     }
     reliable();
   };
-  // 2^l steps as one launch, 2^l at most the device's horizon, the iterations left and the
-  // system's limit; k = steps taken (steps that start after an event take none)
+  // as many steps as one launch as the device's horizon, the iterations left and the system's
+  // limit allow (a graph per count); k = steps taken (steps that start after an event take none)
   auto steps = [&] {
-    const int lim = std::min({C.h_status[ST_SAFE], maxiter - k, S.graph_steps});
-    int l = 0;
-    while (l < 3 && (2 << l) <= lim) ++l;
-    const int cnt = 1 << l;
-    run_graph(S, C.g_step[m][l], [&] {
+    const int cnt = std::max(1, std::min({C.h_status[ST_SAFE], maxiter - k, S.graph_steps}));
+    run_graph(S, C.g_step[m][cnt], [&] {
       // only the last step's final kernel publishes: the host waits for the launch as a whole
       for (int q = 0; q < cnt; ++q) seq_step_direction(S, C, mixed, q == cnt - 1);
     });

@@ -100,7 +100,7 @@ struct CgResult {
   int reliable_updates = 0;    // FP64 true-residual applies of the mixed form
   int confirmations    = 0;    // FP32-corrected confirmations of the mixed form (one FP32 apply each)
   int fp64_fallback = -1;      // mixed form: iteration at which it continued in FP64 (-1: never)
-  int step_launches = 0;       // launches of step sequences (each of 1, 2, 4 or 8 steps)
+  int step_launches = 0;       // launches of step sequences (each of 1 to 8 steps)
   int steps_launched = 0;      // steps in them (those that start after an event take none)
   int host_reads = 0;          // status reads of the host loop (one per launch it waits for)
   bool all_converged = false;
@@ -121,8 +121,9 @@ struct CgState {
   int* h_status = nullptr;  // host-mapped mirror of the status block (written by the final kernels)
   int* d_hstat  = nullptr;  // its device address
   int expect    = 0;        // final kernels launched so far: the mirror's count once they have run
-  // g_step[form][l]: 2^l consecutive steps (form 0 FP64, 1 mixed)
-  hipGraphExec_t g_step[2][4] = {}, g_rel = nullptr, g_conf = nullptr, g_dir[2] = {nullptr, nullptr};
+  // g_step[form][c]: c consecutive steps, 1 <= c <= kMaxGraphSteps (form 0 FP64, 1 mixed)
+  static constexpr int kMaxGraphSteps = 8;
+  hipGraphExec_t g_step[2][kMaxGraphSteps + 1] = {}, g_rel = nullptr, g_conf = nullptr, g_dir[2] = {nullptr, nullptr};
   double *x = nullptr, *rhs = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr, *t = nullptr;
   float4* rec32   = nullptr;  // near-field dipole records of p written by the direction kernels
   double4* rec64 = nullptr;   // (one of them, in the precision of the step's apply; none: nrhs > 1)
@@ -177,7 +178,7 @@ struct System {
   // cgm for solves of several right-hand sides at once (built on first use)
   CgState cg, cgm;
   bool use_graphs = true;
-  int graph_steps = 8;  // most CG steps launched as one sequence (1, 2, 4 or 8)
+  int graph_steps = 8;  // most CG steps launched as one sequence (1 to 8)
   // CG preconditioner: 0 = alpha r (the reference's Jacobi form), 1 = local first-order Neumann
   // alpha r + weight alpha T_loc alpha r over pairs closer than precond_rc (precond.hip)
   int precond = 0;

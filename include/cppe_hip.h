@@ -174,10 +174,10 @@ int cppe_hip_segment_results(cppe_hip_system* sys, int* iters, double* rnorm, in
 /* CG iterations replayed as captured HIP graphs (default 1) or launched kernel by kernel (0). */
 int cppe_hip_set_graphs(cppe_hip_system* sys, int enable);
 /*
- * Most CG steps launched as one sequence without the host in between (1, 2, 4 or 8; default 8).
+ * Most CG steps launched as one sequence without the host in between (1 to 8; default 8).
  * After every decision the device publishes how many steps each snapshot can take before one of
  * them needs the host (an FP64 true residual, or none left active), predicted from their residual
- * reduction rates; the host launches that many steps (rounded down to 1, 2, 4 or 8) and reads the
+ * reduction rates; the host launches that many steps (at most the limit) and reads the
  * status once. A step that starts after such an event changes nothing, so results do not depend
  * on this setting. No counterpart in the reference (a host loop, electric_fields.cc:213-234).
  */
@@ -382,7 +382,8 @@ int cppe_hip_mlp_samples(cppe_hip_mlp* m, long long s0, long long count, float* 
  * losses receives the mean training loss of each epoch. */
 int cppe_hip_mlp_train(cppe_hip_mlp* m, int epochs, int batch, float lr, unsigned seed, double* losses);
 /* Learned guess into the system's resident moments (use guess_mode 1 of cppe_hip_solve_resident);
- * *device_ms receives the time of features + network + guess. */
+ * *device_ms receives the time of features + network + guess, waiting for them. device_ms NULL:
+ * the guess is only enqueued on the system's stream, ahead of the solve that follows. */
 int cppe_hip_ml_guess(cppe_hip_system* sys, cppe_hip_mlp* m, double* device_ms);
 
 #ifdef __cplusplus
