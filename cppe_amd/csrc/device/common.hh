@@ -63,4 +63,17 @@ constexpr float kTholeCutoff32 = 25.0f;
 // 8e-8 (2-norm, relative), below its own rounding error (2e-7).
 constexpr double kFarClass32 = 18.0;
 
+// block-wide sum (256 threads); the result is valid in thread 0
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[k];
+  return s;
+}
+
 }  // namespace cppe_dev

@@ -375,20 +375,52 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void
   }
 }
 
+// Downward pass of one cell: its M2L result unfolded (the coefficients M2L left out restored, as
+// k_fmm_unfold does) and, below the root, its parent's complete expansion translated into it.
+template <int KIND, int P, typename R>
+__device__ __forceinline__ void down_cell(int c, bool child, const int* __restrict__ parent,
+                                          const double* __restrict__ cx, const double* __restrict__ cy,
+                                          const double* __restrict__ cz, R* __restrict__ L) {
+  using O = Ops<KIND, P, R>;
+  R l[O::LS];
+#pragma unroll
+  for (int k = 0; k < O::LS; ++k) l[k] = L[(size_t)c * O::LS + k];
+  O::unfold(l);
+  if (child) {
+    const int pa = parent[c];
+    O::l2l((R)(cx[c] - cx[pa]), (R)(cy[c] - cy[pa]), (R)(cz[c] - cz[pa]), &L[(size_t)pa * O::LS], l);
+  }
+#pragma unroll
+  for (int k = 0; k < O::LS; ++k) L[(size_t)c * O::LS + k] = l[k];
+}
+
+// Cells [c0, c1) of one level below the top levels (their parents complete).
 template <int KIND, int P, typename R>
 __global__ __launch_bounds__(64) void k_fmm_l2l(int c0, int c1, const int* __restrict__ parent,
                                                 const double* __restrict__ cx, const double* __restrict__ cy,
                                                 const double* __restrict__ cz, R* __restrict__ L) {
-  using O     = Ops<KIND, P, R>;
   const int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= c1) return;
-  const int pa = parent[c];
-  R l[O::LS];
-#pragma unroll
-  for (int k = 0; k < O::LS; ++k) l[k] = L[(size_t)c * O::LS + k];
-  O::l2l((R)(cx[c] - cx[pa]), (R)(cy[c] - cy[pa]), (R)(cz[c] - cz[pa]), &L[(size_t)pa * O::LS], l);
-#pragma unroll
-  for (int k = 0; k < O::LS; ++k) L[(size_t)c * O::LS + k] = l[k];
+  down_cell<KIND, P, R>(c, true, parent, cx, cy, cz, L);
+}
+
+// The top levels of the downward pass (root first, few cells each) in one workgroup: level by
+// level with a barrier between, so one launch replaces a launch per level.
+constexpr int kTopLevels = 8, kTopCells = 1024;
+struct TopLevels {
+  int nlev;                    // levels 0 .. nlev - 1
+  int off[kTopLevels + 1];     // first cell of each level, then the end
+};
+
+template <int KIND, int P, typename R>
+__global__ __launch_bounds__(256) void k_fmm_down_top(TopLevels t, const int* __restrict__ parent,
+                                                      const double* __restrict__ cx, const double* __restrict__ cy,
+                                                      const double* __restrict__ cz, R* __restrict__ L) {
+  for (int lev = 0; lev < t.nlev; ++lev) {
+    for (int c = t.off[lev] + threadIdx.x; c < t.off[lev + 1]; c += blockDim.x)
+      down_cell<KIND, P, R>(c, lev > 0, parent, cx, cy, cz, L);
+    __syncthreads();
+  }
 }
 
 struct P2PArgs {
@@ -1229,24 +1261,63 @@ __global__ __launch_bounds__(256) void k_fmm_excl_sub(NearArgs a, const MU4* __r
 
 // out += L2P of each site's leaf expansion (the far-field term when it runs beside the near field)
 // (grid.y = right-hand side: L + y lstride, out + 3 y n)
+// With d.p set (one right-hand side): each block also writes the partials p.out and out.out of
+// its 256 sites -- the CG step's p.Ap and Ap.Ap, summed as the CG's own chunk kernel sums them.
+struct L2pDots {
+  const double* p = nullptr;
+  double* part    = nullptr;
+  int nchunk      = 0;
+};
+
 template <int P, typename R>
 __global__ __launch_bounds__(256) void k_fmm_l2p_add(int n, const int* __restrict__ leaf_of,
                                                      const double4* __restrict__ pos4, const double* __restrict__ cx,
                                                      const double* __restrict__ cy, const double* __restrict__ cz,
                                                      const R* __restrict__ L, double* __restrict__ out,
-                                                     size_t lstride = 0) {
+                                                     size_t lstride = 0, L2pDots d = {}) {
   using O     = Ops<KIND_M1, P, R>;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
   L += blockIdx.y * lstride;
   out += 3 * (size_t)n * blockIdx.y;
-  const int A      = leaf_of[i];
-  const double4 pi = pos4[i];
-  R lx = R(0), ly = R(0), lz = R(0);
-  O::l2p((R)(pi.x - cx[A]), (R)(pi.y - cy[A]), (R)(pi.z - cz[A]), &L[(size_t)A * O::LS], lx, ly, lz);
-  out[i] += (double)lx;
-  out[n + i] += (double)ly;
-  out[2 * (size_t)n + i] += (double)lz;
+  double s = 0.0, q = 0.0;
+  if (i < n) {
+    const int A      = leaf_of[i];
+    const double4 pi = pos4[i];
+    R lx = R(0), ly = R(0), lz = R(0);
+    O::l2p((R)(pi.x - cx[A]), (R)(pi.y - cy[A]), (R)(pi.z - cz[A]), &L[(size_t)A * O::LS], lx, ly, lz);
+    const size_t k1 = n + (size_t)i, k2 = k1 + n;
+    const double a0 = out[i] + (double)lx, a1 = out[k1] + (double)ly, a2 = out[k2] + (double)lz;
+    out[i]  = a0;
+    out[k1] = a1;
+    out[k2] = a2;
+    if (d.p) {
+      s = d.p[i] * a0 + d.p[k1] * a1 + d.p[k2] * a2;
+      q = a0 * a0 + a1 * a1 + a2 * a2;
+    }
+  }
+  if (d.p) {
+    __shared__ double red[4];
+    s = block_sum(s, red);
+    q = block_sum(q, red);
+    if (threadIdx.x == 0) {
+      d.part[2 * (size_t)d.nchunk + blockIdx.x] = s;
+      d.part[3 * (size_t)d.nchunk + blockIdx.x] = q;
+    }
+  }
+}
+
+// The L2P pass of a single right-hand side's apply into out, with the CG step's partials when
+// the step armed them for this output (System::dots).
+template <int P, typename R>
+void launch_l2p_add(System& S, const R* L, double* out, hipStream_t st) {
+  const Octree& T = S.tree;
+  L2pDots d;
+  if (S.dots.p && S.dots.ap == out && S.dots.nchunk == ceil_div(S.n, 256)) {
+    d = L2pDots{S.dots.p, S.dots.part, S.dots.nchunk};
+    S.dots.done = true;
+  }
+  hipLaunchKernelGGL((k_fmm_l2p_add<P, R>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, S.n, T.leaf_of.p,
+                     S.fmm->pos4.p, T.cx.p, T.cy.p, T.cz.p, L, out, (size_t)0, d);
 }
 
 #include "near_sym.hh"
@@ -1348,10 +1419,24 @@ void fmm_down_pass(System& S, R* M, R* L, hipStream_t st) {
     hipLaunchKernelGGL((k_fmm_m2l<KIND, P, R, 64>), dim3(T.ncells), dim3(64), 0, st, T.ncells, T.m2l_off.p,
                        T.m2l_src.p, T.cx.p, T.cy.p, T.cz.p, Mf, L, rec);
   }
-  hipLaunchKernelGGL((k_fmm_unfold<KIND, P, R>), dim3(ceil_div(T.ncells, 64)), dim3(64), 0, st, T.ncells, L);
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[2], st));
-  for (int lev = 1; lev <= T.max_level; ++lev) {
+  // unfold + L2L: the top levels (at most kTopCells cells each) in one workgroup, then one launch
+  // per level
+  TopLevels top{0, {}};
+  top.off[0] = lo[0];
+  while (top.nlev < kTopLevels && top.nlev <= T.max_level && lo[top.nlev + 1] - lo[top.nlev] <= kTopCells) {
+    top.off[top.nlev + 1] = lo[top.nlev + 1];
+    ++top.nlev;
+  }
+  if (top.nlev > 0)
+    hipLaunchKernelGGL((k_fmm_down_top<KIND, P, R>), dim3(1), dim3(256), 0, st, top, T.parent.p, T.cx.p, T.cy.p,
+                       T.cz.p, L);
+  for (int lev = top.nlev; lev <= T.max_level; ++lev) {
     const int c0 = lo[lev], c1 = lo[lev + 1];
+    if (lev == 0) {  // a root level too large for the single workgroup does not occur (one cell)
+      hipLaunchKernelGGL((k_fmm_unfold<KIND, P, R>), dim3(ceil_div(c1 - c0, 64)), dim3(64), 0, st, c1, L);
+      continue;
+    }
     hipLaunchKernelGGL((k_fmm_l2l<KIND, P, R>), dim3(ceil_div(c1 - c0, 64)), dim3(64), 0, st, c0, c1,
                        T.parent.p, T.cx.p, T.cy.p, T.cz.p, L);
   }
@@ -1605,9 +1690,7 @@ void launch_direct_sym(System& S, const NearArgs& a, const MU4* mu4, const R* L,
       hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, false, 1>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a,
                          mu4);
   }
-  if (with_l2p)
-    hipLaunchKernelGGL((k_fmm_l2p_add<P, R>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, S.n, T.leaf_of.p,
-                       F.pos4.p, T.cx.p, T.cy.p, T.cz.p, L, a.out);
+  if (with_l2p) launch_l2p_add<P, R>(S, L, a.out, st);
 }
 
 // Symmetric near field of a B apply (variants 9-11, TS targets per chunk): the pair kernel, the
@@ -1653,9 +1736,7 @@ void launch_near_sym(System& S, const NearArgs& a, const MU4* mu4, const R* L, b
                          mu4);
   }
   launch_sym_gather<P, R>(S, Y, nullptr, a.out);
-  if (with_l2p)
-    hipLaunchKernelGGL((k_fmm_l2p_add<P, R>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, S.n, T.leaf_of.p,
-                       F.pos4.p, T.cx.p, T.cy.p, T.cz.p, L, a.out);
+  if (with_l2p) launch_l2p_add<P, R>(S, L, a.out, st);
 }
 
 template <int P, typename R, bool DAMP, typename MU4>
@@ -1846,9 +1927,7 @@ void fmm_apply_b_prec(System& S, const double* x, double* out, R* M, R* L) {
     fmm_far_pass<KIND_M1, P, R>(S, x, M, L, F.far_stream);
   CPPE_HIP_CHECK(hipEventRecord(F.ev_join, F.far_stream));
   CPPE_HIP_CHECK(hipStreamWaitEvent(S.stream, F.ev_join, 0));
-  const Octree& T = S.tree;
-  hipLaunchKernelGGL((k_fmm_l2p_add<P, R>), dim3(ceil_div(S.n, 256)), dim3(256), 0, S.stream, S.n, T.leaf_of.p,
-                     F.pos4.p, T.cx.p, T.cy.p, T.cz.p, L, out);
+  launch_l2p_add<P, R>(S, L, out, S.stream);
 }
 
 template <int P>

@@ -10,8 +10,10 @@
 // CG keeps its recurrence; convergence is still tested on the unpreconditioned ||r||_2.
 //
 // Pairs are taken from the octree's near lists (every pair of sites in P2P partner leaves closer
-// than the radius; partner lists are symmetric, so T_loc is). Blocks are formed in FP64 and
-// stored in FP32 (6 packed values per pair), products summed in FP64: M stays one fixed operator.
+// than the radius; partner lists are symmetric, so T_loc is). Only the partner index of each pair
+// is stored: the solver's k_sg_prec forms each block where it applies it (a handful of pairs per
+// site, so the arithmetic costs far less than reading stored blocks), from the site records
+// (x, y, z, s) kept here -- see pc_pair in solver.hip for the arithmetic and its symmetry.
 #include <algorithm>
 
 #include "pairs.hh"
@@ -60,28 +62,14 @@ __global__ __launch_bounds__(256) void k_pc_count(PcArgs a, int* __restrict__ cn
   cnt[i] = c;
 }
 
-// T2d_ij packed xx, xy, xz, yy, yz, zz: 3 fT d_a d_b / R^5 - fE delta_ab / R^3
+// Partner index of every pair of site i, in the order of its near list; and the site record.
 __global__ __launch_bounds__(256) void k_pc_fill(PcArgs a, const int* __restrict__ off, int* __restrict__ nj,
-                                                 float* __restrict__ blk) {
+                                                 double4* __restrict__ site) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   int p = off[i];
-  const double si = a.s[i];
-  pc_for_pairs(a, i, [&](int j, double dx, double dy, double dz, double r2) {
-    const double r = sqrt(r2), ri = 1.0 / r, ri3 = ri * ri * ri;
-    double fE = 1.0, fT = 1.0;
-    const double v = a.damping > 0.0 && si > 0.0 && a.s[j] > 0.0 ? a.damping * si * a.s[j] * r : 0.0;
-    if (v > 0.0 && v < kTholeCutoff) thole_ET(v, exp(-v), fE, fT);
-    const double c5 = 3.0 * fT * ri3 * ri * ri, c3 = fE * ri3;
-    nj[p]              = j;
-    blk[6 * (size_t)p] = (float)(c5 * dx * dx - c3);
-    blk[6 * (size_t)p + 1] = (float)(c5 * dx * dy);
-    blk[6 * (size_t)p + 2] = (float)(c5 * dx * dz);
-    blk[6 * (size_t)p + 3] = (float)(c5 * dy * dy - c3);
-    blk[6 * (size_t)p + 4] = (float)(c5 * dy * dz);
-    blk[6 * (size_t)p + 5] = (float)(c5 * dz * dz - c3);
-    ++p;
-  });
+  pc_for_pairs(a, i, [&](int j, double, double, double, double) { nj[p++] = j; });
+  site[i] = make_double4(a.x[i], a.y[i], a.z[i], a.s[i]);
 }
 
 void precond_destroy(System& S) {
@@ -118,8 +106,8 @@ void precond_build(System& S) {
   P->npairs   = tot;
   CPPE_HIP_CHECK(hipMemcpyAsync(P->off.p, cnt.data(), sizeof(int) * (S.n + 1), hipMemcpyHostToDevice, st));
   P->nj.alloc((size_t)std::max(1LL, tot));
-  P->blk.alloc(6 * (size_t)std::max(1LL, tot));
-  hipLaunchKernelGGL(k_pc_fill, dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a, P->off.p, P->nj.p, P->blk.p);
+  P->site.alloc((size_t)S.n);
+  hipLaunchKernelGGL(k_pc_fill, dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a, P->off.p, P->nj.p, P->site.p);
   CPPE_HIP_CHECK(hipStreamSynchronize(st));
 }
 

@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "cg_policy.hh"
+#include "pairs.hh"
 #include "system.hh"
 
 namespace cppe_dev {
@@ -92,19 +93,6 @@ __global__ void k_fill_f64(size_t n, double* __restrict__ a, double v) {
 
 void fill_f64(System& S, double* a, size_t n, double v) {
   if (n) hipLaunchKernelGGL(k_fill_f64, dim3(ceil_div((long long)n, 256)), dim3(256), 0, S.stream, n, a, v);
-}
-
-// block-wide sum (256 threads); the result is valid in thread 0
-__device__ __forceinline__ double block_sum(double v, double* red) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) red[w] = v;
-  __syncthreads();
-  double s = 0.0;
-  if (threadIdx.x == 0)
-    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[k];
-  return s;
 }
 
 // Lane l's share of the chunk partials part[lo, hi): chunks l, l + 64, l + 128, ... summed into
@@ -176,8 +164,8 @@ struct CgVecs {
   double4* w;
   const int* pc_off;
   const int* pc_nj;
-  const float* pc_blk;
-  double pc_weight;
+  const double4* pc_site;  // x, y, z, Thole scale per site
+  double pc_weight, pc_damping;
 };
 
 // p of site i (components p0, p1, p2) into the near-field dipole record the next apply reads
@@ -417,12 +405,30 @@ __global__ __launch_bounds__(256) void k_sg_z(CgVecs v) {
   if (threadIdx.x == 0 && !v.pc_off) v.part[c] = s;
 }
 
+// y += T2d(d) w_j for one close pair, d = r_i - r_j: T2d = 3 fT d d^T / R^5 - fE / R^3 (Thole
+// damped, field_m1.cc:127-193 for one pair). The scalars fE / R^3 and 3 fT / R^5 come from FP32
+// (displacement rounded to FP32, damping product s_i s_j taken in that order-free form), so the
+// pair (j, i) gets exactly the same two scalars as (i, j) and T_loc stays symmetric; d and the
+// products with w are FP64. The preconditioner only has to be one fixed symmetric operator: its
+// blocks agree with the FP64 tensor to about 1e-6 relative.
+__device__ __forceinline__ void pc_pair(const double4& pi, const double4& pj, double damping, const double4& wj,
+                                        double& y0, double& y1, double& y2) {
+  const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+  float c3, cf;
+  geo_m1_f32<true>((float)dx, (float)dy, (float)dz, (float)(damping * (pi.w * pj.w)), c3, cf);
+  const double c5 = (double)cf * (dx * wj.x + dy * wj.y + dz * wj.z), e3 = (double)c3;
+  y0 += c5 * dx - e3 * wj.x;
+  y1 += c5 * dy - e3 * wj.y;
+  y2 += c5 * dz - e3 * wj.z;
+}
+
 // Local preconditioner, after a kernel that wrote w = alpha r: z = w + weight alpha (T_loc w) and
 // the chunk partial r.z into part[slot * nchunk + c]. MODE 0 (after k_sg_init, every segment):
 // also p = z and p's near-field records. MODE 1 (after k_sg_xrz) and MODE 2 (after k_sg_z):
 // active segments, MODE 1 not in a halted step, MODE 2 not while a reliable update is pending.
 // One thread per site walks its pair row in order (a handful of pairs: four in flight at a
-// time), so every site of the chunk gathers at once; the sum over a row has a fixed order.
+// time), so every site of the chunk gathers at once; the sum over a row has a fixed order. Each
+// pair's block is formed here from the two site records (pc_pair), not read from memory.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_sg_prec(CgVecs v, int slot) {
   __shared__ double red[4];
@@ -435,14 +441,11 @@ __global__ __launch_bounds__(256) void k_sg_prec(CgVecs v, int slot) {
       const int ia = i + ci.ia;  // site within its right-hand side
       double y0 = 0.0, y1 = 0.0, y2 = 0.0;
       const int q0 = v.pc_off[ia], q1 = v.pc_off[ia + 1];
+      const double4 pi = v.pc_site[ia];
 #pragma unroll 4
       for (int q = q0; q < q1; ++q) {
-        const double4 wj = v.w[v.pc_nj[q] - ci.ia];
-        const float2* t2 = reinterpret_cast<const float2*>(v.pc_blk + 6 * (size_t)q);
-        const float2 ta = t2[0], tb = t2[1], tc = t2[2];  // xx xy | xz yy | yz zz
-        y0 += (double)ta.x * wj.x + (double)ta.y * wj.y + (double)tb.x * wj.z;
-        y1 += (double)ta.y * wj.x + (double)tb.y * wj.y + (double)tc.x * wj.z;
-        y2 += (double)tb.x * wj.x + (double)tc.x * wj.y + (double)tc.y * wj.z;
+        const int j = v.pc_nj[q];
+        pc_pair(pi, v.pc_site[j], v.pc_damping, v.w[j - ci.ia], y0, y1, y2);
       }
       double a0, a1, a2;
       z_of(v.a6, n, ia, y0, y1, y2, a0, a1, a2);
@@ -764,8 +767,9 @@ static CgVecs vecs(System& S, CgState& C) {
   v.w         = pc ? reinterpret_cast<double4*>(C.wbuf.p) : nullptr;
   v.pc_off    = pc ? S.pc->off.p : nullptr;
   v.pc_nj     = pc ? S.pc->nj.p : nullptr;
-  v.pc_blk    = pc ? S.pc->blk.p : nullptr;
-  v.pc_weight = S.precond_weight;
+  v.pc_site    = pc ? S.pc->site.p : nullptr;
+  v.pc_weight  = S.precond_weight;
+  v.pc_damping = pc ? S.pc->damping : 0.0;
   return v;
 }
 
@@ -839,9 +843,14 @@ static void seq_direction(System& S, CgState& C, bool mixed) {
 static void seq_step_direction(System& S, CgState& C, bool mixed, bool publish) {
   const dim3 g(C.nchunk), b(256);
   if (C.rec32 || C.rec64) S.prepacked = C.p;  // records written by k_sg_init / k_sg_p
+  // one segment of one right-hand side: its chunks are the 256-site blocks of the L2P pass, which
+  // then writes the p.Ap and Ap.Ap partials itself (when the apply ends in one)
+  if (C.nseg == 1 && C.nrhs == 1 && C.na == S.n) S.dots = System::StepDots{C.p, C.Ap, C.part.p, C.nchunk, false};
   apply(S, C, C.p, C.Ap, mixed);
-  S.prepacked = nullptr;
-  hipLaunchKernelGGL(k_sg_pap, g, b, 0, S.stream, vecs(S, C));
+  S.prepacked    = nullptr;
+  const bool dots = S.dots.done;
+  S.dots          = System::StepDots{};
+  if (!dots) hipLaunchKernelGGL(k_sg_pap, g, b, 0, S.stream, vecs(S, C));
   hipLaunchKernelGGL(k_sg_xrz, g, b, 0, S.stream, vecs(S, C));
   launch_prec<1>(S, C, 1);
   launch_final<F_RNB>(S, C, mixed, publish);

@@ -85,8 +85,8 @@ struct FmmState;  // FMM expansions and operator scratch (fmm.hip)
 struct Precond {
   double rc = 0.0, damping = 0.0;
   long long npairs = 0;
-  dbuf<int> off, nj;
-  dbuf<float> blk;
+  dbuf<int> off, nj;  // pairs of site i: partners nj[off[i] .. off[i+1])
+  dbuf<double4> site;  // x, y, z, Thole scale s per site
 };
 struct MlGrid;    // neighbour grid of the learned warm start (ml_features.hip)
 
@@ -198,6 +198,16 @@ struct System {
   // When set, the near-field dipole records of the next B apply's precision already hold this
   // vector (the CG direction kernels write them beside p), and the apply does not pack them.
   const double* prepacked = nullptr;
+  // When p is set (by a CG step around its apply into ap, one right-hand side, chunks of 256 sites
+  // from site 0): the L2P pass that ends the apply also writes the chunk partials p.ap and ap.ap
+  // at part[2 nchunk + c] and part[3 nchunk + c], and sets done.
+  struct StepDots {
+    const double* p = nullptr;
+    const double* ap = nullptr;
+    double* part = nullptr;
+    int nchunk = 0;
+    bool done = false;
+  } dots;
   hipEvent_t* prof = nullptr;  // when set (8 events): [0..4] FMM phase boundaries, [6..7] around the
                                // staged near-field kernel alone
 
