@@ -211,7 +211,7 @@ __device__ __forceinline__ double lane_sum(double v) {
   return v;
 }
 
-// Descriptors 0-55 (FP32 arithmetic) over the neighbour list, kDescLanes lanes per site.
+// Descriptors 0-47, 54 and 55 (FP32 arithmetic) over the neighbour list, kDescLanes lanes per site.
 __global__ __launch_bounds__(256) void k_ml_desc(ListArgs a) {
   const int gid  = blockIdx.x * blockDim.x + threadIdx.x;
   const int i0   = gid / kDescLanes;
@@ -219,12 +219,10 @@ __global__ __launch_bounds__(256) void k_ml_desc(ListArgs a) {
   const bool ok  = i0 < a.n;
   const int i    = ok ? i0 : a.n - 1;  // trailing lanes shadow the last site and store nothing
   const int n = a.n;
-  const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
   const double4 pi = a.pos[i];
   const double Fx = a.F[i], Fy = a.F[n + i], Fz = a.F[2 * (size_t)n + i];
   const double Fn = sqrt(Fx * Fx + Fy * Fy + Fz * Fz) + 1e-30;
-  const double fx = Fx / Fn, fy = Fy / Fn, fz = Fz / Fn;
-  const float ffx = (float)fx, ffy = (float)fy, ffz = (float)fz;
+  const float ffx = (float)(Fx / Fn), ffy = (float)(Fy / Fn), ffz = (float)(Fz / Fn);
   float feat[48];
 #pragma unroll
   for (int k = 0; k < 48; ++k) feat[k] = 0.f;
@@ -277,9 +275,31 @@ __global__ __launch_bounds__(256) void k_ml_desc(ListArgs a) {
   float* Xi = a.X + (size_t)i * kMlpIn;
 #pragma unroll
   for (int k = 0; k < 48; ++k) Xi[k] = feat[k];
-  // molecular frame from the first two exclusion partners (fallback: field frame)
-  double ai[6];
+  Xi[54] = f54;
+  Xi[55] = f55;
+}
+
+// v1 = alpha F, the per-site FP32 records the list kernels gather, and the per-site descriptors
+__global__ void k_ml_v1(ListArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int n = a.n;
+  const double Fx = a.F[i], Fy = a.F[n + i], Fz = a.F[2 * (size_t)n + i];
+  double ai[6], ox, oy, oz;
   site_alpha(a.a6, n, i, ai);
+  amul(ai, Fx, Fy, Fz, ox, oy, oz);
+  a.V[i]                 = ox;
+  a.V[n + i]             = oy;
+  a.V[2 * (size_t)n + i] = oz;
+  a.w1[i]                = make_float4((float)ox, (float)oy, (float)oz, 0.0f);
+  const double Fi = 1.0 / (sqrt(Fx * Fx + Fy * Fy + Fz * Fz) + 1e-30);
+  a.frec[i]       = make_float4((float)(Fx * Fi), (float)(Fy * Fi), (float)(Fz * Fi), a.iso[i]);
+  // descriptors 48-53: the field direction in the molecular frame of the first two exclusion
+  // partners (fallback: field frame), log |F|, alpha_iso / 5, F^ alpha F^ / alpha_iso
+  const double Fn = sqrt(Fx * Fx + Fy * Fy + Fz * Fz) + 1e-30;
+  const double fx = Fx / Fn, fy = Fy / Fn, fz = Fz / Fn;
+  const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
+  float* Xi = a.X + (size_t)i * kMlpIn;
   const double iso_i = (ai[0] + ai[3] + ai[5]) / 3.0;
   const int e0 = a.eoff ? a.eoff[i] : 0, e1 = a.eoff ? a.eoff[i + 1] : 0;
   double e1x = fx, e1y = fy, e1z = fz, e2x = 0, e2y = 0, e2z = 0;
@@ -311,25 +331,6 @@ __global__ __launch_bounds__(256) void k_ml_desc(ListArgs a) {
   double afx, afy, afz;
   amul(ai, fx, fy, fz, afx, afy, afz);
   Xi[53] = (float)((afx * fx + afy * fy + afz * fz) / iso_i);
-  Xi[54] = f54;
-  Xi[55] = f55;
-}
-
-// v1 = alpha F, and the per-site FP32 records the list kernels gather
-__global__ void k_ml_v1(ListArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  const int n = a.n;
-  const double Fx = a.F[i], Fy = a.F[n + i], Fz = a.F[2 * (size_t)n + i];
-  double ai[6], ox, oy, oz;
-  site_alpha(a.a6, n, i, ai);
-  amul(ai, Fx, Fy, Fz, ox, oy, oz);
-  a.V[i]                 = ox;
-  a.V[n + i]             = oy;
-  a.V[2 * (size_t)n + i] = oz;
-  a.w1[i]                = make_float4((float)ox, (float)oy, (float)oz, 0.0f);
-  const double Fi = 1.0 / (sqrt(Fx * Fx + Fy * Fy + Fz * Fz) + 1e-30);
-  a.frec[i]       = make_float4((float)(Fx * Fi), (float)(Fy * Fi), (float)(Fz * Fi), a.iso[i]);
 }
 
 // PASS 1: v2 = alpha_i sum_j K_ij v1_j.  PASS 2: v3 = alpha_i sum_j K_ij v2_j, descriptors 56-63.

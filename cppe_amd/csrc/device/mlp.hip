@@ -330,15 +330,26 @@ void mlp_refresh_bf16(Mlp& M) {
 }
 
 // Forward pass only (the warm-start guess): the arithmetic of k_mlp_tile<false> -- the same MFMA
-// tiles, roundings and layer-3 summation order, so the same outputs -- without the training
-// copies (transposes, pre-activations), so a block needs ~30 KB of LDS instead of ~90 KB and
-// several blocks share a CU; SiLU of layer 2 is taken once per element.
+// tiles in the same k order, the same roundings and layer-3 summation order, so the same outputs --
+// without the training copies. Blocks are persistent over tiles: each wavefront loads its 32
+// columns of W1 and W2 as MFMA operands into registers once, W3 and b3 sit in LDS, and s(Z2) is
+// kept transposed (stride TS + 1) so that the 96 layer-3 threads read it without bank conflicts.
+constexpr int LS2 = TS + 1;
 __global__ __launch_bounds__(256) void k_mlp_fwd(MlpTileArgs a) {
   __shared__ __attribute__((aligned(16))) unsigned short Xs[TS * LX];
   __shared__ __attribute__((aligned(16))) unsigned short H1s[TS * LH];
-  __shared__ float S2[TS * kMlpHid];
-  const int tid = threadIdx.x, wave = tid >> 6;
-  const int col = acc_col();
+  __shared__ float S2t[kMlpHid * LS2];
+  __shared__ float W3s[kMlpHid * kMlpOut + kMlpOut];
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+  const int col = acc_col(), r = l & 31, kh = (l >> 5) * 8;
+  const int n = wave * 32 + col;
+  bf16x8 w1v[kMlpIn / 16], w2v[kMlpHid / 16];
+#pragma unroll
+  for (int k = 0; k < kMlpIn / 16; ++k) w1v[k] = ld8(a.w1t + (wave * 32 + r) * kMlpIn + 16 * k + kh);
+#pragma unroll
+  for (int k = 0; k < kMlpHid / 16; ++k) w2v[k] = ld8(a.w2t + (wave * 32 + r) * kMlpHid + 16 * k + kh);
+  const float bias1 = a.w[kOffB1 + n], bias2 = a.w[kOffB2 + n];
+  for (int e = tid; e < kMlpHid * kMlpOut + kMlpOut; e += 256) W3s[e] = a.w[kOffW3 + e];  // W3 then b3
   for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const long long s0 = (long long)tile * TS;
     for (int e = tid; e < TS * kMlpIn; e += 256) {
@@ -350,24 +361,24 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(MlpTileArgs a) {
     __syncthreads();
     {
       f32x16 acc = f32x16{};
-      mfma_tile(Xs, LX, a.w1t + (wave * 32) * kMlpIn, kMlpIn, kMlpIn, acc);
-      const int n = wave * 32 + col;
-      const float b = a.w[kOffB1 + n];
-      for (int q = 0; q < 16; ++q) H1s[acc_row(q) * LH + n] = f2bf(silu(acc[q] + b));
+#pragma unroll
+      for (int k = 0; k < kMlpIn / 16; ++k)
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(Xs + r * LX + 16 * k + kh), w1v[k], acc, 0, 0, 0);
+      for (int q = 0; q < 16; ++q) H1s[acc_row(q) * LH + n] = f2bf(silu(acc[q] + bias1));
     }
     __syncthreads();
     {
       f32x16 acc = f32x16{};
-      mfma_tile(H1s, LH, a.w2t + (wave * 32) * kMlpHid, kMlpHid, kMlpHid, acc);
-      const int n = wave * 32 + col;
-      const float b = a.w[kOffB2 + n];
-      for (int q = 0; q < 16; ++q) S2[acc_row(q) * kMlpHid + n] = silu(acc[q] + b);
+#pragma unroll
+      for (int k = 0; k < kMlpHid / 16; ++k)
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(H1s + r * LH + 16 * k + kh), w2v[k], acc, 0, 0, 0);
+      for (int q = 0; q < 16; ++q) S2t[n * LS2 + acc_row(q)] = silu(acc[q] + bias2);
     }
     __syncthreads();
     if (tid < TS * kMlpOut) {
       const int s = tid / 3, o = tid % 3;
-      float acc = a.w[kOffB3 + o];
-      for (int h = 0; h < kMlpHid; ++h) acc += S2[s * kMlpHid + h] * a.w[kOffW3 + h * kMlpOut + o];
+      float acc = W3s[kMlpHid * kMlpOut + o];
+      for (int h = 0; h < kMlpHid; ++h) acc += S2t[h * LS2 + s] * W3s[h * kMlpOut + o];
       if (s0 + s < a.ns) a.y[(s0 + s) * 3 + o] = acc;
     }
     __syncthreads();
@@ -387,7 +398,7 @@ void mlp_forward(Mlp& M, const float* X, int ns, float* y, hipStream_t st) {
   a.ns = ns;
   a.ntiles = ceil_div(ns, TS);
   a.y = y;
-  const int grid = std::min(a.ntiles, 4096);
+  const int grid = std::min(a.ntiles, 1024);  // persistent: four blocks per CU (register-bound)
   hipLaunchKernelGGL(k_mlp_fwd, dim3(std::max(grid, 1)), dim3(256), 0, st, a);
 }
 

@@ -638,6 +638,7 @@ void CgState::release_graphs() {
 CgState::~CgState() {
   release_graphs();
   if (h_status) (void)hipHostFree(h_status);
+  if (h_out) (void)hipHostFree(h_out);
 }
 
 // Chunks of <= kChunk sites inside each segment (site ranges seg_off, stacked numbering) and the
@@ -670,6 +671,7 @@ static void cg_layout(System& S, CgState& C, const std::vector<int>& seg_off) {
   C.si.alloc(I_COUNT * (size_t)ns);
   C.status.alloc(ST_COUNT);
   C.params.alloc(P_COUNT);
+  C.params_ok = false;
   C.hist_cap = 0;
   C.hist.release();
   if (!C.h_status) {
@@ -912,6 +914,27 @@ static void read_status(System& S, CgState& C, int steps = 1) {
   std::atomic_thread_fence(std::memory_order_acquire);
 }
 
+// The solve parameters into the device block, unless it already holds exactly these (repeated
+// solves of a system reuse them).
+static void upload_params(System& S, CgState& C, const double* params) {
+  static_assert(P_COUNT <= 8, "host copy of the parameters");
+  if (C.params_ok && std::equal(params, params + P_COUNT, C.h_params)) return;
+  std::copy(params, params + P_COUNT, C.h_params);
+  C.params_ok = true;
+  CPPE_HIP_CHECK(hipMemcpyAsync(C.params.p, C.h_params, sizeof(double) * P_COUNT, hipMemcpyHostToDevice, S.stream));
+}
+
+// A solve's results for the host in one pass into host-mapped memory: the segment tables sd and
+// si and the first nh rows of the residual history (doubles first, then the ints).
+__global__ void k_sg_export(const double* __restrict__ sd, int nsd, const double* __restrict__ hist, long long nhist,
+                            const int* __restrict__ si, int nsi, double* __restrict__ od, int* __restrict__ oi) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nsd) od[t] = sd[t];
+  if (t < nhist) od[nsd + t] = hist[t];
+  if (t < nsi) oi[t] = si[t];
+  __threadfence_system();  // the host reads them once the stream has drained
+}
+
 // Solves B x = rhs for C.rhs into C.x (which holds the initial guess). Results per segment are
 // left in the device state; res.norms holds every segment's residual norms, norms0 segment 0's.
 int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgResult& res) {
@@ -965,7 +988,7 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
   params[P_RESTART]  = 0.0;
   params[P_ETA]      = conf ? S.confirm_eta : 0.0;
   params[P_GAP]      = (S.confirm_forms & 1) ? S.confirm_eta : 0.0;
-  CPPE_HIP_CHECK(hipMemcpyAsync(C.params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
+  upload_params(S, C, params);
   // r0 = rhs - B x0, z0 = alpha r0, p = z0. The mixed form takes r0 from an FP32 apply: it is
   // replaced by an FP64 true residual at the first reliable update, before any convergence test.
   apply(S, C, C.x, C.t, mixed);
@@ -1031,12 +1054,12 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
       C.rec64 = r64;
     }
     params[P_RESTART] = 1.0;
-    CPPE_HIP_CHECK(hipMemcpyAsync(C.params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
+    upload_params(S, C, params);
     hipLaunchKernelGGL(k_sg_init, dim3(C.nchunk), dim3(256), 0, st, vecs(S, C));
     launch_prec<0>(S, C, 0);
     launch_final<F_INIT>(S, C, false);
     params[P_RESTART] = 0.0;
-    CPPE_HIP_CHECK(hipMemcpyAsync(C.params.p, params, sizeof(params), hipMemcpyHostToDevice, st));
+    upload_params(S, C, params);
     ++C.expect;
     read_status(S, C);
     ++nread;
@@ -1075,15 +1098,29 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
   }
   if (mixed && C.h_status[ST_PEND] > 0) reliable();  // maxiter reached with unconfirmed segments
   // per-segment results
-  std::vector<int> si((size_t)I_COUNT * ns);
-  std::vector<double> sd((size_t)D_COUNT * ns);
-  CPPE_HIP_CHECK(hipMemcpyAsync(si.data(), C.si.p, sizeof(int) * si.size(), hipMemcpyDeviceToHost, st));
-  CPPE_HIP_CHECK(hipMemcpyAsync(sd.data(), C.sd.p, sizeof(double) * sd.size(), hipMemcpyDeviceToHost, st));
   const int nh = std::min(k, C.hist_cap);
-  std::vector<double> hist((size_t)nh * ns);
-  if (nh)
-    CPPE_HIP_CHECK(hipMemcpyAsync(hist.data(), C.hist.p, sizeof(double) * hist.size(), hipMemcpyDeviceToHost, st));
+  const int nsd = D_COUNT * ns, nsi = I_COUNT * ns;
+  const long long nhist = (long long)nh * ns;
+  const size_t bytes = sizeof(double) * (nsd + (size_t)nhist) + sizeof(int) * (size_t)nsi;
+  if (C.out_bytes < bytes) {
+    if (C.h_out) CPPE_HIP_CHECK(hipHostFree(C.h_out));
+    C.h_out     = nullptr;
+    C.out_bytes = 0;
+    const size_t cap = std::max<size_t>(bytes, 4096);
+    CPPE_HIP_CHECK(hipHostMalloc((void**)&C.h_out, cap, hipHostMallocMapped | hipHostMallocCoherent));
+    CPPE_HIP_CHECK(hipHostGetDevicePointer((void**)&C.d_out, C.h_out, 0));
+    C.out_bytes = cap;
+  }
+  double* od = reinterpret_cast<double*>(C.d_out);
+  int* oi    = reinterpret_cast<int*>(C.d_out + sizeof(double) * (nsd + (size_t)nhist));
+  const long long nt = std::max<long long>({(long long)nsd, nhist, (long long)nsi});
+  hipLaunchKernelGGL(k_sg_export, dim3(ceil_div(nt, 256)), dim3(256), 0, st, C.sd.p, nsd, C.hist.p, nhist, C.si.p, nsi,
+                     od, oi);
   CPPE_HIP_CHECK(hipStreamSynchronize(st));
+  const double* hd = reinterpret_cast<const double*>(C.h_out);
+  const int* hi    = reinterpret_cast<const int*>(C.h_out + sizeof(double) * (nsd + (size_t)nhist));
+  const std::vector<double> sd(hd, hd + nsd), hist(hd + nsd, hd + nsd + nhist);
+  const std::vector<int> si(hi, hi + nsi);
   res.iters.assign(ns, 0);
   res.rnorm.assign(ns, 0.0);
   res.converged.assign(ns, 0);

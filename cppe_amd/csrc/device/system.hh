@@ -121,6 +121,11 @@ struct CgState {
   int* h_status = nullptr;  // host-mapped mirror of the status block (written by the final kernels)
   int* d_hstat  = nullptr;  // its device address
   int expect    = 0;        // final kernels launched so far: the mirror's count once they have run
+  double h_params[8] = {};  // the solve parameters last uploaded to params (valid: params_ok)
+  bool params_ok     = false;
+  char* h_out        = nullptr;  // host-mapped: a solve's segment tables and residual history
+  char* d_out        = nullptr;  // its device address
+  size_t out_bytes   = 0;
   // g_step[form][c]: c consecutive steps, 1 <= c <= kMaxGraphSteps (form 0 FP64, 1 mixed)
   static constexpr int kMaxGraphSteps = 8;
   hipGraphExec_t g_step[2][kMaxGraphSteps + 1] = {}, g_rel = nullptr, g_conf = nullptr, g_dir[2] = {nullptr, nullptr};
