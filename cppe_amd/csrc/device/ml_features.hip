@@ -186,7 +186,7 @@ struct ListArgs {
   const float* iso;
   float4* frec;  // per site: F / |F| and alpha_iso (written by k_ml_v1)
   float4* w1;    // per site: v1 in FP32 (k_ml_v1)
-  float4* w2;    // per site: v2 in FP32 (k_ml_basis<1>)
+  float4* w2;    // per site: v2 in FP32 (k_ml_desc)
   const double *x, *y, *z, *a6, *F;
   const int* eoff;
   const int* eidx;
@@ -211,7 +211,29 @@ __device__ __forceinline__ double lane_sum(double v) {
   return v;
 }
 
-// Descriptors 0-47, 54 and 55 (FP32 arithmetic) over the neighbour list, kDescLanes lanes per site.
+// One pair of the response bases: acc += K_ij w_j, K_ij = fc(r) T2d_ij, the Thole-damped dipole
+// tensor (3 fT d d^T / r^5 - fE / r^3) of the response operator times the cutoff function, formed
+// in FP32 (sj: Thole scale of site j; di: damping times that of site i); the product summed in FP64.
+__device__ __forceinline__ void basis_acc(const MlPair& q, float r, float fc, float di, double sj, const float4& w,
+                                          double& sx, double& sy, double& sz) {
+  constexpr float kLog2e = 1.44269504088896340736f;
+  const float ri2 = q.ri * q.ri, ri3 = ri2 * q.ri;
+  const float u   = di * (float)sj * r;  // Thole argument; 0: a pair with an undamped site
+  const float e   = u > 0.0f ? __builtin_amdgcn_exp2f(-u * kLog2e) : 0.0f;  // e = 0: fE = fT = 1
+  const float u2  = u * u;
+  const float fE  = 1.0f - (1.0f + u + 0.5f * u2) * e;
+  const float fT  = fE - (u2 * u * (1.0f / 6.0f)) * e;
+  const float c3  = fE * ri3 * fc, c5 = 3.0f * fT * ri3 * ri2 * fc;
+  const float dw  = q.dx * w.x + q.dy * w.y + q.dz * w.z;
+  // K w = c5 (d . w) d - c3 w
+  sx += (double)(c5 * dw * q.dx - c3 * w.x);
+  sy += (double)(c5 * dw * q.dy - c3 * w.y);
+  sz += (double)(c5 * dw * q.dz - c3 * w.z);
+}
+
+// Descriptors 0-47, 54 and 55 (FP32 arithmetic) over the neighbour list, kDescLanes lanes per site,
+// and in the same pass over the list the first response basis v2 = alpha_i sum_j K_ij v1_j (pairs
+// that are not excluded).
 __global__ __launch_bounds__(256) void k_ml_desc(ListArgs a) {
   const int gid  = blockIdx.x * blockDim.x + threadIdx.x;
   const int i0   = gid / kDescLanes;
@@ -223,6 +245,8 @@ __global__ __launch_bounds__(256) void k_ml_desc(ListArgs a) {
   const double Fx = a.F[i], Fy = a.F[n + i], Fz = a.F[2 * (size_t)n + i];
   const double Fn = sqrt(Fx * Fx + Fy * Fy + Fz * Fz) + 1e-30;
   const float ffx = (float)(Fx / Fn), ffy = (float)(Fy / Fn), ffz = (float)(Fz / Fn);
+  const float di   = (float)(a.damping * pi.w);
+  double sx = 0.0, sy = 0.0, sz = 0.0;
   float feat[48];
 #pragma unroll
   for (int k = 0; k < 48; ++k) feat[k] = 0.f;
@@ -232,10 +256,12 @@ __global__ __launch_bounds__(256) void k_ml_desc(ListArgs a) {
   // the lane's pairs in order, two at a time (both indices, then both records, requested first)
   for (long long pb = a.noff[i] + sub; pb < p1; pb += 2 * kDescLanes) {
     const bool has2 = pb + kDescLanes < p1;
-    const int j0    = a.nj[pb] & 0x7fffffff;
-    const int j1    = has2 ? a.nj[pb + kDescLanes] & 0x7fffffff : j0;
+    const int n0    = a.nj[pb];  // negative: an excluded pair (no basis term)
+    const int n1    = has2 ? a.nj[pb + kDescLanes] : n0;
+    const int j0 = n0 & 0x7fffffff, j1 = n1 & 0x7fffffff;
     const double4 pq[2] = {a.pos[j0], a.pos[j1]};
     const float4 fq[2]  = {a.frec[j0], a.frec[j1]};
+    const float4 wq[2]  = {a.w1[j0], a.w1[j1]};
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
     if (b == 1 && !has2) break;
@@ -244,6 +270,7 @@ __global__ __launch_bounds__(256) void k_ml_desc(ListArgs a) {
     const float r   = q.r2 * ri;
     const float ux = q.dx * ri, uy = q.dy * ri, uz = q.dz * ri;
     const float fc  = ml_fc(r);
+    if ((b == 0 ? n0 : n1) >= 0) basis_acc(q, r, fc, di, pq[b].w, wq[b], sx, sy, sz);
     const float4 fj = fq[b];  // F_j / |F_j|, alpha_iso_j
     const float iso = fj.w;
     const float cth = ux * ffx + uy * ffy + uz * ffz;
@@ -271,12 +298,22 @@ __global__ __launch_bounds__(256) void k_ml_desc(ListArgs a) {
   for (int k = 0; k < 48; ++k) feat[k] = lane_sum<kDescLanes>(feat[k]);
   f54 = lane_sum<kDescLanes>(f54);
   f55 = lane_sum<kDescLanes>(f55);
+  sx  = lane_sum<kDescLanes>(sx);
+  sy  = lane_sum<kDescLanes>(sy);
+  sz  = lane_sum<kDescLanes>(sz);
   if (!ok || sub != 0) return;
   float* Xi = a.X + (size_t)i * kMlpIn;
 #pragma unroll
   for (int k = 0; k < 48; ++k) Xi[k] = feat[k];
   Xi[54] = f54;
   Xi[55] = f55;
+  double ai[6], ox, oy, oz;
+  site_alpha(a.a6, n, i, ai);
+  amul(ai, sx, sy, sz, ox, oy, oz);
+  a.V[3 * (size_t)n + i] = ox;
+  a.V[4 * (size_t)n + i] = oy;
+  a.V[5 * (size_t)n + i] = oz;
+  a.w2[i]                = make_float4((float)ox, (float)oy, (float)oz, 0.0f);
 }
 
 // v1 = alpha F, the per-site FP32 records the list kernels gather, and the per-site descriptors
@@ -333,11 +370,8 @@ __global__ void k_ml_v1(ListArgs a) {
   Xi[53] = (float)((afx * fx + afy * fy + afz * fz) / iso_i);
 }
 
-// PASS 1: v2 = alpha_i sum_j K_ij v1_j.  PASS 2: v3 = alpha_i sum_j K_ij v2_j, descriptors 56-63.
-// K_ij = fc(r) T2d_ij, the Thole-damped dipole tensor (3 fT d d^T / r^5 - fE / r^3) fc of the
-// response operator, formed per pair in FP32 (excluded pairs: none); v_j as FP32 records; each
-// pair's product summed in FP64.
-template <int PASS>
+// The second response basis v3 = alpha_i sum_j K_ij v2_j (K as in basis_acc; excluded pairs: none)
+// and descriptors 56-63, kBasisLanes lanes per site.
 __global__ __launch_bounds__(256) void k_ml_basis(ListArgs a) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i0  = gid / kBasisLanes;
@@ -345,10 +379,8 @@ __global__ __launch_bounds__(256) void k_ml_basis(ListArgs a) {
   const bool ok = i0 < a.n;
   const int i   = ok ? i0 : a.n - 1;  // trailing lanes shadow the last site and store nothing
   const int n = a.n;
-  const float4* W = PASS == 1 ? a.w1 : a.w2;
   const double4 pi = a.pos[i];
   const float di   = (float)(a.damping * pi.w);
-  constexpr float kLog2e = 1.44269504088896340736f;
   double sx = 0.0, sy = 0.0, sz = 0.0;
   const long long p1 = ok ? a.noff[i + 1] : 0;
   // the lane's pairs p, p + G, p + 2G, ... in order, four at a time: every index, then every
@@ -367,28 +399,14 @@ __global__ __launch_bounds__(256) void k_ml_basis(ListArgs a) {
     for (int u = 0; u < kU; ++u) {
       const int jj = jb[u] < 0 ? i : jb[u];
       pb4[u]       = a.pos[jj];
-      wb[u]        = W[jj];
+      wb[u]        = a.w2[jj];
     }
 #pragma unroll
     for (int b = 0; b < kU; ++b) {
-    if (jb[b] < 0) continue;
-    const double4 pj = pb4[b];
-    const float4 w   = wb[b];
-    const MlPair q   = ml_pair(pi, pj);
-    const float ri2 = q.ri * q.ri, ri3 = ri2 * q.ri;
-    const float r   = q.r2 * q.ri;
-    const float u   = di * (float)pj.w * r;  // Thole argument; 0: a pair with an undamped site
-    const float e   = u > 0.0f ? __builtin_amdgcn_exp2f(-u * kLog2e) : 0.0f;  // e = 0: fE = fT = 1
-    const float u2  = u * u;
-    const float fE  = 1.0f - (1.0f + u + 0.5f * u2) * e;
-    const float fT  = fE - (u2 * u * (1.0f / 6.0f)) * e;
-    const float fc  = ml_fc(r);
-    const float c3  = fE * ri3 * fc, c5 = 3.0f * fT * ri3 * ri2 * fc;
-    const float dw  = q.dx * w.x + q.dy * w.y + q.dz * w.z;
-    // K w = c5 (d . w) d - c3 w
-    sx += (double)(c5 * dw * q.dx - c3 * w.x);
-    sy += (double)(c5 * dw * q.dy - c3 * w.y);
-    sz += (double)(c5 * dw * q.dz - c3 * w.z);
+      if (jb[b] < 0) continue;
+      const MlPair q = ml_pair(pi, pb4[b]);
+      const float r  = q.r2 * q.ri;
+      basis_acc(q, r, ml_fc(r), di, pb4[b].w, wb[b], sx, sy, sz);
     }
   }
   sx = lane_sum<kBasisLanes>(sx);
@@ -398,13 +416,6 @@ __global__ __launch_bounds__(256) void k_ml_basis(ListArgs a) {
   double ai[6], ox, oy, oz;
   site_alpha(a.a6, n, i, ai);
   amul(ai, sx, sy, sz, ox, oy, oz);
-  if (PASS == 1) {
-    a.V[3 * (size_t)n + i] = ox;
-    a.V[4 * (size_t)n + i] = oy;
-    a.V[5 * (size_t)n + i] = oz;
-    a.w2[i]                = make_float4((float)ox, (float)oy, (float)oz, 0.0f);
-    return;
-  }
   a.V[6 * (size_t)n + i] = ox;
   a.V[7 * (size_t)n + i] = oy;
   a.V[8 * (size_t)n + i] = oz;
@@ -549,8 +560,7 @@ void ml_features(System& S, float* X, double* V) {
              S.alpha6.p, S.v_rhs.p, S.excl_off.p, S.excl_idx.p, X, V};
   hipLaunchKernelGGL(k_ml_v1, dim3(ceil_div(n, 256)), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_ml_desc, dim3(ceil_div((long long)n * kDescLanes, 256)), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(k_ml_basis<1>, dim3(ceil_div((long long)n * kBasisLanes, 256)), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(k_ml_basis<2>, dim3(ceil_div((long long)n * kBasisLanes, 256)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_ml_basis, dim3(ceil_div((long long)n * kBasisLanes, 256)), dim3(256), 0, st, a);
 }
 
 }  // namespace cppe_dev

@@ -335,7 +335,7 @@ void mlp_refresh_bf16(Mlp& M) {
 // columns of W1 and W2 as MFMA operands into registers once, W3 and b3 sit in LDS, and s(Z2) is
 // kept transposed (stride TS + 1) so that the 96 layer-3 threads read it without bank conflicts.
 constexpr int LS2 = TS + 1;
-__global__ __launch_bounds__(256) void k_mlp_fwd(MlpTileArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_mlp_fwd(MlpTileArgs a) {
   __shared__ __attribute__((aligned(16))) unsigned short Xs[TS * LX];
   __shared__ __attribute__((aligned(16))) unsigned short H1s[TS * LH];
   __shared__ float S2t[kMlpHid * LS2];
@@ -350,15 +350,30 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(MlpTileArgs a) {
   for (int k = 0; k < kMlpHid / 16; ++k) w2v[k] = ld8(a.w2t + (wave * 32 + r) * kMlpHid + 16 * k + kh);
   const float bias1 = a.w[kOffB1 + n], bias2 = a.w[kOffB2 + n];
   for (int e = tid; e < kMlpHid * kMlpOut + kMlpOut; e += 256) W3s[e] = a.w[kOffW3 + e];  // W3 then b3
+  // thread tid holds feature f = tid % 64 of samples tid / 64 + 4 k (k < 8) of a tile; the next
+  // tile's are loaded while the current one is evaluated
+  static_assert(TS * kMlpIn == 8 * 256, "eight features per thread and tile");
+  const int f = tid & 63, sb = tid >> 6;
+  const float mean_f = a.mean[f], istd_f = a.inv_std[f];
+  float xr[8];
+  auto fetch = [&](int tile) {
+    const long long s0 = (long long)tile * TS;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const long long sk = s0 + sb + 4 * k;
+      xr[k]              = tile < a.ntiles && sk < a.ns ? a.Xraw[sk * kMlpIn + f] : 0.0f;
+    }
+  };
+  fetch(blockIdx.x);
   for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const long long s0 = (long long)tile * TS;
-    for (int e = tid; e < TS * kMlpIn; e += 256) {
-      const int s = e / kMlpIn, f = e % kMlpIn;
-      unsigned short v = 0;
-      if (s0 + s < a.ns) v = f2bf((a.Xraw[(s0 + s) * kMlpIn + f] - a.mean[f]) * a.inv_std[f]);
-      Xs[s * LX + f] = v;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int s = sb + 4 * k;
+      Xs[s * LX + f] = s0 + s < a.ns ? f2bf((xr[k] - mean_f) * istd_f) : (unsigned short)0;
     }
     __syncthreads();
+    fetch(tile + gridDim.x);
     {
       f32x16 acc = f32x16{};
 #pragma unroll
