@@ -412,8 +412,15 @@ struct TopLevels {
   int off[kTopLevels + 1];     // first cell of each level, then the end
 };
 
+// 512 threads for the dipole-field operators (one pass over a level of up to 512 cells); 256 for
+// the static-field ones, whose FP64 expansions need twice the registers
+template <int KIND>
+constexpr int down_top_threads() {
+  return KIND == KIND_M1 ? 512 : 256;
+}
+
 template <int KIND, int P, typename R>
-__global__ __launch_bounds__(256) void k_fmm_down_top(TopLevels t, const int* __restrict__ parent,
+__global__ __launch_bounds__(down_top_threads<KIND>()) void k_fmm_down_top(TopLevels t, const int* __restrict__ parent,
                                                       const double* __restrict__ cx, const double* __restrict__ cy,
                                                       const double* __restrict__ cz, R* __restrict__ L) {
   for (int lev = 0; lev < t.nlev; ++lev) {
@@ -1429,8 +1436,8 @@ void fmm_down_pass(System& S, R* M, R* L, hipStream_t st) {
     ++top.nlev;
   }
   if (top.nlev > 0)
-    hipLaunchKernelGGL((k_fmm_down_top<KIND, P, R>), dim3(1), dim3(256), 0, st, top, T.parent.p, T.cx.p, T.cy.p,
-                       T.cz.p, L);
+    hipLaunchKernelGGL((k_fmm_down_top<KIND, P, R>), dim3(1), dim3(down_top_threads<KIND>()), 0, st, top, T.parent.p,
+                       T.cx.p, T.cy.p, T.cz.p, L);
   for (int lev = top.nlev; lev <= T.max_level; ++lev) {
     const int c0 = lo[lev], c1 = lo[lev + 1];
     if (lev == 0) {  // a root level too large for the single workgroup does not occur (one cell)
