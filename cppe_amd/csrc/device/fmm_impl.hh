@@ -362,7 +362,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void
   for (int k = 0; k < NCH; ++k) mine[k] = R(0);
 #pragma unroll
   for (int k = 0; k < O::LS; ++k) {
-    if (!O::lkept(k)) continue;  // stored as 0 here, restored by k_fmm_unfold
+    if (!O::lkept(k)) continue;  // stored as 0 here, restored by the downward pass (down_cell)
     const R v = group_sum<G>(l[k]);
     if (gl == (k % G)) mine[k / G] = v;
   }
@@ -1440,7 +1440,7 @@ void fmm_down_pass(System& S, R* M, R* L, hipStream_t st) {
                        T.cx.p, T.cy.p, T.cz.p, L);
   for (int lev = top.nlev; lev <= T.max_level; ++lev) {
     const int c0 = lo[lev], c1 = lo[lev + 1];
-    if (lev == 0) {  // a root level too large for the single workgroup does not occur (one cell)
+    if (lev == 0) {  // more roots than kTopCells (a batch of that many snapshots): unfold only
       hipLaunchKernelGGL((k_fmm_unfold<KIND, P, R>), dim3(ceil_div(c1 - c0, 64)), dim3(64), 0, st, c1, L);
       continue;
     }
