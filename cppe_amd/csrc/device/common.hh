@@ -63,6 +63,16 @@ constexpr float kTholeCutoff32 = 25.0f;
 // 8e-8 (2-norm, relative), below its own rounding error (2e-7).
 constexpr double kFarClass32 = 18.0;
 
+// Work unit of block b of nb: workgroups are dispatched to the device's 8 XCDs in turn (block b to
+// XCD b mod 8), and each XCD takes a contiguous range of units -- consecutive sites, whose
+// neighbours' records mostly lie in the same or an adjacent unit, then stay in one XCD's L2.
+// A bijection of [0, nb).
+__device__ __forceinline__ int xcd_chunk(int b, int nb) {
+  constexpr int kXcd = 8;
+  const int per = nb / kXcd, rem = nb % kXcd, x = b % kXcd;
+  return x * per + min(x, rem) + b / kXcd;
+}
+
 // block-wide sum (256 threads); the result is valid in thread 0
 __device__ __forceinline__ double block_sum(double v, double* red) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);

@@ -256,8 +256,9 @@ void fmm_reserve(System& S) {
 // sources, chunks of 16 / 12 / 8 targets in FP32 and 8 / 6 / 4 in FP64; the system's other
 // launches then use variant 8).
 // Then, for a staged near variant, the whole apply is timed with the far field in series, beside
-// the near field on a second stream, and with only M2L / L2L beside it after the upward pass; the
-// fastest is kept (the overlapped timing reported is the fastest overlapped form); each
+// the near field on a second stream (enqueued after or ahead of it), and with only M2L / L2L beside
+// it after the upward pass; the fastest is kept (the overlapped timing reported is the fastest
+// overlapped form); each
 // overlapped form is timed with the near field in 1, 2 and 4 launches.
 // ms receives the timings: per precision (FP64 then FP32) kNearCand near values, kM2lCand M2L
 // values, then the apply in series and overlapped; choice receives (near variant, near block, m2l
@@ -292,14 +293,14 @@ bool near_variant_allowed(System& S, int variant) {
   return variant != 2 || max_leaf_size(S) <= kMaxLeafVariant2;
 }
 
-// The overlap entry of a launch-shape choice: the far/near form (0, 1, 2) plus 4 log2 of the
-// near-field launches (1, 2 or 4; forms 1 and 2 only).
+// The overlap entry of a launch-shape choice: the far/near form (0 .. 3) plus 4 log2 of the
+// near-field launches (1, 2 or 4; overlapped forms only).
 int encode_overlap(int form, int splits) { return form + 4 * (splits >= 4 ? 2 : splits >= 2 ? 1 : 0); }
 bool decode_overlap(int code, int& form, int& splits) {
   form         = code & 3;
   const int sl = code >> 2;
   splits       = 1 << sl;
-  return code >= 0 && form <= 2 && sl <= 2 && (form > 0 || sl == 0);
+  return code >= 0 && form <= 3 && sl <= 2 && (form > 0 || sl == 0);
 }
 
 void fmm_autotune(System& S, int reps, int* choice, double* ms) {
@@ -363,10 +364,10 @@ void fmm_autotune(System& S, int reps, int* choice, double* ms) {
       m[kNNear + kNM2l]      = timed_total(fp32);
       m[kNNear + kNM2l + 1]  = 0.0;
       F.tune.near_splits[prec] = 1;
-      if (F.tune.near_variant[prec] >= 5) {  // overlapped: the fastest of the two forms x near splits
+      if (F.tune.near_variant[prec] >= 5) {  // overlapped: the fastest of the three forms x near splits
         int bo = 0, bs = 1;
         double bt = 0.0;
-        for (int form = 1; form <= 2; ++form)
+        for (int form = 1; form <= 3; ++form)
           for (int sp = 1; sp <= 4; sp *= 2) {
             F.tune.overlap[prec]     = form;
             F.tune.near_splits[prec] = sp;

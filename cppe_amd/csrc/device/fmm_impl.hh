@@ -46,7 +46,8 @@ struct FmmTune {
   int near_block[2]   = {256, 256};
   int m2l_group[2]    = {64, 64};  // lanes per target cell in M2L (16, 32 or 64)
   int overlap[2]      = {0, 0};    // far field on a second stream beside the near field (staged variants):
-                                   // 0 none, 1 the whole far field, 2 M2L / L2L after the upward pass
+                                   // 0 none, 1 the whole far field, 2 M2L / L2L after the upward pass,
+                                   // 3 the whole far field enqueued ahead of the near field
   int near_splits[2]  = {1, 1};    // overlapped forms: the near field in 1, 2 or 4 launches, so the
                                    // far field's kernels get the device between them
 };
@@ -1283,7 +1284,8 @@ __global__ __launch_bounds__(256) void k_fmm_l2p_add(int n, const int* __restric
                                                      const R* __restrict__ L, double* __restrict__ out,
                                                      size_t lstride = 0, L2pDots d = {}) {
   using O     = Ops<KIND_M1, P, R>;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = xcd_chunk(blockIdx.x, gridDim.x);  // the CG chunk of the same 256 sites
+  const int i = b * blockDim.x + threadIdx.x;
   L += blockIdx.y * lstride;
   out += 3 * (size_t)n * blockIdx.y;
   double s = 0.0, q = 0.0;
@@ -1307,8 +1309,8 @@ __global__ __launch_bounds__(256) void k_fmm_l2p_add(int n, const int* __restric
     s = block_sum(s, red);
     q = block_sum(q, red);
     if (threadIdx.x == 0) {
-      d.part[2 * (size_t)d.nchunk + blockIdx.x] = s;
-      d.part[3 * (size_t)d.nchunk + blockIdx.x] = q;
+      d.part[2 * (size_t)d.nchunk + b] = s;
+      d.part[3 * (size_t)d.nchunk + b] = q;
     }
   }
 }
@@ -1912,7 +1914,8 @@ void fmm_near_pass(System& S, const double* mu, double damping, const R* L, doub
 // starts together with the (latency-bound) upward pass instead of behind it. Overlap 2: the
 // upward pass first, alone on the system's stream, then only M2L / L2L beside the near field --
 // the near field's workgroups otherwise occupy the device while the upward pass's small launches
-// wait for slots, and M2L then ends well after the near field.
+// wait for slots, and M2L then ends well after the near field. Overlap 3: as 1, but the far pass
+// enqueued ahead of the near field, so that the upward pass's workgroups are dispatched first.
 template <int P, typename R>
 void fmm_apply_b_prec(System& S, const double* x, double* out, R* M, R* L) {
   FmmState& F      = *S.fmm;
@@ -1927,10 +1930,11 @@ void fmm_apply_b_prec(System& S, const double* x, double* out, R* M, R* L) {
   if (ovl == 2) fmm_upward_pass<KIND_M1, P, R>(S, x, M, S.stream);
   CPPE_HIP_CHECK(hipEventRecord(F.ev_fork, S.stream));
   CPPE_HIP_CHECK(hipStreamWaitEvent(F.far_stream, F.ev_fork, 0));
+  if (ovl == 3) fmm_far_pass<KIND_M1, P, R>(S, x, M, L, F.far_stream);
   fmm_near_pass<P, R, MODE_BAPPLY>(S, x, dmp, L, out, false);
   if (ovl == 2)
     fmm_down_pass<KIND_M1, P, R>(S, M, L, F.far_stream);
-  else
+  else if (ovl == 1)
     fmm_far_pass<KIND_M1, P, R>(S, x, M, L, F.far_stream);
   CPPE_HIP_CHECK(hipEventRecord(F.ev_join, F.far_stream));
   CPPE_HIP_CHECK(hipStreamWaitEvent(S.stream, F.ev_join, 0));
@@ -1980,6 +1984,7 @@ void fmm_apply_b_multi_prec(System& S, const double* x, double* out, R* M, R* L,
       for (int k = 0; k < K; ++k) fmm_upward_pass<KIND_M1, P, R>(S, x + k * n3, M + k * ms, S.stream);
     CPPE_HIP_CHECK(hipEventRecord(F.ev_fork, S.stream));
     CPPE_HIP_CHECK(hipStreamWaitEvent(F.far_stream, F.ev_fork, 0));
+    if (ovl == 3) far();  // ahead of the near field
   } else {
     far();  // the fused L2P of the near-field kernel reads the local expansions
   }
@@ -1994,7 +1999,7 @@ void fmm_apply_b_multi_prec(System& S, const double* x, double* out, R* M, R* L,
   else
     launch_near<P, R, MODE_BAPPLY, false, MU4Of<R>, K>(S, a, mu4, L, !ovl, ls);
   if (ovl) {
-    far();  // enqueued after the near field so that both start together (see fmm_apply_b_prec)
+    if (ovl != 3) far();  // enqueued after the near field so that both start together (see fmm_apply_b_prec)
     CPPE_HIP_CHECK(hipEventRecord(F.ev_join, F.far_stream));
     CPPE_HIP_CHECK(hipStreamWaitEvent(S.stream, F.ev_join, 0));
     hipLaunchKernelGGL((k_fmm_l2p_add<P, R>), dim3(ceil_div(S.n, 256), K), dim3(256), 0, S.stream, S.n, T.leaf_of.p,

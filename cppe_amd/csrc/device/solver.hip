@@ -189,7 +189,7 @@ struct ChunkIdx {
 // r = rhs - t ; z = alpha r ; p = z ; partials (r.z, r.r)
 __global__ __launch_bounds__(256) void k_sg_init(CgVecs v) {
   __shared__ double red[4];
-  const int c = blockIdx.x, n = v.na;
+  const int c = xcd_chunk(blockIdx.x, v.nchunk), n = v.na;
   const ChunkIdx ci(v, c);
   double rz = 0.0, rr = 0.0;
   for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x) {
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void k_sg_init(CgVecs v) {
 __global__ __launch_bounds__(256) void k_sg_pap(CgVecs v) {
   __shared__ double red[4];
   if (step_halted(v.status)) return;
-  const int c = blockIdx.x, n = v.na;
+  const int c = xcd_chunk(blockIdx.x, v.nchunk), n = v.na;
   const ChunkIdx ci(v, c);
   double s = 0.0, q = 0.0;
   if (v.si[I_ACTIVE * v.nseg + v.cseg[c]])
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kChunk) void k_sg_xrz(CgVecs v) {
   __shared__ double red[4];
   __shared__ double alpha;
   if (step_halted(v.status)) return;
-  const int c = blockIdx.x, n = v.na, g = v.cseg[c];
+  const int c = xcd_chunk(blockIdx.x, v.nchunk), n = v.na, g = v.cseg[c];
   const ChunkIdx ci(v, c);
   const bool active = v.si[I_ACTIVE * v.nseg + g];
   // one site per thread (chunks hold at most kChunk sites): its vectors are loaded before alpha
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(kChunk) void k_sg_xrz(CgVecs v) {
 // confirmations.
 __global__ __launch_bounds__(256) void k_sg_resid(CgVecs v) {
   __shared__ double red[4];
-  const int c = blockIdx.x, n = v.na, g = v.cseg[c];
+  const int c = xcd_chunk(blockIdx.x, v.nchunk), n = v.na, g = v.cseg[c];
   const ChunkIdx ci(v, c);
   double s = 0.0;
   if (v.si[I_NEEDREL * v.nseg + g] || v.si[I_PENDING * v.nseg + g] || v.si[I_ELIGIBLE * v.nseg + g])
@@ -341,7 +341,7 @@ __device__ __forceinline__ bool conf_seg(const CgVecs& v, int g) {
 
 // dv = x - xrel for pending segments with a stored true residual, 0 elsewhere
 __global__ __launch_bounds__(256) void k_sg_defect(CgVecs v) {
-  const int c = blockIdx.x, n = v.na;
+  const int c = xcd_chunk(blockIdx.x, v.nchunk), n = v.na;
   const ChunkIdx ci(v, c);
   const bool on = conf_seg(v, v.cseg[c]);
   for (int i = v.clo[c] + threadIdx.x; i < v.clo[c + 1]; i += blockDim.x)
@@ -355,7 +355,7 @@ __global__ __launch_bounds__(256) void k_sg_defect(CgVecs v) {
 // partials of ||rrel - t||^2 (part[c]) and ||t||^2 (part[nchunk + c]), t = B32 dv, for the same segments
 __global__ __launch_bounds__(256) void k_sg_confirm(CgVecs v) {
   __shared__ double red[4];
-  const int c = blockIdx.x, n = v.na;
+  const int c = xcd_chunk(blockIdx.x, v.nchunk), n = v.na;
   const ChunkIdx ci(v, c);
   double s1 = 0.0, s2 = 0.0;
   if (conf_seg(v, v.cseg[c]))
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(256) void k_sg_confirm(CgVecs v) {
 __global__ __launch_bounds__(256) void k_sg_z(CgVecs v) {
   __shared__ double red[4];
   if (v.status[ST_REL] > 0) return;
-  const int c = blockIdx.x, n = v.na;
+  const int c = xcd_chunk(blockIdx.x, v.nchunk), n = v.na;
   const ChunkIdx ci(v, c);
   double s = 0.0;
   if (v.si[I_ACTIVE * v.nseg + v.cseg[c]])
@@ -433,7 +433,7 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_sg_prec(CgVecs v, int slot) {
   __shared__ double red[4];
   if ((MODE == 1 && step_halted(v.status)) || (MODE == 2 && v.status[ST_REL] > 0)) return;
-  const int c = blockIdx.x, n = v.na;
+  const int c = xcd_chunk(blockIdx.x, v.nchunk), n = v.na;
   const ChunkIdx ci(v, c);
   double s = 0.0;
   if (MODE == 0 || v.si[I_ACTIVE * v.nseg + v.cseg[c]])
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(256) void k_sg_prec(CgVecs v, int slot) {
 // p = z + beta p (active segments)
 __global__ __launch_bounds__(256) void k_sg_p(CgVecs v) {
   if (v.status[ST_REL] > 0) return;
-  const int c = blockIdx.x, n = v.na, g = v.cseg[c];
+  const int c = xcd_chunk(blockIdx.x, v.nchunk), n = v.na, g = v.cseg[c];
   if (!v.si[I_ACTIVE * v.nseg + g]) return;
   const ChunkIdx ci(v, c);
   const double be = v.sd[D_BETA * v.nseg + g];

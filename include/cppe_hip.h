@@ -259,13 +259,15 @@ int cppe_hip_last_solve_info(cppe_hip_system* sys, int* info);
  *      64/128/256. Single-right-hand-side B applies only: the system's other near-field launches
  *      then use variant 8.
  * Then, for a staged or symmetric near variant (5-11), the whole apply is timed with the far field (P2M, M2M,
- * M2L, L2L) in series, on a second stream beside the near field (joined before an L2P pass), and
- * with only M2L / L2L beside the near field after the upward pass (P2M, M2M) has run, each
- * overlapped form with the near field in 1, 2 and 4 launches; the fastest is kept.
+ * M2L, L2L) in series, on a second stream beside the near field (joined before an L2P pass; enqueued
+ * after the near field or ahead of it), and with only M2L / L2L beside the near field after the
+ * upward pass (P2M, M2M) has run, each overlapped form with the near field in 1, 2 and 4 launches;
+ * the fastest is kept.
  * choice[8]: (near variant, near block, m2l group, overlap) for FP64 then FP32; overlap = form +
  *            4 log2(near launches): form 0 = series, 1 = whole far field beside the near field,
- *            2 = upward pass first; with forms 1 and 2 the near field may be issued as 1, 2 or 4
- *            launches of consecutive work items (so 0, 1, 2, 5, 6, 9, 10 are valid).
+ *            2 = upward pass first, 3 = as 1 with the far field enqueued ahead of the near field;
+ *            with forms 1-3 the near field may be issued as 1, 2 or 4 launches of consecutive work
+ *            items (so 0-3, 5-7, 9-11 are valid).
  * ms[78]: per precision, the 34 near timings (order above), the 3 M2L timings (ms/launch), then
  *         the apply in series and overlapped (the faster overlapped form; 0 when not timed).
  * Launch shapes never change results beyond FP summation order.

@@ -91,7 +91,7 @@ def test_multi_rejects_batches():
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("damping", [DAMP, 0.0])
-@pytest.mark.parametrize("overlap", [1, 2])
+@pytest.mark.parametrize("overlap", [1, 2, 3])
 def test_apply_multi_matches_single_applies(precision, damping, overlap):
     """B applied to 3 and 4 vectors together equals one apply per vector (same launch shape)."""
     w = box(1500, 7304)

@@ -69,7 +69,7 @@ def test_bitwise_reproducible_every_form(precision):
     for v in SYMS:
         S.set_tune(tune(v))
         y0 = S.apply_b(x)
-        for o in (0, 1, 2, 5, 6, 9, 10):
+        for o in (0, 1, 2, 3, 5, 6, 7, 9, 10, 11):
             for block in (64, 256):
                 say("sym form", precision, v, o, block)
                 S.set_tune(tune(v, o, block))

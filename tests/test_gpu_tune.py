@@ -57,7 +57,7 @@ def test_autotune_then_solve():
     for prec in ("fp64", "fp32"):
         r = rep[prec]
         assert (r["near_variant"], r["near_block"]) in capi.System.NEAR_CANDIDATES
-        assert r["m2l_group"] in capi.System.M2L_CANDIDATES and r["overlap"] in (0, 1, 2)
+        assert r["m2l_group"] in capi.System.M2L_CANDIDATES and r["overlap"] in (0, 1, 2, 3)
         assert r["near_launches"] in (1, 2, 4) and (r["overlap"] > 0 or r["near_launches"] == 1)
         assert all(t > 0 for t in r["near_ms"].values()) and all(t > 0 for t in r["m2l_ms"].values())
     it, norms, _ = S.solve_resident(0, 1e-8, 100)
@@ -68,9 +68,10 @@ def test_autotune_then_solve():
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_solve_with_each_overlap_form(precision):
-    """Graph-captured solves with the far field beside the near field (overlap 1) and with the
-    upward pass first (overlap 2), each with the near field in 1, 2 and 4 launches (codes + 4, + 8),
-    stop on the FP64 residual with the series form's dipoles."""
+    """Graph-captured solves with the far field beside the near field (overlap 1), with the
+    upward pass first (overlap 2) and with the far field enqueued ahead of the near field (overlap 3),
+    each with the near field in 1, 2 and 4 launches (codes + 4, + 8), stop on the FP64 residual with
+    the series form's dipoles."""
     w, S = system(precision)
     Fn = capi.nuclear_fields(w.pos, w.atoms_z.astype(float), w.atoms_pos)
     F = Fn + capi.System(w.pos, excl=w.excl_csr(), summation="fmm").multipole_fields(w.charges, w.dipoles,
@@ -79,7 +80,7 @@ def test_solve_with_each_overlap_form(precision):
     S.set_tune(np.array([8, 256, 16, 0, 8, 256, 16, 0], dtype=np.int32))
     S.solve_resident(0, 1e-8, 100)
     mu0 = S.download_mu()
-    for o in (1, 2, 5, 6, 9, 10):
+    for o in (1, 2, 3, 5, 6, 7, 9, 10, 11):
         say("overlap", o, precision)
         S.set_tune(np.array([8, 256, 16, o, 8, 256, 16, o], dtype=np.int32))
         it, norms, _ = S.solve_resident(0, 1e-8, 100)
@@ -94,7 +95,7 @@ def test_set_tune_rejects_unknown_shapes():
         S.set_tune([12, 256, 64, 0, 0, 256, 64, 0])
     with pytest.raises(capi.CppeHipError):
         S.set_tune([0, 96, 64, 0, 0, 256, 64, 0])
-    for bad_overlap in (3, 4, 7, 12, 13, -1):  # form 3, split series form, 8 launches, ...
+    for bad_overlap in (4, 8, 12, 13, 16, -1):  # split series form, 8 launches, ...
         with pytest.raises(capi.CppeHipError):
             S.set_tune([5, 256, 64, bad_overlap, 0, 256, 64, 0])
 
