@@ -95,43 +95,43 @@ void fill_f64(System& S, double* a, size_t n, double v) {
   if (n) hipLaunchKernelGGL(k_fill_f64, dim3(ceil_div((long long)n, 256)), dim3(256), 0, S.stream, n, a, v);
 }
 
-// Lane l's share of the chunk partials part[lo, hi): chunks l, l + 64, l + 128, ... summed into
-// four interleaved accumulators (a fixed order, four loads in flight), combined as
-// (a0 + a1) + (a2 + a3). The 64 lane shares are then combined by an xor-shuffle tree.
-__device__ __forceinline__ double lane_chunk_sum(const double* __restrict__ part, int lo, int hi, int lane) {
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  int c = lo + lane;
-  for (; c + 192 < hi; c += 256) {
-    a0 += part[c];
-    a1 += part[c + 64];
-    a2 += part[c + 128];
-    a3 += part[c + 192];
+// Lane l's share of the chunk partials [lo, hi) of NA arrays: its chunks c_m = lo + l + 64 m (m < M)
+// summed into four interleaved accumulators, a_(m mod 4) for m < 4 floor(M / 4) and a_0 for the
+// rest, each in increasing m, combined as (a0 + a1) + (a2 + a3) -- one fixed order. The loads of
+// B chunks of every array are issued before any sum, with addresses clamped into [lo, hi) and the
+// values past a lane's last chunk replaced by 0 (no branch between the loads), so a segment of up
+// to 64 B chunks waits for memory once. The 64 lane shares are then combined by an xor-shuffle
+// tree.
+template <int NA, int B>
+__device__ __forceinline__ void lane_chunk_sums(const double* const (&pa)[NA], int lo, int hi, int lane,
+                                                double (&res)[NA]) {
+  static_assert(B % 4 == 0, "batches start at a multiple of 4");
+  const int M  = hi - lo > lane ? (hi - lo - lane + 63) / 64 : 0;
+  const int M4 = M & ~3;
+  const int Mu = (hi - lo + 63) / 64;  // lane 0's count: the same trip count for every lane
+  double a[NA][4];
+#pragma unroll
+  for (int k = 0; k < NA; ++k) a[k][0] = a[k][1] = a[k][2] = a[k][3] = 0.0;
+  for (int m0 = 0; m0 < Mu; m0 += B) {
+    double x[NA][B];
+#pragma unroll
+    for (int t = 0; t < B; ++t) {
+      const int c = min(lo + lane + 64 * (m0 + t), hi - 1);
+#pragma unroll
+      for (int k = 0; k < NA; ++k) x[k][t] = pa[k][c];
+    }
+#pragma unroll
+    for (int t = 0; t < B; ++t) {
+      const int m = m0 + t;
+#pragma unroll
+      for (int k = 0; k < NA; ++k) {
+        a[k][t & 3] += m < M4 ? x[k][t] : 0.0;
+        a[k][0] += m >= M4 && m < M ? x[k][t] : 0.0;
+      }
+    }
   }
-  for (; c < hi; c += 64) a0 += part[c];
-  return (a0 + a1) + (a2 + a3);
-}
-
-// lane_chunk_sum of two partial arrays at once (the same order for each; twice the loads in flight)
-__device__ __forceinline__ void lane_chunk_sum2(const double* __restrict__ pa, const double* __restrict__ pb, int lo,
-                                                int hi, int lane, double& ra, double& rb) {
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
-  int c = lo + lane;
-  for (; c + 192 < hi; c += 256) {
-    a0 += pa[c];
-    a1 += pa[c + 64];
-    a2 += pa[c + 128];
-    a3 += pa[c + 192];
-    b0 += pb[c];
-    b1 += pb[c + 64];
-    b2 += pb[c + 128];
-    b3 += pb[c + 192];
-  }
-  for (; c < hi; c += 64) {
-    a0 += pa[c];
-    b0 += pb[c];
-  }
-  ra = (a0 + a1) + (a2 + a3);
-  rb = (b0 + b1) + (b2 + b3);
+#pragma unroll
+  for (int k = 0; k < NA; ++k) res[k] = (a[k][0] + a[k][1]) + (a[k][2] + a[k][3]);
 }
 
 __device__ __forceinline__ void z_of(const double* __restrict__ a6, int n, int i, double r0, double r1, double r2,
@@ -270,10 +270,13 @@ __global__ __launch_bounds__(kChunk) void k_sg_xrz(CgVecs v) {
     aq[q] = site ? v.Ap[k] : 0.0;
   }
   // alpha = (r.z) / (p.Ap) of the segment, from k_sg_pap's partials: every block sums them in one
-  // fixed order (lane_chunk_sum, then an xor-shuffle tree), so all blocks of a segment take the
+  // fixed order (lane_chunk_sums, then an xor-shuffle tree), so all blocks of a segment take the
   // same alpha, without a launch of its own
   if (active && threadIdx.x < 64) {
-    double s1 = lane_chunk_sum(v.part + 2 * (size_t)v.nchunk, v.seg_chunk[g], v.seg_chunk[g + 1], threadIdx.x);
+    const double* const pa[1] = {v.part + 2 * (size_t)v.nchunk};
+    double sum[1];
+    lane_chunk_sums<1, 8>(pa, v.seg_chunk[g], v.seg_chunk[g + 1], threadIdx.x, sum);
+    double s1 = sum[0];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s1 += __shfl_xor(s1, off);
     if (threadIdx.x == 0) alpha = v.sd[D_RZ * v.nseg + g] / s1;
@@ -502,22 +505,35 @@ struct CgFinal {
 };
 
 // Per-segment finalisation of the chunk partials: one wavefront per segment (the block's four
-// wavefronts take segments in turn); each lane sums its chunks (lane_chunk_sum), then a fixed
+// wavefronts take segments in turn); each lane sums its chunks (lane_chunk_sums), then a fixed
 // shuffle tree combines the lanes -- deterministic for a given chunk split. Lane 0 then applies
 // the segment's decision (cg_policy.hh); the status block is refreshed at the end.
 // F_INIT with params[P_RESTART] set: the plain-form restart of unfinished segments after the
 // mixed form stagnated (iteration count and history kept). Every mode also publishes the launch
 // horizon ST_SAFE: the smallest number of steps before a segment asks for a replacement, capped by
 // the largest number before every segment has left the active set.
+// Up to kStageSeg segments, the segment tables are copied into LDS first (all threads, one load
+// each) and written back at the end: the decisions read and write them one field after another.
+constexpr int kStageSeg = 256;
 template <int MODE, bool MIXED>
 __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
   __shared__ int cnt[7];
+  __shared__ double stage_d[D_COUNT * kStageSeg];
+  __shared__ int stage_i[I_COUNT * kStageSeg];
+  const int seq0 = f.status[ST_SEQ];
   // F_BETA while a reliable update is pending and F_RNB of a halted step: no update, only the
   // publication below
   const bool skip = (MODE == F_BETA && f.status[ST_REL] > 0) || (MODE == F_RNB && step_halted(f.status));
   if (threadIdx.x < 7) cnt[threadIdx.x] = threadIdx.x == 5 ? kFarSteps : 0;
+  const int ns      = skip ? 0 : f.nseg;
+  const bool staged = ns <= kStageSeg;
+  double* const sd  = staged ? stage_d : f.sd;
+  int* const si     = staged ? stage_i : f.si;
+  if (staged) {
+    for (int k = threadIdx.x; k < D_COUNT * ns; k += blockDim.x) stage_d[k] = f.sd[k];
+    for (int k = threadIdx.x; k < I_COUNT * ns; k += blockDim.x) stage_i[k] = f.si[k];
+  }
   __syncthreads();
-  const int ns = skip ? 0 : f.nseg;
   const bool restart = MODE == F_INIT && f.params[P_RESTART] != 0.0;
   // iterations so far; a restart also counts the step taken before the stagnating replacement
   const int it = f.status[ST_IT] + (restart ? f.status[ST_STEPPED] : 0);
@@ -531,13 +547,24 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
     constexpr bool TWO = MODE == F_INIT || MODE == F_RNB || MODE == F_CONF;
     // mixed steps with the gap bound also need p.Ap and Ap.Ap of the step (k_sg_pap's partials)
     const bool gap = MIXED && MODE == F_RNB && pr.eta_gap > 0.0;
-    if (TWO)
-      lane_chunk_sum2(f.part, f.part + f.nchunk, f.seg_chunk[g], f.seg_chunk[g + 1], lane, s1, s2);
-    else
-      s1 = lane_chunk_sum(f.part, f.seg_chunk[g], f.seg_chunk[g + 1], lane);
-    if (gap)
-      lane_chunk_sum2(f.part + 2 * (size_t)f.nchunk, f.part + 3 * (size_t)f.nchunk, f.seg_chunk[g],
-                      f.seg_chunk[g + 1], lane, s3, s4);
+    const int lo = f.seg_chunk[g], hi = f.seg_chunk[g + 1];
+    const size_t nc = f.nchunk;
+    if (gap) {  // (a mixed F_RNB step: TWO holds)
+      const double* const pa[4] = {f.part, f.part + nc, f.part + 2 * nc, f.part + 3 * nc};
+      double sum[4];
+      lane_chunk_sums<4, 16>(pa, lo, hi, lane, sum);
+      s1 = sum[0], s2 = sum[1], s3 = sum[2], s4 = sum[3];
+    } else if (TWO) {
+      const double* const pa[2] = {f.part, f.part + nc};
+      double sum[2];
+      lane_chunk_sums<2, 16>(pa, lo, hi, lane, sum);
+      s1 = sum[0], s2 = sum[1];
+    } else {
+      const double* const pa[1] = {f.part};
+      double sum[1];
+      lane_chunk_sums<1, 16>(pa, lo, hi, lane, sum);
+      s1 = sum[0];
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       s1 += __shfl_xor(s1, off);
@@ -549,7 +576,7 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
     }
     if (lane != 0) continue;
 
-    const SegRef sg{f.sd, f.si, ns, g};
+    const SegRef sg{sd, si, ns, g};
     if (MODE == F_INIT) {
       if (restart)
         seg_restart(sg, s1, s2, it, pr);
@@ -586,7 +613,7 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
   const bool beta_now = MODE == F_RNB && cnt[1] == 0;  // no reliable update pending: direction now
   if (beta_now && lane == 0)
     for (int g = wv; g < ns; g += 4) {
-      const SegRef sg{f.sd, f.si, ns, g};
+      const SegRef sg{sd, si, ns, g};
       seg_beta(sg, sg.d(D_RZN));
     }
   if (threadIdx.x == 0 && !skip) {
@@ -610,8 +637,13 @@ __global__ __launch_bounds__(256) void k_sg_final(CgFinal f) {
       f.status[ST_STEPPED] = 0;
     }
   }
+  if (staged) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < D_COUNT * ns; k += blockDim.x) f.sd[k] = stage_d[k];
+    for (int k = threadIdx.x; k < I_COUNT * ns; k += blockDim.x) f.si[k] = stage_i[k];
+  }
   if (threadIdx.x == 0) {  // publish: the fields, a system-scope fence, then the count
-    const int seq    = f.status[ST_SEQ] + 1;
+    const int seq    = seq0 + 1;
     f.status[ST_SEQ] = seq;
     if (f.publish) {
       for (int q = 0; q < ST_SEQ; ++q) f.hstat[q] = f.status[q];
