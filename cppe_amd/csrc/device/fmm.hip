@@ -227,6 +227,7 @@ void fmm_reserve(System& S) {
   ensure_buf(F.mu4f, S.n);
   ensure_pos4(S);
   ensure_posf4(S);
+  ensure_excl_near(S);
   ensure_m2l_rec(S, S.stream);  // the synchronisation below orders it before any stream's M2L
   ensure_near_items(S, 0);
   for (int prec = 0; prec < 2; ++prec) {

@@ -79,6 +79,9 @@ struct FmmState {
   dbuf<double> Mc, Lc;    // combined kind (static fields, m = 0 / 2)
   dbuf<double> src10;     // staging of combined sources
   dbuf<double4> pos4;     // near-field source records: x, y, z, alpha_iso^(-1/6)
+  // per site, the exclusion partners whose pairs the near field adds (CSR, built once per tree)
+  dbuf<int> xn_off, xn_idx;
+  bool xn_built = false;
   dbuf<double4> mu4;      // near-field dipole records (FP64 form)
   dbuf<float4> mu4f;      // near-field dipole records (FP32 form)
   // several right-hand sides per apply (kMultiRhs at a time): their expansions and dipole records
@@ -1205,11 +1208,49 @@ __global__ void k_pack_mu4(int n, const double* __restrict__ mu, MU4* __restrict
   out[i] = v;
 }
 
+// Whether exclusion partner k of site i is a pair the near field adds: a valid index other than
+// i, its first occurrence in i's list, and its leaf in the near list of i's leaf A (o0, o1: A's
+// list, sorted by source leaf; otherwise a far-field pair, which stays in).
+__device__ __forceinline__ bool excl_in_near(const NearArgs& a, int i, int k0, int k, int o0, int o1) {
+  const int e = a.eidx[k];
+  if (e < 0 || e >= a.n || e == i) return false;
+  for (int q = k0; q < k; ++q)
+    if (a.eidx[q] == e) return false;
+  const int B = a.leaf_of[e];
+  int lo = o0, hi = o1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a.p2p_src[mid] < B)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < o1 && a.p2p_src[lo] == B;
+}
+
+// The partners excl_in_near accepts, per site in list order: their number into out[i] (off
+// null), or the partners themselves into out[off[i] ...].
+static __global__ __launch_bounds__(256) void k_excl_near(NearArgs a, const int* __restrict__ off,
+                                                         int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int k0 = a.eoff[i], k1 = a.eoff[i + 1];
+  const int A = a.leaf_of[i], o0 = a.p2p_off[A], o1 = a.p2p_off[A + 1];
+  int m = 0;
+  for (int k = k0; k < k1; ++k) {
+    if (!excl_in_near(a, i, k0, k, o0, o1)) continue;
+    if (off) out[off[i] + m] = a.eidx[k];
+    ++m;
+  }
+  if (!off) out[i] = m;
+}
+
 // Takes the excluded near-field pairs out of a near-field sum that included them: for each site
 // i and each distinct exclusion partner e != i whose leaf is in the near list of i's leaf, out[i]
 // -= pair(i, e), evaluated as the staged kernel does (same record, same arithmetic; FP32: offsets
-// from i's leaf centre). K right-hand sides as in the staged kernel.
-template <typename R, bool DAMP, typename MU4, bool B2 = false, int K = 1>
+// from i's leaf centre). K right-hand sides as in the staged kernel. LISTED: a.eoff / a.eidx hold
+// those partners already (FmmState::xn_off / xn_idx), so no partner is tested here.
+template <typename R, bool DAMP, typename MU4, bool B2 = false, int K = 1, bool LISTED = false>
 __global__ __launch_bounds__(256) void k_fmm_excl_sub(NearArgs a, const MU4* __restrict__ mu4) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
@@ -1217,7 +1258,7 @@ __global__ __launch_bounds__(256) void k_fmm_excl_sub(NearArgs a, const MU4* __r
   if (k0 == k1) return;
   const int n      = a.n;
   const int A      = a.leaf_of[i];
-  const int o0 = a.p2p_off[A], o1 = a.p2p_off[A + 1];
+  const int o0 = LISTED ? 0 : a.p2p_off[A], o1 = LISTED ? 0 : a.p2p_off[A + 1];
   const double4 pi = a.pos4[i];
   const double cax = a.cx[A], cay = a.cy[A], caz = a.cz[A];
   const R xi = sizeof(R) == 8 ? (R)pi.x : (R)(pi.x - cax);
@@ -1228,21 +1269,8 @@ __global__ __launch_bounds__(256) void k_fmm_excl_sub(NearArgs a, const MU4* __r
 #pragma unroll
   for (int c = 0; c < 3 * K; ++c) f[c] = R(0);
   for (int k = k0; k < k1; ++k) {
-    const int e = a.eidx[k];
-    if (e < 0 || e >= n || e == i) continue;
-    bool dup = false;
-    for (int q = k0; q < k; ++q) dup |= (a.eidx[q] == e);
-    if (dup) continue;
-    const int B = a.leaf_of[e];
-    int lo = o0, hi = o1;  // near list of A is sorted by source leaf
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (a.p2p_src[mid] < B)
-        lo = mid + 1;
-      else
-        hi = mid;
-    }
-    if (lo == o1 || a.p2p_src[lo] != B) continue;  // a far-field pair: it stays in
+    if (!LISTED && !excl_in_near(a, i, k0, k, o0, o1)) continue;
+    const int e      = a.eidx[k];
     const double4 pj = a.pos4[e];
     StagedRec<R, K> r;
     r.x = sizeof(R) == 8 ? (R)pj.x : (R)(pj.x - cax);
@@ -1541,6 +1569,55 @@ inline void ensure_posf4(System& S) {
                      T.leaf_of.p, T.cx.p, T.cy.p, T.cz.p, F.posf4.p);
 }
 
+// The exclusion partners whose pairs the near field adds (k_excl_near), per site, once per tree;
+// nothing when the system has no exclusions. Allocates: the CG reserves it before any capture.
+inline void ensure_excl_near(System& S) {
+  FmmState& F = *S.fmm;
+  if (F.xn_built || !S.excl_off.p) return;
+  const Octree& T = S.tree;
+  NearArgs a{};
+  a.leaf_of = T.leaf_of.p;
+  a.p2p_off = T.p2p_off.p;
+  a.p2p_src = T.p2p_src.p;
+  a.eoff    = S.excl_off.p;
+  a.eidx    = S.excl_idx.p;
+  a.n       = S.n;
+  const dim3 g(ceil_div(S.n, 256)), b(256);
+  F.xn_off.alloc((size_t)S.n + 1);
+  hipLaunchKernelGGL(k_excl_near, g, b, 0, S.stream, a, nullptr, F.xn_off.p);
+  std::vector<int> off((size_t)S.n + 1, 0);
+  CPPE_HIP_CHECK(hipMemcpyAsync(off.data(), F.xn_off.p, sizeof(int) * S.n, hipMemcpyDeviceToHost, S.stream));
+  CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
+  int tot = 0;
+  for (int i = 0; i < S.n; ++i) {
+    const int c = off[i];
+    off[i]      = tot;
+    tot += c;
+  }
+  off[S.n] = tot;
+  CPPE_HIP_CHECK(hipMemcpyAsync(F.xn_off.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, S.stream));
+  F.xn_idx.alloc((size_t)std::max(tot, 1));
+  hipLaunchKernelGGL(k_excl_near, g, b, 0, S.stream, a, F.xn_off.p, F.xn_idx.p);
+  CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
+  F.xn_built = true;
+}
+
+// k_fmm_excl_sub over the listed partners (ensure_excl_near)
+template <typename R, bool DAMP, typename MU4, int K>
+void launch_excl_sub(System& S, const NearArgs& a, const MU4* mu4, bool b2, hipStream_t st) {
+  if (!a.eoff) return;
+  ensure_excl_near(S);
+  NearArgs ax             = a;
+  ax.eoff                 = S.fmm->xn_off.p;
+  ax.eidx                 = S.fmm->xn_idx.p;
+  constexpr bool kB2      = sizeof(R) == 4 && DAMP;
+  const dim3 g(ceil_div(S.n, 256)), b(256);
+  if (b2)
+    hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, kB2, K, true>), g, b, 0, st, ax, mu4);
+  else
+    hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, false, K, true>), g, b, 0, st, ax, mu4);
+}
+
 void ensure_clusters(System& S);  // fmm.hip
 
 // targets per work item of the symmetric near-field variants 9, 10, 11: FP64 8, 6, 4 and FP32 16,
@@ -1692,13 +1769,7 @@ void launch_direct_sym(System& S, const NearArgs& a, const MU4* mu4, const R* L,
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[7], st));
   hipLaunchKernelGGL(k_direct_sym_gather<R>, dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a, D.tb, D.fwd_off.p, fwd,
                      rev);
-  if (a.eoff) {
-    if (b2)
-      hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, kB2, 1>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a, mu4);
-    else
-      hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, false, 1>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a,
-                         mu4);
-  }
+  launch_excl_sub<R, DAMP, MU4, 1>(S, a, mu4, b2, st);
   if (with_l2p) launch_l2p_add<P, R>(S, L, a.out, st);
 }
 
@@ -1737,13 +1808,7 @@ void launch_near_sym(System& S, const NearArgs& a, const MU4* mu4, const R* L, b
     }
   }
   if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[7], st));
-  if (a.eoff) {
-    if (b2)
-      hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, kB2, 1>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a, mu4);
-    else
-      hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, false, 1>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a,
-                         mu4);
-  }
+  launch_excl_sub<R, DAMP, MU4, 1>(S, a, mu4, b2, st);
   launch_sym_gather<P, R>(S, Y, nullptr, a.out);
   if (with_l2p) launch_l2p_add<P, R>(S, L, a.out, st);
 }
@@ -1831,14 +1896,7 @@ void launch_near(System& S, const NearArgs& a, const MU4* mu4, const R* L, bool 
     else
       go1(std::false_type{});
     if (S.prof) CPPE_HIP_CHECK(hipEventRecord(S.prof[7], st));
-    if (a.eoff) {
-      if (b2)
-        hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, kB2, K>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a,
-                           mu4);
-      else
-        hipLaunchKernelGGL((k_fmm_excl_sub<R, DAMP, MU4, false, K>), dim3(ceil_div(S.n, 256)), dim3(256), 0, st, a,
-                           mu4);
-    }
+    launch_excl_sub<R, DAMP, MU4, K>(S, a, mu4, b2, st);
   } else if constexpr (K > 1) {
     throw hip_error("several right-hand sides per launch need a staged near-field variant");
   } else if (var == 4) {
