@@ -149,3 +149,27 @@ def test_training_and_guess_gain(objective):
     assert it1 <= it0
     B = O.BSystem(test.pos, test.pol, test.excl_csr(), damping=2.1304)
     assert np.linalg.norm(F - B.apply(mu)) < 1e-8
+
+
+def test_untimed_guess_equals_timed():
+    """cppe_hip_ml_guess with device_ms NULL only enqueues the guess on the system's stream: the
+    moments it leaves equal the timed call's bit for bit, and a resident solve right after it (no
+    host synchronisation in between) starts from them and converges."""
+    p, mean, istd = _random_model(7)
+    M = capi.Mlp(p, mean, istd)
+    w = synthetic.water_box(400, 3101, anisotropic=True)
+    F = rhs_of(w)
+    S = capi.System(w.pos, w.pol, w.alpha_iso(), w.excl_csr(), damping=2.1304)
+    S.upload_rhs(F)
+    ms = capi.ml_guess(S, M)
+    assert ms is not None and ms > 0.0
+    g_timed = S.download_mu()
+    assert capi.ml_guess(S, M, timed=False) is None
+    g_untimed = S.download_mu()
+    np.testing.assert_array_equal(g_untimed, g_timed)
+    capi.ml_guess(S, M, timed=False)
+    it, norms, _ = S.solve_resident(1, 1e-8, 200)
+    mu = S.download_mu()
+    B = O.BSystem(w.pos, w.pol, w.excl_csr(), damping=2.1304)
+    print("untimed guess: iterations", it, "last norm", norms[-1])
+    assert norms[-1] < 1e-8 and np.linalg.norm(F - B.apply(mu)) < 1e-8
