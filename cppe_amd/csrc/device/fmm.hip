@@ -217,6 +217,9 @@ void ensure_clusters(System& S) {
 // captured into a graph.
 void fmm_reserve(System& S) {
   FmmState& F     = state(S);
+  // everything below depends on the tree, the order and the near variants only: once reserved for
+  // these variants, a later solve neither allocates nor waits for the stream here
+  if (F.reserved_for[0] == F.tune.near_variant[0] && F.reserved_for[1] == F.tune.near_variant[1]) return;
   const size_t nc = S.tree.ncells;
   const int p     = S.opts.order;
   ensure_buf(F.M64, fmm_mbuf_size(nc, 0, p));
@@ -243,6 +246,8 @@ void fmm_reserve(System& S) {
   }
   if (F.tune.near_variant[0] == 4 || F.tune.near_variant[1] == 4) ensure_clusters(S);
   CPPE_HIP_CHECK(hipStreamSynchronize(S.stream));
+  F.reserved_for[0] = F.tune.near_variant[0];
+  F.reserved_for[1] = F.tune.near_variant[1];
 }
 
 // Autotuner of the B-apply launch shapes: for each precision, every near-field candidate
@@ -396,6 +401,7 @@ void fmm_autotune(System& S, int reps, int* choice, double* ms) {
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   cg_graphs_release(S);  // captured graphs hold the previous launch shapes
+  F.reserved_for[0] = F.reserved_for[1] = -1;  // the sweeps may have rebuilt per-variant structures
 }
 
 void fmm_set_tune(System& S, const int* choice) {
@@ -407,6 +413,7 @@ void fmm_set_tune(System& S, const int* choice) {
     decode_overlap(choice[4 * prec + 3], F.tune.overlap[prec], F.tune.near_splits[prec]);
   }
   cg_graphs_release(S);
+  F.reserved_for[0] = F.reserved_for[1] = -1;
 }
 
 void fmm_destroy(System& S) {

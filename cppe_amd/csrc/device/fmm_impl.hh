@@ -82,6 +82,7 @@ struct FmmState {
   // per site, the exclusion partners whose pairs the near field adds (CSR, built once per tree)
   dbuf<int> xn_off, xn_idx;
   bool xn_built = false;
+  int reserved_for[2] = {-1, -1};  // near variants (FP64, FP32) fmm_reserve last prepared for
   dbuf<double4> mu4;      // near-field dipole records (FP64 form)
   dbuf<float4> mu4f;      // near-field dipole records (FP32 form)
   // several right-hand sides per apply (kMultiRhs at a time): their expansions and dipole records
