@@ -665,6 +665,8 @@ void CgState::release_graphs() {
   drop(g_conf);
   drop(g_dir[0]);
   drop(g_dir[1]);
+  drop(g_init[0]);
+  drop(g_init[1]);
 }
 
 CgState::~CgState() {
@@ -1023,14 +1025,26 @@ int device_cg(System& S, CgState& C, double thresh, int maxiter, bool mixed, CgR
   upload_params(S, C, params);
   // r0 = rhs - B x0, z0 = alpha r0, p = z0. The mixed form takes r0 from an FP32 apply: it is
   // replaced by an FP64 true residual at the first reliable update, before any convergence test.
-  apply(S, C, C.x, C.t, mixed);
-  hipLaunchKernelGGL(k_sg_init, dim3(C.nchunk), dim3(256), 0, st, vecs(S, C));
-  launch_prec<0>(S, C, 0);
-  launch_final<F_INIT>(S, C, mixed);
-  ++C.expect;  // every launch sequence below runs one final kernel per step
-  read_status(S, C);
-  int k = 0, nrel = 0, nlaunch = 0, nsteps = 0, nread = 1, nconf = 0;
+  // The first step goes in the same launch (its horizon is one step: no reduction rate is known
+  // yet); it changes nothing when the initial decision stopped a segment or asked for a true
+  // residual, as any step launched past an event.
   int m = mixed ? 1 : 0;
+  const bool first = maxiter > 0;
+  auto initial = [&] {
+    apply(S, C, C.x, C.t, mixed);
+    hipLaunchKernelGGL(k_sg_init, dim3(C.nchunk), dim3(256), 0, st, vecs(S, C));
+    launch_prec<0>(S, C, 0);
+    launch_final<F_INIT>(S, C, mixed, !first);
+    if (first) seq_step_direction(S, C, mixed, true);
+  };
+  if (first)
+    run_graph(S, C.g_init[m], initial);
+  else
+    initial();
+  C.expect += first ? 2 : 1;  // every launch sequence below runs one final kernel per step
+  read_status(S, C, first ? 2 : 1);
+  int k = first ? C.h_status[ST_IT] + C.h_status[ST_STEPPED] : 0;
+  int nrel = 0, nlaunch = first ? 1 : 0, nsteps = first ? 1 : 0, nread = 1, nconf = 0;
   res.fp64_fallback = -1;
   // Loop invariant at the top: every active segment has completed step, replacement and
   // direction of its last iteration. Pending segments (mixed form) are confirmed together: inside a

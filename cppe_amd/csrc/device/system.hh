@@ -129,6 +129,7 @@ struct CgState {
   // g_step[form][c]: c consecutive steps, 1 <= c <= kMaxGraphSteps (form 0 FP64, 1 mixed)
   static constexpr int kMaxGraphSteps = 8;
   hipGraphExec_t g_step[2][kMaxGraphSteps + 1] = {}, g_rel = nullptr, g_conf = nullptr, g_dir[2] = {nullptr, nullptr};
+  hipGraphExec_t g_init[2] = {nullptr, nullptr};  // initial residual and the first step (form 0 FP64, 1 mixed)
   double *x = nullptr, *rhs = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr, *t = nullptr;
   float4* rec32   = nullptr;  // near-field dipole records of p written by the direction kernels
   double4* rec64 = nullptr;   // (one of them, in the precision of the step's apply; none: nrhs > 1)
